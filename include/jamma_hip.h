@@ -1,0 +1,255 @@
+/*
+ * jamma_hip.h -- C ABI of libjamma_hip.so, the MI355X (gfx950) implementation of
+ * JAMMA's linear-mixed-model association hot path.
+ *
+ * Conventions
+ *  - Every entry point returns int: JM_OK (0) or a negative JM_ERR_* code, and
+ *    never throws. jm_last_error() returns the message of the calling thread's
+ *    last failure.
+ *  - All array arguments are DEVICE pointers to caller-owned memory unless the
+ *    name says *_host. `stream` is a hipStream_t (NULL = default stream); every
+ *    call is asynchronous on that stream.
+ *  - Row-major storage; `ld*` arguments are leading dimensions in elements.
+ *  - Genotypes are int8 A1-allele counts 0/1/2 with -1 for a missing call, in
+ *    SNP-major rows ("g", one row per SNP) or sample-major rows ("gT").
+ *    Padding elements must be 0.
+ *
+ * Reference paths below are relative to the michael-denyer/jamma repository
+ * root (v2.0.1). The reference is Python/JAX; these entry points replace its
+ * JAX/NumPy operators, which a Python host binds through ctypes
+ * (jamma_amd/_lib.py).
+ */
+#ifndef JAMMA_HIP_H
+#define JAMMA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JM_OK 0
+#define JM_ERR_ARG (-1)
+#define JM_ERR_HIP (-2)
+#define JM_ERR_UNSUPPORTED (-3)
+
+#define JM_VERSION 100
+
+/* Library version (JM_VERSION). */
+int jm_version(void);
+
+/* Copies the calling thread's last error message into buf (NUL-terminated). */
+int jm_last_error(char* buf, size_t len);
+
+/* ------------------------------------------------------------------------ */
+/* PLINK decode + pass-1 statistics                                          */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Replaces bed-reader's windowed read in stream_genotype_chunks
+ * (src/jamma/io/plink.py:239-300) and the pass-1 per-SNP statistics
+ * (src/jamma/lmm/runner_streaming.py:248-284, kinship/compute.py:601-611).
+ *
+ * bed: SNP-major .bed payload (magic bytes stripped), bytes_per_snp =
+ *      ceil(n_samples_total / 4). snp_idx (n_snps entries, may be NULL =
+ *      0..n_snps-1) selects the SNP rows to decode; sample_idx (n_out entries,
+ *      may be NULL = 0..n_out-1) selects and orders the samples.
+ * g (may be NULL): n_snps x ld_g int8 output, 0/1/2 = A1 count, -1 missing,
+ *      columns [n_out, ld_g) are written as 0.
+ * counts (may be NULL): n_snps x 4 int32 = (#0, #1, #2, #missing) over the
+ *      selected samples; mean, missing rate, var>0 and HWE follow from them.
+ */
+int jm_decode_bed(const uint8_t* bed, int64_t bytes_per_snp, const int32_t* snp_idx,
+                  int64_t n_snps, const int32_t* sample_idx, int64_t n_out, int8_t* g,
+                  int64_t ld_g, int32_t* counts, void* stream);
+
+/* out (cols x ld_out) = transpose of in (rows x ld_in), int8. Used to form the
+ * sample-major genotype tile that the kinship GEMM reads. */
+int jm_transpose_i8(const int8_t* in, int64_t rows, int64_t cols, int64_t ld_in, int8_t* out,
+                    int64_t ld_out, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Centred kinship (-gk 1)                                                   */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Replaces _accumulate_kinship (src/jamma/kinship/compute.py:46-57) inside the
+ * batch loops of compute_centered_kinship (:205-214) and
+ * compute_kinship_streaming (:663-681), together with impute_and_center
+ * (src/jamma/kinship/missing.py:22-61), which this library applies as exact
+ * corrections (see jm_kinship_finalize).
+ *
+ * S (n x n int32) += X0 X0^T over one tile of n_snps SNPs, where X0 = gT with
+ * missing calls as 0; gT is sample-major (n x ld_gT, ld_gT % 16 == 0). Exact
+ * int32 accumulation on int8 MFMA (v_mfma_i32_16x16x64_i8). Only 128x128 tiles
+ * on or below the block diagonal are written; jm_kinship_finalize mirrors them.
+ */
+int jm_kinship_syrk_i8(const int8_t* gT, int64_t ld_gT, int64_t n, int64_t n_snps, int32_t* S,
+                       void* stream);
+
+/*
+ * Per-tile mean-imputation / centring terms, accumulated over SNP tiles
+ * (the algebra of impute_and_center, kinship/missing.py:48-61):
+ *   mu[j]    = float64 nanmean of SNP j (host computes from jm_decode_bed counts)
+ *   u[i]    += sum_j (g_ij >= 0 ? g_ij mu_j : mu_j^2)
+ *   C[k][i] += sum_{j: g_kj missing} mu_j * (g_ij >= 0 ? g_ij : mu_j / 2)
+ * g is the SNP-major tile (n_snps x ld_g), gT the sample-major one
+ * (n x ld_gT). C (n x n float64) is only touched for samples with missing
+ * calls in the tile.
+ */
+int jm_kinship_impute_terms(const int8_t* g, int64_t ld_g, const int8_t* gT, int64_t ld_gT,
+                            int64_t n, int64_t n_snps, const double* mu, double* u, double* C,
+                            int has_missing, void* stream);
+
+/*
+ * K[i][k] = (S_ik - u_i - u_k + s0 + C[i][k] + C[k][i]) / p_snps
+ * i.e. Xc Xc^T / p with Xc = impute_and_center(X) (kinship/compute.py:217,684).
+ * S is read from the lower tiles written by jm_kinship_syrk_i8. S_sub / u_sub /
+ * C_sub (may be NULL) are subtracted first: the LOCO subtraction
+ * K_c = (S_full - S_c) / (p - p_c) of kinship/compute.py:898-916.
+ */
+int jm_kinship_finalize(const int32_t* S, const double* u, const double* C, double s0,
+                        const int32_t* S_sub, const double* u_sub, const double* C_sub,
+                        double s0_sub, int64_t n, double p_snps, double* K, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Rotation into the kinship eigenbasis                                      */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Replaces UtG = UT @ geno_chunk (src/jamma/lmm/runner_streaming.py:392-410;
+ * also runner_jax.py:236-254 and loco.py:406-423).
+ * Z (n x ldz, row i = eigen-index, column s = SNP) = U^T X, where column s of X
+ * is SNP row s of g with missing calls replaced by mean[s] (float32-derived
+ * mean, runner_streaming.py:322-329,429-434). UT = U^T, n x ld_u float64.
+ * FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 output.
+ */
+int jm_rotate_f64(const double* UT, int64_t ld_u, const int8_t* g, int64_t ld_g,
+                  const double* mean, int64_t n, int64_t n_snps, double* Z, int64_t ldz,
+                  void* stream);
+
+/* Generic C (m x ldc) = A (m x k, lda) * B^T where B is (ncols x k, ldb), all
+ * float64 (UtW = UT @ W and Uty = UT @ y, runner_streaming.py:349-352). */
+int jm_gemm_nt_f64(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t m,
+                   int64_t ncols, int64_t k, double* C, int64_t ldc, void* stream);
+
+/* Splits UT (float64, n x ld_u) into three bfloat16 planes hi/mid/lo with
+ * UT ~= hi + mid + lo (24 significant bits), each n x ld_s (ld_s % 32 == 0). */
+int jm_split_bf16x3(const double* UT, int64_t ld_u, int64_t n, uint16_t* hi, uint16_t* mid,
+                    uint16_t* lo, int64_t ld_s, void* stream);
+
+/*
+ * Mixed-precision rotation: Z (float32, n x ldz) = U^T X using bf16 MFMA
+ * (v_mfma_f32_16x16x32_bf16) with FP32 accumulation:
+ *   hi^T X1 + mid^T X1 + lo^T X1 + hi^T X2
+ * X1 = genotypes with missing -> bf16(mean), X2 = (mean - bf16(mean)) at missing.
+ */
+int jm_rotate_bf16x3(const uint16_t* hi, const uint16_t* mid, const uint16_t* lo, int64_t ld_s,
+                     const int8_t* g, int64_t ld_g, const double* mean, int64_t n,
+                     int64_t n_snps, float* Z, int64_t ldz, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Per-SNP REML/MLE lambda search + Wald/LRT/Score                           */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Spectrum moments. For each column s of Z (float64 when z_is_f32 == 0, else
+ * float32) and each term t = (cluster c, power m) of the compressed spectrum:
+ *   mom[s][b][t] = sum_{i in c} Z[i][s] * V[b][i] * delta_i^m   (b < nv)
+ *   mom[s][nv][t] = sum_{i in c} Z[i][s]^2 * delta_i^m
+ * V (nv x ldv float64) are the partner vectors (rotated covariates, then the
+ * rotated phenotype). Cluster c covers eigen-indices [cl_start, cl_start +
+ * cl_size), has cl_terms powers starting at term cl_toff; delta (n) = d_i -
+ * d_c. mom is (n_snps x (nv+1) x ldm). This is the lambda-independent part of
+ * batch_compute_uab + calc_pab row 0 (likelihood_jax.py:112-186, 342-393).
+ */
+int jm_spectrum_moments(const void* Z, int z_is_f32, int64_t ldz, int64_t n, int64_t n_snps,
+                        const double* V, int64_t ldv, int nv, const int32_t* cl_start,
+                        const int32_t* cl_size, const int32_t* cl_terms,
+                        const int32_t* cl_toff, int n_clusters, const double* delta,
+                        double* mom, int64_t ldm, void* stream);
+
+/* Parameters of one (cohort, phenotype) association problem. Device pointers. */
+typedef struct jm_lmm_problem {
+  int32_t n;           /* analysed samples */
+  int32_t n_cvt;       /* covariate columns incl. intercept (c) */
+  int32_t n_terms;     /* T */
+  int32_t n_clusters;  /* C */
+  int32_t n_grid;      /* lambda grid points (reference default 50) */
+  int32_t n_iter;      /* golden-section iterations = max(n_refine, 20) */
+  double l_min, l_max;
+  const int32_t* term_cluster; /* T */
+  const int32_t* term_m;       /* T */
+  const double* cl_center;     /* C */
+  const double* logdet_mom;    /* T : sum_{i in c} delta_i^m */
+  const double* shared_mom;    /* n_index x T : moments of the lambda-independent
+                                  (covariate/phenotype) pairs, packed GEMMA order;
+                                  rows of genotype pairs are ignored */
+  const int32_t* pair_row;     /* n_index : >= 0 -> row of the per-SNP moments
+                                  (pair involves the genotype), -1 -> shared row */
+  const double* grid_loglam;   /* n_grid : numpy.linspace(log l_min, log l_max) */
+  /* tables filled by jm_lmm_prepare */
+  double* grid_kappa;   /* T x n_grid */
+  double* grid_shared;  /* n_grid x n_index */
+  double* grid_logdet;  /* n_grid */
+  double* null_kappa;   /* T */
+  double* null_shared;  /* n_index */
+  double lambda_null;   /* null-model MLE lambda (Score test) */
+  double logl_null;     /* null-model MLE log-likelihood (LRT) */
+} jm_lmm_problem;
+
+/* Output columns of jm_lmm_assoc (n_snps x JM_NOUT float64). */
+#define JM_OUT_BETA 0
+#define JM_OUT_SE 1
+#define JM_OUT_P_WALD 2
+#define JM_OUT_L_REMLE 3
+#define JM_OUT_LOGL_H1 4
+#define JM_OUT_L_MLE 5
+#define JM_OUT_LOGL_MLE 6
+#define JM_OUT_P_LRT 7
+#define JM_OUT_P_SCORE 8
+#define JM_OUT_BETA_SCORE 9
+#define JM_OUT_SE_SCORE 10
+#define JM_OUT_EVALS 11
+#define JM_NOUT 12
+
+/*
+ * Null model MLE (compute_null_model_mle, src/jamma/lmm/likelihood.py:571-607,
+ * scalar golden section :395-474 over mle_log_likelihood_null :349-392) on one
+ * wavefront. Writes out_host-free device doubles out[0] = lambda0,
+ * out[1] = logl_H0, out[2] = null REML lambda, out[3] = null REML logl.
+ */
+int jm_lmm_null(const jm_lmm_problem* prob, double* out, void* stream);
+
+/* Fills the grid / null tables of prob (needs prob->lambda_null). */
+int jm_lmm_prepare(const jm_lmm_problem* prob, void* stream);
+
+/*
+ * Per-SNP association for n_snps SNPs from their spectrum moments
+ * (xmom: n_snps x (n_cvt+2) x ldm, jm_spectrum_moments with V = [UtW | Uty]).
+ * Replaces, per chunk, batch_compute_iab, golden_section_optimize_lambda
+ * (+_mle), batch_calc_wald_stats, batch_calc_score_stats and
+ * calc_lrt_pvalue_jax (likelihood_jax.py:231-883; call sites
+ * runner_streaming.py:471-554). mode: 1 Wald, 2 LRT, 3 Score, 4 all.
+ * One wavefront per SNP; the reference's 50-point log grid + golden-section
+ * schedule is kept exactly.
+ */
+int jm_lmm_assoc(const jm_lmm_problem* prob, int mode, const double* xmom, int64_t ldm,
+                 int64_t n_snps, double* out, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Host-callable special functions (the device code compiled for the CPU),   */
+/* exported so the CPU test-suite can check them against SciPy.              */
+/* ------------------------------------------------------------------------ */
+
+/* p = I_x(a, b), regularised incomplete beta (jax.scipy.special.betainc). */
+double jm_host_betainc(double a, double b, double x);
+/* chi-square(1) survival function (jax.scipy.stats.chi2.sf(x, 1)). */
+double jm_host_chi2_sf1(double x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JAMMA_HIP_H */

@@ -1,0 +1,102 @@
+"""ctypes binding of libjamma_hip.so (the C ABI declared in include/jamma_hip.h).
+
+The library is required: importing this module raises if it is missing, and
+every call raises RuntimeError with the library's own message on failure.
+There is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int, c_int32, c_int64, c_size_t, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(os.environ.get("JM_LIBRARY", Path(__file__).resolve().parent / "lib" / "libjamma_hip.so"))
+
+JM_NOUT = 12
+OUT = {
+    "beta": 0, "se": 1, "p_wald": 2, "l_remle": 3, "logl_H1": 4, "l_mle": 5, "logl_mle": 6,
+    "p_lrt": 7, "p_score": 8, "beta_score": 9, "se_score": 10, "evals": 11,
+}
+
+
+class LmmProblem(ctypes.Structure):
+    """Mirror of jm_lmm_problem."""
+
+    _fields_ = [
+        ("n", c_int32), ("n_cvt", c_int32), ("n_terms", c_int32), ("n_clusters", c_int32),
+        ("n_grid", c_int32), ("n_iter", c_int32),
+        ("l_min", c_double), ("l_max", c_double),
+        ("term_cluster", c_void_p), ("term_m", c_void_p), ("cl_center", c_void_p),
+        ("logdet_mom", c_void_p), ("shared_mom", c_void_p), ("pair_row", c_void_p),
+        ("grid_loglam", c_void_p),
+        ("grid_kappa", c_void_p), ("grid_shared", c_void_p), ("grid_logdet", c_void_p),
+        ("null_kappa", c_void_p), ("null_shared", c_void_p),
+        ("lambda_null", c_double), ("logl_null", c_double),
+    ]
+
+
+_SIGS = {
+    "jm_version": ([], c_int),
+    "jm_last_error": ([ctypes.c_char_p, c_size_t], c_int),
+    "jm_decode_bed": ([c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64,
+                       c_void_p, c_void_p], c_int),
+    "jm_transpose_i8": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_void_p], c_int),
+    "jm_kinship_syrk_i8": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p], c_int),
+    "jm_kinship_impute_terms": ([c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_void_p,
+                                 c_void_p, c_void_p, c_int, c_void_p], c_int),
+    "jm_kinship_finalize": ([c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p,
+                             c_double, c_int64, c_double, c_void_p, c_void_p], c_int),
+    "jm_rotate_f64": ([c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_int64, c_void_p,
+                       c_int64, c_void_p], c_int),
+    "jm_gemm_nt_f64": ([c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p,
+                        c_int64, c_void_p], c_int),
+    "jm_split_bf16x3": ([c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
+                         c_void_p], c_int),
+    "jm_rotate_bf16x3": ([c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p,
+                          c_int64, c_int64, c_void_p, c_int64, c_void_p], c_int),
+    "jm_spectrum_moments": ([c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int,
+                             c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                             c_int64, c_void_p], c_int),
+    "jm_lmm_null": ([POINTER(LmmProblem), c_void_p, c_void_p], c_int),
+    "jm_lmm_prepare": ([POINTER(LmmProblem), c_void_p], c_int),
+    "jm_lmm_assoc": ([POINTER(LmmProblem), c_int, c_void_p, c_int64, c_int64, c_void_p, c_void_p],
+                     c_int),
+    "jm_host_betainc": ([c_double, c_double, c_double], c_double),
+    "jm_host_chi2_sf1": ([c_double], c_double),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"libjamma_hip.so not found at {LIB_PATH}; build it with "
+            "`python -m jamma_amd.csrc.build` (hipcc, gfx950)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+lib = _load()
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    lib.jm_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what} failed (rc={rc}): {last_error()}")
+
+
+def call(name: str, *args) -> None:
+    """Call an int-returning entry point and raise on a non-zero status."""
+    check(getattr(lib, name)(*args), name)

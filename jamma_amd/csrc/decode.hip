@@ -1,0 +1,119 @@
+// PLINK .bed decoding on the device, fused with the pass-1 per-SNP counts.
+//
+// Replaces the bed-reader windowed read used by stream_genotype_chunks
+// (src/jamma/io/plink.py:239-300; bed-reader 1.0.6, count_A1 = True) and the
+// pass-1 statistics (src/jamma/lmm/runner_streaming.py:248-284,
+// src/jamma/kinship/compute.py:601-611): the host turns the four counts into
+// the float32 nanmean, missing rate and var > 0 exactly as NumPy does.
+//
+// 2-bit code (low bits first, four samples per byte) -> A1 count:
+//   00 -> 2, 01 -> missing (-1), 10 -> 1, 11 -> 0.
+// One block per SNP row; the row's bytes are read once (through L2), the int8
+// output row is written coalesced, and the counts are reduced in the block.
+// The kernel is HBM-bound: ceil(n_total/4) bytes in, n_out bytes out per SNP.
+
+#include "jm_common.h"
+
+namespace jm {
+
+__global__ void __launch_bounds__(256) k_decode_bed(const uint8_t* __restrict__ bed,
+                                                    int64_t bytes_per_snp,
+                                                    const int32_t* __restrict__ snp_idx,
+                                                    int64_t n_snps,
+                                                    const int32_t* __restrict__ sample_idx,
+                                                    int64_t n_out, int8_t* __restrict__ g,
+                                                    int64_t ld_g, int32_t* __restrict__ counts) {
+  __shared__ int red[4][4];
+  const int64_t row = blockIdx.x;
+  if (row >= n_snps) return;
+  const int64_t src = snp_idx ? (int64_t)snp_idx[row] : row;
+  const uint8_t* b = bed + src * bytes_per_snp;
+  int c0 = 0, c1 = 0, c2 = 0, cm = 0;
+  const int64_t lim = g ? ld_g : n_out;
+  for (int64_t o = threadIdx.x; o < lim; o += blockDim.x) {
+    int8_t v = 0;
+    if (o < n_out) {
+      const int64_t t = sample_idx ? (int64_t)sample_idx[o] : o;
+      const int code = (b[t >> 2] >> (2 * (t & 3))) & 3;
+      v = (code == 0) ? 2 : (code == 2 ? 1 : (code == 3 ? 0 : -1));
+      c0 += (v == 0);
+      c1 += (v == 1);
+      c2 += (v == 2);
+      cm += (v < 0);
+    }
+    if (g) g[row * ld_g + o] = v;
+  }
+  if (counts) {
+    // wave reduce then block reduce (fixed order -> deterministic)
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      c0 += __shfl_xor(c0, m, 64);
+      c1 += __shfl_xor(c1, m, 64);
+      c2 += __shfl_xor(c2, m, 64);
+      cm += __shfl_xor(cm, m, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+      red[w][0] = c0;
+      red[w][1] = c1;
+      red[w][2] = c2;
+      red[w][3] = cm;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      int acc = 0;
+      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) acc += red[k][threadIdx.x];
+      counts[row * 4 + threadIdx.x] = acc;
+    }
+  }
+}
+
+// 64x64 LDS-tiled int8 transpose.
+__global__ void __launch_bounds__(256) k_transpose_i8(const int8_t* __restrict__ in, int64_t rows,
+                                                      int64_t cols, int64_t ld_in,
+                                                      int8_t* __restrict__ out, int64_t ld_out) {
+  __shared__ int8_t tile[64][65];
+  const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int k = ty; k < 64; k += 4) {
+    const int64_t r = r0 + k, c = c0 + tx;
+    tile[k][tx] = (r < rows && c < cols) ? in[r * ld_in + c] : (int8_t)0;
+  }
+  __syncthreads();
+  for (int k = ty; k < 64; k += 4) {
+    const int64_t c = c0 + k, r = r0 + tx;  // out row c, column r
+    if (c < cols && r < rows) out[c * ld_out + r] = tile[tx][k];
+  }
+}
+
+}  // namespace jm
+
+using namespace jm;
+
+extern "C" int jm_decode_bed(const uint8_t* bed, int64_t bytes_per_snp, const int32_t* snp_idx,
+                             int64_t n_snps, const int32_t* sample_idx, int64_t n_out, int8_t* g,
+                             int64_t ld_g, int32_t* counts, void* stream) {
+  JM_CHECK_ARG(n_snps >= 0 && n_out >= 0 && bytes_per_snp >= 0, "negative size");
+  if (n_snps == 0) return JM_OK;
+  JM_CHECK_ARG(bed != nullptr, "null bed");
+  JM_CHECK_ARG(g == nullptr || ld_g >= n_out, "ld_g < n_out");
+  JM_CHECK_ARG(sample_idx != nullptr || (n_out + 3) / 4 <= bytes_per_snp,
+               "n_out exceeds the samples of a .bed row");
+  JM_CHECK_ARG(n_snps < (int64_t)1 << 31, "too many SNPs in one call");
+  hipLaunchKernelGGL(k_decode_bed, dim3((unsigned)n_snps), dim3(256), 0, (hipStream_t)stream, bed,
+                     bytes_per_snp, snp_idx, n_snps, sample_idx, n_out, g, ld_g, counts);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int jm_transpose_i8(const int8_t* in, int64_t rows, int64_t cols, int64_t ld_in,
+                               int8_t* out, int64_t ld_out, void* stream) {
+  JM_CHECK_ARG(rows >= 0 && cols >= 0 && ld_in >= cols && ld_out >= rows, "bad sizes");
+  if (rows == 0 || cols == 0) return JM_OK;
+  JM_CHECK_ARG(in && out, "null pointer");
+  const dim3 grid((unsigned)ceil_div(cols, 64), (unsigned)ceil_div(rows, 64));
+  hipLaunchKernelGGL(k_transpose_i8, grid, dim3(256), 0, (hipStream_t)stream, in, rows, cols,
+                     ld_in, out, ld_out);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}

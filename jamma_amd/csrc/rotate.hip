@@ -1,0 +1,402 @@
+// Rotation of genotypes (and covariates / phenotype) into the kinship
+// eigenbasis: Z = U^T X.
+//
+// Replaces UtG = UT @ geno_chunk (src/jamma/lmm/runner_streaming.py:392-410)
+// and UtW = UT @ W, Uty = UT @ y (:349-352), which the reference runs as host
+// FP64 BLAS.
+//
+//  * k_gemm_f64: FP64 MFMA (v_mfma_f64_16x16x4f64), 64x128 block tile,
+//    4 waves of 32x64, LDS-staged A (UT rows) and B (genotype int8 or float64).
+//  * k_rotate_bf16x3: bf16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32
+//    accumulation over the split UT = hi + mid + lo and the genotype split
+//    X = X1 + X2 (X1 = genotype or bf16(mean) at a missing call, X2 = the
+//    bf16 residual of the mean at missing calls):
+//        Z = hi X1 + mid X1 + lo X1 + hi X2.
+//    128x128 block tile, 4 waves of 64x64.
+//
+// Block order is XCD-aware: consecutive hardware block ids go round-robin to
+// the 8 XCDs, so blocks b, b+8, b+16, ... share an XCD (and its L2). Tiles are
+// assigned so that the blocks of one XCD sweep the SNP columns of the same
+// row panel of UT, which then stays L2/MALL-resident while it is consumed.
+//
+// MFMA lane layouts (16x16 outputs): lane l supplies A[l%16][k-group l/16] and
+// B[k-group l/16][l%16]; the accumulator holds D[4*(l/16)+r][l%16], r = 0..3.
+
+#include "jm_common.h"
+
+namespace jm {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+// XCD-aware mapping of a linear block id onto (row block, column block).
+// Hardware block bid runs on XCD bid % 8; XCD x receives the contiguous tile
+// range [off_x, off_x + cnt_x) of the row-panel-major tile order (bijective).
+__device__ __forceinline__ void xcd_tile(int bid, int nrb, int ncb, int& rb, int& cb) {
+  const int nxcd = 8;
+  const int total = nrb * ncb;
+  const int x = bid % nxcd, j = bid / nxcd;
+  const int base = total / nxcd, rem = total % nxcd;
+  const int off = x * base + (x < rem ? x : rem);
+  const int t = off + j;
+  rb = t / ncb;
+  cb = t % ncb;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 GEMM: C[i][s] = sum_k A[i][k] * B(s, k)
+// ---------------------------------------------------------------------------
+constexpr int F_BM = 64, F_BN = 128, F_BK = 32;
+constexpr int F_AS = F_BK + 1;  // padded LDS row stride (doubles)
+
+struct BSrcGeno {  // B(s, k) = g[s][k] (int8, -1 -> mean[s])
+  const int8_t* g;
+  int64_t ld;
+  const double* mean;
+};
+struct BSrcF64 {  // B(s, k) = b[s][k]
+  const double* b;
+  int64_t ld;
+};
+
+template <typename BS>
+struct BTile;
+
+template <>
+struct BTile<BSrcGeno> {
+  static constexpr int kStride = F_BK + 4;  // bytes
+  int8_t v[F_BN * kStride];
+};
+template <>
+struct BTile<BSrcF64> {
+  static constexpr int kStride = F_BK + 1;  // doubles
+  double v[F_BN * kStride];
+};
+
+__device__ __forceinline__ void load_b_tile(BTile<BSrcGeno>& t, const BSrcGeno& src, int64_t s0,
+                                            int64_t k0, int64_t n_cols, int64_t K, int tid) {
+  // 128 rows x 32 bytes: 256 threads x 16 bytes
+  const int r = tid >> 1;
+  const int c = (tid & 1) * 16;
+  const int64_t s = s0 + r;
+  int8_t tmp[16];
+  if (s < n_cols && k0 + c + 16 <= K) {
+    const int4 w = *reinterpret_cast<const int4*>(src.g + s * src.ld + k0 + c);
+    memcpy(tmp, &w, 16);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int64_t k = k0 + c + e;
+      tmp[e] = (s < n_cols && k < K) ? src.g[s * src.ld + k] : (int8_t)0;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 16; ++e) t.v[r * BTile<BSrcGeno>::kStride + c + e] = tmp[e];
+}
+
+__device__ __forceinline__ void load_b_tile(BTile<BSrcF64>& t, const BSrcF64& src, int64_t s0,
+                                            int64_t k0, int64_t n_cols, int64_t K, int tid) {
+  // 128 rows x 32 doubles = 4096: 16 per thread
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int idx = tid + 256 * u;
+    const int r = idx / F_BK, c = idx % F_BK;
+    const int64_t s = s0 + r, k = k0 + c;
+    t.v[r * BTile<BSrcF64>::kStride + c] = (s < n_cols && k < K) ? src.b[s * src.ld + k] : 0.0;
+  }
+}
+
+__device__ __forceinline__ double b_frag(const BTile<BSrcGeno>& t, int col, int k, double mean) {
+  const int8_t g = t.v[col * BTile<BSrcGeno>::kStride + k];
+  return g < 0 ? mean : (double)g;
+}
+__device__ __forceinline__ double b_frag(const BTile<BSrcF64>& t, int col, int k, double) {
+  return t.v[col * BTile<BSrcF64>::kStride + k];
+}
+
+__device__ __forceinline__ double col_mean(const BSrcGeno& src, int64_t s, int64_t n_cols) {
+  return s < n_cols ? src.mean[s] : 0.0;
+}
+__device__ __forceinline__ double col_mean(const BSrcF64&, int64_t, int64_t) { return 0.0; }
+
+template <typename BS>
+__global__ void __launch_bounds__(256) k_gemm_f64(const double* __restrict__ A, int64_t lda,
+                                                  BS bsrc, int64_t M, int64_t n_cols, int64_t K,
+                                                  double* __restrict__ C, int64_t ldc) {
+  __shared__ double As[F_BM * F_AS];
+  __shared__ BTile<BS> Bs;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int nrb = (int)((M + F_BM - 1) / F_BM), ncb = (int)((n_cols + F_BN - 1) / F_BN);
+  int rb, cb;
+  xcd_tile(blockIdx.x, nrb, ncb, rb, cb);
+  const int64_t i0 = (int64_t)rb * F_BM, s0 = (int64_t)cb * F_BN;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+  double means[4];
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) means[tj] = col_mean(bsrc, s0 + wc + tj * 16 + lr, n_cols);
+  d4_t acc[2][4];
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = (d4_t){0.0, 0.0, 0.0, 0.0};
+
+  for (int64_t k0 = 0; k0 < K; k0 += F_BK) {
+    // A tile: 64 x 32 doubles, 8 per thread
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int idx = tid + 256 * u;
+      const int r = idx / F_BK, c = idx % F_BK;
+      const int64_t i = i0 + r, k = k0 + c;
+      As[r * F_AS + c] = (i < M && k < K) ? A[i * lda + k] : 0.0;
+    }
+    load_b_tile(Bs, bsrc, s0, k0, n_cols, K, tid);
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < F_BK; kk += 4) {
+      double a[2];
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti) a[ti] = As[(wr + ti * 16 + lr) * F_AS + kk + lk];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        const double b = b_frag(Bs, wc + tj * 16 + lr, kk + lk, means[tj]);
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+          acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b, acc[ti][tj], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
+        const int64_t s = s0 + wc + tj * 16 + lr;
+        if (i < M && s < n_cols) C[i * ldc + s] = acc[ti][tj][r];
+      }
+}
+
+// ---------------------------------------------------------------------------
+// bf16 split of UT
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ double bf16_to_f64(uint16_t h) {
+  return (double)__uint_as_float(((uint32_t)h) << 16);
+}
+
+__global__ void k_split_bf16x3(const double* __restrict__ UT, int64_t ld_u, int64_t n,
+                               uint16_t* __restrict__ hi, uint16_t* __restrict__ mid,
+                               uint16_t* __restrict__ lo, int64_t ld_s) {
+  const int64_t total = n * ld_s;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / ld_s, k = idx % ld_s;
+    const double u = (k < n) ? UT[i * ld_u + k] : 0.0;
+    const uint16_t h = f32_to_bf16_rne((float)u);
+    const double r1 = u - bf16_to_f64(h);
+    const uint16_t m = f32_to_bf16_rne((float)r1);
+    const double r2 = r1 - bf16_to_f64(m);
+    const uint16_t l = f32_to_bf16_rne((float)r2);
+    hi[idx] = h;
+    mid[idx] = m;
+    lo[idx] = l;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BF16x3 rotation
+// ---------------------------------------------------------------------------
+constexpr int B_BM = 128, B_BN = 128, B_BK = 64;
+constexpr int B_AS = B_BK + 8;   // bf16 elements per LDS row (16-byte aligned rows)
+constexpr int B_GS = B_BK + 16;  // int8 bytes per genotype LDS row
+
+__global__ void __launch_bounds__(256) k_rotate_bf16x3(
+    const uint16_t* __restrict__ Ahi, const uint16_t* __restrict__ Amid,
+    const uint16_t* __restrict__ Alo, int64_t lda, const int8_t* __restrict__ g, int64_t ldg,
+    const double* __restrict__ mean, int64_t M, int64_t n_cols, int64_t K, float* __restrict__ Z,
+    int64_t ldz) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[3][B_BM * B_AS];
+  __shared__ __attribute__((aligned(16))) int8_t Gs[B_BN * B_GS];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int nrb = (int)((M + B_BM - 1) / B_BM), ncb = (int)((n_cols + B_BN - 1) / B_BN);
+  int rb, cb;
+  xcd_tile(blockIdx.x, nrb, ncb, rb, cb);
+  const int64_t i0 = (int64_t)rb * B_BM, s0 = (int64_t)cb * B_BN;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  // per-column bf16 split of the imputation mean: m1 = bf16(mean), m2 = bf16(mean - m1)
+  uint16_t m1[4], m2[4];
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) {
+    const int64_t s = s0 + wc + tj * 16 + lr;
+    const double mu = (s < n_cols) ? mean[s] : 0.0;
+    m1[tj] = f32_to_bf16_rne((float)mu);
+    m2[tj] = f32_to_bf16_rne((float)(mu - bf16_to_f64(m1[tj])));
+  }
+
+  f4_t acc[4][4];
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = (f4_t){0.f, 0.f, 0.f, 0.f};
+
+  const uint16_t* Ap[3] = {Ahi, Amid, Alo};
+  for (int64_t k0 = 0; k0 < K; k0 += B_BK) {
+    // A planes: 128 rows x 64 bf16 = 128 B per row -> 8 x 16 B chunks; 1024 chunks / plane
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int idx = tid + 256 * u;
+        const int r = idx >> 3, c = (idx & 7) * 8;
+        const int64_t i = i0 + r;
+        int4 w = {0, 0, 0, 0};
+        if (i < M && k0 + c + 8 <= lda) w = *reinterpret_cast<const int4*>(Ap[pl] + i * lda + k0 + c);
+        *reinterpret_cast<int4*>(&As[pl][r * B_AS + c]) = w;
+      }
+    }
+    // genotype tile: 128 SNP rows x 64 bytes -> 4 x 16 B per row; 512 chunks
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int idx = tid + 256 * u;
+      const int r = idx >> 2, c = (idx & 3) * 16;
+      const int64_t s = s0 + r;
+      int4 w = {0, 0, 0, 0};
+      if (s < n_cols) {
+        if (k0 + c + 16 <= K) {
+          w = *reinterpret_cast<const int4*>(g + s * ldg + k0 + c);
+        } else {
+          int8_t tmp[16];
+#pragma unroll
+          for (int e = 0; e < 16; ++e) tmp[e] = (k0 + c + e < K) ? g[s * ldg + k0 + c + e] : (int8_t)0;
+          memcpy(&w, tmp, 16);
+        }
+      }
+      *reinterpret_cast<int4*>(&Gs[r * B_GS + c]) = w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < B_BK; kk += 32) {
+      // B fragments: 8 consecutive k for column (wc + tj*16 + lr)
+      bf16x8_t b1[4], b2[4];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        const uint2 raw = *reinterpret_cast<const uint2*>(&Gs[(wc + tj * 16 + lr) * B_GS + kk + 8 * lk]);
+        uint16_t v1[8], v2[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int8_t x = (int8_t)(((e < 4 ? raw.x : raw.y) >> (8 * (e & 3))) & 0xFF);
+          const bool miss = x < 0;
+          // integers 0,1,2 are exact in bf16: 0x0000, 0x3F80, 0x4000
+          const uint16_t xi = (x == 0) ? (uint16_t)0 : (x == 1 ? (uint16_t)0x3F80 : (uint16_t)0x4000);
+          v1[e] = miss ? m1[tj] : xi;
+          v2[e] = miss ? m2[tj] : (uint16_t)0;
+        }
+        memcpy(&b1[tj], v1, 16);
+        memcpy(&b2[tj], v2, 16);
+      }
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) {
+        const int row = wr + ti * 16 + lr;
+        bf16x8_t ah, am, al;
+        memcpy(&ah, &As[0][row * B_AS + kk + 8 * lk], 16);
+        memcpy(&am, &As[1][row * B_AS + kk + 8 * lk], 16);
+        memcpy(&al, &As[2][row * B_AS + kk + 8 * lk], 16);
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) {
+          f4_t c = acc[ti][tj];
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b1[tj], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, b1[tj], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b2[tj], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b1[tj], c, 0, 0, 0);
+          acc[ti][tj] = c;
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
+        const int64_t s = s0 + wc + tj * 16 + lr;
+        if (i < M && s < n_cols) Z[i * ldz + s] = acc[ti][tj][r];
+      }
+}
+
+}  // namespace jm
+
+using namespace jm;
+
+extern "C" int jm_rotate_f64(const double* UT, int64_t ld_u, const int8_t* g, int64_t ld_g,
+                             const double* mean, int64_t n, int64_t n_snps, double* Z, int64_t ldz,
+                             void* stream) {
+  JM_CHECK_ARG(n >= 1 && n_snps >= 0, "bad sizes");
+  if (n_snps == 0) return JM_OK;
+  JM_CHECK_ARG(UT && g && mean && Z, "null pointer");
+  JM_CHECK_ARG(ld_u >= n && ld_g >= n && ldz >= n_snps, "bad leading dimension");
+  JM_CHECK_ARG(ld_g % 16 == 0 && ((uintptr_t)g % 16) == 0, "g must be 16-byte aligned rows");
+  const int nrb = ceil_div(n, F_BM), ncb = ceil_div(n_snps, F_BN);
+  BSrcGeno src{g, ld_g, mean};
+  hipLaunchKernelGGL(k_gemm_f64<BSrcGeno>, dim3(nrb * ncb), dim3(256), 0, (hipStream_t)stream,
+                     UT, ld_u, src, n, n_snps, n, Z, ldz);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int jm_gemm_nt_f64(const double* A, int64_t lda, const double* B, int64_t ldb,
+                              int64_t m, int64_t ncols, int64_t k, double* C, int64_t ldc,
+                              void* stream) {
+  JM_CHECK_ARG(m >= 1 && ncols >= 0 && k >= 1, "bad sizes");
+  if (ncols == 0) return JM_OK;
+  JM_CHECK_ARG(A && B && C, "null pointer");
+  JM_CHECK_ARG(lda >= k && ldb >= k && ldc >= ncols, "bad leading dimension");
+  const int nrb = ceil_div(m, F_BM), ncb = ceil_div(ncols, F_BN);
+  BSrcF64 src{B, ldb};
+  hipLaunchKernelGGL(k_gemm_f64<BSrcF64>, dim3(nrb * ncb), dim3(256), 0, (hipStream_t)stream, A,
+                     lda, src, m, ncols, k, C, ldc);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int jm_split_bf16x3(const double* UT, int64_t ld_u, int64_t n, uint16_t* hi,
+                               uint16_t* mid, uint16_t* lo, int64_t ld_s, void* stream) {
+  JM_CHECK_ARG(n >= 1 && ld_u >= n && ld_s >= n && ld_s % 32 == 0, "bad sizes");
+  JM_CHECK_ARG(UT && hi && mid && lo, "null pointer");
+  hipLaunchKernelGGL(k_split_bf16x3, dim3(2048), dim3(256), 0, (hipStream_t)stream, UT, ld_u, n,
+                     hi, mid, lo, ld_s);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int jm_rotate_bf16x3(const uint16_t* hi, const uint16_t* mid, const uint16_t* lo,
+                                int64_t ld_s, const int8_t* g, int64_t ld_g, const double* mean,
+                                int64_t n, int64_t n_snps, float* Z, int64_t ldz, void* stream) {
+  JM_CHECK_ARG(n >= 1 && n_snps >= 0, "bad sizes");
+  if (n_snps == 0) return JM_OK;
+  JM_CHECK_ARG(hi && mid && lo && g && mean && Z, "null pointer");
+  JM_CHECK_ARG(ld_s >= n && ld_s % 32 == 0 && ld_g >= n && ld_g % 16 == 0 && ldz >= n_snps,
+               "bad leading dimension");
+  JM_CHECK_ARG(((uintptr_t)g % 16) == 0 && ((uintptr_t)hi % 16) == 0 &&
+                   ((uintptr_t)mid % 16) == 0 && ((uintptr_t)lo % 16) == 0,
+               "operands must be 16-byte aligned");
+  const int nrb = ceil_div(n, B_BM), ncb = ceil_div(n_snps, B_BN);
+  hipLaunchKernelGGL(k_rotate_bf16x3, dim3(nrb * ncb), dim3(256), 0, (hipStream_t)stream, hi, mid,
+                     lo, ld_s, g, ld_g, mean, n, n_snps, n, Z, ldz);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}

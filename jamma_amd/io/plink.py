@@ -1,0 +1,141 @@
+"""PLINK binary fileset access with on-device decoding.
+
+Mirrors the metadata/validation interface of src/jamma/io/plink.py
+(get_plink_metadata :92-136, get_chromosome_partitions :139-175,
+validate_plink_dimensions :178-220). Instead of bed-reader's host decode
+(stream_genotype_chunks :239-300), the raw 2-bit payload of a SNP window is
+copied to HBM and decoded there by jm_decode_bed (int8 A1 counts, -1 missing,
+plus per-SNP genotype counts).
+"""
+
+from __future__ import annotations
+
+from pathlib import Path
+from typing import Any
+
+import numpy as np
+import torch
+
+from jamma_amd import _lib
+from jamma_amd.device import ptr, require_gpu, round_up, stream_ptr, to_dev
+
+BED_MAGIC = bytes([0x6C, 0x1B, 0x01])
+
+
+def _read_fam(path: Path) -> np.ndarray:
+    rows = [line.split() for line in open(path) if line.strip()]
+    return np.array([[r[0], r[1]] for r in rows], dtype=str).reshape(-1, 2)
+
+
+def _read_bim(path: Path) -> dict[str, np.ndarray]:
+    chrom, sid, pos, a1, a2 = [], [], [], [], []
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if not t:
+                continue
+            chrom.append(t[0])
+            sid.append(t[1])
+            pos.append(int(t[3]))
+            a1.append(t[4])
+            a2.append(t[5])
+    return {"chromosome": np.array(chrom, dtype=str), "sid": np.array(sid, dtype=str),
+            "bp_position": np.array(pos, dtype=np.int64), "allele_1": np.array(a1, dtype=str),
+            "allele_2": np.array(a2, dtype=str)}
+
+
+def get_plink_metadata(bfile) -> dict[str, Any]:
+    """Dimensions and metadata without reading genotypes (plink.py:92-136)."""
+    bed = Path(f"{bfile}.bed")
+    if not bed.exists():
+        raise FileNotFoundError(f"PLINK .bed file not found: {bed}")
+    iid = _read_fam(Path(f"{bfile}.fam"))
+    bim = _read_bim(Path(f"{bfile}.bim"))
+    return {"n_samples": iid.shape[0], "n_snps": bim["sid"].shape[0], "iid": iid, **bim}
+
+
+def get_chromosome_partitions(bfile) -> dict[str, np.ndarray]:
+    """SNP indices per chromosome in first-appearance order (plink.py:139-175)."""
+    bed = Path(f"{bfile}.bed")
+    if not bed.exists():
+        raise FileNotFoundError(f"PLINK .bed file not found: {bed}")
+    chrom = _read_bim(Path(f"{bfile}.bim"))["chromosome"]
+    _, first = np.unique(chrom, return_index=True)
+    order = [chrom[i] for i in np.sort(first)]
+    return {c: np.where(chrom == c)[0] for c in order}
+
+
+def validate_plink_dimensions(bfile) -> None:
+    """.bed size must be 3 + ceil(n_fam / 4) * n_bim (plink.py:178-220)."""
+    paths = {ext: Path(f"{bfile}{ext}") for ext in (".bed", ".bim", ".fam")}
+    for ext, p in paths.items():
+        if not p.exists():
+            raise FileNotFoundError(f"PLINK {ext} file not found: {p}")
+    n_fam = sum(1 for _ in open(paths[".fam"]))
+    n_bim = sum(1 for _ in open(paths[".bim"]))
+    expected = 3 + (n_fam + 3) // 4 * n_bim
+    actual = paths[".bed"].stat().st_size
+    if actual != expected:
+        raise ValueError(f"PLINK dimension mismatch: .fam has {n_fam} samples, .bim has {n_bim} "
+                         f"SNPs, but .bed file size ({actual} bytes) doesn't match expected "
+                         f"({expected} bytes)")
+
+
+class BedFile:
+    """Memory-mapped SNP-major .bed payload with device-side decoding."""
+
+    def __init__(self, bfile):
+        self.bfile = bfile
+        meta = get_plink_metadata(bfile)
+        self.n_samples = meta["n_samples"]
+        self.n_snps = meta["n_snps"]
+        self.bytes_per_snp = (self.n_samples + 3) // 4
+        path = Path(f"{bfile}.bed")
+        with open(path, "rb") as f:
+            magic = f.read(3)
+        if magic != BED_MAGIC:
+            raise ValueError(f"{path} is not a SNP-major PLINK .bed file")
+        self._mm = np.memmap(path, dtype=np.uint8, mode="r", offset=3,
+                             shape=(self.n_snps * self.bytes_per_snp,))
+
+    def window_to_device(self, start: int, end: int) -> torch.Tensor:
+        """Raw bytes of SNPs [start, end) as a device uint8 tensor."""
+        lo, hi = start * self.bytes_per_snp, end * self.bytes_per_snp
+        host = torch.from_numpy(np.ascontiguousarray(self._mm[lo:hi]))
+        return host.to(require_gpu(), non_blocking=False)
+
+
+def decode(raw_dev: torch.Tensor, bytes_per_snp: int, n_snps: int, sample_idx=None,
+           n_out: int | None = None, snp_idx=None, want_g: bool = True, ld_g: int | None = None,
+           want_counts: bool = True):
+    """jm_decode_bed on a device window -> (g int8 [n_snps, ld_g] | None, counts | None)."""
+    dev = require_gpu()
+    if sample_idx is not None:
+        sample_idx = to_dev(np.asarray(sample_idx, dtype=np.int32), torch.int32, dev)
+        n_out = sample_idx.shape[0] if n_out is None else n_out
+    if n_out is None:
+        raise ValueError("n_out required without sample_idx")
+    if snp_idx is not None:
+        snp_idx = to_dev(np.asarray(snp_idx, dtype=np.int32), torch.int32, dev)
+        n_snps = snp_idx.shape[0]
+    ld = ld_g or round_up(max(n_out, 1), 64)
+    g = torch.empty((max(n_snps, 1), ld), dtype=torch.int8, device=dev) if want_g else None
+    counts = torch.empty((max(n_snps, 1), 4), dtype=torch.int32, device=dev) if want_counts else None
+    if n_snps:
+        _lib.call("jm_decode_bed", ptr(raw_dev), bytes_per_snp, ptr(snp_idx), n_snps,
+                  ptr(sample_idx), n_out, ptr(g), ld if want_g else 0, ptr(counts), stream_ptr())
+    return g, counts, ld
+
+
+def stats_from_counts(counts: np.ndarray, dtype=np.float32):
+    """Per-SNP (mean, n_miss, polymorphic) exactly as NumPy computes them on a
+    ``dtype`` chunk (runner_streaming.py:254-259): nanmean = float(sum/count)
+    with the division in float64, NaN -> 0; var > 0 iff >= 2 distinct calls."""
+    c = np.asarray(counts, dtype=np.int64).reshape(-1, 4)
+    nobs = c[:, 0] + c[:, 1] + c[:, 2]
+    s = c[:, 1] + 2 * c[:, 2]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        mean = np.where(nobs > 0, s / np.maximum(nobs, 1), 0.0)
+    mean = mean.astype(dtype).astype(np.float64)
+    poly = ((c[:, 0] > 0).astype(int) + (c[:, 1] > 0) + (c[:, 2] > 0)) >= 2
+    return mean, c[:, 3].copy(), poly

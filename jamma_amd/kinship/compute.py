@@ -1,0 +1,289 @@
+"""Centred kinship (GEMMA -gk 1) on the device: int8 MFMA + FP64 corrections.
+
+Drop-in for src/jamma/kinship/compute.py:
+  * compute_centered_kinship   :93-226  (in-memory genotypes)
+  * compute_kinship_streaming   :502-690 (two passes over the PLINK file)
+  * compute_loco_kinship(_streaming) :363-499, :693-916 (LOCO by subtraction)
+The 0/1/2 calls are packed to int8 and X0 X0^T is accumulated exactly in
+int32 (jm_kinship_syrk_i8); mean imputation and centring are applied as exact
+FP64 low-rank / sparse corrections (jm_kinship_impute_terms,
+jm_kinship_finalize). Per-SNP statistics come from jm_decode_bed counts.
+"""
+
+from __future__ import annotations
+
+import logging
+from collections.abc import Iterator
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from jamma_amd import _lib
+from jamma_amd.device import ptr, require_gpu, round_up, stream_ptr, to_dev
+from jamma_amd.io.plink import BedFile, decode, get_plink_metadata, stats_from_counts
+
+logger = logging.getLogger("jamma_amd")
+
+
+class KinshipAccumulator:
+    """Device accumulators S (int32), u, C (float64) and s0 over SNP tiles."""
+
+    def __init__(self, n: int, device=None):
+        self.device = device or require_gpu()
+        self.n = n
+        self.S = torch.zeros((n, n), dtype=torch.int32, device=self.device)
+        self.u = torch.zeros((n,), dtype=torch.float64, device=self.device)
+        self.C = None
+        self.s0 = 0.0
+        self.p = 0
+
+    def add_tile(self, g: torch.Tensor, ld_g: int, n_snps: int, mu: np.ndarray,
+                 has_missing: bool) -> None:
+        """g: SNP-major int8 (n_snps x ld_g, -1 missing); mu: float64 nanmean per SNP."""
+        if n_snps == 0:
+            return
+        n = self.n
+        ldt = round_up(n_snps, 64)
+        gT = torch.empty((n, ldt), dtype=torch.int8, device=self.device)
+        if ldt > n_snps:
+            gT[:, n_snps:].zero_()
+        st = stream_ptr()
+        _lib.call("jm_transpose_i8", ptr(g), n_snps, n, ld_g, ptr(gT), ldt, st)
+        _lib.call("jm_kinship_syrk_i8", ptr(gT), ldt, n, n_snps, ptr(self.S), st)
+        mu_d = to_dev(np.asarray(mu, dtype=np.float64), torch.float64, self.device)
+        if has_missing and self.C is None:
+            self.C = torch.zeros((n, n), dtype=torch.float64, device=self.device)
+        _lib.call("jm_kinship_impute_terms", ptr(g), ld_g, ptr(gT), ldt, n, n_snps, ptr(mu_d),
+                  ptr(self.u), ptr(self.C) if has_missing else None, int(has_missing), st)
+        for m in np.asarray(mu, dtype=np.float64):
+            self.s0 += m * m
+        self.p += n_snps
+
+    def finalize(self, p: float | None = None, sub: "KinshipAccumulator | None" = None) -> torch.Tensor:
+        """K = (S - u 1' - 1 u' + s0 + C + C') / p, optionally minus ``sub`` (LOCO)."""
+        p = float(self.p if p is None else p)
+        K = torch.empty((self.n, self.n), dtype=torch.float64, device=self.device)
+        _lib.call("jm_kinship_finalize", ptr(self.S), ptr(self.u), ptr(self.C), self.s0,
+                  ptr(sub.S) if sub else None, ptr(sub.u) if sub else None,
+                  ptr(sub.C) if sub else None, sub.s0 if sub else 0.0, self.n, p, ptr(K),
+                  stream_ptr())
+        return K
+
+
+def _mu64(counts: np.ndarray) -> np.ndarray:
+    """float64 nanmean of integer calls (kinship/missing.py:48-52): sum / count, 0 if none."""
+    mean, _, _ = stats_from_counts(counts, np.float64)
+    return mean
+
+
+def _filter(counts, n, maf_threshold, miss_threshold, stat_dtype):
+    mean, miss, poly = stats_from_counts(counts, stat_dtype)
+    af = mean / 2.0
+    maf = np.minimum(af, 1.0 - af)
+    return (maf >= maf_threshold) & (miss / n <= miss_threshold) & poly
+
+
+def _integral_i8(genotypes: np.ndarray) -> np.ndarray:
+    g = np.asarray(genotypes)
+    nanm = np.isnan(g)
+    vals = np.where(nanm, 0, g)
+    if not np.all((vals == 0) | (vals == 1) | (vals == 2)):
+        raise ValueError("genotypes must be hard calls 0/1/2 (NaN = missing)")
+    return np.where(nanm, -1, vals).astype(np.int8).T.copy()  # SNP-major
+
+
+def _counts(gi8: np.ndarray) -> np.ndarray:
+    return np.stack([(gi8 == 0).sum(1), (gi8 == 1).sum(1), (gi8 == 2).sum(1), (gi8 < 0).sum(1)],
+                    axis=1).astype(np.int64)
+
+
+def centered_kinship_device(genotypes: np.ndarray, batch_size: int = 10000,
+                            maf_threshold: float = 0.0, miss_threshold: float = 1.0) -> torch.Tensor:
+    """compute_centered_kinship on the device -> device float64 K."""
+    g = np.asarray(genotypes)
+    n = g.shape[0]
+    gi8 = _integral_i8(g)
+    counts = _counts(gi8)
+    stat_dtype = np.float32 if g.dtype == np.float32 else np.float64
+    mask = _filter(counts, n, maf_threshold, miss_threshold, stat_dtype)
+    idx = np.where(mask)[0]
+    if len(idx) == 0:
+        raise ValueError(f"No SNPs passed filtering (maf>={maf_threshold}, miss<={miss_threshold},"
+                         f" polymorphic). Original SNP count: {g.shape[1]}")
+    mu = _mu64(counts)
+    acc = KinshipAccumulator(n)
+    ld = round_up(n, 64)
+    for s in range(0, len(idx), batch_size):
+        sel = idx[s:s + batch_size]
+        tile = np.zeros((len(sel), ld), dtype=np.int8)
+        tile[:, :n] = gi8[sel]
+        acc.add_tile(to_dev(tile, torch.int8), ld, len(sel), mu[sel], bool(counts[sel, 3].any()))
+    return acc.finalize(len(idx))
+
+
+def compute_centered_kinship(genotypes: np.ndarray, batch_size: int = 10000,
+                             maf_threshold: float = 0.0, miss_threshold: float = 1.0,
+                             check_memory: bool = True) -> np.ndarray:
+    """Drop-in for kinship/compute.py:93-226 (returns host float64 K)."""
+    del check_memory
+    return centered_kinship_device(genotypes, batch_size, maf_threshold, miss_threshold).cpu().numpy()
+
+
+def _bed_counts(bed: BedFile, chunk_size: int, sample_idx=None, n_out=None) -> np.ndarray:
+    n_out = bed.n_samples if n_out is None else n_out
+    counts = np.zeros((bed.n_snps, 4), dtype=np.int64)
+    for start in range(0, bed.n_snps, chunk_size):
+        end = min(start + chunk_size, bed.n_snps)
+        raw = bed.window_to_device(start, end)
+        _, c, _ = decode(raw, bed.bytes_per_snp, end - start, sample_idx=sample_idx, n_out=n_out,
+                         want_g=False)
+        counts[start:end] = c[: end - start].cpu().numpy()
+    return counts
+
+
+def _streaming_selection(bed, chunk_size, maf_threshold, miss_threshold, ksnps_indices):
+    counts = _bed_counts(bed, chunk_size)
+    mask = _filter(counts, bed.n_samples, maf_threshold, miss_threshold, np.float32)
+    if ksnps_indices is not None:
+        k = np.asarray(ksnps_indices, dtype=np.int64)
+        if len(k) and k.max() >= bed.n_snps:
+            raise ValueError(f"Kinship SNP list index {k.max()} out of range for {bed.n_snps} SNPs")
+        lm = np.zeros(bed.n_snps, dtype=bool)
+        lm[k] = True
+        mask &= lm
+    idx = np.where(mask)[0]
+    if len(idx) == 0:
+        raise ValueError(f"No SNPs passed filtering (maf>={maf_threshold}, miss<={miss_threshold}, "
+                         f"polymorphic). Original SNP count: {bed.n_snps}")
+    return counts, idx
+
+
+def _accumulate_streaming(bed, idx, counts, chunk_size, acc_for):
+    """Pass 2: decode selected SNPs per file chunk and feed acc_for(snp) groups."""
+    n = bed.n_samples
+    ld = round_up(n, 64)
+    mu = _mu64(counts)
+    for start in range(0, bed.n_snps, chunk_size):
+        end = min(start + chunk_size, bed.n_snps)
+        sel = idx[(idx >= start) & (idx < end)]
+        if len(sel) == 0:
+            continue
+        raw = bed.window_to_device(start, end)
+        for acc, part in acc_for(sel):
+            g, _, _ = decode(raw, bed.bytes_per_snp, len(part), n_out=n,
+                             snp_idx=(part - start).astype(np.int32), want_counts=False, ld_g=ld)
+            acc.add_tile(g, ld, len(part), mu[part], bool(counts[part, 3].any()))
+
+
+def kinship_streaming_device(bed_path: Path, chunk_size: int = 10_000, maf_threshold: float = 0.0,
+                             miss_threshold: float = 1.0, ksnps_indices=None) -> torch.Tensor:
+    bed = BedFile(bed_path)
+    counts, idx = _streaming_selection(bed, chunk_size, maf_threshold, miss_threshold,
+                                       ksnps_indices)
+    acc = KinshipAccumulator(bed.n_samples)
+    _accumulate_streaming(bed, idx, counts, chunk_size, lambda sel: [(acc, sel)])
+    return acc.finalize(len(idx))
+
+
+def compute_kinship_streaming(bed_path: Path, chunk_size: int = 10_000, maf_threshold: float = 0.0,
+                              miss_threshold: float = 1.0, check_memory: bool = True,
+                              show_progress: bool = True,
+                              ksnps_indices: np.ndarray | None = None) -> np.ndarray:
+    """Drop-in for kinship/compute.py:502-690 (returns host float64 K)."""
+    del check_memory, show_progress
+    return kinship_streaming_device(bed_path, chunk_size, maf_threshold, miss_threshold,
+                                    ksnps_indices).cpu().numpy()
+
+
+def loco_kinship_streaming_device(bed_path: Path, chunk_size: int = 10_000,
+                                  maf_threshold: float = 0.0, miss_threshold: float = 1.0,
+                                  ksnps_indices=None) -> Iterator[tuple[str, torch.Tensor]]:
+    """Per-chromosome int32 partial Grams; K_c = (S_full - S_c) / (p - p_c)."""
+    bed = BedFile(bed_path)
+    chrom = get_plink_metadata(bed_path)["chromosome"]
+    counts, idx = _streaming_selection(bed, chunk_size, maf_threshold, miss_threshold,
+                                       ksnps_indices)
+    chr_f = chrom[idx]
+    names = sorted(set(chr_f.tolist()))
+    accs = {c: KinshipAccumulator(bed.n_samples) for c in names}
+
+    def split(sel):
+        cs = chrom[sel]
+        return [(accs[c], sel[cs == c]) for c in sorted(set(cs.tolist()))]
+
+    _accumulate_streaming(bed, idx, counts, chunk_size, split)
+    full = KinshipAccumulator(bed.n_samples)
+    for c in names:
+        a = accs[c]
+        full.S += a.S
+        full.u += a.u
+        if a.C is not None:
+            if full.C is None:
+                full.C = torch.zeros_like(a.C)
+            full.C += a.C
+        full.s0 += a.s0
+        full.p += a.p
+    for c in names:
+        p_loco = full.p - accs[c].p
+        if p_loco == 0:
+            raise ValueError(f"Cannot compute LOCO kinship: all {full.p} filtered SNPs are on "
+                             f"chromosome '{c}'.")
+        yield c, full.finalize(p_loco, sub=accs[c])
+        del accs[c]
+
+
+def compute_loco_kinship_streaming(bed_path: Path, chunk_size: int = 10_000,
+                                   maf_threshold: float = 0.0, miss_threshold: float = 1.0,
+                                   check_memory: bool = True, show_progress: bool = True,
+                                   ksnps_indices: np.ndarray | None = None
+                                   ) -> Iterator[tuple[str, np.ndarray]]:
+    """Drop-in for kinship/compute.py:693-916: yields (chr, host K_loco)."""
+    del check_memory, show_progress
+    for c, K in loco_kinship_streaming_device(bed_path, chunk_size, maf_threshold, miss_threshold,
+                                              ksnps_indices):
+        yield c, K.cpu().numpy()
+
+
+def compute_loco_kinship(genotypes: np.ndarray, chromosome_for_each_snp: np.ndarray,
+                         batch_size: int = 10000, maf_threshold: float = 0.0,
+                         miss_threshold: float = 1.0, check_memory: bool = True
+                         ) -> Iterator[tuple[str, np.ndarray]]:
+    """Drop-in for kinship/compute.py:363-499 (in-memory LOCO)."""
+    del check_memory
+    g = np.asarray(genotypes)
+    n = g.shape[0]
+    chrom = np.asarray(chromosome_for_each_snp)
+    gi8 = _integral_i8(g)
+    counts = _counts(gi8)
+    stat_dtype = np.float32 if g.dtype == np.float32 else np.float64
+    mask = _filter(counts, n, maf_threshold, miss_threshold, stat_dtype)
+    idx = np.where(mask)[0]
+    if len(idx) == 0:
+        raise ValueError("No SNPs passed filtering")
+    mu = _mu64(counts)
+    ld = round_up(n, 64)
+    names = sorted(set(chrom[idx].tolist()))
+    accs = {c: KinshipAccumulator(n) for c in names}
+    for c in names:
+        sel_c = idx[chrom[idx] == c]
+        for s in range(0, len(sel_c), batch_size):
+            sel = sel_c[s:s + batch_size]
+            tile = np.zeros((len(sel), ld), dtype=np.int8)
+            tile[:, :n] = gi8[sel]
+            accs[c].add_tile(to_dev(tile, torch.int8), ld, len(sel), mu[sel],
+                             bool(counts[sel, 3].any()))
+    full = KinshipAccumulator(n)
+    for c in names:
+        full.S += accs[c].S
+        full.u += accs[c].u
+        if accs[c].C is not None:
+            full.C = accs[c].C.clone() if full.C is None else full.C + accs[c].C
+        full.s0 += accs[c].s0
+        full.p += accs[c].p
+    for c in names:
+        p_loco = full.p - accs[c].p
+        if p_loco == 0:
+            raise ValueError(f"Cannot compute LOCO kinship: all {full.p} filtered SNPs are on "
+                             f"chromosome '{c}'. LOCO requires SNPs on multiple chromosomes.")
+        yield c, full.finalize(p_loco, sub=accs[c]).cpu().numpy()

@@ -1,0 +1,232 @@
+"""Device association engine: resident cohort state + per-phenotype problem.
+
+A ``Cohort`` keeps, in HBM, what every association request against one kinship
+shares: the eigenvalues and their compressed-spectrum plan, U^T in FP64 and
+(for the mixed path) its bf16 hi/mid/lo split. An ``AssocProblem`` adds one
+phenotype and covariate set: the rotated covariates/phenotype, their spectrum
+moments, the null model (device) and the grid tables. ``associate`` then runs
+one SNP tile through rotation -> spectrum moments -> lambda search + tests.
+
+Reference flow being replaced: src/jamma/lmm/runner_streaming.py:334-566
+(eigendecomposition reuse, UtW/Uty, null model, per-chunk UtG rotation and
+the JAX chain of likelihood_jax.py).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from jamma_amd import _lib
+from jamma_amd.device import ptr, require_gpu, round_up, stream_ptr, to_dev
+from jamma_amd.lmm.spectrum import SpectrumPlan, plan_spectrum
+
+PRECISIONS = ("f64", "mixed")
+MAX_CVT = 20
+
+
+def get_ab_index(a: int, b: int, n_cvt: int) -> int:
+    """GEMMA packed index (src/jamma/lmm/likelihood.py:25-43), 1-based a, b."""
+    cols = n_cvt + 2
+    a1, b1 = (a, b) if a <= b else (b, a)
+    return (2 * cols - a1 + 2) * (a1 - 1) // 2 + b1 - a1
+
+
+class Cohort:
+    """Eigendecomposition of one kinship, resident on the device."""
+
+    def __init__(self, eigenvalues, eigenvectors, precision: str = "mixed", device=None):
+        if precision not in PRECISIONS + ("both",):
+            raise ValueError(f"precision must be one of {PRECISIONS + ('both',)}")
+        self.device = device or require_gpu()
+        d = eigenvalues.detach().cpu().numpy() if isinstance(eigenvalues, torch.Tensor) \
+            else np.asarray(eigenvalues, dtype=np.float64)
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        self.n = n = d.shape[0]
+        self.d = d
+        self.plan: SpectrumPlan = plan_spectrum(d)
+        U = to_dev(eigenvectors, torch.float64, self.device)
+        if tuple(U.shape) != (n, n):
+            raise ValueError(f"eigenvectors must be ({n}, {n}), got {tuple(U.shape)}")
+        self.UT = U.t().contiguous()  # UT[i][k] = U[k][i]
+        del U
+        self.ld_u = n
+        self.precision = precision
+        self.split = None
+        if precision in ("mixed", "both"):
+            self.ld_s = round_up(n, 64)
+            planes = torch.empty((3, n, self.ld_s), dtype=torch.int16, device=self.device)
+            _lib.call("jm_split_bf16x3", ptr(self.UT), self.ld_u, n, ptr(planes[0]), ptr(planes[1]),
+                      ptr(planes[2]), self.ld_s, stream_ptr())
+            self.split = planes
+        p = self.plan
+        dev = self.device
+        self.cl_start = to_dev(p.cl_start, torch.int32, dev)
+        self.cl_size = to_dev(p.cl_size, torch.int32, dev)
+        self.cl_terms = to_dev(p.cl_terms, torch.int32, dev)
+        self.cl_toff = to_dev(p.cl_toff, torch.int32, dev)
+        self.cl_center = to_dev(p.cl_center, torch.float64, dev)
+        self.delta = to_dev(p.idx_delta, torch.float64, dev)
+        self.term_cluster = to_dev(p.term_cluster, torch.int32, dev)
+        self.term_m = to_dev(p.term_m, torch.int32, dev)
+
+    @property
+    def n_terms(self) -> int:
+        return self.plan.n_terms
+
+    def hbm_bytes(self) -> int:
+        b = self.UT.numel() * 8
+        if self.split is not None:
+            b += self.split.numel() * 2
+        return b
+
+    # -- rotation -----------------------------------------------------------
+    def rotate_dense(self, rows: torch.Tensor) -> torch.Tensor:
+        """UT @ rows^T for rows (m x n) float64 -> (n x m) float64."""
+        rows = rows.contiguous()
+        m = rows.shape[0]
+        out = torch.empty((self.n, m), dtype=torch.float64, device=self.device)
+        _lib.call("jm_gemm_nt_f64", ptr(self.UT), self.ld_u, ptr(rows), rows.shape[1], self.n, m,
+                  self.n, ptr(out), m, stream_ptr())
+        return out
+
+    def rotate_genotypes(self, g: torch.Tensor, ld_g: int, means: torch.Tensor, n_snps: int,
+                         precision: str) -> tuple[torch.Tensor, bool, int]:
+        """Z = U^T X for a genotype tile (SNP-major int8 rows, -1 missing -> means)."""
+        ldz = max(1, n_snps)
+        if precision == "f64":
+            Z = torch.empty((self.n, ldz), dtype=torch.float64, device=self.device)
+            _lib.call("jm_rotate_f64", ptr(self.UT), self.ld_u, ptr(g), ld_g, ptr(means), self.n,
+                      n_snps, ptr(Z), ldz, stream_ptr())
+            return Z, False, ldz
+        if precision == "mixed":
+            if self.split is None:
+                raise ValueError("cohort was built without the bf16 split (precision='f64')")
+            Z = torch.empty((self.n, ldz), dtype=torch.float32, device=self.device)
+            _lib.call("jm_rotate_bf16x3", ptr(self.split[0]), ptr(self.split[1]),
+                      ptr(self.split[2]), self.ld_s, ptr(g), ld_g, ptr(means), self.n, n_snps,
+                      ptr(Z), ldz, stream_ptr())
+            return Z, True, ldz
+        raise ValueError(f"unknown precision {precision!r}")
+
+    def moments(self, Z: torch.Tensor, z_is_f32: bool, ldz: int, n_cols: int, V: torch.Tensor,
+                nv: int, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Spectrum moments (n_cols x (nv+1) x T) of the columns of Z."""
+        T = self.n_terms
+        if out is None:
+            out = torch.empty((n_cols, nv + 1, T), dtype=torch.float64, device=self.device)
+        _lib.call("jm_spectrum_moments", ptr(Z), int(z_is_f32), ldz, self.n, n_cols, ptr(V),
+                  V.shape[1], nv, ptr(self.cl_start), ptr(self.cl_size), ptr(self.cl_terms),
+                  ptr(self.cl_toff), len(self.plan.cl_size), ptr(self.delta), ptr(out), T,
+                  stream_ptr())
+        return out
+
+
+class AssocProblem:
+    """One phenotype (+ covariates) against a resident cohort."""
+
+    def __init__(self, cohort: Cohort, phenotype, covariates=None, l_min: float = 1e-5,
+                 l_max: float = 1e5, n_grid: int = 50, n_refine: int = 10):
+        self.cohort = cohort
+        dev = cohort.device
+        n = cohort.n
+        y = np.asarray(phenotype.detach().cpu() if isinstance(phenotype, torch.Tensor) else phenotype,
+                       dtype=np.float64)
+        if y.shape != (n,):
+            raise ValueError(f"phenotype must have {n} entries")
+        W = np.ones((n, 1)) if covariates is None else np.asarray(covariates, dtype=np.float64)
+        if W.ndim != 2 or W.shape[0] != n:
+            raise ValueError("covariates must be (n_samples, n_cvt)")
+        c = W.shape[1]
+        if n <= c + 1:
+            raise ValueError(f"Over-parameterized model: {n} samples with {c} covariates leaves "
+                             f"df={n - c - 1} (need at least {c + 2} samples)")
+        if c > MAX_CVT:
+            raise ValueError(f"at most {MAX_CVT} covariate columns are supported, got {c}")
+        self.n_cvt = c
+        self.n = n
+        T = cohort.n_terms
+        ni = (c + 3) * (c + 2) // 2
+        self.n_index = ni
+        # rotated covariates + phenotype: R = U^T [W | y]   (n x (c+1))
+        rows = to_dev(np.vstack([W.T, y[None, :]]), torch.float64, dev)
+        R = cohort.rotate_dense(rows)
+        self.V = R.t().contiguous()  # partner rows: UtW_1..UtW_c, Uty   ((c+1) x n)
+        Zs = torch.cat([R, torch.ones((n, 1), dtype=torch.float64, device=dev)], dim=1).contiguous()
+        smom = cohort.moments(Zs, False, c + 2, c + 2, self.V, c + 1)  # ((c+2), (c+2), T)
+        # shared (lambda-independent) pair rows in packed GEMMA order; genotype pairs -> row -1
+        pair_row = np.full(ni, -1, dtype=np.int32)
+        shared_src = np.zeros((ni, 2), dtype=np.int64)  # (column, row) of smom
+        xc = c + 1  # genotype column (1-based)
+        yc = c + 2
+        for a in range(1, c + 3):
+            for b in range(a, c + 3):
+                j = get_ab_index(a, b, c)
+                if a == xc or b == xc:
+                    other = b if a == xc else a
+                    pair_row[j] = (c + 1) if other == xc else (c if other == yc else other - 1)
+                else:
+                    col = a - 1 if a <= c else c
+                    if a == b:
+                        shared_src[j] = (col, c + 1)  # self row
+                    else:
+                        shared_src[j] = (col, (b - 1) if b <= c else c)
+        idx_col = torch.as_tensor(shared_src[:, 0], device=dev)
+        idx_row = torch.as_tensor(shared_src[:, 1], device=dev)
+        shared = smom[idx_col, idx_row].contiguous()
+        shared[torch.as_tensor(pair_row >= 0, device=dev)] = 0.0
+        self.shared_mom = shared                       # (ni, T)
+        self.logdet_mom = smom[c + 1, c + 1].contiguous()  # (T,)
+        self.pair_row = to_dev(pair_row, torch.int32, dev)
+        loglam = np.linspace(np.log(l_min), np.log(l_max), n_grid)
+        self.grid_loglam = to_dev(loglam, torch.float64, dev)
+        self.grid_kappa = torch.empty((T, n_grid), dtype=torch.float64, device=dev)
+        self.grid_shared = torch.empty((n_grid, ni), dtype=torch.float64, device=dev)
+        self.grid_logdet = torch.empty((n_grid,), dtype=torch.float64, device=dev)
+        self.null_kappa = torch.empty((T,), dtype=torch.float64, device=dev)
+        self.null_shared = torch.empty((ni,), dtype=torch.float64, device=dev)
+        pr = _lib.LmmProblem()
+        pr.n, pr.n_cvt, pr.n_terms = n, c, T
+        pr.n_clusters = len(cohort.plan.cl_size)
+        pr.n_grid, pr.n_iter = n_grid, max(n_refine, 20)
+        pr.l_min, pr.l_max = l_min, l_max
+        pr.term_cluster, pr.term_m = ptr(cohort.term_cluster), ptr(cohort.term_m)
+        pr.cl_center = ptr(cohort.cl_center)
+        pr.logdet_mom, pr.shared_mom = ptr(self.logdet_mom), ptr(self.shared_mom)
+        pr.pair_row, pr.grid_loglam = ptr(self.pair_row), ptr(self.grid_loglam)
+        pr.grid_kappa, pr.grid_shared = ptr(self.grid_kappa), ptr(self.grid_shared)
+        pr.grid_logdet = ptr(self.grid_logdet)
+        pr.null_kappa, pr.null_shared = ptr(self.null_kappa), ptr(self.null_shared)
+        self.prob = pr
+        # null model MLE on the device (likelihood.py:571-607)
+        nullout = torch.zeros(4, dtype=torch.float64, device=dev)
+        _lib.call("jm_lmm_null", ctypes_ref(pr), ptr(nullout), stream_ptr())
+        lam0, logl0 = nullout[:2].cpu().tolist()
+        pr.lambda_null, pr.logl_null = lam0, logl0
+        self.lambda_null, self.logl_H0 = lam0, logl0
+        _lib.call("jm_lmm_prepare", ctypes_ref(pr), stream_ptr())
+
+    def assoc_moments(self, xmom: torch.Tensor, n_snps: int, mode: int,
+                      out: torch.Tensor | None = None) -> torch.Tensor:
+        """lambda search + tests from per-SNP moments (n_snps x (c+2) x T)."""
+        if out is None:
+            out = torch.empty((max(n_snps, 1), _lib.JM_NOUT), dtype=torch.float64,
+                              device=self.cohort.device)
+        if n_snps:
+            _lib.call("jm_lmm_assoc", ctypes_ref(self.prob), int(mode), ptr(xmom),
+                      self.cohort.n_terms, n_snps, ptr(out), stream_ptr())
+        return out
+
+    def associate(self, g: torch.Tensor, ld_g: int, means: torch.Tensor, n_snps: int, mode: int,
+                  precision: str = "mixed") -> torch.Tensor:
+        """One SNP tile: rotation -> moments -> lambda search + tests (device out)."""
+        co = self.cohort
+        Z, f32, ldz = co.rotate_genotypes(g, ld_g, means, n_snps, precision)
+        xmom = co.moments(Z, f32, ldz, n_snps, self.V, self.n_cvt + 1)
+        del Z
+        return self.assoc_moments(xmom, n_snps, mode)
+
+
+def ctypes_ref(pr):
+    import ctypes
+    return ctypes.byref(pr)
