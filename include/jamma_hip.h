@@ -121,13 +121,14 @@ int jm_kinship_finalize(const int32_t* S, const double* u, const double* C, doub
  * Replaces UtG = UT @ geno_chunk (src/jamma/lmm/runner_streaming.py:392-410;
  * also runner_jax.py:236-254 and loco.py:406-423).
  * Z (n x ldz, row i = eigen-index, column s = SNP) = U^T X, where column s of X
- * is SNP row s of g with missing calls replaced by mean[s] (float32-derived
- * mean, runner_streaming.py:322-329,429-434). UT = U^T, n x ld_u float64.
- * FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 output.
+ * is SNP row r(s) of g with missing calls replaced by mean[r(s)] (float32-derived
+ * mean, runner_streaming.py:322-329,429-434); r(s) = col_idx[s], or s when
+ * col_idx is NULL (the gather lets one launch serve several requests).
+ * UT = U^T, n x ld_u float64. FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 output.
  */
 int jm_rotate_f64(const double* UT, int64_t ld_u, const int8_t* g, int64_t ld_g,
-                  const double* mean, int64_t n, int64_t n_snps, double* Z, int64_t ldz,
-                  void* stream);
+                  const double* mean, const int32_t* col_idx, int64_t n, int64_t n_snps,
+                  double* Z, int64_t ldz, void* stream);
 
 /* Generic C (m x ldc) = A (m x k, lda) * B^T where B is (ncols x k, ldb), all
  * float64 (UtW = UT @ W and Uty = UT @ y, runner_streaming.py:349-352). */
@@ -144,10 +145,11 @@ int jm_split_bf16x3(const double* UT, int64_t ld_u, int64_t n, uint16_t* hi, uin
  * (v_mfma_f32_16x16x32_bf16) with FP32 accumulation:
  *   hi^T X1 + mid^T X1 + lo^T X1 + hi^T X2
  * X1 = genotypes with missing -> bf16(mean), X2 = (mean - bf16(mean)) at missing.
+ * col_idx as in jm_rotate_f64.
  */
 int jm_rotate_bf16x3(const uint16_t* hi, const uint16_t* mid, const uint16_t* lo, int64_t ld_s,
-                     const int8_t* g, int64_t ld_g, const double* mean, int64_t n,
-                     int64_t n_snps, float* Z, int64_t ldz, void* stream);
+                     const int8_t* g, int64_t ld_g, const double* mean, const int32_t* col_idx,
+                     int64_t n, int64_t n_snps, float* Z, int64_t ldz, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Per-SNP REML/MLE lambda search + Wald/LRT/Score                           */
@@ -169,6 +171,26 @@ int jm_spectrum_moments(const void* Z, int z_is_f32, int64_t ldz, int64_t n, int
                         const int32_t* cl_size, const int32_t* cl_terms,
                         const int32_t* cl_toff, int n_clusters, const double* delta,
                         double* mom, int64_t ldm, void* stream);
+
+/* Per-column descriptor of jm_spectrum_moments_multi: the partner rows
+ * (nv x ldv float64) and the output block ((nv+1) x ldm float64) of one column. */
+typedef struct jm_mom_col {
+  const double* V;
+  int64_t ldv;
+  int32_t nv;
+  int32_t reserved;
+  double* out;
+  int64_t ldm;
+} jm_mom_col;
+
+/* jm_spectrum_moments with a per-column descriptor (device array cols[n_snps]),
+ * so columns of different requests (phenotypes / covariate sets) on the same
+ * cohort share one launch. nv_max = max nv over the columns. */
+int jm_spectrum_moments_multi(const void* Z, int z_is_f32, int64_t ldz, int64_t n, int64_t n_snps,
+                              const jm_mom_col* cols, int nv_max, const int32_t* cl_start,
+                              const int32_t* cl_size, const int32_t* cl_terms,
+                              const int32_t* cl_toff, int n_clusters, const double* delta,
+                              void* stream);
 
 /* Parameters of one (cohort, phenotype) association problem. Device pointers. */
 typedef struct jm_lmm_problem {
@@ -237,6 +259,18 @@ int jm_lmm_prepare(const jm_lmm_problem* prob, void* stream);
  */
 int jm_lmm_assoc(const jm_lmm_problem* prob, int mode, const double* xmom, int64_t ldm,
                  int64_t n_snps, double* out, void* stream);
+
+/*
+ * Batched form of jm_lmm_assoc for the serving loop: SNP s is tested under
+ * problem probs[snp_prob[s]] (device array of n_probs problems, all on one
+ * cohort so they share n_terms) with mode modes[snp_prob[s]], from the moments
+ * block starting at snp_xmom[s] (row stride ldm); results go to row s of out.
+ * max_cvt = max n_cvt over the problems. One launch serves every request
+ * admitted on the cohort; a SNP's result does not depend on the batch.
+ */
+int jm_lmm_assoc_multi(const jm_lmm_problem* probs, const int32_t* modes, int n_probs, int max_cvt,
+                       int n_terms, const int32_t* snp_prob, const double* const* snp_xmom,
+                       int64_t ldm, int64_t n_snps, double* out, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Host-callable special functions (the device code compiled for the CPU),   */

@@ -21,6 +21,13 @@ OUT = {
 }
 
 
+class MomCol(ctypes.Structure):
+    """Mirror of jm_mom_col."""
+
+    _fields_ = [("V", c_void_p), ("ldv", c_int64), ("nv", c_int32), ("reserved", c_int32),
+                ("out", c_void_p), ("ldm", c_int64)]
+
+
 class LmmProblem(ctypes.Structure):
     """Mirror of jm_lmm_problem."""
 
@@ -48,17 +55,22 @@ _SIGS = {
                                  c_void_p, c_void_p, c_int, c_void_p], c_int),
     "jm_kinship_finalize": ([c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p,
                              c_double, c_int64, c_double, c_void_p, c_void_p], c_int),
-    "jm_rotate_f64": ([c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_int64, c_void_p,
-                       c_int64, c_void_p], c_int),
+    "jm_rotate_f64": ([c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_int64,
+                       c_void_p, c_int64, c_void_p], c_int),
     "jm_gemm_nt_f64": ([c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p,
                         c_int64, c_void_p], c_int),
     "jm_split_bf16x3": ([c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
                          c_void_p], c_int),
     "jm_rotate_bf16x3": ([c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p,
-                          c_int64, c_int64, c_void_p, c_int64, c_void_p], c_int),
+                          c_void_p, c_int64, c_int64, c_void_p, c_int64, c_void_p], c_int),
     "jm_spectrum_moments": ([c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                              c_int64, c_void_p], c_int),
+    "jm_spectrum_moments_multi": ([c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p, c_int,
+                                   c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                   c_void_p], c_int),
+    "jm_lmm_assoc_multi": ([c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64,
+                            c_int64, c_void_p, c_void_p], c_int),
     "jm_lmm_null": ([POINTER(LmmProblem), c_void_p, c_void_p], c_int),
     "jm_lmm_prepare": ([POINTER(LmmProblem), c_void_p], c_int),
     "jm_lmm_assoc": ([POINTER(LmmProblem), c_int, c_void_p, c_int64, c_int64, c_void_p, c_void_p],
@@ -74,7 +86,7 @@ def _load():
     if not LIB_PATH.exists():
         raise ImportError(
             f"libjamma_hip.so not found at {LIB_PATH}; build it with "
-            "`python -m jamma_amd.csrc.build` (hipcc, gfx950)")
+            "`python jamma_amd/csrc/build.py` (hipcc, gfx950)")
     lib = ctypes.CDLL(str(LIB_PATH))
     for name, (args, res) in _SIGS.items():
         fn = getattr(lib, name)

@@ -1,6 +1,6 @@
 """Build libjamma_hip.so for gfx950 from the sources in this directory.
 
-Usage: python -m jamma_amd.csrc.build   (or build() from __graft_entry__)
+Usage: python jamma_amd/csrc/build.py   (or build() from __graft_entry__)
 Objects go to jamma_amd/csrc/_build/, the library to jamma_amd/lib/.
 """
 

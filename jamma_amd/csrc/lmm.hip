@@ -425,29 +425,40 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx) {
 constexpr int kMaxLevels = 24;  // n_cvt <= 20 -> levels 0..n_cvt
 
 // Main association kernel: one wavefront per SNP, grid-stride over SNPs.
-// Per-wave LDS: kappa (2T) | P (ni x 64) | iab level logs (kMaxLevels).
-__global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr, int mode, const double* xmom,
-                                                   int64_t ldm, int64_t n_snps, double* out) {
+// Per-wave LDS: kappa (2T) | P (ni_max x 64) | iab level logs (kMaxLevels).
+// MULTI: SNP s belongs to problem snp_prob[s] (its own phenotype, covariates,
+// null model and mode) and its moments start at snp_xmom[s]; all problems of
+// one launch share the cohort (same T). Otherwise every SNP uses pr_one.
+template <bool MULTI>
+__global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mode_one,
+                                                   const double* xmom, int64_t ldm, int64_t n_snps,
+                                                   double* out, const jm_lmm_problem* probs,
+                                                   const int32_t* modes, const int32_t* snp_prob,
+                                                   const double* const* snp_xmom, int T_all,
+                                                   int ni_max) {
   extern __shared__ double smem[];
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
-  const int nc = pr.n_cvt, T = pr.n_terms, n = pr.n, G = pr.n_grid;
-  const int ni = (nc + 3) * (nc + 2) / 2;
-  const int per_wave = 2 * T + ni * 64 + kMaxLevels;
+  const int per_wave = 2 * T_all + ni_max * 64 + kMaxLevels;
   double* kap = smem + (size_t)wib * per_wave;
-  double* P = kap + 2 * T;
-  double* iab_ld = P + ni * 64;
-  const bool do_reml = (mode == 1 || mode == 4);
-  const bool do_mle = (mode == 2 || mode == 4);
-  const bool do_score = (mode == 3 || mode == 4);
-  const double df = (double)(n - nc - 1);
+  double* P = kap + 2 * T_all;
+  double* iab_ld = P + ni_max * 64;
   const double phi = 0.6180339887498949;
-  const int nrows = nc + 2;  // per-SNP moment rows: W_1..W_c, y, xx
   const int h = lane >> 5;
 
   for (int64_t s = (int64_t)blockIdx.x * wpb + wib; s < n_snps; s += (int64_t)gridDim.x * wpb) {
-    const double* xm = xmom + s * (int64_t)nrows * ldm;
+    const int pidx = MULTI ? snp_prob[s] : 0;
+    const jm_lmm_problem pr = MULTI ? probs[pidx] : pr_one;
+    const int mode = MULTI ? modes[pidx] : mode_one;
+    const int nc = pr.n_cvt, T = pr.n_terms, n = pr.n, G = pr.n_grid;
+    const int ni = (nc + 3) * (nc + 2) / 2;
+    const bool do_reml = (mode == 1 || mode == 4);
+    const bool do_mle = (mode == 2 || mode == 4);
+    const bool do_score = (mode == 3 || mode == 4);
+    const double df = (double)(n - nc - 1);
+    const int nrows = nc + 2;  // per-SNP moment rows: W_1..W_c, y, xx
+    const double* xm = MULTI ? snp_xmom[s] : xmom + s * (int64_t)nrows * ldm;
     int evals = 0;
     // ---- Iab (lambda = 0): row 0 = sum of the m = 0 terms ----
     {
@@ -656,6 +667,35 @@ extern "C" int jm_lmm_null(const jm_lmm_problem* prob, double* out, void* stream
   return JM_OK;
 }
 
+static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* xmom, int64_t ldm,
+                        int64_t n_snps, double* out, const jm_lmm_problem* probs,
+                        const int32_t* modes, const int32_t* snp_prob,
+                        const double* const* snp_xmom, int T, int max_cvt, void* stream) {
+  const int ni_max = (max_cvt + 3) * (max_cvt + 2) / 2;
+  const size_t per_wave = (size_t)(2 * T + ni_max * 64 + kMaxLevels) * sizeof(double);
+  int wpb = (int)((64 * 1024) / per_wave);
+  if (wpb > 4) wpb = 4;
+  if (wpb < 1) {
+    JM_CHECK_ARG(per_wave <= 160 * 1024, "problem too large for the LDS budget (T=%d)", T);
+    wpb = 1;
+  }
+  int64_t blocks = (n_snps + wpb - 1) / wpb;
+  const int64_t max_blocks = 256 * 16;
+  if (blocks > max_blocks) blocks = max_blocks;
+  jm_lmm_problem dummy;
+  memset(&dummy, 0, sizeof(dummy));
+  if (one)
+    hipLaunchKernelGGL(k_lmm_assoc<false>, dim3((unsigned)blocks), dim3(64 * wpb), per_wave * wpb,
+                       (hipStream_t)stream, *one, mode_one, xmom, ldm, n_snps, out, nullptr,
+                       nullptr, nullptr, nullptr, T, ni_max);
+  else
+    hipLaunchKernelGGL(k_lmm_assoc<true>, dim3((unsigned)blocks), dim3(64 * wpb), per_wave * wpb,
+                       (hipStream_t)stream, dummy, 0, nullptr, ldm, n_snps, out, probs, modes,
+                       snp_prob, snp_xmom, T, ni_max);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
 extern "C" int jm_lmm_assoc(const jm_lmm_problem* prob, int mode, const double* xmom,
                             int64_t ldm, int64_t n_snps, double* out, void* stream) {
   int rc = check_problem(prob);
@@ -668,22 +708,21 @@ extern "C" int jm_lmm_assoc(const jm_lmm_problem* prob, int mode, const double* 
   JM_CHECK_ARG(prob->grid_kappa && prob->grid_shared && prob->grid_logdet && prob->null_kappa &&
                    prob->null_shared,
                "tables not prepared");
-  const size_t per_wave = lds_per_wave(prob);
-  int wpb = (int)((64 * 1024) / per_wave);
-  if (wpb > 4) wpb = 4;
-  if (wpb < 1) {
-    JM_CHECK_ARG(per_wave <= 160 * 1024, "problem too large for the LDS budget (T=%d)",
-                 prob->n_terms);
-    wpb = 1;
-  }
-  const int64_t waves_needed = n_snps;
-  int64_t blocks = (waves_needed + wpb - 1) / wpb;
-  const int64_t max_blocks = 256 * 16;
-  if (blocks > max_blocks) blocks = max_blocks;
-  hipLaunchKernelGGL(k_lmm_assoc, dim3((unsigned)blocks), dim3(64 * wpb), per_wave * wpb,
-                     (hipStream_t)stream, *prob, mode, xmom, ldm, n_snps, out);
-  JM_CHECK_LAUNCH();
-  return JM_OK;
+  return launch_assoc(prob, mode, xmom, ldm, n_snps, out, nullptr, nullptr, nullptr, nullptr,
+                      prob->n_terms, prob->n_cvt, stream);
+}
+
+extern "C" int jm_lmm_assoc_multi(const jm_lmm_problem* probs, const int32_t* modes, int n_probs,
+                                  int max_cvt, int n_terms, const int32_t* snp_prob,
+                                  const double* const* snp_xmom, int64_t ldm, int64_t n_snps,
+                                  double* out, void* stream) {
+  JM_CHECK_ARG(n_snps >= 0 && n_probs >= 1, "bad sizes");
+  if (n_snps == 0) return JM_OK;
+  JM_CHECK_ARG(probs && modes && snp_prob && snp_xmom && out, "null pointer");
+  JM_CHECK_ARG(max_cvt >= 1 && max_cvt <= 20, "max_cvt out of range [1,20]");
+  JM_CHECK_ARG(n_terms >= 1 && ldm >= n_terms, "bad n_terms / ldm");
+  return launch_assoc(nullptr, 0, nullptr, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
+                      n_terms, max_cvt, stream);
 }
 
 extern "C" double jm_host_betainc(double a, double b, double x) { return betainc_reg(a, b, x); }

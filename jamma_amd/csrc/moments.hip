@@ -23,20 +23,36 @@ constexpr int kMomMaxTerms = 16;  // MAX_TERMS_PER_CLUSTER on the host side
 constexpr int kMomGroup = 6;      // partner rows accumulated per pass
 constexpr int kMomTerms = kMomMaxTerms;
 
-template <typename ZT>
+template <typename ZT, bool MULTI>
 __global__ void __launch_bounds__(64) k_moments(const ZT* __restrict__ Z, int64_t ldz, int64_t n,
-                                                int64_t n_snps, const double* __restrict__ V,
-                                                int64_t ldv, int nv,
+                                                int64_t n_snps, const double* __restrict__ V_one,
+                                                int64_t ldv_one, int nv_one,
+                                                const jm_mom_col* __restrict__ cols, int nv_max,
                                                 const int32_t* __restrict__ cl_start,
                                                 const int32_t* __restrict__ cl_size,
                                                 const int32_t* __restrict__ cl_terms,
                                                 const int32_t* __restrict__ cl_toff,
                                                 int n_clusters,
                                                 const double* __restrict__ delta,
-                                                double* __restrict__ mom, int64_t ldm) {
+                                                double* __restrict__ mom_one, int64_t ldm_one) {
   const int64_t s = (int64_t)blockIdx.x * 64 + threadIdx.x;
   const bool active = s < n_snps;
   const int64_t sc = active ? s : 0;
+  // this column's partner rows and output block
+  const double* V = V_one;
+  int64_t ldv = ldv_one, ldm = ldm_one;
+  int nv = nv_one;
+  double* ms;
+  if (MULTI) {
+    const jm_mom_col d = cols[sc];
+    V = d.V;
+    ldv = d.ldv;
+    nv = d.nv;
+    ldm = d.ldm;
+    ms = d.out;
+  } else {
+    ms = mom_one + sc * (int64_t)(nv_one + 1) * ldm_one;
+  }
   // cluster range of this group: clusters whose start lies in [i_lo, i_hi)
   const int64_t i_lo = n * blockIdx.y / gridDim.y;
   const int64_t i_hi = n * (blockIdx.y + 1) / gridDim.y;
@@ -57,9 +73,10 @@ __global__ void __launch_bounds__(64) k_moments(const ZT* __restrict__ Z, int64_
     c1 = lo;
   }
   const int nrows = nv + 1;
-  double* ms = mom + sc * (int64_t)nrows * ldm;
-  for (int g0 = 0; g0 < nrows; g0 += kMomGroup) {
-    const int ng = (nrows - g0 < kMomGroup) ? (nrows - g0) : kMomGroup;
+  const int nrows_max = (MULTI ? nv_max : nv_one) + 1;
+  for (int g0 = 0; g0 < nrows_max; g0 += kMomGroup) {
+    int ng = nrows - g0;
+    ng = ng < 0 ? 0 : (ng > kMomGroup ? kMomGroup : ng);
     for (int c = c0; c < c1; ++c) {
       const int i0 = cl_start[c];
       const int sz = cl_size[c];
@@ -126,13 +143,39 @@ extern "C" int jm_spectrum_moments(const void* Z, int z_is_f32, int64_t ldz, int
   const dim3 grid((unsigned)((n_snps + 63) / 64), (unsigned)groups);
   hipStream_t st = (hipStream_t)stream;
   if (z_is_f32)
-    hipLaunchKernelGGL(k_moments<float>, grid, dim3(64), 0, st, (const float*)Z, ldz, n, n_snps,
-                       V, ldv, nv, cl_start, cl_size, cl_terms, cl_toff, n_clusters, delta, mom,
-                       ldm);
+    hipLaunchKernelGGL((k_moments<float, false>), grid, dim3(64), 0, st, (const float*)Z, ldz, n,
+                       n_snps, V, ldv, nv, nullptr, 0, cl_start, cl_size, cl_terms, cl_toff,
+                       n_clusters, delta, mom, ldm);
   else
-    hipLaunchKernelGGL(k_moments<double>, grid, dim3(64), 0, st, (const double*)Z, ldz, n,
-                       n_snps, V, ldv, nv, cl_start, cl_size, cl_terms, cl_toff, n_clusters,
-                       delta, mom, ldm);
+    hipLaunchKernelGGL((k_moments<double, false>), grid, dim3(64), 0, st, (const double*)Z, ldz,
+                       n, n_snps, V, ldv, nv, nullptr, 0, cl_start, cl_size, cl_terms, cl_toff,
+                       n_clusters, delta, mom, ldm);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int jm_spectrum_moments_multi(const void* Z, int z_is_f32, int64_t ldz, int64_t n,
+                                         int64_t n_snps, const jm_mom_col* cols, int nv_max,
+                                         const int32_t* cl_start, const int32_t* cl_size,
+                                         const int32_t* cl_terms, const int32_t* cl_toff,
+                                         int n_clusters, const double* delta, void* stream) {
+  JM_CHECK_ARG(n >= 1 && n_snps >= 0 && nv_max >= 0 && n_clusters >= 1, "bad sizes");
+  JM_CHECK_ARG(ldz >= n_snps, "ldz < n_snps");
+  if (n_snps == 0) return JM_OK;
+  JM_CHECK_ARG(Z && cols && cl_start && cl_size && cl_terms && cl_toff && delta, "null pointer");
+  int groups = (int)((n + 255) / 256);
+  if (groups > 64) groups = 64;
+  if (groups < 1) groups = 1;
+  const dim3 grid((unsigned)((n_snps + 63) / 64), (unsigned)groups);
+  hipStream_t st = (hipStream_t)stream;
+  if (z_is_f32)
+    hipLaunchKernelGGL((k_moments<float, true>), grid, dim3(64), 0, st, (const float*)Z, ldz, n,
+                       n_snps, nullptr, 0, 0, cols, nv_max, cl_start, cl_size, cl_terms, cl_toff,
+                       n_clusters, delta, nullptr, 0);
+  else
+    hipLaunchKernelGGL((k_moments<double, true>), grid, dim3(64), 0, st, (const double*)Z, ldz, n,
+                       n_snps, nullptr, 0, 0, cols, nv_max, cl_start, cl_size, cl_terms, cl_toff,
+                       n_clusters, delta, nullptr, 0);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

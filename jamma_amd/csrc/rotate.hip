@@ -50,10 +50,11 @@ __device__ __forceinline__ void xcd_tile(int bid, int nrb, int ncb, int& rb, int
 constexpr int F_BM = 64, F_BN = 128, F_BK = 32;
 constexpr int F_AS = F_BK + 1;  // padded LDS row stride (doubles)
 
-struct BSrcGeno {  // B(s, k) = g[s][k] (int8, -1 -> mean[s])
+struct BSrcGeno {  // B(s, k) = g[r(s)][k] (int8, -1 -> mean[r(s)]), r(s) = col ? col[s] : s
   const int8_t* g;
   int64_t ld;
   const double* mean;
+  const int32_t* col;
 };
 struct BSrcF64 {  // B(s, k) = b[s][k]
   const double* b;
@@ -80,15 +81,16 @@ __device__ __forceinline__ void load_b_tile(BTile<BSrcGeno>& t, const BSrcGeno& 
   const int r = tid >> 1;
   const int c = (tid & 1) * 16;
   const int64_t s = s0 + r;
+  const int64_t sr = (s < n_cols && src.col) ? (int64_t)src.col[s] : s;
   int8_t tmp[16];
   if (s < n_cols && k0 + c + 16 <= K) {
-    const int4 w = *reinterpret_cast<const int4*>(src.g + s * src.ld + k0 + c);
+    const int4 w = *reinterpret_cast<const int4*>(src.g + sr * src.ld + k0 + c);
     memcpy(tmp, &w, 16);
   } else {
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const int64_t k = k0 + c + e;
-      tmp[e] = (s < n_cols && k < K) ? src.g[s * src.ld + k] : (int8_t)0;
+      tmp[e] = (s < n_cols && k < K) ? src.g[sr * src.ld + k] : (int8_t)0;
     }
   }
 #pragma unroll
@@ -116,7 +118,8 @@ __device__ __forceinline__ double b_frag(const BTile<BSrcF64>& t, int col, int k
 }
 
 __device__ __forceinline__ double col_mean(const BSrcGeno& src, int64_t s, int64_t n_cols) {
-  return s < n_cols ? src.mean[s] : 0.0;
+  if (s >= n_cols) return 0.0;
+  return src.mean[src.col ? (int64_t)src.col[s] : s];
 }
 __device__ __forceinline__ double col_mean(const BSrcF64&, int64_t, int64_t) { return 0.0; }
 
@@ -222,8 +225,8 @@ constexpr int B_GS = B_BK + 16;  // int8 bytes per genotype LDS row
 __global__ void __launch_bounds__(256) k_rotate_bf16x3(
     const uint16_t* __restrict__ Ahi, const uint16_t* __restrict__ Amid,
     const uint16_t* __restrict__ Alo, int64_t lda, const int8_t* __restrict__ g, int64_t ldg,
-    const double* __restrict__ mean, int64_t M, int64_t n_cols, int64_t K, float* __restrict__ Z,
-    int64_t ldz) {
+    const double* __restrict__ mean, const int32_t* __restrict__ col, int64_t M, int64_t n_cols,
+    int64_t K, float* __restrict__ Z, int64_t ldz) {
   __shared__ __attribute__((aligned(16))) uint16_t As[3][B_BM * B_AS];
   __shared__ __attribute__((aligned(16))) int8_t Gs[B_BN * B_GS];
   const int tid = threadIdx.x;
@@ -240,7 +243,7 @@ __global__ void __launch_bounds__(256) k_rotate_bf16x3(
 #pragma unroll
   for (int tj = 0; tj < 4; ++tj) {
     const int64_t s = s0 + wc + tj * 16 + lr;
-    const double mu = (s < n_cols) ? mean[s] : 0.0;
+    const double mu = (s < n_cols) ? mean[col ? (int64_t)col[s] : s] : 0.0;
     m1[tj] = f32_to_bf16_rne((float)mu);
     m2[tj] = f32_to_bf16_rne((float)(mu - bf16_to_f64(m1[tj])));
   }
@@ -274,12 +277,13 @@ __global__ void __launch_bounds__(256) k_rotate_bf16x3(
       const int64_t s = s0 + r;
       int4 w = {0, 0, 0, 0};
       if (s < n_cols) {
+        const int64_t sr = col ? (int64_t)col[s] : s;
         if (k0 + c + 16 <= K) {
-          w = *reinterpret_cast<const int4*>(g + s * ldg + k0 + c);
+          w = *reinterpret_cast<const int4*>(g + sr * ldg + k0 + c);
         } else {
           int8_t tmp[16];
 #pragma unroll
-          for (int e = 0; e < 16; ++e) tmp[e] = (k0 + c + e < K) ? g[s * ldg + k0 + c + e] : (int8_t)0;
+          for (int e = 0; e < 16; ++e) tmp[e] = (k0 + c + e < K) ? g[sr * ldg + k0 + c + e] : (int8_t)0;
           memcpy(&w, tmp, 16);
         }
       }
@@ -343,15 +347,15 @@ __global__ void __launch_bounds__(256) k_rotate_bf16x3(
 using namespace jm;
 
 extern "C" int jm_rotate_f64(const double* UT, int64_t ld_u, const int8_t* g, int64_t ld_g,
-                             const double* mean, int64_t n, int64_t n_snps, double* Z, int64_t ldz,
-                             void* stream) {
+                             const double* mean, const int32_t* col_idx, int64_t n, int64_t n_snps,
+                             double* Z, int64_t ldz, void* stream) {
   JM_CHECK_ARG(n >= 1 && n_snps >= 0, "bad sizes");
   if (n_snps == 0) return JM_OK;
   JM_CHECK_ARG(UT && g && mean && Z, "null pointer");
   JM_CHECK_ARG(ld_u >= n && ld_g >= n && ldz >= n_snps, "bad leading dimension");
   JM_CHECK_ARG(ld_g % 16 == 0 && ((uintptr_t)g % 16) == 0, "g must be 16-byte aligned rows");
   const int nrb = ceil_div(n, F_BM), ncb = ceil_div(n_snps, F_BN);
-  BSrcGeno src{g, ld_g, mean};
+  BSrcGeno src{g, ld_g, mean, col_idx};
   hipLaunchKernelGGL(k_gemm_f64<BSrcGeno>, dim3(nrb * ncb), dim3(256), 0, (hipStream_t)stream,
                      UT, ld_u, src, n, n_snps, n, Z, ldz);
   JM_CHECK_LAUNCH();
@@ -385,7 +389,8 @@ extern "C" int jm_split_bf16x3(const double* UT, int64_t ld_u, int64_t n, uint16
 
 extern "C" int jm_rotate_bf16x3(const uint16_t* hi, const uint16_t* mid, const uint16_t* lo,
                                 int64_t ld_s, const int8_t* g, int64_t ld_g, const double* mean,
-                                int64_t n, int64_t n_snps, float* Z, int64_t ldz, void* stream) {
+                                const int32_t* col_idx, int64_t n, int64_t n_snps, float* Z,
+                                int64_t ldz, void* stream) {
   JM_CHECK_ARG(n >= 1 && n_snps >= 0, "bad sizes");
   if (n_snps == 0) return JM_OK;
   JM_CHECK_ARG(hi && mid && lo && g && mean && Z, "null pointer");
@@ -396,7 +401,7 @@ extern "C" int jm_rotate_bf16x3(const uint16_t* hi, const uint16_t* mid, const u
                "operands must be 16-byte aligned");
   const int nrb = ceil_div(n, B_BM), ncb = ceil_div(n_snps, B_BN);
   hipLaunchKernelGGL(k_rotate_bf16x3, dim3(nrb * ncb), dim3(256), 0, (hipStream_t)stream, hi, mid,
-                     lo, ld_s, g, ld_g, mean, n, n_snps, n, Z, ldz);
+                     lo, ld_s, g, ld_g, mean, col_idx, n, n_snps, n, Z, ldz);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

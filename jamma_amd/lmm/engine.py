@@ -91,21 +91,23 @@ class Cohort:
         return out
 
     def rotate_genotypes(self, g: torch.Tensor, ld_g: int, means: torch.Tensor, n_snps: int,
-                         precision: str) -> tuple[torch.Tensor, bool, int]:
-        """Z = U^T X for a genotype tile (SNP-major int8 rows, -1 missing -> means)."""
+                         precision: str, col_idx: torch.Tensor | None = None
+                         ) -> tuple[torch.Tensor, bool, int]:
+        """Z = U^T X for a genotype tile (SNP-major int8 rows, -1 missing -> means).
+        With col_idx, column s uses genotype row / mean col_idx[s]."""
         ldz = max(1, n_snps)
         if precision == "f64":
             Z = torch.empty((self.n, ldz), dtype=torch.float64, device=self.device)
-            _lib.call("jm_rotate_f64", ptr(self.UT), self.ld_u, ptr(g), ld_g, ptr(means), self.n,
-                      n_snps, ptr(Z), ldz, stream_ptr())
+            _lib.call("jm_rotate_f64", ptr(self.UT), self.ld_u, ptr(g), ld_g, ptr(means),
+                      ptr(col_idx), self.n, n_snps, ptr(Z), ldz, stream_ptr())
             return Z, False, ldz
         if precision == "mixed":
             if self.split is None:
                 raise ValueError("cohort was built without the bf16 split (precision='f64')")
             Z = torch.empty((self.n, ldz), dtype=torch.float32, device=self.device)
             _lib.call("jm_rotate_bf16x3", ptr(self.split[0]), ptr(self.split[1]),
-                      ptr(self.split[2]), self.ld_s, ptr(g), ld_g, ptr(means), self.n, n_snps,
-                      ptr(Z), ldz, stream_ptr())
+                      ptr(self.split[2]), self.ld_s, ptr(g), ld_g, ptr(means), ptr(col_idx),
+                      self.n, n_snps, ptr(Z), ldz, stream_ptr())
             return Z, True, ldz
         raise ValueError(f"unknown precision {precision!r}")
 

@@ -397,9 +397,10 @@ def assoc_chunk(mode, n_cvt, d, UtW, Uty, Utx, Hi_null=None, logl_H0=None,
     return out
 
 
-def snp_pass1(genotypes_valid):
-    """Pass-1 float32 statistics, runner_streaming.py:248-263."""
-    g32 = np.asarray(genotypes_valid, dtype=np.float32)
+def snp_pass1(genotypes_valid, dtype=np.float32):
+    """Pass-1 statistics on a ``dtype`` chunk (float32 in runner_streaming.py:248-263,
+    float64 in the LOCO per-chromosome runner, loco.py:305-324)."""
+    g32 = np.asarray(genotypes_valid, dtype=dtype)
     miss = np.sum(np.isnan(g32), axis=0)
     with warnings.catch_warnings():
         warnings.simplefilter("ignore", RuntimeWarning)
@@ -412,7 +413,7 @@ def snp_pass1(genotypes_valid):
 
 def run_lmm(genotypes, phenotypes, kinship=None, covariates=None, eigenvalues=None,
             eigenvectors=None, maf_threshold=0.01, miss_threshold=0.05, l_min=1e-5,
-            l_max=1e5, n_grid=50, n_refine=10, lmm_mode=1, chunk=512):
+            l_max=1e5, n_grid=50, n_refine=10, lmm_mode=1, chunk=512, stat_dtype=np.float32):
     """run_lmm_association_streaming semantics (runner_streaming.py:94-628) on an
     in-memory (n_samples, n_snps) genotype array with NaN for missing.
 
@@ -429,7 +430,7 @@ def run_lmm(genotypes, phenotypes, kinship=None, covariates=None, eigenvalues=No
         covariates = covariates[valid, :]
     G = genotypes[valid, :]
     n = y.shape[0]
-    means, miss, var = snp_pass1(G)
+    means, miss, var = snp_pass1(G, stat_dtype)
     af = means / 2.0
     maf = np.minimum(af, 1.0 - af)
     mask = (maf >= maf_threshold) & (miss / n <= miss_threshold) & (var > 0)
@@ -464,3 +465,25 @@ def run_lmm(genotypes, phenotypes, kinship=None, covariates=None, eigenvalues=No
     for k in parts[0]:
         res[k] = np.concatenate([pp[k] for pp in parts])
     return res
+
+
+def run_lmm_loco(genotypes, chromosomes, phenotypes, covariates=None, maf_threshold=0.01,
+                 miss_threshold=0.05, lmm_mode=1):
+    """run_lmm_loco semantics (loco.py:54-593): LOCO kinships over all samples with the
+    LMM thresholds, chromosomes in sorted order, float64 per-chromosome statistics.
+    Returns {chr: run_lmm result dict (snp_indices relative to that chromosome)}."""
+    from oracle import kinship as ok
+
+    chromosomes = np.asarray(chromosomes)
+    loco = ok.loco_kinship(genotypes, chromosomes, maf_threshold, miss_threshold)
+    out = {}
+    for c in sorted(set(chromosomes.tolist())):
+        if c not in loco:
+            continue
+        sel = np.where(chromosomes == c)[0]
+        r = run_lmm(genotypes[:, sel], phenotypes, kinship=loco[c], covariates=covariates,
+                    maf_threshold=maf_threshold, miss_threshold=miss_threshold,
+                    lmm_mode=lmm_mode, stat_dtype=np.float64)
+        r["global_indices"] = sel[r["snp_indices"]]
+        out[c] = r
+    return out

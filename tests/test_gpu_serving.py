@@ -1,0 +1,137 @@
+"""Serving loop (GPU): continuous batching over resident cohorts.
+
+Each request's result must be the same alone and in the mix (same device
+instructions per SNP, so bit-identical), match the CPU oracle on the same
+inputs, and admission must respect the HBM budget.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import kinship as ok
+from oracle import lmm as ol
+from tests.util import balding_nichols
+
+pytestmark = pytest.mark.gpu
+
+
+def _cohort(name, n, p, seed, precision="mixed"):
+    from jamma_amd.device import round_up, to_dev
+    from jamma_amd.io.plink import stats_from_counts
+    from jamma_amd.serve import ResidentCohort
+
+    G = balding_nichols(n, p, miss=0.01, seed=seed)
+    K = ok.centered_kinship(G)
+    d, U = ol.eigendecompose(K)
+    ld = round_up(n, 64)
+    g8 = np.zeros((p, ld), np.int8)
+    g8[:, :n] = np.where(np.isnan(G), -1, G).astype(np.int8).T
+    counts = np.stack([(G == 0).sum(0), (G == 1).sum(0), (G == 2).sum(0), np.isnan(G).sum(0)], 1)
+    mean32, _, _ = stats_from_counts(counts, np.float32)
+    rc = ResidentCohort(name, to_dev(g8, torch.int8), ld, to_dev(mean32, torch.float64), d, U,
+                        precision=precision)
+    return rc, G, d, U, mean32
+
+
+def _requests(rng, cohorts, k):
+    from jamma_amd.serve import Request
+
+    out = []
+    for i in range(k):
+        name, (rc, G, *_) = list(cohorts.items())[i % len(cohorts)]
+        n = rc.n
+        ncov = int(rng.integers(0, 4))
+        W = np.column_stack([np.ones(n), rng.normal(size=(n, ncov))]) if ncov else None
+        y = rng.normal(size=n) + 0.3 * np.nan_to_num(G[:, i % 50])
+        length = int(rng.integers(20, 400))
+        start = int(rng.integers(0, rc.p - length))
+        out.append(Request(name, y, start, start + length, covariates=W, mode=4))
+    return out
+
+
+def _clone(r):
+    from jamma_amd.serve import Request
+
+    return Request(r.cohort, r.phenotype, r.snp_start, r.snp_end, r.covariates, r.mode)
+
+
+@pytest.fixture(scope="module")
+def cohorts():
+    return {"a": _cohort("a", 300, 900, 1), "b": _cohort("b", 517, 700, 2)}
+
+
+def test_alone_equals_mix_and_oracle(cohorts):
+    from jamma_amd.serve import Server
+
+    rng = np.random.default_rng(0)
+    reqs = _requests(rng, cohorts, 8)
+    alone = []
+    for r in reqs:
+        srv = Server(slice_snps=64, max_batch_snps=256)
+        for rc, *_ in cohorts.values():
+            srv.add_cohort(rc)
+        with srv:
+            alone.append(srv.submit(_clone(r)).result(timeout=120))
+    srv = Server(slice_snps=64, max_batch_snps=256)
+    for rc, *_ in cohorts.values():
+        srv.add_cohort(rc)
+    with srv:
+        futs = [srv.submit(_clone(r)) for r in reqs]
+        mixed = [f.result(timeout=300) for f in futs]
+        stats = srv.latency_stats()
+    assert stats["max_concurrent"] >= 2, stats
+    for a, m in zip(alone, mixed):
+        assert np.array_equal(a, m, equal_nan=True)
+    # oracle on the same inputs
+    for r, res in zip(reqs, mixed):
+        rc, G, d, U, mean32 = cohorts[r.cohort]
+        X = G[:, r.snp_start:r.snp_end]
+        X = np.where(np.isnan(X), mean32[r.snp_start:r.snp_end][None, :], X)
+        W = r.covariates if r.covariates is not None else np.ones((rc.n, 1))
+        UT = U.T
+        UtW, Uty = UT @ W, UT @ r.phenotype
+        lam0, ll0 = ol.null_model_mle(d, UtW, Uty, W.shape[1])
+        ref = ol.assoc_chunk(4, W.shape[1], d, UtW, Uty, UT @ X, 1.0 / (lam0 * d + 1.0), ll0)
+        for k, col in (("p_wald", 2), ("p_lrt", 7), ("p_score", 8)):
+            dl = np.abs(np.log10(res[:, col]) - np.log10(ref[k]))
+            assert np.nanmax(dl) <= 1e-4, (r.cohort, k, np.nanmax(dl))
+
+
+def test_admission_budget_queues_and_completes(cohorts):
+    from jamma_amd.serve import Request, Server
+    from jamma_amd.serve.server import _Active
+
+    rng = np.random.default_rng(1)
+    reqs = _requests(rng, cohorts, 5)
+    rc = cohorts["a"][0]
+    one = max(_Active.cost_bytes(cohorts[r.cohort][0], r, 64) for r in reqs)
+    srv = Server(hbm_budget_bytes=int(one * 1.5), slice_snps=64)
+    for c, *_ in cohorts.values():
+        srv.add_cohort(c)
+    with srv:
+        futs = [srv.submit(r) for r in reqs]
+        res = [f.result(timeout=300) for f in futs]
+        stats = srv.latency_stats()
+    assert stats["max_concurrent"] < len(reqs)  # the budget forces queueing
+    assert all(np.isfinite(x[:, 2]).all() for x in res)
+    tiny = Server(hbm_budget_bytes=1024)
+    tiny.add_cohort(rc)
+    with pytest.raises(MemoryError):
+        tiny.submit(Request("a", np.zeros(rc.n), 0, rc.p, covariates=None))
+
+
+def test_bad_request_fails_alone(cohorts):
+    from jamma_amd.serve import Request, Server
+
+    rc = cohorts["a"][0]
+    srv = Server(slice_snps=64)
+    srv.add_cohort(rc)
+    with srv:
+        bad = srv.submit(Request("a", np.zeros(rc.n + 1), 0, 100))
+        good = srv.submit(Request("a", np.random.default_rng(2).normal(size=rc.n), 0, 100))
+        with pytest.raises(ValueError):
+            bad.result(timeout=60)
+        assert good.result(timeout=120).shape == (100, 12)
