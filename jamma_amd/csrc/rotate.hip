@@ -24,6 +24,8 @@
 
 #include "jm_common.h"
 
+#include <cstdlib>
+
 namespace jm {
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
@@ -342,6 +344,151 @@ __global__ void __launch_bounds__(256) k_rotate_bf16x3(
       }
 }
 
+// ---------------------------------------------------------------------------
+// BF16x3 rotation, v2: 256x128 block tile (8 waves of 64x64), BK = 32,
+// genotypes converted to bf16 X1/X2 once per stage in LDS, next K-slab
+// prefetched into registers while the current one is multiplied.
+// ---------------------------------------------------------------------------
+constexpr int V2_BM = 256, V2_BN = 128, V2_BK = 32;
+constexpr int V2_AS = V2_BK + 8;  // bf16 per LDS row (80 B, 16-byte aligned)
+constexpr int V2_BS = V2_BK + 8;
+
+__device__ __forceinline__ uint16_t geno_bf16(int8_t x, uint16_t m1) {
+  // 0 -> 0x0000, 1 -> 0x3F80, 2 -> 0x4000 (exact), missing -> bf16(mean)
+  return x < 0 ? m1 : (x == 0 ? (uint16_t)0 : (uint16_t)(0x3F00 + 0x80 * (int)x));
+}
+
+__global__ void __launch_bounds__(512) k_rotate_bf16x3_v2(
+    const uint16_t* __restrict__ Ahi, const uint16_t* __restrict__ Amid,
+    const uint16_t* __restrict__ Alo, int64_t lda, const int8_t* __restrict__ g, int64_t ldg,
+    const double* __restrict__ mean, const int32_t* __restrict__ col, int64_t M, int64_t n_cols,
+    int64_t K, float* __restrict__ Z, int64_t ldz) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[3][V2_BM * V2_AS];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][V2_BN * V2_BS];
+  __shared__ uint16_t Bm[2][V2_BN];  // per column bf16(mean), residual
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int nrb = (int)((M + V2_BM - 1) / V2_BM), ncb = (int)((n_cols + V2_BN - 1) / V2_BN);
+  int rb, cb;
+  xcd_tile(blockIdx.x, nrb, ncb, rb, cb);
+  const int64_t i0 = (int64_t)rb * V2_BM, s0 = (int64_t)cb * V2_BN;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+  if (tid < V2_BN) {
+    const int64_t s = s0 + tid;
+    const double mu = (s < n_cols) ? mean[col ? (int64_t)col[s] : s] : 0.0;
+    const uint16_t m1 = f32_to_bf16_rne((float)mu);
+    Bm[0][tid] = m1;
+    Bm[1][tid] = f32_to_bf16_rne((float)(mu - bf16_to_f64(m1)));
+  }
+  // per-thread load slots: A: 3 planes x 256 rows x 64 B = 3072 x 16 B -> 6 per thread
+  //                        B: 128 rows x 32 B -> 256 x 16 B: threads 0..255, one each
+  const int b_r = tid >> 1, b_c = (tid & 1) * 16;
+  const int64_t b_s = s0 + b_r;
+  const int64_t b_row = (b_s < n_cols) ? (col ? (int64_t)col[b_s] : b_s) : -1;
+  int4 ra[6];
+  int4 rbv = {0, 0, 0, 0};
+  const uint16_t* Ap[3] = {Ahi, Amid, Alo};
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+      const int idx = tid + 512 * u;        // 0..3071
+      const int pl = idx >> 10;             // plane
+      const int r = (idx & 1023) >> 2;      // row 0..255
+      const int c = (idx & 3) * 8;          // bf16 column 0,8,16,24
+      const int64_t i = i0 + r;
+      int4 w = {0, 0, 0, 0};
+      if (i < M && k0 + c + 8 <= lda) w = *reinterpret_cast<const int4*>(Ap[pl] + i * lda + k0 + c);
+      ra[u] = w;
+    }
+    if (tid < 256) {
+      int4 w = {0, 0, 0, 0};
+      if (b_row >= 0) {
+        if (k0 + b_c + 16 <= K) {
+          w = *reinterpret_cast<const int4*>(g + b_row * ldg + k0 + b_c);
+        } else {
+          int8_t tmp[16];
+#pragma unroll
+          for (int e = 0; e < 16; ++e)
+            tmp[e] = (k0 + b_c + e < K) ? g[b_row * ldg + k0 + b_c + e] : (int8_t)0;
+          memcpy(&w, tmp, 16);
+        }
+      }
+      rbv = w;
+    }
+  };
+  auto lstore = [&]() {
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+      const int idx = tid + 512 * u;
+      const int pl = idx >> 10;
+      const int r = (idx & 1023) >> 2;
+      const int c = (idx & 3) * 8;
+      *reinterpret_cast<int4*>(&As[pl][r * V2_AS + c]) = ra[u];
+    }
+    if (tid < 256) {
+      const uint16_t m1 = Bm[0][b_r], m2 = Bm[1][b_r];
+      int8_t x[16];
+      memcpy(x, &rbv, 16);
+      uint16_t v1[16], v2[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        v1[e] = geno_bf16(x[e], m1);
+        v2[e] = x[e] < 0 ? m2 : (uint16_t)0;
+      }
+      memcpy(&Bs[0][b_r * V2_BS + b_c], v1, 32);
+      memcpy(&Bs[1][b_r * V2_BS + b_c], v2, 32);
+    }
+  };
+  f4_t acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
+  __syncthreads();  // Bm visible
+  gload(0);
+  for (int64_t k0 = 0; k0 < K; k0 += V2_BK) {
+    lstore();
+    __syncthreads();
+    if (k0 + V2_BK < K) gload(k0 + V2_BK);  // prefetch the next slab
+    bf16x8_t b1[4], b2[4];
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) {
+      const int cidx = (wc + tj * 16 + lr) * V2_BS + 8 * lk;
+      memcpy(&b1[tj], &Bs[0][cidx], 16);
+      memcpy(&b2[tj], &Bs[1][cidx], 16);
+    }
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) {
+      const int aidx = (wr + ti * 16 + lr) * V2_AS + 8 * lk;
+      bf16x8_t ah, am, al;
+      memcpy(&ah, &As[0][aidx], 16);
+      memcpy(&am, &As[1][aidx], 16);
+      memcpy(&al, &As[2][aidx], 16);
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        f4_t c = acc[ti][tj];
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b1[tj], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, b1[tj], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b2[tj], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b1[tj], c, 0, 0, 0);
+        acc[ti][tj] = c;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
+        const int64_t s = s0 + wc + tj * 16 + lr;
+        if (i < M && s < n_cols) Z[i * ldz + s] = acc[ti][tj][r];
+      }
+}
+
 }  // namespace jm
 
 using namespace jm;
@@ -399,9 +546,19 @@ extern "C" int jm_rotate_bf16x3(const uint16_t* hi, const uint16_t* mid, const u
   JM_CHECK_ARG(((uintptr_t)g % 16) == 0 && ((uintptr_t)hi % 16) == 0 &&
                    ((uintptr_t)mid % 16) == 0 && ((uintptr_t)lo % 16) == 0,
                "operands must be 16-byte aligned");
-  const int nrb = ceil_div(n, B_BM), ncb = ceil_div(n_snps, B_BN);
-  hipLaunchKernelGGL(k_rotate_bf16x3, dim3(nrb * ncb), dim3(256), 0, (hipStream_t)stream, hi, mid,
-                     lo, ld_s, g, ld_g, mean, col_idx, n, n_snps, n, Z, ldz);
+  static const int version = [] {
+    const char* v = getenv("JM_ROT_BF16_V");
+    return v ? atoi(v) : 1;
+  }();
+  if (version == 2) {
+    const int nrb = ceil_div(n, V2_BM), ncb = ceil_div(n_snps, V2_BN);
+    hipLaunchKernelGGL(k_rotate_bf16x3_v2, dim3(nrb * ncb), dim3(512), 0, (hipStream_t)stream, hi,
+                       mid, lo, ld_s, g, ld_g, mean, col_idx, n, n_snps, n, Z, ldz);
+  } else {
+    const int nrb = ceil_div(n, B_BM), ncb = ceil_div(n_snps, B_BN);
+    hipLaunchKernelGGL(k_rotate_bf16x3, dim3(nrb * ncb), dim3(256), 0, (hipStream_t)stream, hi,
+                       mid, lo, ld_s, g, ld_g, mean, col_idx, n, n_snps, n, Z, ldz);
+  }
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

@@ -32,6 +32,33 @@ def get_ab_index(a: int, b: int, n_cvt: int) -> int:
     return (2 * cols - a1 + 2) * (a1 - 1) // 2 + b1 - a1
 
 
+def pair_tables(n_cvt: int) -> tuple[np.ndarray, np.ndarray]:
+    """Where each packed GEMMA pair (a, b) gets its moments.
+
+    pair_row[j] >= 0: the pair involves the genotype (column n_cvt+1); its moments
+    are row pair_row[j] of the per-SNP block, whose rows are the genotype's
+    partners W_1..W_c (rows 0..c-1), y (row c) and the genotype itself (row c+1).
+    pair_row[j] = -1: a covariate/phenotype pair; shared_src[j] = (column, row)
+    of the shared moment block computed from Z = [UtW | Uty | 1] with partners
+    [UtW | Uty] (row c+1 = the column with itself).
+    """
+    c = n_cvt
+    ni = (c + 3) * (c + 2) // 2
+    pair_row = np.full(ni, -1, dtype=np.int32)
+    shared_src = np.zeros((ni, 2), dtype=np.int64)
+    xc, yc = c + 1, c + 2
+    for a in range(1, c + 3):
+        for b in range(a, c + 3):
+            j = get_ab_index(a, b, c)
+            if a == xc or b == xc:
+                other = b if a == xc else a
+                pair_row[j] = (c + 1) if other == xc else (c if other == yc else other - 1)
+            else:
+                col = a - 1 if a <= c else c
+                shared_src[j] = (col, c + 1) if a == b else (col, (b - 1) if b <= c else c)
+    return pair_row, shared_src
+
+
 class Cohort:
     """Eigendecomposition of one kinship, resident on the device."""
 
@@ -156,23 +183,7 @@ class AssocProblem:
         self.V = R.t().contiguous()  # partner rows: UtW_1..UtW_c, Uty   ((c+1) x n)
         Zs = torch.cat([R, torch.ones((n, 1), dtype=torch.float64, device=dev)], dim=1).contiguous()
         smom = cohort.moments(Zs, False, c + 2, c + 2, self.V, c + 1)  # ((c+2), (c+2), T)
-        # shared (lambda-independent) pair rows in packed GEMMA order; genotype pairs -> row -1
-        pair_row = np.full(ni, -1, dtype=np.int32)
-        shared_src = np.zeros((ni, 2), dtype=np.int64)  # (column, row) of smom
-        xc = c + 1  # genotype column (1-based)
-        yc = c + 2
-        for a in range(1, c + 3):
-            for b in range(a, c + 3):
-                j = get_ab_index(a, b, c)
-                if a == xc or b == xc:
-                    other = b if a == xc else a
-                    pair_row[j] = (c + 1) if other == xc else (c if other == yc else other - 1)
-                else:
-                    col = a - 1 if a <= c else c
-                    if a == b:
-                        shared_src[j] = (col, c + 1)  # self row
-                    else:
-                        shared_src[j] = (col, (b - 1) if b <= c else c)
+        pair_row, shared_src = pair_tables(c)
         idx_col = torch.as_tensor(shared_src[:, 0], device=dev)
         idx_row = torch.as_tensor(shared_src[:, 1], device=dev)
         shared = smom[idx_col, idx_row].contiguous()

@@ -104,6 +104,45 @@ class _Active:
         return self.next >= self.req.snp_end
 
 
+class AdmissionQueue:
+    """First-fit admission in arrival order against a byte budget (host logic).
+
+    ``try_admit(cost, create)`` is called for queued requests in order; a request
+    is admitted when ``used + cost <= budget`` and ``create`` succeeds. A request
+    whose ``create`` raises fails alone (its error is returned) without holding
+    budget; requests that do not fit stay queued in order.
+    """
+
+    def __init__(self, budget: int):
+        self.budget = int(budget)
+        self.used = 0
+        self.queue: list = []
+
+    def fits_ever(self, cost: int) -> bool:
+        return cost <= self.budget
+
+    def admit(self, cost_of, create):
+        """-> (admitted [(item, obj, cost)], failed [(item, exc)])."""
+        admitted, failed, keep = [], [], []
+        for item in self.queue:
+            cost = cost_of(item)
+            if self.used + cost <= self.budget:
+                try:
+                    obj = create(item)
+                except Exception as e:  # noqa: BLE001 - reported to the request
+                    failed.append((item, e))
+                    continue
+                self.used += cost
+                admitted.append((item, obj, cost))
+            else:
+                keep.append(item)
+        self.queue = keep
+        return admitted, failed
+
+    def release(self, cost: int) -> None:
+        self.used -= cost
+
+
 class Server:
     """Scheduler thread owning the device work of every resident cohort."""
 
@@ -116,9 +155,7 @@ class Server:
         if hbm_budget_bytes is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget_bytes = int(free * 0.5)
-        self.budget = hbm_budget_bytes
-        self.used = 0
-        self._queue: list[Request] = []
+        self.adm = AdmissionQueue(hbm_budget_bytes)
         self._active: dict[str, list[_Active]] = {}
         self._cv = threading.Condition()
         self._ids = itertools.count()
@@ -143,12 +180,12 @@ class Server:
             raise ValueError(f"SNP range [{req.snp_start}, {req.snp_end}) outside [0, {rc.p})")
         if req.mode not in (1, 2, 3, 4):
             raise ValueError("mode must be 1..4")
-        if _Active.cost_bytes(rc, req, self.slice_snps) > self.budget:
+        if not self.adm.fits_ever(_Active.cost_bytes(rc, req, self.slice_snps)):
             raise MemoryError("request can never fit in the HBM budget")
         req.id = next(self._ids)
         req.t_submit = time.perf_counter()
         with self._cv:
-            self._queue.append(req)
+            self.adm.queue.append(req)
             self._cv.notify()
         return req.future
 
@@ -174,24 +211,20 @@ class Server:
     # -- scheduler --------------------------------------------------------------
     def _admit(self) -> None:
         """First-fit admission in arrival order against the HBM budget."""
-        keep = []
-        for req in self._queue:
+
+        def create(req):
             rc = self.cohorts[req.cohort]
-            cost = _Active.cost_bytes(rc, req, self.slice_snps)
-            if self.used + cost <= self.budget:
-                try:
-                    with torch.cuda.stream(rc.stream):
-                        act = _Active(req, rc, self.slice_snps)
-                except Exception as e:  # bad request: fail it, keep serving
-                    req.future.set_exception(e)
-                    continue
-                act.cost = cost
-                self.used += cost
-                req.t_admit = time.perf_counter()
-                self._active[req.cohort].append(act)
-            else:
-                keep.append(req)
-        self._queue = keep
+            with torch.cuda.stream(rc.stream):
+                return _Active(req, rc, self.slice_snps)
+
+        admitted, failed = self.adm.admit(
+            lambda r: _Active.cost_bytes(self.cohorts[r.cohort], r, self.slice_snps), create)
+        for req, exc in failed:
+            req.future.set_exception(exc)
+        for req, act, cost in admitted:
+            act.cost = cost
+            req.t_admit = time.perf_counter()
+            self._active[req.cohort].append(act)
 
     def _step_cohort(self, name: str) -> list[_Active]:
         acts = self._active[name]
@@ -263,9 +296,9 @@ class Server:
     def _loop(self) -> None:
         while True:
             with self._cv:
-                while not self._stop and not self._queue and not any(self._active.values()):
+                while not self._stop and not self.adm.queue and not any(self._active.values()):
                     self._cv.wait()
-                if self._stop and not self._queue and not any(self._active.values()):
+                if self._stop and not self.adm.queue and not any(self._active.values()):
                     return
                 self._admit()
             self.max_concurrent = max(self.max_concurrent,
@@ -277,12 +310,9 @@ class Server:
                     a.req.t_done = time.perf_counter()
                     self._active[name].remove(a)
                     with self._cv:
-                        self.used -= a.cost
+                        self.adm.release(a.cost)
                     self.completed.append(a.req)
                     a.req.future.set_result(res)
-            if not any(self._active.values()) and self._queue and self.used == 0:
-                # nothing admitted and nothing running: the queue head is stuck
-                pass
 
     # -- reporting -------------------------------------------------------------
     def latency_stats(self) -> dict:

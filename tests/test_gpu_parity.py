@@ -250,3 +250,29 @@ def test_gwas_end_to_end(tmp_path):
     gem = load_assoc(GOLDEN / "gemma_synthetic" / "gemma_assoc.assoc.txt")
     np.testing.assert_allclose(got["p_wald"], gem["p_wald"], rtol=1e-4)
     np.testing.assert_allclose(got["se"], gem["se"], rtol=1e-5)
+
+
+def test_loco_runner_matches_oracle(tmp_path):
+    from jamma_amd.lmm.loco import run_lmm_loco, run_lmm_loco_multi
+
+    G = balding_nichols(240, 600, miss=0.01, seed=11)
+    chrom = np.array([str(1 + (j * 3) // 600) for j in range(600)])
+    y, W = phenotype(G, n_cov=2, seed=11)
+    y[5] = np.nan
+    pre = tmp_path / "loco"
+    opl.write_plink(pre, G, phenotypes=y, chromosomes=chrom)
+    res = run_lmm_loco(pre, y, covariates=W, lmm_mode=4, precision="f64")
+    ref = ol.run_lmm_loco(G, chrom, y, covariates=W, lmm_mode=4)
+    order = np.concatenate([ref[c]["global_indices"] for c in sorted(ref)])
+    assert [r.rs for r in res] == [f"rs{j:06d}" for j in order]
+    cat = {k: np.concatenate([ref[c][k] for c in sorted(ref)]) for k in ALL}
+    _assert_matches_oracle(_arrays(res, ALL), cat, ALL)
+    # two phenotypes sharing the LOCO eigendecompositions
+    y2 = y + 0.5 * np.nan_to_num(G[:, 3])
+    multi = run_lmm_loco_multi(pre, np.column_stack([y, y2]), covariates=W, lmm_mode=1,
+                               precision="f64")
+    assert np.allclose(_arrays(multi[0], ("p_wald",))["p_wald"], cat["p_wald"], rtol=1e-6)
+    ref2 = ol.run_lmm_loco(G, chrom, y2, covariates=W, lmm_mode=1)
+    p2 = np.concatenate([ref2[c]["p_wald"] for c in sorted(ref2)])
+    np.testing.assert_allclose(-np.log10(_arrays(multi[1], ("p_wald",))["p_wald"]), -np.log10(p2),
+                               atol=1e-6)

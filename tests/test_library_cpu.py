@@ -156,3 +156,46 @@ def test_writer_all_columns_order():
     cols = format_line(r, "all").split("\t")
     assert cols[6] == "0.250"
     assert cols[7:] == [f"{v:.6e}" for v in (1.5, 0.25, -10.0, 2.0, 3.0, 1e-3, 4e-3, 2e-3)]
+
+
+@pytest.mark.parametrize("n_cvt", [1, 2, 4, 7])
+def test_pair_tables_reproduce_pab_row0(n_cvt):
+    """The moment layout the device kernels use (engine.pair_tables) rebuilds the
+    oracle's Pab row 0 for every packed pair at any lambda (NumPy emulation)."""
+    from jamma_amd.lmm.engine import pair_tables
+
+    rng = np.random.default_rng(n_cvt)
+    n = 150
+    d = np.sort(rng.uniform(0.0, 4.0, n))
+    d[:2] = 0.0
+    pl = plan_spectrum(d)
+    UtW = rng.normal(size=(n, n_cvt))
+    Uty = rng.normal(size=n)
+    Utx = rng.normal(size=(n, 3))
+
+    def moments(z, partners):  # rows: partners..., z*z
+        out = np.zeros((len(partners) + 1, pl.n_terms))
+        for t in range(pl.n_terms):
+            c, m = pl.term_cluster[t], pl.term_m[t]
+            s, e = pl.cl_start[c], pl.cl_start[c] + pl.cl_size[c]
+            w = pl.idx_delta[s:e] ** m
+            for b, v in enumerate(partners):
+                out[b, t] = np.sum(z[s:e] * v[s:e] * w)
+            out[-1, t] = np.sum(z[s:e] ** 2 * w)
+        return out
+
+    partners = [UtW[:, k] for k in range(n_cvt)] + [Uty]
+    Zs = partners + [np.ones(n)]
+    smom = np.stack([moments(z, partners) for z in Zs])  # (c+2, c+2, T)
+    pair_row, shared_src = pair_tables(n_cvt)
+    dc = pl.cl_center[pl.term_cluster]
+    for lam in (1e-5, 0.3, 7.0, 1e5):
+        h = 1 / (1 + lam * dc)
+        kap = h * (-(lam * h)) ** pl.term_m
+        ref = ol.pab_batch(n_cvt, 1.0 / (lam * d + 1.0), UtW, Uty, Utx)[:, 0, :]
+        for s in range(3):
+            xm = moments(Utx[:, s], partners)
+            for j in range(len(pair_row)):
+                row = xm[pair_row[j]] if pair_row[j] >= 0 else smom[shared_src[j][0], shared_src[j][1]]
+                got = np.sum(kap * row)
+                assert abs(got - ref[s, j]) <= 1e-11 * max(1.0, abs(ref[s, j])), (lam, s, j)
