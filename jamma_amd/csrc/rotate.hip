@@ -20,7 +20,10 @@
 // row panel of UT, which then stays L2/MALL-resident while it is consumed.
 //
 // MFMA lane layouts (16x16 outputs): lane l supplies A[l%16][k-group l/16] and
-// B[k-group l/16][l%16]; the accumulator holds D[4*(l/16)+r][l%16], r = 0..3.
+// B[k-group l/16][l%16]. The accumulator element r (0..3) of lane l holds
+//   bf16 / int8 forms:  D[4*(l/16) + r][l%16]
+//   f64 16x16x4 form:   D[(l/16) + 4*r][l%16]
+// (both measured on gfx950 with a single-wave layout check).
 
 #include "jm_common.h"
 
@@ -180,7 +183,7 @@ __global__ void __launch_bounds__(256) k_gemm_f64(const double* __restrict__ A, 
     for (int tj = 0; tj < 4; ++tj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
+        const int64_t i = i0 + wr + ti * 16 + lk + 4 * r;  // f64 output layout
         const int64_t s = s0 + wc + tj * 16 + lr;
         if (i < M && s < n_cols) C[i * ldc + s] = acc[ti][tj][r];
       }
@@ -546,10 +549,9 @@ extern "C" int jm_rotate_bf16x3(const uint16_t* hi, const uint16_t* mid, const u
   JM_CHECK_ARG(((uintptr_t)g % 16) == 0 && ((uintptr_t)hi % 16) == 0 &&
                    ((uintptr_t)mid % 16) == 0 && ((uintptr_t)lo % 16) == 0,
                "operands must be 16-byte aligned");
-  static const int version = [] {
-    const char* v = getenv("JM_ROT_BF16_V");
-    return v ? atoi(v) : 1;
-  }();
+  // tile variant: 1 = 128x128 / 256 threads, 2 = 256x128 / 512 threads (JM_ROT_BF16_V)
+  const char* venv = getenv("JM_ROT_BF16_V");
+  const int version = venv ? atoi(venv) : 1;
   if (version == 2) {
     const int nrb = ceil_div(n, V2_BM), ncb = ceil_div(n_snps, V2_BN);
     hipLaunchKernelGGL(k_rotate_bf16x3_v2, dim3(nrb * ncb), dim3(512), 0, (hipStream_t)stream, hi,
