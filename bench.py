@@ -295,7 +295,7 @@ def main() -> int:
                    "n": n, "p": p, "n_cvt": args.ncov + 1, "tile_snps": args.tile},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["bf16"] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK["bf16"], "traffic": None,
-                     "kernel": "k_rotate_bf16x3", "algorithmic": "2 n^2 FLOP per SNP"},
+                     "kernel": "k_rotate_bf16x3_v2", "algorithmic": "2 n^2 FLOP per SNP"},
         "cpu_baseline": cpu,
         "fp64_path": results.get("f64"),
         "mixed_path": mixed,

@@ -549,9 +549,10 @@ extern "C" int jm_rotate_bf16x3(const uint16_t* hi, const uint16_t* mid, const u
   JM_CHECK_ARG(((uintptr_t)g % 16) == 0 && ((uintptr_t)hi % 16) == 0 &&
                    ((uintptr_t)mid % 16) == 0 && ((uintptr_t)lo % 16) == 0,
                "operands must be 16-byte aligned");
-  // tile variant: 1 = 128x128 / 256 threads, 2 = 256x128 / 512 threads (JM_ROT_BF16_V)
+  // tile variant: 2 = 256x128 / 512 threads with register prefetch (default),
+  // 1 = 128x128 / 256 threads (JM_ROT_BF16_V=1)
   const char* venv = getenv("JM_ROT_BF16_V");
-  const int version = venv ? atoi(venv) : 1;
+  const int version = venv ? atoi(venv) : 2;
   if (version == 2) {
     const int nrb = ceil_div(n, V2_BM), ncb = ceil_div(n_snps, V2_BN);
     hipLaunchKernelGGL(k_rotate_bf16x3_v2, dim3(nrb * ncb), dim3(512), 0, (hipStream_t)stream, hi,
