@@ -34,7 +34,7 @@ extern "C" {
 #define JM_ERR_HIP (-2)
 #define JM_ERR_UNSUPPORTED (-3)
 
-#define JM_VERSION 100
+#define JM_VERSION 101
 
 /* Library version (JM_VERSION). */
 int jm_version(void);
@@ -219,6 +219,8 @@ typedef struct jm_lmm_problem {
   double* null_shared;  /* n_index */
   double lambda_null;   /* null-model MLE lambda (Score test) */
   double logl_null;     /* null-model MLE log-likelihood (LRT) */
+  const double* cl_weight;     /* C : cluster sizes (weight of log|1+lambda d_c| in
+                                  logdet_h; equals logdet_mom of the m = 0 term) */
 } jm_lmm_problem;
 
 /* Output columns of jm_lmm_assoc (n_snps x JM_NOUT float64). */

@@ -83,7 +83,7 @@ __host__ __device__ static double lgamma_diff(double a, double b) {
 }
 
 // Regularised incomplete beta I_x(a, b), a, b > 0.
-__host__ __device__ double betainc_reg(double a, double b, double x) {
+__host__ __device__ __attribute__((noinline)) double betainc_reg(double a, double b, double x) {
   if (!(x > 0.0)) return (x == 0.0) ? 0.0 : NAN;
   if (!(x < 1.0)) return (x == 1.0) ? 1.0 : NAN;
   const double lg = (a >= b) ? lgamma_diff(a, b) - lgamma(b) : lgamma_diff(b, a) - lgamma(a);
@@ -93,7 +93,7 @@ __host__ __device__ double betainc_reg(double a, double b, double x) {
 }
 
 // chi-square(1) survival: P(X > x) = erfc(sqrt(x / 2)).
-__host__ __device__ double chi2_sf1(double x) {
+__host__ __device__ __attribute__((noinline)) double chi2_sf1(double x) {
   if (!(x > 0.0)) return (x == 0.0) ? 1.0 : NAN;
   return erfc(sqrt(0.5 * x));
 }
@@ -423,203 +423,534 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx) {
 }
 
 constexpr int kMaxLevels = 24;  // n_cvt <= 20 -> levels 0..n_cvt
+constexpr int kXR = 8;          // per-SNP moment rows held in registers
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ const double* uniform_ptr(const double* p) {
+  const uint64_t u = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return (const double*)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ double bcast(double v, int src) {
+  const uint64_t u = __double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, src);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), src);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Butterfly reduce-scatter of 8 per-lane values: on return every lane holds
+// the full wave sum of value q = 4*b5 + 2*b4 + b3 (10 exchanges).
+__device__ __forceinline__ double reduce8(double (&v)[8], int lane) {
+  {
+    const bool up = lane & 32;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double send = up ? v[i] : v[i + 4];
+      const double keep = up ? v[i + 4] : v[i];
+      v[i] = keep + shfl_xor_d(send, 32);
+    }
+  }
+  {
+    const bool up = lane & 16;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const double send = up ? v[i] : v[i + 2];
+      const double keep = up ? v[i + 2] : v[i];
+      v[i] = keep + shfl_xor_d(send, 16);
+    }
+  }
+  {
+    const bool up = lane & 8;
+    const double send = up ? v[0] : v[1];
+    const double keep = up ? v[1] : v[0];
+    v[0] = keep + shfl_xor_d(send, 8);
+  }
+  v[0] = v[0] + shfl_xor_d(v[0], 4);
+  v[0] = v[0] + shfl_xor_d(v[0], 2);
+  v[0] = v[0] + shfl_xor_d(v[0], 1);
+  return v[0];
+}
+
+// Sum over the 32 lanes of a half-wave (both halves at once).
+__device__ __forceinline__ double half_sum(double v) {
+#pragma unroll
+  for (int m = 16; m >= 1; m >>= 1) v += shfl_xor_d(v, m);
+  return v;
+}
+
+// x^m for 0 <= m < 16 by binary powering.
+__device__ __forceinline__ double ipow16(double x, int m) {
+  double r = 1.0, b = x;
+  r = (m & 1) ? r * b : r;
+  b = b * b;
+  r = (m & 2) ? r * b : r;
+  b = b * b;
+  r = (m & 4) ? r * b : r;
+  b = b * b;
+  r = (m & 8) ? r * b : r;
+  return r;
+}
+
+// Packed index of the pair (genotype, partner) for per-SNP moment row r
+// (rows: W_1..W_c, y, x): W_b -> (b, c+1), y -> (c+1, c+2), x -> (c+1, c+1).
+__device__ __forceinline__ int xpair_index(int r, int nc) {
+  if (r < nc) return ab_index(r + 1, nc + 1, nc);
+  if (r == nc) return ab_index(nc + 1, nc + 2, nc);
+  return ab_index(nc + 1, nc + 1, nc);
+}
+
+// Per-lane term data: lane l owns the KT contiguous terms l*KT .. l*KT+KT-1.
+template <int KT>
+struct Terms {
+  double dc[KT], L[KT], inv_m[KT];
+  int m[KT];
+  bool act[KT];
+};
 
 // Main association kernel: one wavefront per SNP, grid-stride over SNPs.
-// Per-wave LDS: kappa (2T) | P (ni_max x 64) | iab level logs (kMaxLevels).
-// MULTI: SNP s belongs to problem snp_prob[s] (its own phenotype, covariates,
+// Per-wave LDS: P (ni_max x 64 doubles; the grid stage's per-lane columns,
+// reused by the golden stage as two packed Pab rows) | iab logs | slot table.
+// MULTI: SNP s belongs to problem snp_prob[s] (own phenotype, covariates,
 // null model and mode) and its moments start at snp_xmom[s]; all problems of
-// one launch share the cohort (same T). Otherwise every SNP uses pr_one.
-template <bool MULTI>
+// one launch share the cohort (same terms). Otherwise every SNP uses pr_one.
+//
+// An evaluation of REML at lambda_A and MLE at lambda_B (golden stage):
+//   1. lanes over terms: kappa_t for both lambdas and the m >= 1 part of
+//      logdet_h (lane l owns the KT contiguous terms l*KT .. of each block of
+//      64*KT terms; T > 64*KT runs several blocks, kappa of blocks after the
+//      first is recomputed per pair chunk);
+//   2. dot products with the SNP's moment rows (XR of them in registers, the
+//      rest and the shared rows from memory), reduced 8 values (4 pairs x 2
+//      lambdas) at a time by reduce8 into two packed Pab rows in LDS;
+//   3. the Pab recursion, half-wave A on the REML row and half-wave B on the
+//      MLE row, parallel over the pairs of each level;
+//   4. one round of logs (level pivots, P_yy, log|1 + lambda d_c| per cluster)
+//      spread over the lanes, then a half-wave sum.
+// Stages: 0 = Iab (lambda = 0, kappa_t = [m_t == 0]) followed by the grid;
+// 1, 2 = first golden probes; then n_iter golden steps; the final evaluation
+// at the bracket midpoints (Wald / LRT); the Score evaluation at lambda_null.
+template <int KT, int XR, bool MB, bool MULTI>
 __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mode_one,
                                                    const double* xmom, int64_t ldm, int64_t n_snps,
                                                    double* out, const jm_lmm_problem* probs,
                                                    const int32_t* modes, const int32_t* snp_prob,
-                                                   const double* const* snp_xmom, int T_all,
-                                                   int ni_max) {
+                                                   const double* const* snp_xmom, int ni_max) {
   extern __shared__ double smem[];
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
-  const int per_wave = 2 * T_all + ni_max * 64 + kMaxLevels;
-  double* kap = smem + (size_t)wib * per_wave;
-  double* P = kap + 2 * T_all;
-  double* iab_ld = P + ni_max * 64;
+  const int per_wave = ni_max * 64 + kMaxLevels + ni_max + 1;
+  double* P = smem + (size_t)wib * per_wave;  // grid columns / two packed rows
+  double* iab_ld = P + ni_max * 64;           // kMaxLevels
+  int* slot = (int*)(iab_ld + kMaxLevels);    // memory-phase slots (<= ni_max + 1 ints)
+  double* PA = P;                              // row A: ni entries + logdet slot
+  double* PB = P + ni_max + 1;                 // row B
   const double phi = 0.6180339887498949;
-  const int h = lane >> 5;
+  const int h = lane >> 5;  // half-wave: 0 -> REML / lambda A, 1 -> MLE / lambda B
+  const int q = lane & 31;
 
   for (int64_t s = (int64_t)blockIdx.x * wpb + wib; s < n_snps; s += (int64_t)gridDim.x * wpb) {
     const int pidx = MULTI ? snp_prob[s] : 0;
     const jm_lmm_problem pr = MULTI ? probs[pidx] : pr_one;
     const int mode = MULTI ? modes[pidx] : mode_one;
-    const int nc = pr.n_cvt, T = pr.n_terms, n = pr.n, G = pr.n_grid;
+    const int nc = pr.n_cvt, T = pr.n_terms, n = pr.n, G = pr.n_grid, C = pr.n_clusters;
     const int ni = (nc + 3) * (nc + 2) / 2;
     const bool do_reml = (mode == 1 || mode == 4);
     const bool do_mle = (mode == 2 || mode == 4);
     const bool do_score = (mode == 3 || mode == 4);
+    const bool do_golden = do_reml || do_mle;
     const double df = (double)(n - nc - 1);
+    const double nn = (double)n;
+    const double cr = 0.5 * df * (log(df) - log(2.0 * M_PI) - 1.0);
+    const double cm = 0.5 * nn * (log(nn) - log(2.0 * M_PI) - 1.0);
     const int nrows = nc + 2;  // per-SNP moment rows: W_1..W_c, y, xx
-    const double* xm = MULTI ? snp_xmom[s] : xmom + s * (int64_t)nrows * ldm;
-    int evals = 0;
-    // ---- Iab (lambda = 0): row 0 = sum of the m = 0 terms ----
-    {
-      for (int j = lane; j < ni; j += 64) {
-        const int r = pr.pair_row[j];
-        const double* row = (r >= 0) ? xm + (int64_t)r * ldm : pr.shared_mom + (int64_t)j * T;
-        double acc = 0.0;
-        for (int t = 0; t < T; ++t)
-          if (pr.term_m[t] == 0) acc += row[t];
-        P[j * 64] = acc;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      PabSummary oi;
-      pab_recursion(P, 64, nc, oi, nullptr, iab_ld);
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-    }
+    const double* xm = uniform_ptr(MULTI ? snp_xmom[s] : xmom + s * (int64_t)nrows * ldm);
 
-    // ---- grid stage: lane g evaluates REML and MLE at lambda_g ----
+    // ---- the SNP's register rows (lane l owns terms l*KT .. l*KT+KT-1) ----
+    double xr[XR][KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int t = lane * KT + k;
+#pragma unroll
+      for (int r = 0; r < XR; ++r) xr[r][k] = (r < nrows && t < T) ? xm[(int64_t)r * ldm + t] : 0.0;
+    }
+    // memory-phase slots: genotype rows >= XR, then the shared pairs
+    const int nxm = nrows > XR ? nrows - XR : 0;
+    const int n_mem = nxm + (nc + 1) * (nc + 2) / 2;
+    for (int u = lane; u < n_mem; u += 64) {
+      int j;
+      if (u < nxm) {
+        j = xpair_index(XR + u, nc);
+      } else {  // (u - nxm)-th pair (a <= b) over the indices {1..c, c+2}
+        int k = u - nxm, a = 1;
+        for (;; ++a) {
+          if (a == nc + 1) continue;
+          const int len = (nc + 2 - a) + 1 - (a <= nc ? 1 : 0);
+          if (k < len) break;
+          k -= len;
+        }
+        int b = a;
+        for (;; ++b) {
+          if (b == nc + 1) continue;
+          if (k == 0) break;
+          --k;
+        }
+        j = ab_index(a, b, nc);
+      }
+      slot[u] = j;
+    }
+    wave_sync();
+
+    double* os = out + s * JM_NOUT;
+    if (lane < JM_NOUT) os[lane] = NAN;  // columns a mode does not compute stay NaN
+    int evals = 0;
     double best_r = -INFINITY, best_m = -INFINITY;
     int bi_r = 0x7fffffff, bi_m = 0x7fffffff;
-    if (do_reml || do_mle) {
-      for (int g0 = 0; g0 < G; g0 += 64) {
-        const int g = g0 + lane;
-        if (g < G) {
-          for (int j = 0; j < ni; ++j) {
-            const int r = pr.pair_row[j];
-            double acc;
-            if (r >= 0) {
-              const double* row = xm + (int64_t)r * ldm;
-              acc = 0.0;
-              for (int t = 0; t < T; ++t) acc += pr.grid_kappa[(int64_t)t * G + g] * row[t];
-            } else {
-              acc = pr.grid_shared[(int64_t)g * ni + j];
+    double aA = 0, bA = 0, cA = 0, dA = 0, fcA = 0, fdA = 0;
+    double aB = 0, bB = 0, cB = 0, dB = 0, fcB = 0, fdB = 0;
+    bool klA = false, klB = false;
+    double lam_reml = NAN, lam_mle = NAN;
+    const int st_final = 3 + pr.n_iter, st_score = 4 + pr.n_iter;
+
+    for (int stage = 0;;) {
+      // ---- the lambdas of this stage ----
+      const bool lam0 = stage == 0;
+      double lamA = 0.0, lamB = 0.0;
+      if (stage == 1) {
+        lamA = exp(cA);
+        lamB = exp(cB);
+      } else if (stage == 2) {
+        lamA = exp(dA);
+        lamB = exp(dB);
+      } else if (stage >= 3 && stage < st_final) {
+        klA = fcA > fdA;
+        klB = fcB > fdB;
+        const double naA = klA ? aA : cA, nbA = klA ? dA : bA;
+        const double naB = klB ? aB : cB, nbB = klB ? dB : bB;
+        const double ncA = nbA - phi * (nbA - naA), ndA = naA + phi * (nbA - naA);
+        const double ncB = nbB - phi * (nbB - naB), ndB = naB + phi * (nbB - naB);
+        aA = naA; bA = nbA; cA = ncA; dA = ndA;
+        aB = naB; bB = nbB; cB = ncB; dB = ndB;
+        lamA = exp(klA ? cA : dA);
+        lamB = exp(klB ? cB : dB);
+      } else if (stage == st_final) {
+        lam_reml = exp((aA + bA) / 2.0);
+        lam_mle = exp((aB + bB) / 2.0);
+        lamA = lam_reml;
+        lamB = lam_mle;
+      } else if (stage == st_score) {
+        lamA = lamB = pr.lambda_null;
+      }
+
+      // ---- 1. kappa and the m >= 1 logdet part, per block of 64*KT terms ----
+      const int TB = 64 * KT;
+      const int nblk = MB ? (T + TB - 1) / TB : 1;  // MB: T > 64 * KT
+      auto kappa_block = [&](int tb, double (&kA)[KT], double (&kB)[KT], double& ldA,
+                             double& ldB) {
+        double pA = 1.0, pB = 1.0, hA = 0.0, hB = 0.0, nzA = 0.0, nzB = 0.0;
+        ldA = ldB = 0.0;
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+          const int t = tb * TB + lane * KT + k;
+          const bool act = t < T;
+          const int tc = act ? t : 0;
+          const int mk = act ? pr.term_m[tc] : 0;
+          if (k == 0 || mk == 0) {
+            const double dck = pr.cl_center[pr.term_cluster[tc]];
+            hA = 1.0 / fma(lamA, dck, 1.0);
+            hB = 1.0 / fma(lamB, dck, 1.0);
+            nzA = -lamA * hA;
+            nzB = -lamB * hB;
+          }
+          if (k == 0) {
+            pA = ipow16(nzA, mk);
+            pB = ipow16(nzB, mk);
+          } else {
+            pA = mk == 0 ? 1.0 : pA * nzA;
+            pB = mk == 0 ? 1.0 : pB * nzB;
+          }
+          if (lam0) {
+            kA[k] = kB[k] = (act && mk == 0) ? 1.0 : 0.0;
+          } else {
+            kA[k] = act ? hA * pA : 0.0;
+            kB[k] = act ? hB * pB : 0.0;
+            if (mk > 0) {
+              const double wl = pr.logdet_mom[tc] / (double)mk;
+              ldA -= pA * wl;
+              ldB -= pB * wl;
             }
-            P[j * 64 + lane] = acc;
-          }
-          PabSummary o;
-          pab_recursion(P + lane, 64, nc, o, iab_ld, nullptr);
-          const double ldh = pr.grid_logdet[g];
-          const double fr = reml_value(o, nc, n, ldh);
-          const double fm = mle_value(o.pyy1, n, ldh);
-          if (bi_r == 0x7fffffff || better(fr, g, best_r, bi_r)) {
-            best_r = fr;
-            bi_r = g;
-          }
-          if (bi_m == 0x7fffffff || better(fm, g, best_m, bi_m)) {
-            best_m = fm;
-            bi_m = g;
           }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-      }
-      evals += G;
-      wave_argmax(best_r, bi_r);
-      wave_argmax(best_m, bi_m);
-    }
-
-    // ---- golden-section refinement: REML in half 0, MLE in half 1 ----
-    double my_lam = NAN, my_logl = NAN;
-    PabSummary ofin;  // final Pab summary of this half (REML half -> Wald)
-    ofin.pxx = ofin.pxy = ofin.pyy = ofin.pyy1 = ofin.hiw = NAN;
-    if (do_reml || do_mle) {
-      const int bi = h ? bi_m : bi_r;
-      const int ilo = bi > 0 ? bi - 1 : 0;
-      const int ihi = bi + 1 < G ? bi + 1 : G - 1;
-      double a = pr.grid_loglam[ilo];
-      double b = pr.grid_loglam[ihi];
-      double c = b - phi * (b - a);
-      double d = a + phi * (b - a);
-      PabSummary o;
-      auto eval = [&](double loglam_mine) -> double {
-        const double lam_mine = exp(loglam_mine);
-        const double lam_r = shfl_d(lam_mine, 0);
-        const double lam_m = shfl_d(lam_mine, 32);
-        const double ldh = eval_two(pr, nc, ni, T, lane, kap, P, lam_r, lam_m, xm, ldm, nullptr);
-        pab_recursion(P + h, 64, nc, o, h ? nullptr : iab_ld, nullptr);
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        return h ? mle_value(o.pyy1, n, ldh) : reml_value(o, nc, n, ldh);
       };
-      double fc = eval(c);
-      double fd = eval(d);
-      for (int it = 0; it < pr.n_iter; ++it) {
-        const bool keep_left = fc > fd;
-        const double na = keep_left ? a : c;
-        const double nb = keep_left ? d : b;
-        const double ncp = nb - phi * (nb - na);
-        const double ndp = na + phi * (nb - na);
-        const double nl = eval(keep_left ? ncp : ndp);
-        const double nfc = keep_left ? nl : fd;
-        const double nfd = keep_left ? fc : nl;
-        a = na;
-        b = nb;
-        c = ncp;
-        d = ndp;
-        fc = nfc;
-        fd = nfd;
+      double kA0[KT], kB0[KT], ldA0, ldB0;
+      kappa_block(0, kA0, kB0, ldA0, ldB0);
+      // ---- 2. dot products, reduced 4 pairs at a time into PA / PB ----
+      const int qv = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+#pragma unroll
+      for (int e0 = 0; e0 < XR; e0 += 4) {
+        double v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = 0.0;
+        for (int tb = 0; tb < nblk; ++tb) {
+          double kA[KT], kB[KT], la, lb;
+          if (tb == 0) {
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+              kA[k] = kA0[k];
+              kB[k] = kB0[k];
+            }
+          } else {
+            kappa_block(tb, kA, kB, la, lb);
+          }
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int r = e0 + e;
+            if (r < XR) {
+#pragma unroll
+              for (int k = 0; k < KT; ++k) {
+                const int t = tb * TB + lane * KT + k;
+                const double x = (tb == 0) ? xr[r < XR ? r : 0][k]
+                                           : ((r < nrows && t < T) ? xm[(int64_t)r * ldm + t] : 0.0);
+                v[2 * e] = fma(kA[k], x, v[2 * e]);
+                v[2 * e + 1] = fma(kB[k], x, v[2 * e + 1]);
+              }
+            }
+          }
+        }
+        const double r = reduce8(v, lane);
+        const int e = e0 + (qv >> 1);
+        if ((lane & 7) == 0 && e < nrows && e < XR) ((qv & 1) ? PB : PA)[xpair_index(e, nc)] = r;
       }
-      const double log_opt = (a + b) / 2.0;
-      const double fin = eval(log_opt);
-      ofin = o;
-      my_lam = exp(log_opt);
-      my_logl = fin;
-      evals += 2 * (pr.n_iter + 3);
-    }
-    const double lam_mle = shfl_d(my_lam, 32);
-    const double logl_mle = shfl_d(my_logl, 32);
+      for (int u0 = 0; u0 <= n_mem; u0 += 4) {
+        double v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = 0.0;
+        for (int tb = 0; tb < nblk; ++tb) {
+          double kA[KT], kB[KT], la, lb;
+          if (tb == 0) {
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+              kA[k] = kA0[k];
+              kB[k] = kB0[k];
+            }
+            la = ldA0;
+            lb = ldB0;
+          } else {
+            kappa_block(tb, kA, kB, la, lb);
+          }
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int u = u0 + e;
+            if (u < n_mem) {
+              const double* row = (u < nxm) ? xm + (int64_t)(XR + u) * ldm
+                                            : pr.shared_mom + (int64_t)slot[u] * T;
+#pragma unroll
+              for (int k = 0; k < KT; ++k) {
+                const int t = tb * TB + lane * KT + k;
+                const double x = t < T ? row[t] : 0.0;
+                v[2 * e] = fma(kA[k], x, v[2 * e]);
+                v[2 * e + 1] = fma(kB[k], x, v[2 * e + 1]);
+              }
+            } else if (u == n_mem) {
+              v[2 * e] += la;
+              v[2 * e + 1] += lb;
+            }
+          }
+        }
+        const double r = reduce8(v, lane);
+        const int u = u0 + (qv >> 1);
+        if ((lane & 7) == 0 && u <= n_mem) {
+          double* dst = (qv & 1) ? PB : PA;
+          dst[u < n_mem ? slot[u] : ni] = r;  // slot ni: m >= 1 part of logdet_h
+        }
+      }
+      wave_sync();
+      // ---- 3. Pab recursion, half A on PA, half B on PB ----
+      double* Ph = h ? PB : PA;
+      double piv = 0.0, pyy_lvl = 0.0;
+      for (int p = 1; p <= nc + 1; ++p) {
+        const double d = Ph[ab_index(p, p, nc)];
+        if (q == p - 1) piv = d;
+        if (p == nc + 1) pyy_lvl = Ph[ab_index(nc + 2, nc + 2, nc)];
+        const double inv = (d != 0.0) ? 1.0 / d : 0.0;
+        const int Lw = nc + 2 - p;
+        const int np = Lw * (Lw + 1) / 2;
+        for (int idx = q; idx < np; idx += 32) {
+          int ap = 0, rem = idx;
+          while (rem >= Lw - ap) {
+            rem -= Lw - ap;
+            ++ap;
+          }
+          const int a = p + 1 + ap, b = a + rem;
+          const double pa = Ph[ab_index(a, p, nc)], pb = Ph[ab_index(b, p, nc)];
+          const int iab = ab_index(a, b, nc);
+          Ph[iab] = Ph[iab] - pa * pb * inv;
+        }
+        wave_sync();
+      }
+      // ---- 4. logs: pivots (A), P_yy (A: lane c+1, B: lane 32), clusters ----
+      const double pyy1 = Ph[ab_index(nc + 2, nc + 2, nc)];
+      const double lam_h = h ? lamB : lamA;
+      const int first = h ? 1 : nc + 2;  // first cluster lane of this half
+      const int per = 32 - first;
+      const int n_cl = lam0 ? 0 : C;
+      double contrib = 0.0, lgp = 0.0;
+      for (int c0 = 0;; c0 += per) {
+        double arg = 1.0, w = 0.0;
+        int kind = 0;  // 1: pivot, 2: P_yy, 3: cluster
+        if (c0 == 0 && h == 0 && q <= nc) {
+          kind = 1;
+          arg = piv > 0.0 ? piv : 1.0;
+        } else if (c0 == 0 && q == first - 1) {
+          kind = 2;
+          arg = clamp_pyy(pyy1);
+        } else if (q >= first && c0 + q - first < n_cl) {
+          const int c = c0 + q - first;
+          kind = 3;
+          arg = fabs(fma(lam_h, pr.cl_center[c], 1.0));
+          w = pr.cl_weight[c];
+        }
+        const double lg = log(arg);
+        if (kind == 1) contrib += lam0 ? lg : lg - iab_ld[q];
+        if (kind == 1 && lam0) iab_ld[q] = lg;
+        if (kind == 2) lgp = lg;
+        if (kind == 3) contrib += w * lg;
+        if (c0 + per >= n_cl) break;
+      }
+      wave_sync();
+      const double S = half_sum(contrib);
+      const double fA = cr - 0.5 * (bcast(S, 0) + PA[ni]) - 0.5 * df * bcast(lgp, nc + 1);
+      const double fB = cm - 0.5 * (bcast(S, 32) + PB[ni]) - 0.5 * nn * bcast(lgp, 32);
+      const double pyyA = bcast(pyy_lvl, 0);
 
-    // ---- Score at the null lambda ----
-    double sb = NAN, sse = NAN, ps = NAN;
-    if (do_score) {
-      eval_two(pr, nc, ni, T, lane, kap, P, pr.lambda_null, pr.lambda_null, xm, ldm,
-               pr.null_kappa);
-      PabSummary o;
-      pab_recursion(P, 64, nc, o, nullptr, nullptr);
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      const double pxyy = clamp_pyy(o.pyy1);
-      double tau;
-      beta_se(df, o.pxx, o.pxy, pxyy, sb, sse, tau);
-      if (o.pxx > 0.0) {
-        const double f = (double)n * (o.pxy * o.pxy) / (o.pyy * o.pxx);
-        ps = f_pvalue(df, f);
+      // ---- consume the stage ----
+      if (stage == 0) {
+        // grid stage: lane g evaluates REML and MLE at lambda_g
+        if (do_golden) {
+          for (int g0 = 0; g0 < G; g0 += 64) {
+            const int g = g0 + lane;
+            const bool gv = g < G;
+            double acc[kXR];
+#pragma unroll
+            for (int r = 0; r < kXR; ++r) acc[r] = 0.0;
+            for (int t = 0; t < T; ++t) {
+              const double kg = gv ? pr.grid_kappa[(int64_t)t * G + g] : 0.0;
+#pragma unroll
+              for (int r = 0; r < kXR; ++r)
+                if (r < nrows) acc[r] = fma(kg, xm[(int64_t)r * ldm + t], acc[r]);
+            }
+            wave_sync();
+#pragma unroll
+            for (int r = 0; r < kXR; ++r)
+              if (r < nrows) P[xpair_index(r, nc) * 64 + lane] = acc[r];
+            for (int r = kXR; r < nrows; ++r) {
+              double a = 0.0;
+              for (int t = 0; t < T; ++t)
+                a = fma(gv ? pr.grid_kappa[(int64_t)t * G + g] : 0.0, xm[(int64_t)r * ldm + t], a);
+              P[xpair_index(r, nc) * 64 + lane] = a;
+            }
+            for (int u = nxm; u < n_mem; ++u) {
+              const int j = slot[u];
+              P[j * 64 + lane] = gv ? pr.grid_shared[(int64_t)g * ni + j] : 0.0;
+            }
+            wave_sync();
+            if (gv) {
+              PabSummary o;
+              pab_recursion(P + lane, 64, nc, o, iab_ld, nullptr);
+              const double ldh = pr.grid_logdet[g];
+              const double fr = reml_value(o, nc, n, ldh);
+              const double fm = mle_value(o.pyy1, n, ldh);
+              if (bi_r == 0x7fffffff || better(fr, g, best_r, bi_r)) {
+                best_r = fr;
+                bi_r = g;
+              }
+              if (bi_m == 0x7fffffff || better(fm, g, best_m, bi_m)) {
+                best_m = fm;
+                bi_m = g;
+              }
+            }
+            wave_sync();
+          }
+          evals += G;
+          wave_argmax(best_r, bi_r);
+          wave_argmax(best_m, bi_m);
+          const int iloA = bi_r > 0 ? bi_r - 1 : 0, ihiA = bi_r + 1 < G ? bi_r + 1 : G - 1;
+          const int iloB = bi_m > 0 ? bi_m - 1 : 0, ihiB = bi_m + 1 < G ? bi_m + 1 : G - 1;
+          aA = pr.grid_loglam[iloA];
+          bA = pr.grid_loglam[ihiA];
+          aB = pr.grid_loglam[iloB];
+          bB = pr.grid_loglam[ihiB];
+          cA = bA - phi * (bA - aA);
+          dA = aA + phi * (bA - aA);
+          cB = bB - phi * (bB - aB);
+          dB = aB + phi * (bB - aB);
+          stage = 1;
+        } else {
+          stage = st_score;
+        }
+      } else if (stage == 1) {
+        fcA = fA;
+        fcB = fB;
+        stage = 2;
+      } else if (stage == 2) {
+        fdA = fA;
+        fdB = fB;
+        stage = 3;
+      } else if (stage < st_final) {
+        const double nfcA = klA ? fA : fdA, nfdA = klA ? fcA : fA;
+        const double nfcB = klB ? fB : fdB, nfdB = klB ? fcB : fB;
+        fcA = nfcA;
+        fdA = nfdA;
+        fcB = nfcB;
+        fdB = nfdB;
+        ++stage;
+      } else if (stage == st_final) {
+        // Wald at the REML optimum (likelihood_jax.py:601-675), LRT at the MLE optimum
+        evals += 2 * (pr.n_iter + 3);
+        if (lane == 0 && do_reml) {
+          const double pxx = PA[ab_index(nc + 1, nc + 1, nc)];
+          const double pxy = PA[ab_index(nc + 1, nc + 2, nc)];
+          const double pxyy = clamp_pyy(PA[ab_index(nc + 2, nc + 2, nc)]);
+          double beta, se, tau;
+          beta_se(df, pxx, pxy, pxyy, beta, se, tau);
+          os[JM_OUT_BETA] = beta;
+          os[JM_OUT_SE] = se;
+          os[JM_OUT_P_WALD] = (pxx > 0.0) ? f_pvalue(df, (pyyA - pxyy) * tau) : NAN;
+          os[JM_OUT_L_REMLE] = lam_reml;
+          os[JM_OUT_LOGL_H1] = fA;
+        }
+        if (lane == 0 && do_mle) {
+          os[JM_OUT_L_MLE] = lam_mle;
+          os[JM_OUT_LOGL_MLE] = fB;
+          os[JM_OUT_P_LRT] = chi2_sf1(fmax(2.0 * (fB - pr.logl_null), 0.0));
+        }
+        if (!do_score) break;
+        stage = st_score;
+      } else {  // score at the null lambda (likelihood_jax.py:706-773)
+        evals += 1;
+        if (lane == 0) {
+          const double pxx = PA[ab_index(nc + 1, nc + 1, nc)];
+          const double pxy = PA[ab_index(nc + 1, nc + 2, nc)];
+          const double pxyy = clamp_pyy(PA[ab_index(nc + 2, nc + 2, nc)]);
+          double sb, sse, tau;
+          beta_se(df, pxx, pxy, pxyy, sb, sse, tau);
+          os[JM_OUT_BETA_SCORE] = sb;
+          os[JM_OUT_SE_SCORE] = sse;
+          os[JM_OUT_P_SCORE] = (pxx > 0.0) ? f_pvalue(df, nn * (pxy * pxy) / (pyyA * pxx)) : NAN;
+        }
+        break;
       }
-      evals += 1;
+      wave_sync();
     }
 
-    if (lane == 0) {
-      double* os = out + s * JM_NOUT;
-      os[JM_OUT_EVALS] = (double)evals;
-      if (do_reml) {
-        double beta, se, tau;
-        const double pxyy = clamp_pyy(ofin.pyy1);
-        beta_se(df, ofin.pxx, ofin.pxy, pxyy, beta, se, tau);
-        double pw = NAN;
-        if (ofin.pxx > 0.0) pw = f_pvalue(df, (ofin.pyy - pxyy) * tau);
-        os[JM_OUT_BETA] = beta;
-        os[JM_OUT_SE] = se;
-        os[JM_OUT_P_WALD] = pw;
-        os[JM_OUT_L_REMLE] = my_lam;
-        os[JM_OUT_LOGL_H1] = my_logl;
-      } else {
-        os[JM_OUT_BETA] = os[JM_OUT_SE] = os[JM_OUT_P_WALD] = NAN;
-        os[JM_OUT_L_REMLE] = os[JM_OUT_LOGL_H1] = NAN;
-      }
-      if (do_mle) {
-        os[JM_OUT_L_MLE] = lam_mle;
-        os[JM_OUT_LOGL_MLE] = logl_mle;
-        os[JM_OUT_P_LRT] = chi2_sf1(fmax(2.0 * (logl_mle - pr.logl_null), 0.0));
-      } else {
-        os[JM_OUT_L_MLE] = os[JM_OUT_LOGL_MLE] = os[JM_OUT_P_LRT] = NAN;
-      }
-      os[JM_OUT_P_SCORE] = ps;
-      os[JM_OUT_BETA_SCORE] = sb;
-      os[JM_OUT_SE_SCORE] = sse;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) os[JM_OUT_EVALS] = (double)evals;
+    wave_sync();
   }
 }
 
@@ -667,31 +998,65 @@ extern "C" int jm_lmm_null(const jm_lmm_problem* prob, double* out, void* stream
   return JM_OK;
 }
 
+template <int KT, int XR, bool MB>
+static void launch_kt(const jm_lmm_problem* one, int mode_one, const double* xmom, int64_t ldm,
+                      int64_t n_snps, double* out, const jm_lmm_problem* probs,
+                      const int32_t* modes, const int32_t* snp_prob, const double* const* snp_xmom,
+                      int ni_max, unsigned blocks, int wpb, size_t lds, hipStream_t st) {
+  if (one) {
+    hipLaunchKernelGGL((k_lmm_assoc<KT, XR, MB, false>), dim3(blocks), dim3(64 * wpb), lds, st, *one,
+                       mode_one, xmom, ldm, n_snps, out, nullptr, nullptr, nullptr, nullptr,
+                       ni_max);
+  } else {
+    jm_lmm_problem dummy;
+    memset(&dummy, 0, sizeof(dummy));
+    hipLaunchKernelGGL((k_lmm_assoc<KT, XR, MB, true>), dim3(blocks), dim3(64 * wpb), lds, st, dummy,
+                       0, nullptr, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom, ni_max);
+  }
+}
+
 static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* xmom, int64_t ldm,
                         int64_t n_snps, double* out, const jm_lmm_problem* probs,
                         const int32_t* modes, const int32_t* snp_prob,
                         const double* const* snp_xmom, int T, int max_cvt, void* stream) {
+  JM_CHECK_ARG(T >= 1, "n_terms=%d < 1", T);
   const int ni_max = (max_cvt + 3) * (max_cvt + 2) / 2;
-  const size_t per_wave = (size_t)(2 * T + ni_max * 64 + kMaxLevels) * sizeof(double);
+  const size_t per_wave = (size_t)(ni_max * 64 + kMaxLevels + ni_max + 1) * sizeof(double);
   int wpb = (int)((64 * 1024) / per_wave);
   if (wpb > 4) wpb = 4;
   if (wpb < 1) {
-    JM_CHECK_ARG(per_wave <= 160 * 1024, "problem too large for the LDS budget (T=%d)", T);
+    JM_CHECK_ARG(per_wave <= 160 * 1024, "problem too large for the LDS budget (n_cvt=%d)",
+                 max_cvt);
     wpb = 1;
   }
   int64_t blocks = (n_snps + wpb - 1) / wpb;
   const int64_t max_blocks = 256 * 16;
   if (blocks > max_blocks) blocks = max_blocks;
-  jm_lmm_problem dummy;
-  memset(&dummy, 0, sizeof(dummy));
-  if (one)
-    hipLaunchKernelGGL(k_lmm_assoc<false>, dim3((unsigned)blocks), dim3(64 * wpb), per_wave * wpb,
-                       (hipStream_t)stream, *one, mode_one, xmom, ldm, n_snps, out, nullptr,
-                       nullptr, nullptr, nullptr, T, ni_max);
+  const unsigned nb = (unsigned)blocks;
+  const size_t lds = per_wave * wpb;
+  hipStream_t st = (hipStream_t)stream;
+  const int kt = (T + 63) / 64;  // > 8: blocks of 512 terms
+  if (kt <= 1)
+    launch_kt<1, 8, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
+                    ni_max, nb, wpb, lds, st);
+  else if (kt <= 2)
+    launch_kt<2, 8, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
+                    ni_max, nb, wpb, lds, st);
+  else if (kt <= 3)
+    launch_kt<3, 8, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
+                    ni_max, nb, wpb, lds, st);
+  else if (kt <= 4)
+    launch_kt<4, 8, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
+                    ni_max, nb, wpb, lds, st);
+  else if (kt <= 6)
+    launch_kt<6, 4, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
+                    ni_max, nb, wpb, lds, st);
+  else if (kt <= 8)
+    launch_kt<8, 4, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
+                    ni_max, nb, wpb, lds, st);
   else
-    hipLaunchKernelGGL(k_lmm_assoc<true>, dim3((unsigned)blocks), dim3(64 * wpb), per_wave * wpb,
-                       (hipStream_t)stream, dummy, 0, nullptr, ldm, n_snps, out, probs, modes,
-                       snp_prob, snp_xmom, T, ni_max);
+    launch_kt<8, 4, true>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
+                    ni_max, nb, wpb, lds, st);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

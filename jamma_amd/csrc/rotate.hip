@@ -492,6 +492,302 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_v2(
       }
 }
 
+
+// ---------------------------------------------------------------------------
+// BF16x3 rotation, 256x256 block tile (default)
+// ---------------------------------------------------------------------------
+// 4 waves, each owning a 128x128 output tile (8x8 MFMA tiles, accumulators in
+// AGPRs). Per 32-deep K step the block stages the three UT planes (3 x 256 x
+// 32 bf16 = 48 KB) and the int8 genotypes (256 x 32 B = 8 KB) straight from
+// global memory into LDS with global_load_lds_dwordx4 (no register staging),
+// double buffered, one barrier per step. Genotype fragments are widened to
+// bf16 X1 / X2 per fragment. LDS rows are XOR-swizzled in 16-byte chunks so
+// that the ds_read_b128 (A) and ds_read_b64 (B) fragment reads of 16 lanes hit
+// 64 distinct banks: A chunk c of row r sits at c ^ ((r >> 2) & 3), genotype
+// chunk c of column j at c ^ ((j >> 3) & 1).
+constexpr int V3_BM = 256, V3_BN = 256, V3_BK = 32;
+constexpr int V3_APLANE = V3_BM * V3_BK * 2;      // 16 KB per plane
+constexpr int V3_BBYTES = V3_BN * V3_BK;          // 8 KB
+constexpr int V3_STAGE = 3 * V3_APLANE + V3_BBYTES;  // 56 KB
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ void lds_dma16(const void* gsrc, uint8_t* lds_dst) {
+  __builtin_amdgcn_global_load_lds(gsrc, (lds_void*)lds_dst, 16, 0, 0);
+}
+
+// 8 genotype bytes -> bf16x8 X1 (0/1/2 exact, missing -> m1) and X2 (missing -> m2).
+__device__ __forceinline__ void widen8(uint64_t bytes, uint32_t m1, uint32_t m2, bf16x8_t& x1,
+                                       bf16x8_t& x2) {
+  uint32_t w1[4], w2[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const uint32_t b0 = (uint32_t)(bytes >> (16 * e)) & 0xFFu;
+    const uint32_t b1 = (uint32_t)(bytes >> (16 * e + 8)) & 0xFFu;
+    const uint32_t v0 = (b0 & 0x80u) ? m1 : (b0 ? 0x3F00u + (b0 << 7) : 0u);
+    const uint32_t v1 = (b1 & 0x80u) ? m1 : (b1 ? 0x3F00u + (b1 << 7) : 0u);
+    const uint32_t r0 = (b0 & 0x80u) ? m2 : 0u;
+    const uint32_t r1 = (b1 & 0x80u) ? m2 : 0u;
+    w1[e] = v0 | (v1 << 16);
+    w2[e] = r0 | (r1 << 16);
+  }
+  memcpy(&x1, w1, 16);
+  memcpy(&x2, w2, 16);
+}
+
+__global__ void __launch_bounds__(256) k_rotate_bf16x3_v3(
+    const uint16_t* __restrict__ Ahi, const uint16_t* __restrict__ Amid,
+    const uint16_t* __restrict__ Alo, int64_t lda, const int8_t* __restrict__ g, int64_t ldg,
+    const double* __restrict__ mean, const int32_t* __restrict__ col, int64_t M, int64_t n_cols,
+    int64_t K, float* __restrict__ Z, int64_t ldz) {
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * V3_STAGE];
+  __shared__ uint32_t Bm[2][V3_BN];  // bf16(mean), bf16 residual of the mean
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int nrb = (int)((M + V3_BM - 1) / V3_BM), ncb = (int)((n_cols + V3_BN - 1) / V3_BN);
+  int rb, cb;
+  xcd_tile(blockIdx.x, nrb, ncb, rb, cb);
+  const int64_t i0 = (int64_t)rb * V3_BM, s0 = (int64_t)cb * V3_BN;
+  const int wr = (wave >> 1) * 128, wc = (wave & 1) * 128;
+  const int lr = lane & 15, lk = lane >> 4;
+  {
+    const int64_t sc = s0 + tid;
+    const double mu = (sc < n_cols) ? mean[col ? (int64_t)col[sc] : sc] : 0.0;
+    const uint16_t h1 = f32_to_bf16_rne((float)mu);
+    Bm[0][tid] = h1;
+    Bm[1][tid] = f32_to_bf16_rne((float)(mu - bf16_to_f64(h1)));
+  }
+  // per-lane global sources of the DMA loads (fixed over K; the K offset is added per step)
+  // A: instruction q (0..15) of plane p covers rows 16q .. 16q+15; lane -> (row, chunk)
+  const uint16_t* Ap[3] = {Ahi, Amid, Alo};
+  const int a_r = lane >> 2, a_cs = lane & 3;  // row within 16, stored chunk
+  // B: instruction q (0..7) covers columns 32q .. 32q+31; lane -> (column, chunk)
+  const int b_j = lane >> 1, b_cs = lane & 1;
+  // each wave issues A instructions q = wave*4 .. wave*4+3 of every plane, B q = wave*2, +1
+  int64_t a_off[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int row = 16 * (wave * 4 + u) + a_r;
+    int64_t gi = i0 + row;
+    if (gi >= M) gi = M - 1;  // rows past the end load a valid row; never stored
+    const int c = a_cs ^ ((row >> 2) & 3);
+    a_off[u] = gi * lda + 8 * c;
+  }
+  int64_t b_off[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int j = 32 * (wave * 2 + u) + b_j;
+    int64_t sc = s0 + j;
+    if (sc >= n_cols) sc = n_cols - 1;
+    const int64_t grow = col ? (int64_t)col[sc] : sc;
+    const int c = b_cs ^ ((j >> 3) & 1);
+    b_off[u] = grow * ldg + 16 * c;
+  }
+  auto issue = [&](int stage, int64_t k0) {
+    uint8_t* base = lds + stage * V3_STAGE;
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        lds_dma16(Ap[pl] + a_off[u] + k0, base + pl * V3_APLANE + (wave * 4 + u) * 1024);
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      lds_dma16(g + b_off[u] + k0, base + 3 * V3_APLANE + (wave * 2 + u) * 1024);
+  };
+
+  f4_t acc[8][8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
+
+  const int nk = (int)((K + V3_BK - 1) / V3_BK);
+  issue(0, 0);
+  __syncthreads();  // Bm visible
+  uint32_t m1[8], m2[8];
+#pragma unroll
+  for (int tj = 0; tj < 8; ++tj) {
+    m1[tj] = Bm[0][wc + 16 * tj + lr];
+    m2[tj] = Bm[1][wc + 16 * tj + lr];
+  }
+  for (int kt = 0; kt < nk; ++kt) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA loads of stage kt have landed
+    __syncthreads();                // ... and everyone's; stage kt-1 is free again
+    if (kt + 1 < nk) issue((kt + 1) & 1, (int64_t)(kt + 1) * V3_BK);
+    const uint8_t* base = lds + (kt & 1) * V3_STAGE;
+    // genotype fragments: raw bytes, widened to X1 once and to X2 in the hi pass
+    uint64_t gb[8];
+    bf16x8_t x1[8];
+#pragma unroll
+    for (int tj = 0; tj < 8; ++tj) {
+      const int j = wc + 16 * tj + lr;
+      const int c = (lk >> 1) ^ ((j >> 3) & 1);
+      memcpy(&gb[tj], base + 3 * V3_APLANE + j * 32 + 16 * c + 8 * (lk & 1), 8);
+      bf16x8_t unused;
+      widen8(gb[tj], m1[tj], m2[tj], x1[tj], unused);
+    }
+    // planes lo, mid, hi (small terms first)
+#pragma unroll 1
+    for (int pl = 2; pl >= 0; --pl) {
+      bf16x8_t af[8];
+#pragma unroll
+      for (int ti = 0; ti < 8; ++ti) {
+        const int r = wr + 16 * ti + lr;
+        const int c = lk ^ ((r >> 2) & 3);
+        memcpy(&af[ti], base + pl * V3_APLANE + r * 64 + 16 * c, 16);
+      }
+      if (pl == 0) {
+#pragma unroll
+        for (int tj = 0; tj < 8; ++tj) {
+          bf16x8_t x2, unused;
+          widen8(gb[tj], m1[tj], m2[tj], unused, x2);
+#pragma unroll
+          for (int ti = 0; ti < 8; ++ti)
+            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ti], x2, acc[ti][tj], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int ti = 0; ti < 8; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 8; ++tj)
+          acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ti], x1[tj], acc[ti][tj], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int ti = 0; ti < 8; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 8; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
+        const int64_t sc = s0 + wc + tj * 16 + lr;
+        if (i < M && sc < n_cols) Z[i * ldz + sc] = acc[ti][tj][r];
+      }
+}
+
+// Same staging as k_rotate_bf16x3_v3 (256x256 block tile, DMA into double-
+// buffered swizzled LDS), 8 waves in a 4 x 2 grid of 64x128 wave tiles: 128
+// accumulators per lane, so two waves share each SIMD and hide each other's
+// LDS latency.
+__global__ void __launch_bounds__(512) k_rotate_bf16x3_v4(
+    const uint16_t* __restrict__ Ahi, const uint16_t* __restrict__ Amid,
+    const uint16_t* __restrict__ Alo, int64_t lda, const int8_t* __restrict__ g, int64_t ldg,
+    const double* __restrict__ mean, const int32_t* __restrict__ col, int64_t M, int64_t n_cols,
+    int64_t K, float* __restrict__ Z, int64_t ldz) {
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * V3_STAGE];
+  __shared__ uint32_t Bm[2][V3_BN];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;  // 8 waves
+  const int nrb = (int)((M + V3_BM - 1) / V3_BM), ncb = (int)((n_cols + V3_BN - 1) / V3_BN);
+  int rb, cb;
+  xcd_tile(blockIdx.x, nrb, ncb, rb, cb);
+  const int64_t i0 = (int64_t)rb * V3_BM, s0 = (int64_t)cb * V3_BN;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 128;
+  const int lr = lane & 15, lk = lane >> 4;
+  if (tid < V3_BN) {
+    const int64_t sc = s0 + tid;
+    const double mu = (sc < n_cols) ? mean[col ? (int64_t)col[sc] : sc] : 0.0;
+    const uint16_t h1 = f32_to_bf16_rne((float)mu);
+    Bm[0][tid] = h1;
+    Bm[1][tid] = f32_to_bf16_rne((float)(mu - bf16_to_f64(h1)));
+  }
+  const uint16_t* Ap[3] = {Ahi, Amid, Alo};
+  // A: 48 DMA instructions per stage (3 planes x 16 row groups), 6 per wave;
+  // source offsets are recomputed per issue to keep registers for the MFMAs
+  const int a_rl = lane >> 2, a_cs = lane & 3;
+  // B: 8 DMA instructions per stage (32 columns each), one per wave
+  int64_t b_off;
+  {
+    const int j = 32 * wave + (lane >> 1);
+    int64_t sc = s0 + j;
+    if (sc >= n_cols) sc = n_cols - 1;
+    const int64_t grow = col ? (int64_t)col[sc] : sc;
+    const int c = (lane & 1) ^ ((j >> 3) & 1);
+    b_off = grow * ldg + 16 * c;
+  }
+  auto issue = [&](int stage, int64_t k0) {
+    uint8_t* base = lds + stage * V3_STAGE;
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+      const int q = wave * 6 + u;
+      const int row = 16 * (q & 15) + a_rl;
+      int64_t gi = i0 + row;
+      if (gi >= M) gi = M - 1;  // rows past the end load a valid row; never stored
+      const int c = a_cs ^ ((row >> 2) & 3);
+      lds_dma16(Ap[q >> 4] + gi * lda + 8 * c + k0, base + (q >> 4) * V3_APLANE + (q & 15) * 1024);
+    }
+    lds_dma16(g + b_off + k0, base + 3 * V3_APLANE + wave * 1024);
+  };
+
+  f4_t acc[4][8];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
+
+  const int nk = (int)((K + V3_BK - 1) / V3_BK);
+  issue(0, 0);
+  __syncthreads();  // Bm visible
+  uint32_t mm[8];  // bf16(mean) | residual << 16
+#pragma unroll
+  for (int tj = 0; tj < 8; ++tj) mm[tj] = Bm[0][wc + 16 * tj + lr] | (Bm[1][wc + 16 * tj + lr] << 16);
+  for (int kt = 0; kt < nk; ++kt) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (kt + 1 < nk) issue((kt + 1) & 1, (int64_t)(kt + 1) * V3_BK);
+    const uint8_t* base = lds + (kt & 1) * V3_STAGE;
+    auto gbytes = [&](int tj) -> uint64_t {
+      const int j = wc + 16 * tj + lr;
+      const int c = (lk >> 1) ^ ((j >> 3) & 1);
+      uint64_t v;
+      memcpy(&v, base + 3 * V3_APLANE + j * 32 + 16 * c + 8 * (lk & 1), 8);
+      return v;
+    };
+    bf16x8_t x1[8];
+#pragma unroll
+    for (int tj = 0; tj < 8; ++tj) {
+      bf16x8_t unused;
+      widen8(gbytes(tj), mm[tj] & 0xFFFFu, mm[tj] >> 16, x1[tj], unused);
+    }
+    // planes lo, mid, hi x X1, and hi x X2 (small terms first)
+#pragma unroll 1
+    for (int pl = 2; pl >= 0; --pl) {
+      bf16x8_t af[4];
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) {
+        const int r = wr + 16 * ti + lr;
+        const int c = lk ^ ((r >> 2) & 3);
+        memcpy(&af[ti], base + pl * V3_APLANE + r * 64 + 16 * c, 16);
+      }
+      if (pl == 0) {
+#pragma unroll
+        for (int tj = 0; tj < 8; ++tj) {
+          bf16x8_t x2, unused;
+          widen8(gbytes(tj), mm[tj] & 0xFFFFu, mm[tj] >> 16, unused, x2);
+#pragma unroll
+          for (int ti = 0; ti < 4; ++ti)
+            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ti], x2, acc[ti][tj], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 8; ++tj)
+          acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ti], x1[tj], acc[ti][tj], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 8; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
+        const int64_t sc = s0 + wc + tj * 16 + lr;
+        if (i < M && sc < n_cols) Z[i * ldz + sc] = acc[ti][tj][r];
+      }
+}
+
 }  // namespace jm
 
 using namespace jm;
@@ -549,11 +845,21 @@ extern "C" int jm_rotate_bf16x3(const uint16_t* hi, const uint16_t* mid, const u
   JM_CHECK_ARG(((uintptr_t)g % 16) == 0 && ((uintptr_t)hi % 16) == 0 &&
                    ((uintptr_t)mid % 16) == 0 && ((uintptr_t)lo % 16) == 0,
                "operands must be 16-byte aligned");
-  // tile variant: 2 = 256x128 / 512 threads with register prefetch (default),
-  // 1 = 128x128 / 256 threads (JM_ROT_BF16_V=1)
+  // tile variant (JM_ROT_BF16_V): 3 = 256x256, DMA-staged, double buffered (default);
+  // 2 = 256x128 / 512 threads with register prefetch; 1 = 128x128 / 256 threads
   const char* venv = getenv("JM_ROT_BF16_V");
-  const int version = venv ? atoi(venv) : 2;
-  if (version == 2) {
+  const int version = venv ? atoi(venv) : 3;
+  if (version == 4) {
+    JM_CHECK_ARG(ld_g >= ((n + 31) / 32) * 32, "ld_g must be >= n rounded up to 32");
+    const int nrb = ceil_div(n, V3_BM), ncb = ceil_div(n_snps, V3_BN);
+    hipLaunchKernelGGL(k_rotate_bf16x3_v4, dim3(nrb * ncb), dim3(512), 0, (hipStream_t)stream, hi,
+                       mid, lo, ld_s, g, ld_g, mean, col_idx, n, n_snps, n, Z, ldz);
+  } else if (version == 3) {
+    JM_CHECK_ARG(ld_g >= ((n + 31) / 32) * 32, "ld_g must be >= n rounded up to 32");
+    const int nrb = ceil_div(n, V3_BM), ncb = ceil_div(n_snps, V3_BN);
+    hipLaunchKernelGGL(k_rotate_bf16x3_v3, dim3(nrb * ncb), dim3(256), 0, (hipStream_t)stream, hi,
+                       mid, lo, ld_s, g, ld_g, mean, col_idx, n, n_snps, n, Z, ldz);
+  } else if (version == 2) {
     const int nrb = ceil_div(n, V2_BM), ncb = ceil_div(n_snps, V2_BN);
     hipLaunchKernelGGL(k_rotate_bf16x3_v2, dim3(nrb * ncb), dim3(512), 0, (hipStream_t)stream, hi,
                        mid, lo, ld_s, g, ld_g, mean, col_idx, n, n_snps, n, Z, ldz);

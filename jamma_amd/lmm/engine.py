@@ -93,6 +93,7 @@ class Cohort:
         self.cl_terms = to_dev(p.cl_terms, torch.int32, dev)
         self.cl_toff = to_dev(p.cl_toff, torch.int32, dev)
         self.cl_center = to_dev(p.cl_center, torch.float64, dev)
+        self.cl_weight = to_dev(p.cl_size.astype(np.float64), torch.float64, dev)
         self.delta = to_dev(p.idx_delta, torch.float64, dev)
         self.term_cluster = to_dev(p.term_cluster, torch.int32, dev)
         self.term_m = to_dev(p.term_m, torch.int32, dev)
@@ -204,7 +205,7 @@ class AssocProblem:
         pr.n_grid, pr.n_iter = n_grid, max(n_refine, 20)
         pr.l_min, pr.l_max = l_min, l_max
         pr.term_cluster, pr.term_m = ptr(cohort.term_cluster), ptr(cohort.term_m)
-        pr.cl_center = ptr(cohort.cl_center)
+        pr.cl_center, pr.cl_weight = ptr(cohort.cl_center), ptr(cohort.cl_weight)
         pr.logdet_mom, pr.shared_mom = ptr(self.logdet_mom), ptr(self.shared_mom)
         pr.pair_row, pr.grid_loglam = ptr(self.pair_row), ptr(self.grid_loglam)
         pr.grid_kappa, pr.grid_shared = ptr(self.grid_kappa), ptr(self.grid_shared)

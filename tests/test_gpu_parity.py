@@ -87,7 +87,7 @@ def test_loco_kinship_matches_oracle():
 
 
 # ------------------------------------------------------------------ rotation
-@pytest.mark.parametrize("variant", ["1", "2"])
+@pytest.mark.parametrize("variant", ["1", "2", "3", "4"])
 @pytest.mark.parametrize("n,p", [(100, 37), (333, 700), (520, 1100)])
 def test_rotation_f64_and_mixed(n, p, variant, monkeypatch):
     monkeypatch.setenv("JM_ROT_BF16_V", variant)

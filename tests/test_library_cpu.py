@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(_lib.lib, name), name
     assert declared == set(_lib.EXPORTED)
-    assert _lib.lib.jm_version() == 100
+    assert _lib.lib.jm_version() == 101
 
 
 def test_error_reporting_without_gpu_work():
