@@ -136,12 +136,13 @@ def main() -> int:
         e = min(s + ktile, pf)
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         gt = gf[s:e]
-        ldt = round_up(e - s, 64)
+        ldt = round_up(e - s, 128)
         gT = torch.zeros((n, ldt), dtype=torch.int8, device=dev)
+        gT0 = torch.zeros((n, ldt), dtype=torch.int8, device=dev)
         _lib.call("jm_transpose_i8", gt.data_ptr(), e - s, n, ld, gT.data_ptr(), ldt,
-                  torch.cuda.current_stream().cuda_stream)
+                  gT0.data_ptr(), torch.cuda.current_stream().cuda_stream)
         ev0.record()
-        _lib.call("jm_kinship_syrk_i8", gT.data_ptr(), ldt, n, e - s, acc.S.data_ptr(),
+        _lib.call("jm_kinship_syrk_i8", gT0.data_ptr(), ldt, n, e - s, acc.S.data_ptr(),
                   torch.cuda.current_stream().cuda_stream)
         ev1.record()
         mu_d = to_dev(mu64[s:e], torch.float64)
@@ -303,7 +304,9 @@ def main() -> int:
         else None,
         "parity": parity,
         "kinship": {"seconds": kin_s, "int8_gemm_ms": syrk_ms,
-                    "int8_gemm_frac_of_peak": (2.0 * n * n * pf / (syrk_ms * 1e-3)) / PEAK["int8"]},
+                    "int8_gemm_frac_of_peak": (float(n) * (n + 1) * pf / (syrk_ms * 1e-3))
+                    / PEAK["int8"],
+                    "int8_gemm_ops": "n(n+1)p (symmetric rank-p update, SYRK convention)"},
         "eigendecomposition_s": eig_s,
         "spectrum_terms": cohort.n_terms,
     }
@@ -356,10 +359,13 @@ def serving(args) -> int:
         st = torch.cuda.current_stream().cuda_stream
         for s in range(0, p, 8192):
             e = min(s + 8192, p)
-            ldt = round_up(e - s, 64)
+            ldt = round_up(e - s, 128)
             gT = torch.zeros((n, ldt), dtype=torch.int8, device=dev)
-            _lib.call("jm_transpose_i8", g[s:e].data_ptr(), e - s, n, ld, gT.data_ptr(), ldt, st)
-            _lib.call("jm_kinship_syrk_i8", gT.data_ptr(), ldt, n, e - s, acc.S.data_ptr(), st)
+            gT0 = torch.zeros((n, ldt), dtype=torch.int8, device=dev)
+            _lib.call("jm_transpose_i8", g[s:e].data_ptr(), e - s, n, ld, gT.data_ptr(), ldt,
+                      gT0.data_ptr(), st)
+            _lib.call("jm_kinship_syrk_i8", gT0.data_ptr(), ldt, n, e - s, acc.S.data_ptr(), st)
+            del gT0
             if acc.C is None:
                 acc.C = torch.zeros((n, n), dtype=torch.float64, device=dev)
             mu_d = torch.as_tensor(mu64[s:e], device=dev)

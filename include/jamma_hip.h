@@ -64,14 +64,12 @@ int jm_decode_bed(const uint8_t* bed, int64_t bytes_per_snp, const int32_t* snp_
                   int64_t n_snps, const int32_t* sample_idx, int64_t n_out, int8_t* g,
                   int64_t ld_g, int32_t* counts, void* stream);
 
-/* out (cols x ld_out) = transpose of in (rows x ld_in), int8. Used to form the
- * sample-major genotype tile that the kinship GEMM reads. */
+/* out (cols x ld_out) = transpose of in (rows x ld_in), int8, and/or out0, the
+ * same transpose with missing calls (-1) written as 0. out holds the sample-major
+ * genotype tile the imputation terms read, out0 the operand of the kinship
+ * GEMM. Either output may be NULL; columns [rows, ld_out) are not written. */
 int jm_transpose_i8(const int8_t* in, int64_t rows, int64_t cols, int64_t ld_in, int8_t* out,
-                    int64_t ld_out, void* stream);
-
-/* ------------------------------------------------------------------------ */
-/* Centred kinship (-gk 1)                                                   */
-/* ------------------------------------------------------------------------ */
+                    int64_t ld_out, int8_t* out0, void* stream);
 
 /*
  * Replaces _accumulate_kinship (src/jamma/kinship/compute.py:46-57) inside the
@@ -80,12 +78,13 @@ int jm_transpose_i8(const int8_t* in, int64_t rows, int64_t cols, int64_t ld_in,
  * (src/jamma/kinship/missing.py:22-61), which this library applies as exact
  * corrections (see jm_kinship_finalize).
  *
- * S (n x n int32) += X0 X0^T over one tile of n_snps SNPs, where X0 = gT with
- * missing calls as 0; gT is sample-major (n x ld_gT, ld_gT % 16 == 0). Exact
- * int32 accumulation on int8 MFMA (v_mfma_i32_16x16x64_i8). Only 128x128 tiles
- * on or below the block diagonal are written; jm_kinship_finalize mirrors them.
+ * S (n x n int32) += X0 X0^T over one tile of n_snps SNPs. gT0 is X0^T, the
+ * sample-major tile with missing calls as 0 (jm_transpose_i8's out0), n x ld_gT
+ * with ld_gT a multiple of 128 and columns [n_snps, ld_gT) zero. Exact int32
+ * accumulation on int8 MFMA (v_mfma_i32_16x16x64_i8). Only the lower triangle
+ * (i >= k) of S is guaranteed; jm_kinship_finalize mirrors it.
  */
-int jm_kinship_syrk_i8(const int8_t* gT, int64_t ld_gT, int64_t n, int64_t n_snps, int32_t* S,
+int jm_kinship_syrk_i8(const int8_t* gT0, int64_t ld_gT, int64_t n, int64_t n_snps, int32_t* S,
                        void* stream);
 
 /*
@@ -105,7 +104,7 @@ int jm_kinship_impute_terms(const int8_t* g, int64_t ld_g, const int8_t* gT, int
 /*
  * K[i][k] = (S_ik - u_i - u_k + s0 + C[i][k] + C[k][i]) / p_snps
  * i.e. Xc Xc^T / p with Xc = impute_and_center(X) (kinship/compute.py:217,684).
- * S is read from the lower tiles written by jm_kinship_syrk_i8. S_sub / u_sub /
+ * S is read from the lower triangle written by jm_kinship_syrk_i8. S_sub / u_sub /
  * C_sub (may be NULL) are subtracted first: the LOCO subtraction
  * K_c = (S_full - S_c) / (p - p_c) of kinship/compute.py:898-916.
  */
@@ -135,21 +134,28 @@ int jm_rotate_f64(const double* UT, int64_t ld_u, const int8_t* g, int64_t ld_g,
 int jm_gemm_nt_f64(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t m,
                    int64_t ncols, int64_t k, double* C, int64_t ldc, void* stream);
 
-/* Splits UT (float64, n x ld_u) into three bfloat16 planes hi/mid/lo with
- * UT ~= hi + mid + lo (24 significant bits), each n x ld_s (ld_s % 32 == 0). */
-int jm_split_bf16x3(const double* UT, int64_t ld_u, int64_t n, uint16_t* hi, uint16_t* mid,
-                    uint16_t* lo, int64_t ld_s, void* stream);
-
 /*
- * Mixed-precision rotation: Z (float32, n x ldz) = U^T X using bf16 MFMA
+ * Mixed-precision rotation (the same Z = U^T X as jm_rotate_f64) on bf16 MFMA
  * (v_mfma_f32_16x16x32_bf16) with FP32 accumulation:
- *   hi^T X1 + mid^T X1 + lo^T X1 + hi^T X2
+ *   Z = lo^T X1 + mid^T X1 + hi^T X2 + hi^T X1,
+ * UT = hi + mid + lo (three bfloat16 planes, 24 significant bits),
  * X1 = genotypes with missing -> bf16(mean), X2 = (mean - bf16(mean)) at missing.
- * col_idx as in jm_rotate_f64.
+ * Both operands are pre-blocked so that every copy into LDS is one contiguous
+ * kilobyte:
+ *   jm_pack_ut_bf16x3: UT (float64, n x ld_u) -> planes (jm_packed_ut_bytes(n)
+ *     bytes, 16-byte aligned), once per cohort;
+ *   jm_pack_genotypes: SNP rows r(s) of g (col_idx as in jm_rotate_f64) and
+ *     their means -> gp (jm_packed_geno_bytes(n, n_snps) bytes) and mm
+ *     (n_snps x uint32: bf16(mean) | bf16 residual << 16), per SNP tile;
+ *   jm_rotate_bf16x3: Z (float32, n x ldz) from planes, gp, mm.
  */
-int jm_rotate_bf16x3(const uint16_t* hi, const uint16_t* mid, const uint16_t* lo, int64_t ld_s,
-                     const int8_t* g, int64_t ld_g, const double* mean, const int32_t* col_idx,
-                     int64_t n, int64_t n_snps, float* Z, int64_t ldz, void* stream);
+int64_t jm_packed_ut_bytes(int64_t n);
+int64_t jm_packed_geno_bytes(int64_t n, int64_t n_snps);
+int jm_pack_ut_bf16x3(const double* UT, int64_t ld_u, int64_t n, uint16_t* planes, void* stream);
+int jm_pack_genotypes(const int8_t* g, int64_t ld_g, const int32_t* col_idx, const double* mean,
+                      int64_t n, int64_t n_snps, uint8_t* gp, uint32_t* mm, void* stream);
+int jm_rotate_bf16x3(const uint16_t* planes, const uint8_t* gp, const uint32_t* mm, int64_t n,
+                     int64_t n_snps, float* Z, int64_t ldz, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Per-SNP REML/MLE lambda search + Wald/LRT/Score                           */

@@ -50,7 +50,8 @@ _SIGS = {
     "jm_last_error": ([ctypes.c_char_p, c_size_t], c_int),
     "jm_decode_bed": ([c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64,
                        c_void_p, c_void_p], c_int),
-    "jm_transpose_i8": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_void_p], c_int),
+    "jm_transpose_i8": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_void_p,
+                         c_void_p], c_int),
     "jm_kinship_syrk_i8": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p], c_int),
     "jm_kinship_impute_terms": ([c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_void_p,
                                  c_void_p, c_void_p, c_int, c_void_p], c_int),
@@ -60,10 +61,13 @@ _SIGS = {
                        c_void_p, c_int64, c_void_p], c_int),
     "jm_gemm_nt_f64": ([c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p,
                         c_int64, c_void_p], c_int),
-    "jm_split_bf16x3": ([c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
-                         c_void_p], c_int),
-    "jm_rotate_bf16x3": ([c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p,
-                          c_void_p, c_int64, c_int64, c_void_p, c_int64, c_void_p], c_int),
+    "jm_packed_ut_bytes": ([c_int64], c_int64),
+    "jm_packed_geno_bytes": ([c_int64, c_int64], c_int64),
+    "jm_pack_ut_bf16x3": ([c_void_p, c_int64, c_int64, c_void_p, c_void_p], c_int),
+    "jm_pack_genotypes": ([c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_int64, c_void_p,
+                           c_void_p, c_void_p], c_int),
+    "jm_rotate_bf16x3": ([c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_int64,
+                          c_void_p], c_int),
     "jm_spectrum_moments": ([c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                              c_int64, c_void_p], c_int),

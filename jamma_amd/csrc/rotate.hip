@@ -7,12 +7,13 @@
 //
 //  * k_gemm_f64: FP64 MFMA (v_mfma_f64_16x16x4f64), 64x128 block tile,
 //    4 waves of 32x64, LDS-staged A (UT rows) and B (genotype int8 or float64).
-//  * k_rotate_bf16x3: bf16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32
+//  * k_rotate_bf16x3_packed: bf16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32
 //    accumulation over the split UT = hi + mid + lo and the genotype split
 //    X = X1 + X2 (X1 = genotype or bf16(mean) at a missing call, X2 = the
 //    bf16 residual of the mean at missing calls):
-//        Z = hi X1 + mid X1 + lo X1 + hi X2.
-//    128x128 block tile, 4 waves of 64x64.
+//        Z = lo X1 + mid X1 + hi X2 + hi X1.
+//    256x256 block tile, both operands pre-blocked (k_pack_ut, k_pack_geno)
+//    and DMA-copied into double-buffered LDS.
 //
 // Block order is XCD-aware: consecutive hardware block ids go round-robin to
 // the 8 XCDs, so blocks b, b+8, b+16, ... share an XCD (and its L2). Tiles are
@@ -27,7 +28,6 @@
 
 #include "jm_common.h"
 
-#include <cstdlib>
 
 namespace jm {
 
@@ -201,310 +201,17 @@ __device__ __forceinline__ double bf16_to_f64(uint16_t h) {
   return (double)__uint_as_float(((uint32_t)h) << 16);
 }
 
-__global__ void k_split_bf16x3(const double* __restrict__ UT, int64_t ld_u, int64_t n,
-                               uint16_t* __restrict__ hi, uint16_t* __restrict__ mid,
-                               uint16_t* __restrict__ lo, int64_t ld_s) {
-  const int64_t total = n * ld_s;
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = idx / ld_s, k = idx % ld_s;
-    const double u = (k < n) ? UT[i * ld_u + k] : 0.0;
-    const uint16_t h = f32_to_bf16_rne((float)u);
-    const double r1 = u - bf16_to_f64(h);
-    const uint16_t m = f32_to_bf16_rne((float)r1);
-    const double r2 = r1 - bf16_to_f64(m);
-    const uint16_t l = f32_to_bf16_rne((float)r2);
-    hi[idx] = h;
-    mid[idx] = m;
-    lo[idx] = l;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // BF16x3 rotation
 // ---------------------------------------------------------------------------
-constexpr int B_BM = 128, B_BN = 128, B_BK = 64;
-constexpr int B_AS = B_BK + 8;   // bf16 elements per LDS row (16-byte aligned rows)
-constexpr int B_GS = B_BK + 16;  // int8 bytes per genotype LDS row
-
-__global__ void __launch_bounds__(256) k_rotate_bf16x3(
-    const uint16_t* __restrict__ Ahi, const uint16_t* __restrict__ Amid,
-    const uint16_t* __restrict__ Alo, int64_t lda, const int8_t* __restrict__ g, int64_t ldg,
-    const double* __restrict__ mean, const int32_t* __restrict__ col, int64_t M, int64_t n_cols,
-    int64_t K, float* __restrict__ Z, int64_t ldz) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[3][B_BM * B_AS];
-  __shared__ __attribute__((aligned(16))) int8_t Gs[B_BN * B_GS];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int nrb = (int)((M + B_BM - 1) / B_BM), ncb = (int)((n_cols + B_BN - 1) / B_BN);
-  int rb, cb;
-  xcd_tile(blockIdx.x, nrb, ncb, rb, cb);
-  const int64_t i0 = (int64_t)rb * B_BM, s0 = (int64_t)cb * B_BN;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-  const int lr = lane & 15, lk = lane >> 4;
-
-  // per-column bf16 split of the imputation mean: m1 = bf16(mean), m2 = bf16(mean - m1)
-  uint16_t m1[4], m2[4];
-#pragma unroll
-  for (int tj = 0; tj < 4; ++tj) {
-    const int64_t s = s0 + wc + tj * 16 + lr;
-    const double mu = (s < n_cols) ? mean[col ? (int64_t)col[s] : s] : 0.0;
-    m1[tj] = f32_to_bf16_rne((float)mu);
-    m2[tj] = f32_to_bf16_rne((float)(mu - bf16_to_f64(m1[tj])));
-  }
-
-  f4_t acc[4][4];
-#pragma unroll
-  for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = (f4_t){0.f, 0.f, 0.f, 0.f};
-
-  const uint16_t* Ap[3] = {Ahi, Amid, Alo};
-  for (int64_t k0 = 0; k0 < K; k0 += B_BK) {
-    // A planes: 128 rows x 64 bf16 = 128 B per row -> 8 x 16 B chunks; 1024 chunks / plane
-#pragma unroll
-    for (int pl = 0; pl < 3; ++pl) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int idx = tid + 256 * u;
-        const int r = idx >> 3, c = (idx & 7) * 8;
-        const int64_t i = i0 + r;
-        int4 w = {0, 0, 0, 0};
-        if (i < M && k0 + c + 8 <= lda) w = *reinterpret_cast<const int4*>(Ap[pl] + i * lda + k0 + c);
-        *reinterpret_cast<int4*>(&As[pl][r * B_AS + c]) = w;
-      }
-    }
-    // genotype tile: 128 SNP rows x 64 bytes -> 4 x 16 B per row; 512 chunks
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int idx = tid + 256 * u;
-      const int r = idx >> 2, c = (idx & 3) * 16;
-      const int64_t s = s0 + r;
-      int4 w = {0, 0, 0, 0};
-      if (s < n_cols) {
-        const int64_t sr = col ? (int64_t)col[s] : s;
-        if (k0 + c + 16 <= K) {
-          w = *reinterpret_cast<const int4*>(g + sr * ldg + k0 + c);
-        } else {
-          int8_t tmp[16];
-#pragma unroll
-          for (int e = 0; e < 16; ++e) tmp[e] = (k0 + c + e < K) ? g[sr * ldg + k0 + c + e] : (int8_t)0;
-          memcpy(&w, tmp, 16);
-        }
-      }
-      *reinterpret_cast<int4*>(&Gs[r * B_GS + c]) = w;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < B_BK; kk += 32) {
-      // B fragments: 8 consecutive k for column (wc + tj*16 + lr)
-      bf16x8_t b1[4], b2[4];
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj) {
-        const uint2 raw = *reinterpret_cast<const uint2*>(&Gs[(wc + tj * 16 + lr) * B_GS + kk + 8 * lk]);
-        uint16_t v1[8], v2[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int8_t x = (int8_t)(((e < 4 ? raw.x : raw.y) >> (8 * (e & 3))) & 0xFF);
-          const bool miss = x < 0;
-          // integers 0,1,2 are exact in bf16: 0x0000, 0x3F80, 0x4000
-          const uint16_t xi = (x == 0) ? (uint16_t)0 : (x == 1 ? (uint16_t)0x3F80 : (uint16_t)0x4000);
-          v1[e] = miss ? m1[tj] : xi;
-          v2[e] = miss ? m2[tj] : (uint16_t)0;
-        }
-        memcpy(&b1[tj], v1, 16);
-        memcpy(&b2[tj], v2, 16);
-      }
-#pragma unroll
-      for (int ti = 0; ti < 4; ++ti) {
-        const int row = wr + ti * 16 + lr;
-        bf16x8_t ah, am, al;
-        memcpy(&ah, &As[0][row * B_AS + kk + 8 * lk], 16);
-        memcpy(&am, &As[1][row * B_AS + kk + 8 * lk], 16);
-        memcpy(&al, &As[2][row * B_AS + kk + 8 * lk], 16);
-#pragma unroll
-        for (int tj = 0; tj < 4; ++tj) {
-          f4_t c = acc[ti][tj];
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b1[tj], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, b1[tj], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b2[tj], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b1[tj], c, 0, 0, 0);
-          acc[ti][tj] = c;
-        }
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
-        const int64_t s = s0 + wc + tj * 16 + lr;
-        if (i < M && s < n_cols) Z[i * ldz + s] = acc[ti][tj][r];
-      }
-}
-
-// ---------------------------------------------------------------------------
-// BF16x3 rotation, v2: 256x128 block tile (8 waves of 64x64), BK = 32,
-// genotypes converted to bf16 X1/X2 once per stage in LDS, next K-slab
-// prefetched into registers while the current one is multiplied.
-// ---------------------------------------------------------------------------
-constexpr int V2_BM = 256, V2_BN = 128, V2_BK = 32;
-constexpr int V2_AS = V2_BK + 8;  // bf16 per LDS row (80 B, 16-byte aligned)
-constexpr int V2_BS = V2_BK + 8;
-
-__device__ __forceinline__ uint16_t geno_bf16(int8_t x, uint16_t m1) {
-  // 0 -> 0x0000, 1 -> 0x3F80, 2 -> 0x4000 (exact), missing -> bf16(mean)
-  return x < 0 ? m1 : (x == 0 ? (uint16_t)0 : (uint16_t)(0x3F00 + 0x80 * (int)x));
-}
-
-__global__ void __launch_bounds__(512) k_rotate_bf16x3_v2(
-    const uint16_t* __restrict__ Ahi, const uint16_t* __restrict__ Amid,
-    const uint16_t* __restrict__ Alo, int64_t lda, const int8_t* __restrict__ g, int64_t ldg,
-    const double* __restrict__ mean, const int32_t* __restrict__ col, int64_t M, int64_t n_cols,
-    int64_t K, float* __restrict__ Z, int64_t ldz) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[3][V2_BM * V2_AS];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][V2_BN * V2_BS];
-  __shared__ uint16_t Bm[2][V2_BN];  // per column bf16(mean), residual
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int nrb = (int)((M + V2_BM - 1) / V2_BM), ncb = (int)((n_cols + V2_BN - 1) / V2_BN);
-  int rb, cb;
-  xcd_tile(blockIdx.x, nrb, ncb, rb, cb);
-  const int64_t i0 = (int64_t)rb * V2_BM, s0 = (int64_t)cb * V2_BN;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-  const int lr = lane & 15, lk = lane >> 4;
-  if (tid < V2_BN) {
-    const int64_t s = s0 + tid;
-    const double mu = (s < n_cols) ? mean[col ? (int64_t)col[s] : s] : 0.0;
-    const uint16_t m1 = f32_to_bf16_rne((float)mu);
-    Bm[0][tid] = m1;
-    Bm[1][tid] = f32_to_bf16_rne((float)(mu - bf16_to_f64(m1)));
-  }
-  // per-thread load slots: A: 3 planes x 256 rows x 64 B = 3072 x 16 B -> 6 per thread
-  //                        B: 128 rows x 32 B -> 256 x 16 B: threads 0..255, one each
-  const int b_r = tid >> 1, b_c = (tid & 1) * 16;
-  const int64_t b_s = s0 + b_r;
-  const int64_t b_row = (b_s < n_cols) ? (col ? (int64_t)col[b_s] : b_s) : -1;
-  int4 ra[6];
-  int4 rbv = {0, 0, 0, 0};
-  const uint16_t* Ap[3] = {Ahi, Amid, Alo};
-  auto gload = [&](int64_t k0) {
-#pragma unroll
-    for (int u = 0; u < 6; ++u) {
-      const int idx = tid + 512 * u;        // 0..3071
-      const int pl = idx >> 10;             // plane
-      const int r = (idx & 1023) >> 2;      // row 0..255
-      const int c = (idx & 3) * 8;          // bf16 column 0,8,16,24
-      const int64_t i = i0 + r;
-      int4 w = {0, 0, 0, 0};
-      if (i < M && k0 + c + 8 <= lda) w = *reinterpret_cast<const int4*>(Ap[pl] + i * lda + k0 + c);
-      ra[u] = w;
-    }
-    if (tid < 256) {
-      int4 w = {0, 0, 0, 0};
-      if (b_row >= 0) {
-        if (k0 + b_c + 16 <= K) {
-          w = *reinterpret_cast<const int4*>(g + b_row * ldg + k0 + b_c);
-        } else {
-          int8_t tmp[16];
-#pragma unroll
-          for (int e = 0; e < 16; ++e)
-            tmp[e] = (k0 + b_c + e < K) ? g[b_row * ldg + k0 + b_c + e] : (int8_t)0;
-          memcpy(&w, tmp, 16);
-        }
-      }
-      rbv = w;
-    }
-  };
-  auto lstore = [&]() {
-#pragma unroll
-    for (int u = 0; u < 6; ++u) {
-      const int idx = tid + 512 * u;
-      const int pl = idx >> 10;
-      const int r = (idx & 1023) >> 2;
-      const int c = (idx & 3) * 8;
-      *reinterpret_cast<int4*>(&As[pl][r * V2_AS + c]) = ra[u];
-    }
-    if (tid < 256) {
-      const uint16_t m1 = Bm[0][b_r], m2 = Bm[1][b_r];
-      int8_t x[16];
-      memcpy(x, &rbv, 16);
-      uint16_t v1[16], v2[16];
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        v1[e] = geno_bf16(x[e], m1);
-        v2[e] = x[e] < 0 ? m2 : (uint16_t)0;
-      }
-      memcpy(&Bs[0][b_r * V2_BS + b_c], v1, 32);
-      memcpy(&Bs[1][b_r * V2_BS + b_c], v2, 32);
-    }
-  };
-  f4_t acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
-  __syncthreads();  // Bm visible
-  gload(0);
-  for (int64_t k0 = 0; k0 < K; k0 += V2_BK) {
-    lstore();
-    __syncthreads();
-    if (k0 + V2_BK < K) gload(k0 + V2_BK);  // prefetch the next slab
-    bf16x8_t b1[4], b2[4];
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj) {
-      const int cidx = (wc + tj * 16 + lr) * V2_BS + 8 * lk;
-      memcpy(&b1[tj], &Bs[0][cidx], 16);
-      memcpy(&b2[tj], &Bs[1][cidx], 16);
-    }
-#pragma unroll
-    for (int ti = 0; ti < 4; ++ti) {
-      const int aidx = (wr + ti * 16 + lr) * V2_AS + 8 * lk;
-      bf16x8_t ah, am, al;
-      memcpy(&ah, &As[0][aidx], 16);
-      memcpy(&am, &As[1][aidx], 16);
-      memcpy(&al, &As[2][aidx], 16);
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj) {
-        f4_t c = acc[ti][tj];
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b1[tj], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, b1[tj], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b2[tj], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b1[tj], c, 0, 0, 0);
-        acc[ti][tj] = c;
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
-        const int64_t s = s0 + wc + tj * 16 + lr;
-        if (i < M && s < n_cols) Z[i * ldz + s] = acc[ti][tj][r];
-      }
-}
-
-
-// ---------------------------------------------------------------------------
-// BF16x3 rotation, 256x256 block tile (default)
-// ---------------------------------------------------------------------------
-// 4 waves, each owning a 128x128 output tile (8x8 MFMA tiles, accumulators in
-// AGPRs). Per 32-deep K step the block stages the three UT planes (3 x 256 x
-// 32 bf16 = 48 KB) and the int8 genotypes (256 x 32 B = 8 KB) straight from
-// global memory into LDS with global_load_lds_dwordx4 (no register staging),
-// double buffered, one barrier per step. Genotype fragments are widened to
-// bf16 X1 / X2 per fragment. LDS rows are XOR-swizzled in 16-byte chunks so
-// that the ds_read_b128 (A) and ds_read_b64 (B) fragment reads of 16 lanes hit
-// 64 distinct banks: A chunk c of row r sits at c ^ ((r >> 2) & 3), genotype
-// chunk c of column j at c ^ ((j >> 3) & 1).
+// Per 32-deep K step a 256x256 block stages the three UT planes (3 x 256 x 32
+// bf16 = 48 KB) and the int8 genotypes (256 x 32 B = 8 KB) from global memory
+// into LDS with global_load_lds_dwordx4 (no register staging), double
+// buffered. LDS rows are XOR-swizzled in 16-byte chunks so that the
+// ds_read_b128 (A) and ds_read_b64 (B) fragment reads of 16 lanes hit 64
+// distinct banks: A chunk c of row r sits at c ^ ((r >> 2) & 3) (64-byte rows:
+// the 4-bank group of a chunk is 4 (r & 3) + c), genotype chunk c of column j
+// at c ^ ((j >> 3) & 1).
 constexpr int V3_BM = 256, V3_BN = 256, V3_BK = 32;
 constexpr int V3_APLANE = V3_BM * V3_BK * 2;      // 16 KB per plane
 constexpr int V3_BBYTES = V3_BN * V3_BK;          // 8 KB
@@ -516,225 +223,159 @@ __device__ __forceinline__ void lds_dma16(const void* gsrc, uint8_t* lds_dst) {
   __builtin_amdgcn_global_load_lds(gsrc, (lds_void*)lds_dst, 16, 0, 0);
 }
 
-// 8 genotype bytes -> bf16x8 X1 (0/1/2 exact, missing -> m1) and X2 (missing -> m2).
-__device__ __forceinline__ void widen8(uint64_t bytes, uint32_t m1, uint32_t m2, bf16x8_t& x1,
-                                       bf16x8_t& x2) {
+// Byte-permute form of widen8. The 2-bit genotype code indexes a 4-entry table
+// (0, 1, 2, missing -> x & 3 = 0, 1, 2, 3); v_perm_b32 picks the low and high
+// bytes of each bf16 from per-column tables:
+//   X1: lo bytes {00, 80, 00, lo(m1)}, hi bytes {00, 3F, 40, hi(m1)}  (0, 1.0, 2.0, m1)
+//   X2: lo bytes {00, 00, 00, lo(m2)}, hi bytes {00, 00, 00, hi(m2)}
+// mm = m1 | m2 << 16. About 18 VALU per 8 genotypes instead of ~70.
+__device__ __forceinline__ void widen8p(uint64_t bytes, uint32_t mm, bf16x8_t& x1, bf16x8_t* x2) {
+  const uint32_t t1lo = ((mm & 0xFFu) << 24) | 0x8000u;
+  const uint32_t t1hi = (((mm >> 8) & 0xFFu) << 24) | 0x403F00u;
   uint32_t w1[4], w2[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    const uint32_t b0 = (uint32_t)(bytes >> (16 * e)) & 0xFFu;
-    const uint32_t b1 = (uint32_t)(bytes >> (16 * e + 8)) & 0xFFu;
-    const uint32_t v0 = (b0 & 0x80u) ? m1 : (b0 ? 0x3F00u + (b0 << 7) : 0u);
-    const uint32_t v1 = (b1 & 0x80u) ? m1 : (b1 ? 0x3F00u + (b1 << 7) : 0u);
-    const uint32_t r0 = (b0 & 0x80u) ? m2 : 0u;
-    const uint32_t r1 = (b1 & 0x80u) ? m2 : 0u;
-    w1[e] = v0 | (v1 << 16);
-    w2[e] = r0 | (r1 << 16);
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t idx = (uint32_t)(bytes >> (32 * h)) & 0x03030303u;
+    const uint32_t selA = __builtin_amdgcn_perm(idx, idx, 0x01010000u) + 0x04000400u;
+    const uint32_t selB = __builtin_amdgcn_perm(idx, idx, 0x03030202u) + 0x04000400u;
+    w1[2 * h] = __builtin_amdgcn_perm(t1hi, t1lo, selA);
+    w1[2 * h + 1] = __builtin_amdgcn_perm(t1hi, t1lo, selB);
+    if (x2) {
+      const uint32_t t2lo = ((mm >> 16) & 0xFFu) << 24;
+      const uint32_t t2hi = ((mm >> 24) & 0xFFu) << 24;
+      w2[2 * h] = __builtin_amdgcn_perm(t2hi, t2lo, selA);
+      w2[2 * h + 1] = __builtin_amdgcn_perm(t2hi, t2lo, selB);
+    }
   }
   memcpy(&x1, w1, 16);
-  memcpy(&x2, w2, 16);
+  if (x2) memcpy(x2, w2, 16);
 }
 
-__global__ void __launch_bounds__(256) k_rotate_bf16x3_v3(
-    const uint16_t* __restrict__ Ahi, const uint16_t* __restrict__ Amid,
-    const uint16_t* __restrict__ Alo, int64_t lda, const int8_t* __restrict__ g, int64_t ldg,
-    const double* __restrict__ mean, const int32_t* __restrict__ col, int64_t M, int64_t n_cols,
-    int64_t K, float* __restrict__ Z, int64_t ldz) {
+// ---------------------------------------------------------------------------
+// Packed operands for the BF16x3 rotation
+// ---------------------------------------------------------------------------
+// The rotation streams UT (three bf16 planes) and the genotype tile through
+// LDS in 256 x 32 blocks. Both are stored pre-blocked so that every DMA of
+// the rotation kernel copies one contiguous kilobyte:
+//   UT planes: plane-major, then panel P (256 rows), then K step s (32
+//     columns): a 16 KB block of 256 rows x 64 B, row r's 16-byte chunk c at
+//     r * 64 + 16 * (c ^ ((r >> 2) & 3)). Rows >= n and columns >= n are 0.
+//   genotypes: column block B (256 SNPs), then K step s (32 samples): an 8 KB
+//     block of 256 SNPs x 32 B, SNP j's chunk c at j * 32 + 16 * (c ^ ((j >> 3)
+//     & 1)). The 2-bit codes are those of the decoded tile (0/1/2, missing
+//     0xFF); SNPs >= n_snps and samples >= n are 0.
+// The swizzles are those of the LDS stage, so a block copies verbatim.
+constexpr int64_t UTP_BLOCK = 16384, GP_BLOCK = 8192;
+
+__global__ void k_pack_ut(const double* __restrict__ UT, int64_t ld_u, int64_t n, int64_t npan,
+                          int64_t nks, uint16_t* __restrict__ planes) {
+  const int64_t per_plane = npan * nks * (UTP_BLOCK / 2);  // bf16 elements per plane
+  const int64_t chunks = npan * nks * 1024;                // 16-byte chunks per plane
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < chunks;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t blk = q >> 10;
+    const int pos = (int)(q & 1023);  // chunk slot within the 16 KB block
+    const int r = pos >> 2, cs = pos & 3;
+    const int c = cs ^ ((r >> 2) & 3);
+    const int64_t P = blk / nks, sk = blk % nks;
+    const int64_t i = P * 256 + r, k0 = sk * 32 + 8 * c;
+    uint16_t h[8], m[8], l[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const double u = (i < n && k0 + e < n) ? UT[i * ld_u + k0 + e] : 0.0;
+      h[e] = f32_to_bf16_rne((float)u);
+      const double r1 = u - bf16_to_f64(h[e]);
+      m[e] = f32_to_bf16_rne((float)r1);
+      const double r2 = r1 - bf16_to_f64(m[e]);
+      l[e] = f32_to_bf16_rne((float)r2);
+    }
+    const int64_t off = q * 8;
+    memcpy(planes + off, h, 16);
+    memcpy(planes + per_plane + off, m, 16);
+    memcpy(planes + 2 * per_plane + off, l, 16);
+  }
+}
+
+__global__ void k_pack_geno(const int8_t* __restrict__ g, int64_t ld_g,
+                            const int32_t* __restrict__ col, const double* __restrict__ mean,
+                            int64_t n, int64_t n_snps, int64_t nks, uint8_t* __restrict__ gp,
+                            uint32_t* __restrict__ mm) {
+  const int64_t nblk = (n_snps + 255) / 256;
+  const int64_t chunks = nblk * nks * 512;  // 16-byte chunks
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < chunks;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t blk = q >> 9;
+    const int pos = (int)(q & 511);
+    const int j = pos >> 1, cs = pos & 1;
+    const int c = cs ^ ((j >> 3) & 1);
+    const int64_t B = blk / nks, sk = blk % nks;
+    const int64_t snp = B * 256 + j, k0 = sk * 32 + 16 * c;
+    uint8_t v[16];
+    if (snp < n_snps) {
+      const int8_t* row = g + (col ? (int64_t)col[snp] : snp) * ld_g;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = (k0 + e < n) ? (uint8_t)row[k0 + e] : (uint8_t)0;
+      if (sk == 0 && c == 0) {
+        const double mu = mean[col ? (int64_t)col[snp] : snp];
+        const uint16_t h1 = f32_to_bf16_rne((float)mu);
+        mm[snp] = (uint32_t)h1 | ((uint32_t)f32_to_bf16_rne((float)(mu - bf16_to_f64(h1))) << 16);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = 0;
+    }
+    memcpy(gp + q * 16, v, 16);
+  }
+}
+
+// BF16x3 rotation on packed operands: 256x256 block tile, 8 waves in a 4 x 2
+// grid of 64x128 wave tiles (128 accumulators per lane, two waves per SIMD),
+// both operands DMA-copied block by block into double-buffered LDS, one
+// barrier per 32-deep K step. Genotype fragments are widened to X1 / X2 with
+// widen8p; per K step: lo x X1, mid x X1, hi x X2, hi x X1 (small terms first).
+__global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
+    const uint16_t* __restrict__ planes, int64_t per_plane, const uint8_t* __restrict__ gp,
+    const uint32_t* __restrict__ mmv, int64_t M, int64_t n_cols, int64_t nks,
+    float* __restrict__ Z, int64_t ldz) {
   __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * V3_STAGE];
-  __shared__ uint32_t Bm[2][V3_BN];  // bf16(mean), bf16 residual of the mean
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int nrb = (int)((M + V3_BM - 1) / V3_BM), ncb = (int)((n_cols + V3_BN - 1) / V3_BN);
   int rb, cb;
   xcd_tile(blockIdx.x, nrb, ncb, rb, cb);
   const int64_t i0 = (int64_t)rb * V3_BM, s0 = (int64_t)cb * V3_BN;
-  const int wr = (wave >> 1) * 128, wc = (wave & 1) * 128;
-  const int lr = lane & 15, lk = lane >> 4;
-  {
-    const int64_t sc = s0 + tid;
-    const double mu = (sc < n_cols) ? mean[col ? (int64_t)col[sc] : sc] : 0.0;
-    const uint16_t h1 = f32_to_bf16_rne((float)mu);
-    Bm[0][tid] = h1;
-    Bm[1][tid] = f32_to_bf16_rne((float)(mu - bf16_to_f64(h1)));
-  }
-  // per-lane global sources of the DMA loads (fixed over K; the K offset is added per step)
-  // A: instruction q (0..15) of plane p covers rows 16q .. 16q+15; lane -> (row, chunk)
-  const uint16_t* Ap[3] = {Ahi, Amid, Alo};
-  const int a_r = lane >> 2, a_cs = lane & 3;  // row within 16, stored chunk
-  // B: instruction q (0..7) covers columns 32q .. 32q+31; lane -> (column, chunk)
-  const int b_j = lane >> 1, b_cs = lane & 1;
-  // each wave issues A instructions q = wave*4 .. wave*4+3 of every plane, B q = wave*2, +1
-  int64_t a_off[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int row = 16 * (wave * 4 + u) + a_r;
-    int64_t gi = i0 + row;
-    if (gi >= M) gi = M - 1;  // rows past the end load a valid row; never stored
-    const int c = a_cs ^ ((row >> 2) & 3);
-    a_off[u] = gi * lda + 8 * c;
-  }
-  int64_t b_off[2];
-#pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    const int j = 32 * (wave * 2 + u) + b_j;
-    int64_t sc = s0 + j;
-    if (sc >= n_cols) sc = n_cols - 1;
-    const int64_t grow = col ? (int64_t)col[sc] : sc;
-    const int c = b_cs ^ ((j >> 3) & 1);
-    b_off[u] = grow * ldg + 16 * c;
-  }
-  auto issue = [&](int stage, int64_t k0) {
-    uint8_t* base = lds + stage * V3_STAGE;
-#pragma unroll
-    for (int pl = 0; pl < 3; ++pl)
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        lds_dma16(Ap[pl] + a_off[u] + k0, base + pl * V3_APLANE + (wave * 4 + u) * 1024);
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-      lds_dma16(g + b_off[u] + k0, base + 3 * V3_APLANE + (wave * 2 + u) * 1024);
-  };
-
-  f4_t acc[8][8];
-#pragma unroll
-  for (int a = 0; a < 8; ++a)
-#pragma unroll
-    for (int b = 0; b < 8; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
-
-  const int nk = (int)((K + V3_BK - 1) / V3_BK);
-  issue(0, 0);
-  __syncthreads();  // Bm visible
-  uint32_t m1[8], m2[8];
-#pragma unroll
-  for (int tj = 0; tj < 8; ++tj) {
-    m1[tj] = Bm[0][wc + 16 * tj + lr];
-    m2[tj] = Bm[1][wc + 16 * tj + lr];
-  }
-  for (int kt = 0; kt < nk; ++kt) {
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA loads of stage kt have landed
-    __syncthreads();                // ... and everyone's; stage kt-1 is free again
-    if (kt + 1 < nk) issue((kt + 1) & 1, (int64_t)(kt + 1) * V3_BK);
-    const uint8_t* base = lds + (kt & 1) * V3_STAGE;
-    // genotype fragments: raw bytes, widened to X1 once and to X2 in the hi pass
-    uint64_t gb[8];
-    bf16x8_t x1[8];
-#pragma unroll
-    for (int tj = 0; tj < 8; ++tj) {
-      const int j = wc + 16 * tj + lr;
-      const int c = (lk >> 1) ^ ((j >> 3) & 1);
-      memcpy(&gb[tj], base + 3 * V3_APLANE + j * 32 + 16 * c + 8 * (lk & 1), 8);
-      bf16x8_t unused;
-      widen8(gb[tj], m1[tj], m2[tj], x1[tj], unused);
-    }
-    // planes lo, mid, hi (small terms first)
-#pragma unroll 1
-    for (int pl = 2; pl >= 0; --pl) {
-      bf16x8_t af[8];
-#pragma unroll
-      for (int ti = 0; ti < 8; ++ti) {
-        const int r = wr + 16 * ti + lr;
-        const int c = lk ^ ((r >> 2) & 3);
-        memcpy(&af[ti], base + pl * V3_APLANE + r * 64 + 16 * c, 16);
-      }
-      if (pl == 0) {
-#pragma unroll
-        for (int tj = 0; tj < 8; ++tj) {
-          bf16x8_t x2, unused;
-          widen8(gb[tj], m1[tj], m2[tj], unused, x2);
-#pragma unroll
-          for (int ti = 0; ti < 8; ++ti)
-            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ti], x2, acc[ti][tj], 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int ti = 0; ti < 8; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < 8; ++tj)
-          acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ti], x1[tj], acc[ti][tj], 0, 0, 0);
-    }
-  }
-#pragma unroll
-  for (int ti = 0; ti < 8; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < 8; ++tj)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
-        const int64_t sc = s0 + wc + tj * 16 + lr;
-        if (i < M && sc < n_cols) Z[i * ldz + sc] = acc[ti][tj][r];
-      }
-}
-
-// Same staging as k_rotate_bf16x3_v3 (256x256 block tile, DMA into double-
-// buffered swizzled LDS), 8 waves in a 4 x 2 grid of 64x128 wave tiles: 128
-// accumulators per lane, so two waves share each SIMD and hide each other's
-// LDS latency.
-__global__ void __launch_bounds__(512) k_rotate_bf16x3_v4(
-    const uint16_t* __restrict__ Ahi, const uint16_t* __restrict__ Amid,
-    const uint16_t* __restrict__ Alo, int64_t lda, const int8_t* __restrict__ g, int64_t ldg,
-    const double* __restrict__ mean, const int32_t* __restrict__ col, int64_t M, int64_t n_cols,
-    int64_t K, float* __restrict__ Z, int64_t ldz) {
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * V3_STAGE];
-  __shared__ uint32_t Bm[2][V3_BN];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;  // 8 waves
-  const int nrb = (int)((M + V3_BM - 1) / V3_BM), ncb = (int)((n_cols + V3_BN - 1) / V3_BN);
-  int rb, cb;
-  xcd_tile(blockIdx.x, nrb, ncb, rb, cb);
-  const int64_t i0 = (int64_t)rb * V3_BM, s0 = (int64_t)cb * V3_BN;
   const int wr = (wave >> 1) * 64, wc = (wave & 1) * 128;
   const int lr = lane & 15, lk = lane >> 4;
-  if (tid < V3_BN) {
-    const int64_t sc = s0 + tid;
-    const double mu = (sc < n_cols) ? mean[col ? (int64_t)col[sc] : sc] : 0.0;
-    const uint16_t h1 = f32_to_bf16_rne((float)mu);
-    Bm[0][tid] = h1;
-    Bm[1][tid] = f32_to_bf16_rne((float)(mu - bf16_to_f64(h1)));
-  }
-  const uint16_t* Ap[3] = {Ahi, Amid, Alo};
-  // A: 48 DMA instructions per stage (3 planes x 16 row groups), 6 per wave;
-  // source offsets are recomputed per issue to keep registers for the MFMAs
-  const int a_rl = lane >> 2, a_cs = lane & 3;
-  // B: 8 DMA instructions per stage (32 columns each), one per wave
-  int64_t b_off;
-  {
-    const int j = 32 * wave + (lane >> 1);
-    int64_t sc = s0 + j;
-    if (sc >= n_cols) sc = n_cols - 1;
-    const int64_t grow = col ? (int64_t)col[sc] : sc;
-    const int c = (lane & 1) ^ ((j >> 3) & 1);
-    b_off = grow * ldg + 16 * c;
-  }
-  auto issue = [&](int stage, int64_t k0) {
+  // DMA sources: plane pl's block (rb, s) and genotype block (cb, s)
+  const uint8_t* ap = (const uint8_t*)planes + (int64_t)rb * nks * UTP_BLOCK + lane * 16;
+  const uint8_t* bp = gp + (int64_t)cb * nks * GP_BLOCK + lane * 16;
+  const int64_t pstride = per_plane * 2;  // bytes per plane
+  auto issue = [&](int stage, int64_t sk) {
     uint8_t* base = lds + stage * V3_STAGE;
 #pragma unroll
     for (int u = 0; u < 6; ++u) {
-      const int q = wave * 6 + u;
-      const int row = 16 * (q & 15) + a_rl;
-      int64_t gi = i0 + row;
-      if (gi >= M) gi = M - 1;  // rows past the end load a valid row; never stored
-      const int c = a_cs ^ ((row >> 2) & 3);
-      lds_dma16(Ap[q >> 4] + gi * lda + 8 * c + k0, base + (q >> 4) * V3_APLANE + (q & 15) * 1024);
+      const int q = wave * 6 + u;  // 0..47: plane q >> 4, kilobyte q & 15
+      lds_dma16(ap + (q >> 4) * pstride + sk * UTP_BLOCK + (q & 15) * 1024,
+                base + (q >> 4) * V3_APLANE + (q & 15) * 1024);
     }
-    lds_dma16(g + b_off + k0, base + 3 * V3_APLANE + wave * 1024);
+    lds_dma16(bp + sk * GP_BLOCK + wave * 1024, base + 3 * V3_APLANE + wave * 1024);
   };
-
+  uint32_t mm[8];
+#pragma unroll
+  for (int tj = 0; tj < 8; ++tj) {
+    const int64_t sc = s0 + wc + 16 * tj + lr;
+    mm[tj] = sc < n_cols ? mmv[sc] : 0u;
+  }
   f4_t acc[4][8];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 8; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
-
-  const int nk = (int)((K + V3_BK - 1) / V3_BK);
   issue(0, 0);
-  __syncthreads();  // Bm visible
-  uint32_t mm[8];  // bf16(mean) | residual << 16
-#pragma unroll
-  for (int tj = 0; tj < 8; ++tj) mm[tj] = Bm[0][wc + 16 * tj + lr] | (Bm[1][wc + 16 * tj + lr] << 16);
-  for (int kt = 0; kt < nk; ++kt) {
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (kt + 1 < nk) issue((kt + 1) & 1, (int64_t)(kt + 1) * V3_BK);
+  for (int kt = 0; kt < nks; ++kt) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of stage kt have landed
+    __syncthreads();                // ... and everyone's; stage kt-1 is free
+    if (kt + 1 < nks) issue((kt + 1) & 1, kt + 1);
     const uint8_t* base = lds + (kt & 1) * V3_STAGE;
     auto gbytes = [&](int tj) -> uint64_t {
       const int j = wc + 16 * tj + lr;
@@ -745,11 +386,7 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_v4(
     };
     bf16x8_t x1[8];
 #pragma unroll
-    for (int tj = 0; tj < 8; ++tj) {
-      bf16x8_t unused;
-      widen8(gbytes(tj), mm[tj] & 0xFFFFu, mm[tj] >> 16, x1[tj], unused);
-    }
-    // planes lo, mid, hi x X1, and hi x X2 (small terms first)
+    for (int tj = 0; tj < 8; ++tj) widen8p(gbytes(tj), mm[tj], x1[tj], nullptr);
 #pragma unroll 1
     for (int pl = 2; pl >= 0; --pl) {
       bf16x8_t af[4];
@@ -763,7 +400,7 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_v4(
 #pragma unroll
         for (int tj = 0; tj < 8; ++tj) {
           bf16x8_t x2, unused;
-          widen8(gbytes(tj), mm[tj] & 0xFFFFu, mm[tj] >> 16, unused, x2);
+          widen8p(gbytes(tj), mm[tj], unused, &x2);
 #pragma unroll
           for (int ti = 0; ti < 4; ++ti)
             acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ti], x2, acc[ti][tj], 0, 0, 0);
@@ -823,51 +460,57 @@ extern "C" int jm_gemm_nt_f64(const double* A, int64_t lda, const double* B, int
   return JM_OK;
 }
 
-extern "C" int jm_split_bf16x3(const double* UT, int64_t ld_u, int64_t n, uint16_t* hi,
-                               uint16_t* mid, uint16_t* lo, int64_t ld_s, void* stream) {
-  JM_CHECK_ARG(n >= 1 && ld_u >= n && ld_s >= n && ld_s % 32 == 0, "bad sizes");
-  JM_CHECK_ARG(UT && hi && mid && lo, "null pointer");
-  hipLaunchKernelGGL(k_split_bf16x3, dim3(2048), dim3(256), 0, (hipStream_t)stream, UT, ld_u, n,
-                     hi, mid, lo, ld_s);
+extern "C" int64_t jm_packed_ut_bytes(int64_t n) {
+  if (n < 1) return 0;
+  return 3 * ((n + 255) / 256) * ((n + 31) / 32) * UTP_BLOCK;
+}
+
+extern "C" int64_t jm_packed_geno_bytes(int64_t n, int64_t n_snps) {
+  if (n < 1 || n_snps < 1) return 0;
+  return ((n_snps + 255) / 256) * ((n + 31) / 32) * GP_BLOCK;
+}
+
+extern "C" int jm_pack_ut_bf16x3(const double* UT, int64_t ld_u, int64_t n, uint16_t* planes,
+                                 void* stream) {
+  JM_CHECK_ARG(n >= 1 && ld_u >= n, "bad sizes");
+  JM_CHECK_ARG(UT && planes && ((uintptr_t)planes % 16) == 0, "null or misaligned pointer");
+  const int64_t npan = (n + 255) / 256, nks = (n + 31) / 32;
+  const int64_t chunks = npan * nks * 1024;
+  const unsigned grid = (unsigned)(chunks < 65536 ? (chunks + 255) / 256 : 256 * 64);
+  hipLaunchKernelGGL(k_pack_ut, dim3(grid), dim3(256), 0, (hipStream_t)stream, UT, ld_u, n, npan,
+                     nks, planes);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }
 
-extern "C" int jm_rotate_bf16x3(const uint16_t* hi, const uint16_t* mid, const uint16_t* lo,
-                                int64_t ld_s, const int8_t* g, int64_t ld_g, const double* mean,
-                                const int32_t* col_idx, int64_t n, int64_t n_snps, float* Z,
-                                int64_t ldz, void* stream) {
+extern "C" int jm_pack_genotypes(const int8_t* g, int64_t ld_g, const int32_t* col_idx,
+                                 const double* mean, int64_t n, int64_t n_snps, uint8_t* gp,
+                                 uint32_t* mm, void* stream) {
+  JM_CHECK_ARG(n >= 1 && n_snps >= 0 && ld_g >= n, "bad sizes");
+  if (n_snps == 0) return JM_OK;
+  JM_CHECK_ARG(g && mean && gp && mm && ((uintptr_t)gp % 16) == 0, "null or misaligned pointer");
+  const int64_t nks = (n + 31) / 32;
+  const int64_t chunks = ((n_snps + 255) / 256) * nks * 512;
+  const unsigned grid = (unsigned)(chunks < 65536 ? (chunks + 255) / 256 : 256 * 64);
+  hipLaunchKernelGGL(k_pack_geno, dim3(grid), dim3(256), 0, (hipStream_t)stream, g, ld_g, col_idx,
+                     mean, n, n_snps, nks, gp, mm);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int jm_rotate_bf16x3(const uint16_t* planes, const uint8_t* gp, const uint32_t* mm,
+                                int64_t n, int64_t n_snps, float* Z, int64_t ldz, void* stream) {
   JM_CHECK_ARG(n >= 1 && n_snps >= 0, "bad sizes");
   if (n_snps == 0) return JM_OK;
-  JM_CHECK_ARG(hi && mid && lo && g && mean && Z, "null pointer");
-  JM_CHECK_ARG(ld_s >= n && ld_s % 32 == 0 && ld_g >= n && ld_g % 16 == 0 && ldz >= n_snps,
-               "bad leading dimension");
-  JM_CHECK_ARG(((uintptr_t)g % 16) == 0 && ((uintptr_t)hi % 16) == 0 &&
-                   ((uintptr_t)mid % 16) == 0 && ((uintptr_t)lo % 16) == 0,
-               "operands must be 16-byte aligned");
-  // tile variant (JM_ROT_BF16_V): 3 = 256x256, DMA-staged, double buffered (default);
-  // 2 = 256x128 / 512 threads with register prefetch; 1 = 128x128 / 256 threads
-  const char* venv = getenv("JM_ROT_BF16_V");
-  const int version = venv ? atoi(venv) : 3;
-  if (version == 4) {
-    JM_CHECK_ARG(ld_g >= ((n + 31) / 32) * 32, "ld_g must be >= n rounded up to 32");
-    const int nrb = ceil_div(n, V3_BM), ncb = ceil_div(n_snps, V3_BN);
-    hipLaunchKernelGGL(k_rotate_bf16x3_v4, dim3(nrb * ncb), dim3(512), 0, (hipStream_t)stream, hi,
-                       mid, lo, ld_s, g, ld_g, mean, col_idx, n, n_snps, n, Z, ldz);
-  } else if (version == 3) {
-    JM_CHECK_ARG(ld_g >= ((n + 31) / 32) * 32, "ld_g must be >= n rounded up to 32");
-    const int nrb = ceil_div(n, V3_BM), ncb = ceil_div(n_snps, V3_BN);
-    hipLaunchKernelGGL(k_rotate_bf16x3_v3, dim3(nrb * ncb), dim3(256), 0, (hipStream_t)stream, hi,
-                       mid, lo, ld_s, g, ld_g, mean, col_idx, n, n_snps, n, Z, ldz);
-  } else if (version == 2) {
-    const int nrb = ceil_div(n, V2_BM), ncb = ceil_div(n_snps, V2_BN);
-    hipLaunchKernelGGL(k_rotate_bf16x3_v2, dim3(nrb * ncb), dim3(512), 0, (hipStream_t)stream, hi,
-                       mid, lo, ld_s, g, ld_g, mean, col_idx, n, n_snps, n, Z, ldz);
-  } else {
-    const int nrb = ceil_div(n, B_BM), ncb = ceil_div(n_snps, B_BN);
-    hipLaunchKernelGGL(k_rotate_bf16x3, dim3(nrb * ncb), dim3(256), 0, (hipStream_t)stream, hi,
-                       mid, lo, ld_s, g, ld_g, mean, col_idx, n, n_snps, n, Z, ldz);
-  }
+  JM_CHECK_ARG(planes && gp && mm && Z, "null pointer");
+  JM_CHECK_ARG(ldz >= n_snps, "ldz < n_snps");
+  JM_CHECK_ARG(((uintptr_t)planes % 16) == 0 && ((uintptr_t)gp % 16) == 0,
+               "packed operands must be 16-byte aligned");
+  const int64_t nks = (n + 31) / 32;
+  const int64_t per_plane = jm_packed_ut_bytes(n) / 6;  // bf16 elements per plane
+  const int nrb = ceil_div(n, V3_BM), ncb = ceil_div(n_snps, V3_BN);
+  hipLaunchKernelGGL(k_rotate_bf16x3_packed, dim3(nrb * ncb), dim3(512), 0, (hipStream_t)stream,
+                     planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

@@ -44,13 +44,16 @@ class KinshipAccumulator:
         if n_snps == 0:
             return
         n = self.n
-        ldt = round_up(n_snps, 64)
+        ldt = round_up(n_snps, 128)
         gT = torch.empty((n, ldt), dtype=torch.int8, device=self.device)
+        gT0 = torch.empty((n, ldt), dtype=torch.int8, device=self.device)
         if ldt > n_snps:
             gT[:, n_snps:].zero_()
+            gT0[:, n_snps:].zero_()
         st = stream_ptr()
-        _lib.call("jm_transpose_i8", ptr(g), n_snps, n, ld_g, ptr(gT), ldt, st)
-        _lib.call("jm_kinship_syrk_i8", ptr(gT), ldt, n, n_snps, ptr(self.S), st)
+        _lib.call("jm_transpose_i8", ptr(g), n_snps, n, ld_g, ptr(gT), ldt, ptr(gT0), st)
+        _lib.call("jm_kinship_syrk_i8", ptr(gT0), ldt, n, n_snps, ptr(self.S), st)
+        del gT0
         mu_d = to_dev(np.asarray(mu, dtype=np.float64), torch.float64, self.device)
         if has_missing and self.C is None:
             self.C = torch.zeros((n, n), dtype=torch.float64, device=self.device)

@@ -81,11 +81,10 @@ class Cohort:
         self.precision = precision
         self.split = None
         if precision in ("mixed", "both"):
-            self.ld_s = round_up(n, 64)
-            planes = torch.empty((3, n, self.ld_s), dtype=torch.int16, device=self.device)
-            _lib.call("jm_split_bf16x3", ptr(self.UT), self.ld_u, n, ptr(planes[0]), ptr(planes[1]),
-                      ptr(planes[2]), self.ld_s, stream_ptr())
-            self.split = planes
+            nbytes = int(_lib.lib.jm_packed_ut_bytes(n))
+            planes = torch.empty((nbytes // 2,), dtype=torch.int16, device=self.device)
+            _lib.call("jm_pack_ut_bf16x3", ptr(self.UT), self.ld_u, n, ptr(planes), stream_ptr())
+            self.split = planes  # packed hi / mid / lo bf16 planes
         p = self.plan
         dev = self.device
         self.cl_start = to_dev(p.cl_start, torch.int32, dev)
@@ -133,9 +132,15 @@ class Cohort:
             if self.split is None:
                 raise ValueError("cohort was built without the bf16 split (precision='f64')")
             Z = torch.empty((self.n, ldz), dtype=torch.float32, device=self.device)
-            _lib.call("jm_rotate_bf16x3", ptr(self.split[0]), ptr(self.split[1]),
-                      ptr(self.split[2]), self.ld_s, ptr(g), ld_g, ptr(means), ptr(col_idx),
-                      self.n, n_snps, ptr(Z), ldz, stream_ptr())
+            if n_snps:
+                gbytes = int(_lib.lib.jm_packed_geno_bytes(self.n, n_snps))
+                gp = torch.empty((gbytes,), dtype=torch.uint8, device=self.device)
+                mm = torch.empty((n_snps,), dtype=torch.int32, device=self.device)
+                st = stream_ptr()
+                _lib.call("jm_pack_genotypes", ptr(g), ld_g, ptr(col_idx), ptr(means), self.n,
+                          n_snps, ptr(gp), ptr(mm), st)
+                _lib.call("jm_rotate_bf16x3", ptr(self.split), ptr(gp), ptr(mm), self.n, n_snps,
+                          ptr(Z), ldz, st)
             return Z, True, ldz
         raise ValueError(f"unknown precision {precision!r}")
 

@@ -87,10 +87,8 @@ def test_loco_kinship_matches_oracle():
 
 
 # ------------------------------------------------------------------ rotation
-@pytest.mark.parametrize("variant", ["1", "2", "3", "4"])
-@pytest.mark.parametrize("n,p", [(100, 37), (333, 700), (520, 1100)])
-def test_rotation_f64_and_mixed(n, p, variant, monkeypatch):
-    monkeypatch.setenv("JM_ROT_BF16_V", variant)
+@pytest.mark.parametrize("n,p", [(100, 37), (333, 700), (520, 1100), (1000, 257)])
+def test_rotation_f64_and_mixed(n, p):
     from jamma_amd.lmm.engine import Cohort
     from jamma_amd.device import round_up, to_dev
 
@@ -112,6 +110,13 @@ def test_rotation_f64_and_mixed(n, p, variant, monkeypatch):
     Z32, _, _ = co.rotate_genotypes(gd, ld, md, p, "mixed")
     err = np.abs(Z32.cpu().numpy().astype(np.float64) - ref)
     assert err.max() < 1e-5 * np.abs(ref).max()
+    # gathered columns (serving batches): repeated and permuted SNP rows
+    idx = rng.integers(0, p, size=p + 45).astype(np.int32)
+    cd = to_dev(idx, torch.int32)
+    for prec, tol in (("f64", 1e-12), ("mixed", 1e-5)):
+        Zg, _, _ = co.rotate_genotypes(gd, ld, md, len(idx), prec, col_idx=cd)
+        assert np.max(np.abs(Zg.cpu().numpy().astype(np.float64) - ref[:, idx])) < \
+            tol * np.abs(ref).max(), prec
 
 
 # ------------------------------------------------------------------ association

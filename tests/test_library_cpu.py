@@ -22,7 +22,7 @@ from tests.util import GOLDEN
 
 def test_library_exports_every_declared_symbol():
     header = (ROOT / "include" / "jamma_hip.h").read_text()
-    declared = set(re.findall(r"^(?:int|double)\s+(jm_\w+)\s*\(", header, flags=re.M))
+    declared = set(re.findall(r"^(?:int|double|int64_t)\s+(jm_\w+)\s*\(", header, flags=re.M))
     assert declared, "no declarations parsed"
     for name in declared:
         assert hasattr(_lib.lib, name), name
