@@ -76,7 +76,8 @@ def main() -> int:
     ap.add_argument("--mode", type=int, default=4)
     ap.add_argument("--tile", type=int, default=8192)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=2048,
+                    help="SNPs of the CPU baseline sample (about 15 s of host work at n=10k)")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 path (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--workload", default="precision", choices=["precision", "serving", "copy"],
@@ -195,11 +196,12 @@ def main() -> int:
         for s, e in tiles:
             evs = None
             if timers is not None:
-                evs = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+                evs = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
             co = cohort
             if evs:
                 evs[0].record()
-            Z, f32, ldz = co.rotate_genotypes(gf[s:e], ld, means[s:e], e - s, precision)
+            Z, f32, ldz = co.rotate_genotypes(gf[s:e], ld, means[s:e], e - s, precision,
+                                              timer=(evs[4], evs[5]) if evs else None)
             if evs:
                 evs[1].record()
             xm = co.moments(Z, f32, ldz, e - s, prob.V, prob.n_cvt + 1)
@@ -223,14 +225,14 @@ def main() -> int:
             outs = run_step(precision, timers if k == 0 else None)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
-        kern = np.array([[e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2]), e[2].elapsed_time(e[3])]
-                         for e in timers])
+        kern = np.array([[e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2]), e[2].elapsed_time(e[3]),
+                          e[4].elapsed_time(e[5])] for e in timers])
         out_all[precision] = torch.cat(outs).cpu().numpy()
         results[precision] = {"s_per_step": dt, "snps_per_s": pf / dt,
                               "kernel_ms_per_tile": kern.mean(axis=0).tolist(),
                               "kernel_ms_total": kern.sum(axis=0).tolist()}
         log(f"{precision}: {dt * 1e3:.1f} ms/step = {pf / dt:,.0f} SNPs/s; per-tile ms "
-            f"rotate/moments/assoc = {np.round(kern.mean(axis=0), 3).tolist()}")
+            f"rotate(+pack)/moments/assoc/rotation kernel = {np.round(kern.mean(axis=0), 3).tolist()}")
 
     # ---- parity of the two device paths ----
     parity = {}
@@ -258,8 +260,11 @@ def main() -> int:
         lam0, ll0 = ol.null_model_mle(d, UtW, Uty, W.shape[1])
         Hi0 = 1.0 / (lam0 * d + 1.0)
         t0 = time.perf_counter()
-        Utx = UT @ Xs
-        ref = ol.assoc_chunk(4, W.shape[1], d, UtW, Uty, Utx, Hi0, ll0)
+        parts = []
+        for c0 in range(0, len(sample), 256):  # bounded memory per oracle call
+            Utx = UT @ Xs[:, c0:c0 + 256]
+            parts.append(ol.assoc_chunk(4, W.shape[1], d, UtW, Uty, Utx, Hi0, ll0))
+        ref = {k: np.concatenate([r[k] for r in parts]) for k in parts[0]}
         cpu_s = time.perf_counter() - t0
         cores = len(os.sched_getaffinity(0))
         cpu = {"value": args.cpu_sample / cpu_s, "unit": "SNPs/s", "cores": cores, "kind": "port",
@@ -274,9 +279,10 @@ def main() -> int:
         log(f"cpu baseline {cpu['value']:.1f} SNPs/s on {cores} threads")
 
     mixed = results["mixed"]
-    rot_ms = mixed["kernel_ms_per_tile"][0]
+    rot_ms = mixed["kernel_ms_per_tile"][3]  # the rotation kernel alone
     tile_snps = pf / len(tiles)
     achieved = 2.0 * n * n * tile_snps / (rot_ms * 1e-3)
+    traffic = hbm_traffic("k_rotate_bf16x3_packed", n, args.tile)
     line = {
         "metric": METRIC,
         "value": mixed["snps_per_s"],
@@ -295,8 +301,12 @@ def main() -> int:
                                f"{args.ncov} covariates + intercept, -lmm {args.mode}",
                    "n": n, "p": p, "n_cvt": args.ncov + 1, "tile_snps": args.tile},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["bf16"] / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK["bf16"], "traffic": None,
-                     "kernel": "k_rotate_bf16x3_v2", "algorithmic": "2 n^2 FLOP per SNP"},
+                     "unit": "TFLOP/s", "frac": achieved / PEAK["bf16"], "traffic": traffic,
+                     "kernel": "k_rotate_bf16x3_packed",
+                     "algorithmic": "2 n^2 FLOP per SNP (U^T x); the BF16x3 split issues 4 bf16 "
+                                    "MFMA products per algorithmic FLOP",
+                     "mfma_issue_frac": 4.0 * achieved / PEAK["bf16"],
+                     "launch_ms": rot_ms, "snps_per_launch": tile_snps},
         "cpu_baseline": cpu,
         "fp64_path": results.get("f64"),
         "mixed_path": mixed,
@@ -312,6 +322,21 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def hbm_traffic(kernel: str, n: int, tile: int):
+    """HBM bytes per launch of `kernel` at (n, tile) from the committed rocprofv3 counter
+    summary (profiles/hbm_traffic.txt: FETCH_SIZE x fetch scale + WRITE_SIZE x write scale,
+    the scales measured on a copy kernel of known size). None when not measured."""
+    path = Path(__file__).resolve().parent / "profiles" / "hbm_traffic.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        f = line.split()
+        if len(f) >= 5 and not line.startswith("#") and f[0] == kernel and \
+                int(f[1]) == n and int(f[2]) == tile:
+            return float(f[3]) + float(f[4])
+    return None
 
 
 def copy_calibration() -> int:

@@ -161,22 +161,50 @@ int jm_rotate_bf16x3(const uint16_t* planes, const uint8_t* gp, const uint32_t* 
 /* Per-SNP REML/MLE lambda search + Wald/LRT/Score                           */
 /* ------------------------------------------------------------------------ */
 
+/* Device description of a compressed spectrum (host struct of device pointers).
+ * Cluster c (sorted eigen-indices, centre d_c) owns the terms toff_c ..
+ * toff_c + M_c - 1 and is cut into pieces (segments) of at most a fixed number
+ * of eigen-indices; piece q covers [seg_start, seg_start + seg_len) of cluster
+ * seg_cluster and its partial moments occupy slots seg_slot .. seg_slot + M - 1
+ * of the workspace. */
+typedef struct jm_spectrum_plan {
+  int32_t n;          /* eigenvalues */
+  int32_t n_terms;    /* T */
+  int32_t n_clusters; /* C */
+  int32_t n_seg;      /* pieces */
+  int32_t n_slots;    /* sum over pieces of M of their cluster */
+  int32_t reserved;
+  const int32_t* cl_terms;    /* C : M_c */
+  const int32_t* cl_toff;     /* C : first term of c */
+  const int32_t* cl_seg0;     /* C : first piece of c */
+  const int32_t* cl_nseg;     /* C : number of pieces of c */
+  const int32_t* seg_start;   /* n_seg */
+  const int32_t* seg_len;     /* n_seg */
+  const int32_t* seg_cluster; /* n_seg */
+  const int32_t* seg_slot;    /* n_seg */
+  const double* delta;        /* n : d_i - d_c(i) */
+  const int32_t* term_cluster; /* T */
+  const int32_t* term_m;       /* T */
+} jm_spectrum_plan;
+
+/* Workspace (bytes) of jm_spectrum_moments for n_snps columns and nv partner
+ * rows (jm_spectrum_moments_multi: nv = nv_max). */
+int64_t jm_moments_work_bytes(const jm_spectrum_plan* plan, int64_t n_snps, int nv);
+
 /*
- * Spectrum moments. For each column s of Z (float64 when z_is_f32 == 0, else
- * float32) and each term t = (cluster c, power m) of the compressed spectrum:
+ * Spectrum moments. For each column s of Z (n x ldz; float64 when z_is_f32 ==
+ * 0, else float32) and each term t = (cluster c, power m) of the plan:
  *   mom[s][b][t] = sum_{i in c} Z[i][s] * V[b][i] * delta_i^m   (b < nv)
  *   mom[s][nv][t] = sum_{i in c} Z[i][s]^2 * delta_i^m
  * V (nv x ldv float64) are the partner vectors (rotated covariates, then the
- * rotated phenotype). Cluster c covers eigen-indices [cl_start, cl_start +
- * cl_size), has cl_terms powers starting at term cl_toff; delta (n) = d_i -
- * d_c. mom is (n_snps x (nv+1) x ldm). This is the lambda-independent part of
- * batch_compute_uab + calc_pab row 0 (likelihood_jax.py:112-186, 342-393).
+ * rotated phenotype); mom is (n_snps x (nv+1) x ldm); work holds
+ * jm_moments_work_bytes(plan, n_snps, nv) bytes of per-piece partials. This is
+ * the lambda-independent part of batch_compute_uab + calc_pab row 0
+ * (likelihood_jax.py:112-186, 342-393).
  */
-int jm_spectrum_moments(const void* Z, int z_is_f32, int64_t ldz, int64_t n, int64_t n_snps,
-                        const double* V, int64_t ldv, int nv, const int32_t* cl_start,
-                        const int32_t* cl_size, const int32_t* cl_terms,
-                        const int32_t* cl_toff, int n_clusters, const double* delta,
-                        double* mom, int64_t ldm, void* stream);
+int jm_spectrum_moments(const void* Z, int z_is_f32, int64_t ldz, int64_t n_snps,
+                        const double* V, int64_t ldv, int nv, const jm_spectrum_plan* plan,
+                        double* work, double* mom, int64_t ldm, void* stream);
 
 /* Per-column descriptor of jm_spectrum_moments_multi: the partner rows
  * (nv x ldv float64) and the output block ((nv+1) x ldm float64) of one column. */
@@ -192,11 +220,9 @@ typedef struct jm_mom_col {
 /* jm_spectrum_moments with a per-column descriptor (device array cols[n_snps]),
  * so columns of different requests (phenotypes / covariate sets) on the same
  * cohort share one launch. nv_max = max nv over the columns. */
-int jm_spectrum_moments_multi(const void* Z, int z_is_f32, int64_t ldz, int64_t n, int64_t n_snps,
-                              const jm_mom_col* cols, int nv_max, const int32_t* cl_start,
-                              const int32_t* cl_size, const int32_t* cl_terms,
-                              const int32_t* cl_toff, int n_clusters, const double* delta,
-                              void* stream);
+int jm_spectrum_moments_multi(const void* Z, int z_is_f32, int64_t ldz, int64_t n_snps,
+                              const jm_mom_col* cols, int nv_max, const jm_spectrum_plan* plan,
+                              double* work, void* stream);
 
 /* Parameters of one (cohort, phenotype) association problem. Device pointers. */
 typedef struct jm_lmm_problem {

@@ -45,6 +45,19 @@ class LmmProblem(ctypes.Structure):
     ]
 
 
+class SpectrumPlanC(ctypes.Structure):
+    """Mirror of jm_spectrum_plan."""
+
+    _fields_ = [
+        ("n", c_int32), ("n_terms", c_int32), ("n_clusters", c_int32), ("n_seg", c_int32),
+        ("n_slots", c_int32), ("reserved", c_int32),
+        ("cl_terms", c_void_p), ("cl_toff", c_void_p), ("cl_seg0", c_void_p), ("cl_nseg", c_void_p),
+        ("seg_start", c_void_p), ("seg_len", c_void_p), ("seg_cluster", c_void_p),
+        ("seg_slot", c_void_p), ("delta", c_void_p), ("term_cluster", c_void_p),
+        ("term_m", c_void_p),
+    ]
+
+
 _SIGS = {
     "jm_version": ([], c_int),
     "jm_last_error": ([ctypes.c_char_p, c_size_t], c_int),
@@ -68,12 +81,11 @@ _SIGS = {
                            c_void_p, c_void_p], c_int),
     "jm_rotate_bf16x3": ([c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_int64,
                           c_void_p], c_int),
-    "jm_spectrum_moments": ([c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int,
-                             c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
-                             c_int64, c_void_p], c_int),
-    "jm_spectrum_moments_multi": ([c_void_p, c_int, c_int64, c_int64, c_int64, c_void_p, c_int,
-                                   c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
-                                   c_void_p], c_int),
+    "jm_moments_work_bytes": ([c_void_p, c_int64, c_int], c_int64),
+    "jm_spectrum_moments": ([c_void_p, c_int, c_int64, c_int64, c_void_p, c_int64, c_int, c_void_p,
+                             c_void_p, c_void_p, c_int64, c_void_p], c_int),
+    "jm_spectrum_moments_multi": ([c_void_p, c_int, c_int64, c_int64, c_void_p, c_int, c_void_p,
+                                   c_void_p, c_void_p], c_int),
     "jm_lmm_assoc_multi": ([c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64,
                             c_int64, c_void_p, c_void_p], c_int),
     "jm_lmm_null": ([POINTER(LmmProblem), c_void_p, c_void_p], c_int),

@@ -444,6 +444,37 @@ __device__ __forceinline__ double bcast(double v, int src) {
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+// Value of the butterfly partner for lane bit BIT, without going through
+// LDS: bits 0/1 quad_perm DPP (lane ^ 1, lane ^ 2), bit 2 row_half_mirror
+// (i <-> 7 - i), bit 3 row_mirror (i <-> 15 - i), bits 4/5 the gfx950
+// permlane16 / permlane32 swaps (lane ^ 16, lane ^ 32). Every partner differs
+// from the lane in bit BIT and agrees on the higher bits, which is all a
+// butterfly (and the index bookkeeping of the reduce-scatter) needs.
+template <int BIT>
+__device__ __forceinline__ uint32_t xpartner32(uint32_t v, int lane) {
+  if constexpr (BIT == 0) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+  if constexpr (BIT == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+  if constexpr (BIT == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);
+  if constexpr (BIT == 3) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);
+  if constexpr (BIT == 4) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane & 16) ? r[0] : r[1];
+  }
+  if constexpr (BIT == 5) {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane & 32) ? r[0] : r[1];
+  }
+  return v;
+}
+
+template <int BIT>
+__device__ __forceinline__ double xpartner(double v, int lane) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = xpartner32<BIT>((uint32_t)u, lane);
+  const uint32_t hi = xpartner32<BIT>((uint32_t)(u >> 32), lane);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // Butterfly reduce-scatter of 8 per-lane values: on return every lane holds
 // the full wave sum of value q = 4*b5 + 2*b4 + b3 (10 exchanges).
 __device__ __forceinline__ double reduce8(double (&v)[8], int lane) {
@@ -453,7 +484,7 @@ __device__ __forceinline__ double reduce8(double (&v)[8], int lane) {
     for (int i = 0; i < 4; ++i) {
       const double send = up ? v[i] : v[i + 4];
       const double keep = up ? v[i + 4] : v[i];
-      v[i] = keep + shfl_xor_d(send, 32);
+      v[i] = keep + xpartner<5>(send, lane);
     }
   }
   {
@@ -462,25 +493,28 @@ __device__ __forceinline__ double reduce8(double (&v)[8], int lane) {
     for (int i = 0; i < 2; ++i) {
       const double send = up ? v[i] : v[i + 2];
       const double keep = up ? v[i + 2] : v[i];
-      v[i] = keep + shfl_xor_d(send, 16);
+      v[i] = keep + xpartner<4>(send, lane);
     }
   }
   {
     const bool up = lane & 8;
     const double send = up ? v[0] : v[1];
     const double keep = up ? v[1] : v[0];
-    v[0] = keep + shfl_xor_d(send, 8);
+    v[0] = keep + xpartner<3>(send, lane);
   }
-  v[0] = v[0] + shfl_xor_d(v[0], 4);
-  v[0] = v[0] + shfl_xor_d(v[0], 2);
-  v[0] = v[0] + shfl_xor_d(v[0], 1);
+  v[0] = v[0] + xpartner<2>(v[0], lane);
+  v[0] = v[0] + xpartner<1>(v[0], lane);
+  v[0] = v[0] + xpartner<0>(v[0], lane);
   return v[0];
 }
 
 // Sum over the 32 lanes of a half-wave (both halves at once).
-__device__ __forceinline__ double half_sum(double v) {
-#pragma unroll
-  for (int m = 16; m >= 1; m >>= 1) v += shfl_xor_d(v, m);
+__device__ __forceinline__ double half_sum(double v, int lane) {
+  v += xpartner<4>(v, lane);
+  v += xpartner<3>(v, lane);
+  v += xpartner<2>(v, lane);
+  v += xpartner<1>(v, lane);
+  v += xpartner<0>(v, lane);
   return v;
 }
 
@@ -503,6 +537,25 @@ __device__ __forceinline__ int xpair_index(int r, int nc) {
   if (r < nc) return ab_index(r + 1, nc + 1, nc);
   if (r == nc) return ab_index(nc + 1, nc + 2, nc);
   return ab_index(nc + 1, nc + 1, nc);
+}
+
+// Packed index of the k-th pair (a <= b) over the covariate / phenotype indices
+// {1..c, c+2} (the pairs that do not involve the genotype), in packed order.
+__device__ __forceinline__ int shared_pair_index(int k, int nc) {
+  int a = 1;
+  for (;; ++a) {
+    if (a == nc + 1) continue;
+    const int len = (nc + 2 - a) + 1 - (a <= nc ? 1 : 0);
+    if (k < len) break;
+    k -= len;
+  }
+  int b = a;
+  for (;; ++b) {
+    if (b == nc + 1) continue;
+    if (k == 0) break;
+    --k;
+  }
+  return ab_index(a, b, nc);
 }
 
 // Per-lane term data: lane l owns the KT contiguous terms l*KT .. l*KT+KT-1.
@@ -540,13 +593,26 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
                                                    const double* xmom, int64_t ldm, int64_t n_snps,
                                                    double* out, const jm_lmm_problem* probs,
                                                    const int32_t* modes, const int32_t* snp_prob,
-                                                   const double* const* snp_xmom, int ni_max) {
+                                                   const double* const* snp_xmom, int ni_max,
+                                                   int srow_n) {
   extern __shared__ double smem[];
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
   const int per_wave = ni_max * 64 + kMaxLevels + ni_max + 1;
-  double* P = smem + (size_t)wib * per_wave;  // grid columns / two packed rows
+  // srow_n > 0 (single problem): the block keeps the srow_n shared pair rows
+  // (covariate / phenotype moments) in LDS, in slot order, ahead of the waves
+  double* srows = smem;
+  const size_t srow_len = MULTI ? 0 : (size_t)srow_n * pr_one.n_terms;
+  if (!MULTI && srow_n > 0) {
+    const int T1 = pr_one.n_terms, nc1 = pr_one.n_cvt;
+    for (size_t idx = threadIdx.x; idx < srow_len; idx += blockDim.x) {
+      const int k = (int)(idx / T1), t = (int)(idx % T1);
+      srows[idx] = pr_one.shared_mom[(int64_t)shared_pair_index(k, nc1) * T1 + t];
+    }
+    __syncthreads();
+  }
+  double* P = smem + srow_len + (size_t)wib * per_wave;  // grid columns / two packed rows
   double* iab_ld = P + ni_max * 64;           // kMaxLevels
   int* slot = (int*)(iab_ld + kMaxLevels);    // memory-phase slots (<= ni_max + 1 ints)
   double* PA = P;                              // row A: ni entries + logdet slot
@@ -587,21 +653,8 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
       int j;
       if (u < nxm) {
         j = xpair_index(XR + u, nc);
-      } else {  // (u - nxm)-th pair (a <= b) over the indices {1..c, c+2}
-        int k = u - nxm, a = 1;
-        for (;; ++a) {
-          if (a == nc + 1) continue;
-          const int len = (nc + 2 - a) + 1 - (a <= nc ? 1 : 0);
-          if (k < len) break;
-          k -= len;
-        }
-        int b = a;
-        for (;; ++b) {
-          if (b == nc + 1) continue;
-          if (k == 0) break;
-          --k;
-        }
-        j = ab_index(a, b, nc);
+      } else {
+        j = shared_pair_index(u - nxm, nc);
       }
       slot[u] = j;
     }
@@ -747,7 +800,16 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const int u = u0 + e;
-            if (u < n_mem) {
+            if (u < n_mem && u >= nxm && srow_len) {  // shared row from LDS
+              const double* row = srows + (size_t)(u - nxm) * T;
+#pragma unroll
+              for (int k = 0; k < KT; ++k) {
+                const int t = tb * TB + lane * KT + k;
+                const double x = t < T ? row[t] : 0.0;
+                v[2 * e] = fma(kA[k], x, v[2 * e]);
+                v[2 * e + 1] = fma(kB[k], x, v[2 * e + 1]);
+              }
+            } else if (u < n_mem) {
               const double* row = (u < nxm) ? xm + (int64_t)(XR + u) * ldm
                                             : pr.shared_mom + (int64_t)slot[u] * T;
 #pragma unroll
@@ -824,7 +886,7 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
         if (c0 + per >= n_cl) break;
       }
       wave_sync();
-      const double S = half_sum(contrib);
+      const double S = half_sum(contrib, lane);
       const double fA = cr - 0.5 * (bcast(S, 0) + PA[ni]) - 0.5 * df * bcast(lgp, nc + 1);
       const double fB = cm - 0.5 * (bcast(S, 32) + PB[ni]) - 0.5 * nn * bcast(lgp, 32);
       const double pyyA = bcast(pyy_lvl, 0);
@@ -1002,16 +1064,17 @@ template <int KT, int XR, bool MB>
 static void launch_kt(const jm_lmm_problem* one, int mode_one, const double* xmom, int64_t ldm,
                       int64_t n_snps, double* out, const jm_lmm_problem* probs,
                       const int32_t* modes, const int32_t* snp_prob, const double* const* snp_xmom,
-                      int ni_max, unsigned blocks, int wpb, size_t lds, hipStream_t st) {
+                      int ni_max, unsigned blocks, int wpb, size_t lds, int srow_n,
+                      hipStream_t st) {
   if (one) {
     hipLaunchKernelGGL((k_lmm_assoc<KT, XR, MB, false>), dim3(blocks), dim3(64 * wpb), lds, st, *one,
                        mode_one, xmom, ldm, n_snps, out, nullptr, nullptr, nullptr, nullptr,
-                       ni_max);
+                       ni_max, srow_n);
   } else {
     jm_lmm_problem dummy;
     memset(&dummy, 0, sizeof(dummy));
     hipLaunchKernelGGL((k_lmm_assoc<KT, XR, MB, true>), dim3(blocks), dim3(64 * wpb), lds, st, dummy,
-                       0, nullptr, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom, ni_max);
+                       0, nullptr, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom, ni_max, 0);
   }
 }
 
@@ -1022,10 +1085,15 @@ static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* x
   JM_CHECK_ARG(T >= 1, "n_terms=%d < 1", T);
   const int ni_max = (max_cvt + 3) * (max_cvt + 2) / 2;
   const size_t per_wave = (size_t)(ni_max * 64 + kMaxLevels + ni_max + 1) * sizeof(double);
-  int wpb = (int)((64 * 1024) / per_wave);
+  // single problem: the shared pair rows go to LDS when they take <= 48 KB
+  const int n_sh = (max_cvt + 1) * (max_cvt + 2) / 2;
+  const size_t sh_bytes = (size_t)n_sh * T * sizeof(double);
+  const int srow_n = (one && sh_bytes <= 48 * 1024) ? n_sh : 0;
+  const size_t sh = srow_n ? sh_bytes : 0;
+  int wpb = (int)((80 * 1024 - sh) / per_wave);
   if (wpb > 4) wpb = 4;
   if (wpb < 1) {
-    JM_CHECK_ARG(per_wave <= 160 * 1024, "problem too large for the LDS budget (n_cvt=%d)",
+    JM_CHECK_ARG(sh + per_wave <= 160 * 1024, "problem too large for the LDS budget (n_cvt=%d)",
                  max_cvt);
     wpb = 1;
   }
@@ -1033,30 +1101,30 @@ static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* x
   const int64_t max_blocks = 256 * 16;
   if (blocks > max_blocks) blocks = max_blocks;
   const unsigned nb = (unsigned)blocks;
-  const size_t lds = per_wave * wpb;
+  const size_t lds = sh + per_wave * wpb;
   hipStream_t st = (hipStream_t)stream;
   const int kt = (T + 63) / 64;  // > 8: blocks of 512 terms
   if (kt <= 1)
     launch_kt<1, 8, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, st);
+                    ni_max, nb, wpb, lds, srow_n, st);
   else if (kt <= 2)
     launch_kt<2, 8, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, st);
+                    ni_max, nb, wpb, lds, srow_n, st);
   else if (kt <= 3)
     launch_kt<3, 8, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, st);
+                    ni_max, nb, wpb, lds, srow_n, st);
   else if (kt <= 4)
     launch_kt<4, 8, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, st);
+                    ni_max, nb, wpb, lds, srow_n, st);
   else if (kt <= 6)
     launch_kt<6, 4, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, st);
+                    ni_max, nb, wpb, lds, srow_n, st);
   else if (kt <= 8)
     launch_kt<8, 4, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, st);
+                    ni_max, nb, wpb, lds, srow_n, st);
   else
     launch_kt<8, 4, true>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, st);
+                    ni_max, nb, wpb, lds, srow_n, st);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

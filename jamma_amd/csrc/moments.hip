@@ -10,10 +10,10 @@
 // is then sum_t kappa_t(lambda) * mom[.][.][t] (see lmm.hip).
 //
 // Layout: Z is (n x ldz), SNP s contiguous within a row, so one thread per
-// SNP reads coalesced rows while walking i; accumulators for the current
-// cluster stay in registers and are flushed at the cluster end. Clusters are
-// split into index-range groups across grid.y for parallelism; the summation
-// order inside a cluster is fixed, so results are bitwise reproducible.
+// SNP reads coalesced rows while walking i. Clusters are cut into pieces of at
+// most a fixed number of eigen-indices (jm_spectrum_plan); k_mom_seg writes
+// one partial per piece and k_mom_fold adds the pieces of each cluster in
+// order, so the work is balanced and the results are bitwise reproducible.
 
 #include "jm_common.h"
 
@@ -23,102 +23,115 @@ constexpr int kMomMaxTerms = 16;  // MAX_TERMS_PER_CLUSTER on the host side
 constexpr int kMomGroup = 6;      // partner rows accumulated per pass
 constexpr int kMomTerms = kMomMaxTerms;
 
+// Partial moments of one cluster piece (grid.y) for 64 SNP columns (grid.x):
+//   work[s][b][seg_slot + m] = sum_{i in piece} Z[i][s] V[b][i] delta_i^m
+// (row b == nv: Z[i][s]^2). Pieces hold at most a fixed number of
+// eigen-indices, so the blocks carry balanced work however the spectrum
+// clusters. The summation order inside a piece is fixed.
 template <typename ZT, bool MULTI>
-__global__ void __launch_bounds__(64) k_moments(const ZT* __restrict__ Z, int64_t ldz, int64_t n,
+__global__ void __launch_bounds__(64) k_mom_seg(const ZT* __restrict__ Z, int64_t ldz,
                                                 int64_t n_snps, const double* __restrict__ V_one,
                                                 int64_t ldv_one, int nv_one,
                                                 const jm_mom_col* __restrict__ cols, int nv_max,
-                                                const int32_t* __restrict__ cl_start,
-                                                const int32_t* __restrict__ cl_size,
-                                                const int32_t* __restrict__ cl_terms,
-                                                const int32_t* __restrict__ cl_toff,
-                                                int n_clusters,
-                                                const double* __restrict__ delta,
-                                                double* __restrict__ mom_one, int64_t ldm_one) {
+                                                jm_spectrum_plan pl, double* __restrict__ work) {
   const int64_t s = (int64_t)blockIdx.x * 64 + threadIdx.x;
   const bool active = s < n_snps;
   const int64_t sc = active ? s : 0;
-  // this column's partner rows and output block
+  const int seg = blockIdx.y;
   const double* V = V_one;
-  int64_t ldv = ldv_one, ldm = ldm_one;
+  int64_t ldv = ldv_one;
   int nv = nv_one;
-  double* ms;
   if (MULTI) {
     const jm_mom_col d = cols[sc];
     V = d.V;
     ldv = d.ldv;
     nv = d.nv;
-    ldm = d.ldm;
-    ms = d.out;
-  } else {
-    ms = mom_one + sc * (int64_t)(nv_one + 1) * ldm_one;
   }
-  // cluster range of this group: clusters whose start lies in [i_lo, i_hi)
-  const int64_t i_lo = n * blockIdx.y / gridDim.y;
-  const int64_t i_hi = n * (blockIdx.y + 1) / gridDim.y;
-  int c0 = 0, c1 = n_clusters;
-  {  // first cluster with start >= i_lo
-    int lo = 0, hi = n_clusters;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (cl_start[mid] < i_lo) lo = mid + 1; else hi = mid;
-    }
-    c0 = lo;
-    lo = c0;
-    hi = n_clusters;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (cl_start[mid] < i_hi) lo = mid + 1; else hi = mid;
-    }
-    c1 = lo;
-  }
-  const int nrows = nv + 1;
-  const int nrows_max = (MULTI ? nv_max : nv_one) + 1;
-  for (int g0 = 0; g0 < nrows_max; g0 += kMomGroup) {
-    int ng = nrows - g0;
-    ng = ng < 0 ? 0 : (ng > kMomGroup ? kMomGroup : ng);
-    for (int c = c0; c < c1; ++c) {
-      const int i0 = cl_start[c];
-      const int sz = cl_size[c];
-      const int M = cl_terms[c];
-      double acc[kMomGroup][kMomTerms];
+  const int nvm = MULTI ? nv_max : nv_one;
+  const int i0 = pl.seg_start[seg], len = pl.seg_len[seg];
+  const int M = pl.cl_terms[pl.seg_cluster[seg]];
+  const int slot0 = pl.seg_slot[seg];
+  __shared__ double tr[kMomTerms * 65];  // term x column, padded
+  for (int g0 = 0; g0 <= nvm; g0 += kMomGroup) {
+    // rows of this group present in every column of the block (MULTI: nv
+    // differs per column; missing rows are written as 0 and never read)
+    int ng = nvm + 1 - g0;
+    ng = ng > kMomGroup ? kMomGroup : ng;
+    double acc[kMomGroup][kMomTerms];
 #pragma unroll
-      for (int b = 0; b < kMomGroup; ++b)
+    for (int b = 0; b < kMomGroup; ++b)
 #pragma unroll
-        for (int m = 0; m < kMomTerms; ++m) acc[b][m] = 0.0;
-      for (int i = i0; i < i0 + sz; ++i) {
-        const double z = (double)Z[(int64_t)i * ldz + sc];
-        const double dl = delta[i];
-        double prod[kMomGroup];
+      for (int m = 0; m < kMomTerms; ++m) acc[b][m] = 0.0;
+    for (int i = i0; i < i0 + len; ++i) {
+      const double z = (double)Z[(int64_t)i * ldz + sc];
+      const double dl = pl.delta[i];
+      double prod[kMomGroup];
 #pragma unroll
-        for (int b = 0; b < kMomGroup; ++b) {
-          const int row = g0 + b;
-          prod[b] = 0.0;
-          if (b < ng) prod[b] = (row < nv) ? z * V[(int64_t)row * ldv + i] : z * z;
-        }
-        double pw = 1.0;
-#pragma unroll
-        for (int m = 0; m < kMomTerms; ++m) {
-          if (m < M) {
-#pragma unroll
-            for (int b = 0; b < kMomGroup; ++b) acc[b][m] += prod[b] * pw;
-            pw *= dl;
-          }
-        }
+      for (int b = 0; b < kMomGroup; ++b) {
+        const int row = g0 + b;
+        prod[b] = 0.0;
+        if (b < ng && row <= nv) prod[b] = (row < nv) ? z * V[(int64_t)row * ldv + i] : z * z;
       }
-      if (active) {
-        const int toff = cl_toff[c];
+      double pw = 1.0;
 #pragma unroll
-        for (int b = 0; b < kMomGroup; ++b) {
-          if (b < ng) {
-            double* dst = ms + (int64_t)(g0 + b) * ldm + toff;
+      for (int m = 0; m < kMomTerms; ++m) {
+        if (m < M) {
 #pragma unroll
-            for (int m = 0; m < kMomTerms; ++m)
-              if (m < M) dst[m] = acc[b][m];
-          }
+          for (int b = 0; b < kMomGroup; ++b) acc[b][m] += prod[b] * pw;
+          pw *= dl;
         }
       }
     }
+    // transpose through LDS, one row at a time, so that the M consecutive slots
+    // of a (column, row) are written by consecutive lanes
+#pragma unroll
+    for (int b = 0; b < kMomGroup; ++b) {
+      if (b < ng) {
+#pragma unroll
+        for (int m = 0; m < kMomTerms; ++m) tr[m * 65 + threadIdx.x] = acc[b][m];
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * M; e += 64) {
+          const int col = e / M, m = e % M;
+          const int64_t sc2 = (int64_t)blockIdx.x * 64 + col;
+          if (sc2 < n_snps)
+            work[(sc2 * (nvm + 1) + g0 + b) * pl.n_slots + slot0 + m] = tr[m * 65 + col];
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// mom[s][b][t] = sum over the pieces p of the cluster c of term t (in order)
+// of work[s][b][seg_slot[seg0_c + p] + m_t]. Thread per (column, row, term),
+// term fastest: reads and writes of a (column, row) are contiguous.
+template <bool MULTI>
+__global__ void k_mom_fold(int64_t n_snps, int nv_one, const jm_mom_col* __restrict__ cols,
+                           int nv_max, jm_spectrum_plan pl, const double* __restrict__ work,
+                           double* __restrict__ mom_one, int64_t ldm_one) {
+  const int nvm = MULTI ? nv_max : nv_one;
+  const int T = pl.n_terms;
+  const int64_t total = n_snps * (int64_t)(nvm + 1) * T;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int t = (int)(q % T);
+    const int64_t sb = q / T;  // s * (nvm + 1) + b
+    const int b = (int)(sb % (nvm + 1));
+    const int64_t s = sb / (nvm + 1);
+    double* dst;
+    if (MULTI) {
+      const jm_mom_col d = cols[s];
+      if (b > d.nv) continue;
+      dst = d.out + (int64_t)b * d.ldm;
+    } else {
+      dst = mom_one + sb * ldm_one;
+    }
+    const double* src = work + sb * pl.n_slots;
+    const int c = pl.term_cluster[t], m = pl.term_m[t];
+    const int sg0 = pl.cl_seg0[c], ns = pl.cl_nseg[c];
+    double v = src[pl.seg_slot[sg0] + m];
+    for (int p = 1; p < ns; ++p) v += src[pl.seg_slot[sg0 + p] + m];
+    dst[t] = v;
   }
 }
 
@@ -126,56 +139,73 @@ __global__ void __launch_bounds__(64) k_moments(const ZT* __restrict__ Z, int64_
 
 using namespace jm;
 
-extern "C" int jm_spectrum_moments(const void* Z, int z_is_f32, int64_t ldz, int64_t n,
-                                   int64_t n_snps, const double* V, int64_t ldv, int nv,
-                                   const int32_t* cl_start, const int32_t* cl_size,
-                                   const int32_t* cl_terms, const int32_t* cl_toff,
-                                   int n_clusters, const double* delta, double* mom, int64_t ldm,
-                                   void* stream) {
-  JM_CHECK_ARG(n >= 1 && n_snps >= 0 && nv >= 0 && n_clusters >= 1, "bad sizes");
+static int check_plan(const jm_spectrum_plan* pl) {
+  JM_CHECK_ARG(pl != nullptr, "null plan");
+  JM_CHECK_ARG(pl->n >= 1 && pl->n_terms >= 1 && pl->n_clusters >= 1 && pl->n_seg >= 1 &&
+                   pl->n_slots >= 1,
+               "bad plan sizes");
+  JM_CHECK_ARG(pl->cl_terms && pl->cl_toff && pl->cl_seg0 && pl->cl_nseg && pl->seg_start &&
+                   pl->seg_len && pl->seg_cluster && pl->seg_slot && pl->delta &&
+                   pl->term_cluster && pl->term_m,
+               "missing plan array");
+  return JM_OK;
+}
+
+extern "C" int64_t jm_moments_work_bytes(const jm_spectrum_plan* plan, int64_t n_snps, int nv) {
+  if (!plan || n_snps < 1 || nv < 0) return 0;
+  return n_snps * (int64_t)(nv + 1) * plan->n_slots * (int64_t)sizeof(double);
+}
+
+extern "C" int jm_spectrum_moments(const void* Z, int z_is_f32, int64_t ldz, int64_t n_snps,
+                                   const double* V, int64_t ldv, int nv,
+                                   const jm_spectrum_plan* plan, double* work, double* mom,
+                                   int64_t ldm, void* stream) {
+  int rc = check_plan(plan);
+  if (rc) return rc;
+  JM_CHECK_ARG(n_snps >= 0 && nv >= 0, "bad sizes");
   JM_CHECK_ARG(ldz >= n_snps, "ldz < n_snps");
+  JM_CHECK_ARG(ldm >= plan->n_terms, "ldm < n_terms");
   if (n_snps == 0) return JM_OK;
-  JM_CHECK_ARG(Z && cl_start && cl_size && cl_terms && cl_toff && delta && mom, "null pointer");
-  JM_CHECK_ARG(nv == 0 || (V && ldv >= n), "V missing or ldv < n");
-  int groups = (int)((n + 255) / 256);
-  if (groups > 64) groups = 64;
-  if (groups < 1) groups = 1;
-  const dim3 grid((unsigned)((n_snps + 63) / 64), (unsigned)groups);
+  JM_CHECK_ARG(Z && work && mom, "null pointer");
+  JM_CHECK_ARG(nv == 0 || (V && ldv >= plan->n), "V missing or ldv < n");
+  const jm_spectrum_plan pl = *plan;
+  const dim3 grid((unsigned)((n_snps + 63) / 64), (unsigned)pl.n_seg);
   hipStream_t st = (hipStream_t)stream;
   if (z_is_f32)
-    hipLaunchKernelGGL((k_moments<float, false>), grid, dim3(64), 0, st, (const float*)Z, ldz, n,
-                       n_snps, V, ldv, nv, nullptr, 0, cl_start, cl_size, cl_terms, cl_toff,
-                       n_clusters, delta, mom, ldm);
+    hipLaunchKernelGGL((k_mom_seg<float, false>), grid, dim3(64), 0, st, (const float*)Z, ldz,
+                       n_snps, V, ldv, nv, nullptr, 0, pl, work);
   else
-    hipLaunchKernelGGL((k_moments<double, false>), grid, dim3(64), 0, st, (const double*)Z, ldz,
-                       n, n_snps, V, ldv, nv, nullptr, 0, cl_start, cl_size, cl_terms, cl_toff,
-                       n_clusters, delta, mom, ldm);
+    hipLaunchKernelGGL((k_mom_seg<double, false>), grid, dim3(64), 0, st, (const double*)Z, ldz,
+                       n_snps, V, ldv, nv, nullptr, 0, pl, work);
+  JM_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_mom_fold<false>), dim3(2048), dim3(256), 0, st, n_snps, nv, nullptr, 0, pl,
+                     work, mom, ldm);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }
 
-extern "C" int jm_spectrum_moments_multi(const void* Z, int z_is_f32, int64_t ldz, int64_t n,
+extern "C" int jm_spectrum_moments_multi(const void* Z, int z_is_f32, int64_t ldz,
                                          int64_t n_snps, const jm_mom_col* cols, int nv_max,
-                                         const int32_t* cl_start, const int32_t* cl_size,
-                                         const int32_t* cl_terms, const int32_t* cl_toff,
-                                         int n_clusters, const double* delta, void* stream) {
-  JM_CHECK_ARG(n >= 1 && n_snps >= 0 && nv_max >= 0 && n_clusters >= 1, "bad sizes");
+                                         const jm_spectrum_plan* plan, double* work,
+                                         void* stream) {
+  int rc = check_plan(plan);
+  if (rc) return rc;
+  JM_CHECK_ARG(n_snps >= 0 && nv_max >= 0, "bad sizes");
   JM_CHECK_ARG(ldz >= n_snps, "ldz < n_snps");
   if (n_snps == 0) return JM_OK;
-  JM_CHECK_ARG(Z && cols && cl_start && cl_size && cl_terms && cl_toff && delta, "null pointer");
-  int groups = (int)((n + 255) / 256);
-  if (groups > 64) groups = 64;
-  if (groups < 1) groups = 1;
-  const dim3 grid((unsigned)((n_snps + 63) / 64), (unsigned)groups);
+  JM_CHECK_ARG(Z && cols && work, "null pointer");
+  const jm_spectrum_plan pl = *plan;
+  const dim3 grid((unsigned)((n_snps + 63) / 64), (unsigned)pl.n_seg);
   hipStream_t st = (hipStream_t)stream;
   if (z_is_f32)
-    hipLaunchKernelGGL((k_moments<float, true>), grid, dim3(64), 0, st, (const float*)Z, ldz, n,
-                       n_snps, nullptr, 0, 0, cols, nv_max, cl_start, cl_size, cl_terms, cl_toff,
-                       n_clusters, delta, nullptr, 0);
+    hipLaunchKernelGGL((k_mom_seg<float, true>), grid, dim3(64), 0, st, (const float*)Z, ldz,
+                       n_snps, nullptr, 0, 0, cols, nv_max, pl, work);
   else
-    hipLaunchKernelGGL((k_moments<double, true>), grid, dim3(64), 0, st, (const double*)Z, ldz, n,
-                       n_snps, nullptr, 0, 0, cols, nv_max, cl_start, cl_size, cl_terms, cl_toff,
-                       n_clusters, delta, nullptr, 0);
+    hipLaunchKernelGGL((k_mom_seg<double, true>), grid, dim3(64), 0, st, (const double*)Z, ldz,
+                       n_snps, nullptr, 0, 0, cols, nv_max, pl, work);
+  JM_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_mom_fold<true>), dim3(2048), dim3(256), 0, st, n_snps, 0, cols, nv_max,
+                     pl, work, nullptr, 0);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

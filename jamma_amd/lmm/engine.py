@@ -96,6 +96,18 @@ class Cohort:
         self.delta = to_dev(p.idx_delta, torch.float64, dev)
         self.term_cluster = to_dev(p.term_cluster, torch.int32, dev)
         self.term_m = to_dev(p.term_m, torch.int32, dev)
+        # device plan for the moments kernels (cluster pieces + partial slots)
+        self._plan_dev = [to_dev(a, torch.int32, dev) for a in
+                          (p.cl_seg0, p.cl_nseg, p.seg_start, p.seg_len, p.seg_cluster, p.seg_slot)]
+        pc = _lib.SpectrumPlanC()
+        pc.n, pc.n_terms, pc.n_clusters = n, p.n_terms, len(p.cl_size)
+        pc.n_seg, pc.n_slots = len(p.seg_start), p.n_slots
+        pc.cl_terms, pc.cl_toff = ptr(self.cl_terms), ptr(self.cl_toff)
+        (pc.cl_seg0, pc.cl_nseg, pc.seg_start, pc.seg_len, pc.seg_cluster,
+         pc.seg_slot) = [ptr(a) for a in self._plan_dev]
+        pc.delta = ptr(self.delta)
+        pc.term_cluster, pc.term_m = ptr(self.term_cluster), ptr(self.term_m)
+        self.plan_c = pc
 
     @property
     def n_terms(self) -> int:
@@ -118,15 +130,20 @@ class Cohort:
         return out
 
     def rotate_genotypes(self, g: torch.Tensor, ld_g: int, means: torch.Tensor, n_snps: int,
-                         precision: str, col_idx: torch.Tensor | None = None
+                         precision: str, col_idx: torch.Tensor | None = None, timer=None
                          ) -> tuple[torch.Tensor, bool, int]:
         """Z = U^T X for a genotype tile (SNP-major int8 rows, -1 missing -> means).
-        With col_idx, column s uses genotype row / mean col_idx[s]."""
+        With col_idx, column s uses genotype row / mean col_idx[s]. timer: optional pair
+        of torch.cuda.Event recorded around the rotation kernel alone."""
         ldz = max(1, n_snps)
         if precision == "f64":
             Z = torch.empty((self.n, ldz), dtype=torch.float64, device=self.device)
+            if timer:
+                timer[0].record()
             _lib.call("jm_rotate_f64", ptr(self.UT), self.ld_u, ptr(g), ld_g, ptr(means),
                       ptr(col_idx), self.n, n_snps, ptr(Z), ldz, stream_ptr())
+            if timer:
+                timer[1].record()
             return Z, False, ldz
         if precision == "mixed":
             if self.split is None:
@@ -139,8 +156,12 @@ class Cohort:
                 st = stream_ptr()
                 _lib.call("jm_pack_genotypes", ptr(g), ld_g, ptr(col_idx), ptr(means), self.n,
                           n_snps, ptr(gp), ptr(mm), st)
+                if timer:
+                    timer[0].record()
                 _lib.call("jm_rotate_bf16x3", ptr(self.split), ptr(gp), ptr(mm), self.n, n_snps,
                           ptr(Z), ldz, st)
+                if timer:
+                    timer[1].record()
             return Z, True, ldz
         raise ValueError(f"unknown precision {precision!r}")
 
@@ -150,11 +171,17 @@ class Cohort:
         T = self.n_terms
         if out is None:
             out = torch.empty((n_cols, nv + 1, T), dtype=torch.float64, device=self.device)
-        _lib.call("jm_spectrum_moments", ptr(Z), int(z_is_f32), ldz, self.n, n_cols, ptr(V),
-                  V.shape[1], nv, ptr(self.cl_start), ptr(self.cl_size), ptr(self.cl_terms),
-                  ptr(self.cl_toff), len(self.plan.cl_size), ptr(self.delta), ptr(out), T,
-                  stream_ptr())
+        if n_cols:
+            work = self.moments_work(n_cols, nv)
+            _lib.call("jm_spectrum_moments", ptr(Z), int(z_is_f32), ldz, n_cols, ptr(V),
+                      V.shape[1], nv, ctypes_ref(self.plan_c), ptr(work), ptr(out), T,
+                      stream_ptr())
         return out
+
+    def moments_work(self, n_cols: int, nv: int) -> torch.Tensor:
+        """Workspace of the moments kernels for n_cols columns and nv partner rows."""
+        nbytes = int(_lib.lib.jm_moments_work_bytes(ctypes_ref(self.plan_c), n_cols, nv))
+        return torch.empty((max(nbytes, 8) // 8,), dtype=torch.float64, device=self.device)
 
 
 class AssocProblem:

@@ -47,6 +47,50 @@ class SpectrumPlan:
     idx_cluster: np.ndarray  # (n,) int32  cluster of eigen-index i
     idx_delta: np.ndarray    # (n,) float64  d_i - d_c
     radius: float
+    # pieces of at most seg_max eigen-indices (balanced work for the moments)
+    seg_max: int = 0
+    seg_start: np.ndarray = None   # (S,) int32
+    seg_len: np.ndarray = None     # (S,) int32
+    seg_cluster: np.ndarray = None  # (S,) int32
+    seg_slot: np.ndarray = None    # (S,) int32  first partial slot of the piece
+    cl_seg0: np.ndarray = None     # (C,) int32  first piece of cluster c
+    cl_nseg: np.ndarray = None     # (C,) int32
+    n_slots: int = 0
+
+
+def segment_plan(pl: "SpectrumPlan", seg_max: int | None = None) -> "SpectrumPlan":
+    """Cut every cluster into near-equal pieces of at most seg_max eigen-indices
+    (default max(64, min(512, ceil(n / 64)))); fills the seg_* fields in place."""
+    n = pl.n
+    if seg_max is None:
+        seg_max = max(64, min(512, -(-n // 64)))
+    starts, lens, clus, seg0, nseg = [], [], [], [], []
+    for c in range(len(pl.cl_size)):
+        size, st = int(pl.cl_size[c]), int(pl.cl_start[c])
+        k = -(-size // seg_max)
+        seg0.append(len(starts))
+        nseg.append(k)
+        base, extra = divmod(size, k)
+        pos = st
+        for q in range(k):
+            ln = base + (1 if q < extra else 0)
+            starts.append(pos)
+            lens.append(ln)
+            clus.append(c)
+            pos += ln
+    clus = np.array(clus, dtype=np.int32)
+    m = pl.cl_terms[clus].astype(np.int64)
+    slot = np.zeros(len(clus), dtype=np.int64)
+    slot[1:] = np.cumsum(m)[:-1]
+    pl.seg_max = int(seg_max)
+    pl.seg_start = np.array(starts, dtype=np.int32)
+    pl.seg_len = np.array(lens, dtype=np.int32)
+    pl.seg_cluster = clus
+    pl.seg_slot = slot.astype(np.int32)
+    pl.cl_seg0 = np.array(seg0, dtype=np.int32)
+    pl.cl_nseg = np.array(nseg, dtype=np.int32)
+    pl.n_slots = int(m.sum())
+    return pl
 
 
 def _terms_for(rc: float, eps: float) -> int:
@@ -120,7 +164,8 @@ def plan_spectrum(d: np.ndarray, eps: float = 1e-16,
     term_m = (np.arange(t, dtype=np.int32) - np.repeat(toff, mterms)).astype(np.int32)
     idx_cluster = np.repeat(np.arange(len(cl), dtype=np.int32), size)
     idx_delta = d - center[idx_cluster]
-    return SpectrumPlan(n=d.shape[0], d=d, cl_start=start, cl_size=size, cl_center=center,
-                        cl_terms=mterms, cl_toff=toff, n_terms=int(t),
-                        term_cluster=term_cluster, term_m=term_m, idx_cluster=idx_cluster,
-                        idx_delta=idx_delta, radius=r)
+    pl = SpectrumPlan(n=d.shape[0], d=d, cl_start=start, cl_size=size, cl_center=center,
+                      cl_terms=mterms, cl_toff=toff, n_terms=int(t),
+                      term_cluster=term_cluster, term_m=term_m, idx_cluster=idx_cluster,
+                      idx_delta=idx_delta, radius=r)
+    return segment_plan(pl)

@@ -31,7 +31,7 @@ import torch
 
 from jamma_amd import _lib
 from jamma_amd.device import ptr, require_gpu, round_up, to_dev
-from jamma_amd.lmm.engine import AssocProblem, Cohort
+from jamma_amd.lmm.engine import ctypes_ref, AssocProblem, Cohort
 
 
 @dataclass
@@ -267,10 +267,9 @@ class Server:
             desc_arr = desc
             desc_dev = to_dev(np.frombuffer(desc_arr, dtype=np.uint8), torch.uint8)
             nv_max = max(a.prob.n_cvt for a in acts) + 1
-            pl = co.plan
-            _lib.call("jm_spectrum_moments_multi", ptr(Z), int(f32), ldz, co.n, B, ptr(desc_dev),
-                      nv_max, ptr(co.cl_start), ptr(co.cl_size), ptr(co.cl_terms), ptr(co.cl_toff),
-                      len(pl.cl_size), ptr(co.delta), rc.stream.cuda_stream)
+            work = co.moments_work(B, nv_max)
+            _lib.call("jm_spectrum_moments_multi", ptr(Z), int(f32), ldz, B, ptr(desc_dev),
+                      nv_max, ctypes_ref(co.plan_c), ptr(work), rc.stream.cuda_stream)
             probs = (_lib.LmmProblem * len(acts))(*[a.prob.prob for a in acts])
             probs_dev = to_dev(np.frombuffer(probs, dtype=np.uint8), torch.uint8)
             modes = to_dev(np.array([a.req.mode for a in acts], dtype=np.int32), torch.int32)

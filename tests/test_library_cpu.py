@@ -199,3 +199,38 @@ def test_pair_tables_reproduce_pab_row0(n_cvt):
                 row = xm[pair_row[j]] if pair_row[j] >= 0 else smom[shared_src[j][0], shared_src[j][1]]
                 got = np.sum(kap * row)
                 assert abs(got - ref[s, j]) <= 1e-11 * max(1.0, abs(ref[s, j])), (lam, s, j)
+
+
+@pytest.mark.parametrize("seg_max", [None, 7, 64])
+def test_segment_plan_pieces_fold_to_cluster_moments(seg_max):
+    """Cluster pieces cover each cluster exactly, in order, with at most seg_max
+    indices; per-piece partial sums folded in piece order equal the direct
+    cluster sums (NumPy emulation of k_mom_seg + k_mom_fold)."""
+    from jamma_amd.lmm.spectrum import segment_plan
+
+    d = _kinship_spectrum(seed=2)
+    pl = plan_spectrum(d)
+    if seg_max is not None:
+        pl = segment_plan(pl, seg_max)
+    assert pl.seg_len.min() >= 1 and pl.seg_len.max() <= pl.seg_max
+    for c in range(len(pl.cl_size)):
+        s0, k = pl.cl_seg0[c], pl.cl_nseg[c]
+        assert np.all(pl.seg_cluster[s0:s0 + k] == c)
+        assert pl.seg_start[s0] == pl.cl_start[c]
+        assert np.sum(pl.seg_len[s0:s0 + k]) == pl.cl_size[c]
+        assert np.all(np.diff(pl.seg_start[s0:s0 + k]) == pl.seg_len[s0:s0 + k - 1])
+    assert pl.n_slots == int(np.sum(pl.cl_terms[pl.seg_cluster]))
+    rng = np.random.default_rng(3)
+    z = rng.normal(size=len(d))
+    work = np.zeros(pl.n_slots)
+    for q in range(len(pl.seg_start)):
+        c = pl.seg_cluster[q]
+        sl = slice(pl.seg_start[q], pl.seg_start[q] + pl.seg_len[q])
+        for m in range(pl.cl_terms[c]):
+            work[pl.seg_slot[q] + m] = np.sum(z[sl] * pl.idx_delta[sl] ** m)
+    for t in range(pl.n_terms):
+        c, m = pl.term_cluster[t], pl.term_m[t]
+        folded = sum(work[pl.seg_slot[pl.cl_seg0[c] + p] + m] for p in range(pl.cl_nseg[c]))
+        sl = slice(pl.cl_start[c], pl.cl_start[c] + pl.cl_size[c])
+        direct = np.sum(z[sl] * pl.idx_delta[sl] ** m)
+        assert abs(folded - direct) <= 1e-12 * max(1.0, np.sum(np.abs(z[sl] * pl.idx_delta[sl] ** m)))
