@@ -3,12 +3,17 @@
 Mirrors eigendecompose_kinship (src/jamma/lmm/eigen.py:28-128): ascending
 eigenvalues, |d| < threshold set to 0, warnings for negative / many zero
 eigenvalues. The reference calls host LAPACK dsyevd through numpy.linalg.eigh;
-here the FP64 symmetric eigensolver of the ROCm stack (rocSOLVER syevd, reached
-through torch.linalg.eigh) runs on the resident device matrix.
+here rocSOLVER's FP64 divide-and-conquer solver (rocsolver_dsyevd) runs on the
+resident device matrix. It is called directly rather than through
+torch.linalg.eigh: the hipSOLVER layer torch goes through sizes its workspace in
+32-bit integers and fails for cohorts of ~30k samples, while rocSOLVER manages
+its own workspace through the rocBLAS handle.
 """
 
 from __future__ import annotations
 
+import ctypes
+import os
 import warnings
 
 import numpy as np
@@ -16,13 +21,74 @@ import torch
 
 from jamma_amd.device import require_gpu, to_dev
 
+_RB_EVECT_ORIGINAL = 211  # rocblas_evect_original
+_RB_FILL_LOWER = 122      # rocblas_fill_lower
+_SOLVER = {}
+
+
+def _solver(device: torch.device):
+    """(rocsolver lib, rocblas handle) for a device, created once per process."""
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    if idx in _SOLVER:
+        return _SOLVER[idx]
+    libdir = os.path.join(os.path.dirname(torch.__file__), "lib")
+    # torch's own rocBLAS/rocSOLVER: already mapped in-process, same HIP runtime
+    rb = ctypes.CDLL(os.path.join(libdir, "librocblas.so"), mode=ctypes.RTLD_GLOBAL)
+    rs = ctypes.CDLL(os.path.join(libdir, "librocsolver.so"), mode=ctypes.RTLD_GLOBAL)
+    rb.rocblas_create_handle.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    rb.rocblas_create_handle.restype = ctypes.c_int
+    rb.rocblas_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    rb.rocblas_set_stream.restype = ctypes.c_int
+    rs.rocsolver_dsyevd.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p]
+    rs.rocsolver_dsyevd.restype = ctypes.c_int
+    h = ctypes.c_void_p()
+    with torch.cuda.device(idx):
+        st = rb.rocblas_create_handle(ctypes.byref(h))
+    if st != 0:
+        raise RuntimeError(f"rocblas_create_handle failed with status {st}")
+    _SOLVER[idx] = (rb, rs, h)
+    return _SOLVER[idx]
+
+
+def _dsyevd(Kd: torch.Tensor):
+    """Eigen-pairs of a symmetric device float64 matrix: (d ascending, U columns)."""
+    n = Kd.shape[0]
+    rb, rs, h = _solver(Kd.device)
+    A = Kd.clone(memory_format=torch.contiguous_format)  # overwritten with eigenvectors
+    D = torch.empty(n, dtype=torch.float64, device=Kd.device)
+    E = torch.empty(max(n, 1), dtype=torch.float64, device=Kd.device)
+    info = torch.zeros(1, dtype=torch.int32, device=Kd.device)
+    with torch.cuda.device(Kd.device):
+        stream = torch.cuda.current_stream()
+        st = rb.rocblas_set_stream(h, ctypes.c_void_p(stream.cuda_stream))
+        if st != 0:
+            raise RuntimeError(f"rocblas_set_stream failed with status {st}")
+        st = rs.rocsolver_dsyevd(h, _RB_EVECT_ORIGINAL, _RB_FILL_LOWER, n, A.data_ptr(), n,
+                                 D.data_ptr(), E.data_ptr(), info.data_ptr())
+        if st != 0:
+            raise RuntimeError(f"rocsolver_dsyevd failed with status {st} (n={n})")
+        code = int(info.item())  # synchronises the stream
+    if code != 0:
+        raise RuntimeError(f"rocsolver_dsyevd did not converge: info={code} (n={n})")
+    del E
+    # column-major result: A's row j (row-major view) is eigenvector j, so U = A^T
+    return D, A.t()
+
 
 def eigh_device(K, threshold: float = 1e-10):
-    """-> (d host float64 ascending thresholded, U device float64 (n x n))."""
+    """-> (d host float64 ascending thresholded, U device float64 (n x n)).
+
+    U[:, j] is the eigenvector of d[j]; U is a transposed view whose transpose
+    (U^T, eigenvectors as rows) is contiguous.
+    """
     Kd = to_dev(K, torch.float64, require_gpu())
     if Kd.ndim != 2 or Kd.shape[0] != Kd.shape[1]:
         raise ValueError(f"Kinship matrix must be square, got shape {tuple(Kd.shape)}")
-    d, U = torch.linalg.eigh(Kd)
+    if Kd.shape[0] == 0:
+        raise ValueError("Kinship matrix is empty")
+    d, U = _dsyevd(Kd)
     del Kd
     d = d.cpu().numpy()
     n_neg = int(np.sum(d < -threshold))

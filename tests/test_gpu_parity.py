@@ -53,6 +53,27 @@ def test_decode_bed_exact():
     assert np.array_equal(counts, exp)
 
 
+# ------------------------------------------------------------------ eigendecomposition
+@pytest.mark.parametrize("n", [1, 7, 300, 1031])
+def test_eigh_device_matches_lapack(n):
+    """rocsolver_dsyevd on the device vs LAPACK dsyevd (numpy): same ascending
+    spectrum, U diag(d) U^T reproduces K, U orthonormal."""
+    from jamma_amd.lmm.eigen import eigh_device
+
+    G = balding_nichols(n, 3 * n + 5, seed=n)
+    K = ok.centered_kinship(G) if n > 1 else np.array([[0.7]])
+    d, U = eigh_device(K)
+    d_ref = np.linalg.eigvalsh(K)
+    d_ref = np.where(np.abs(d_ref) < 1e-10, 0.0, d_ref)
+    scale = max(1.0, float(np.abs(d_ref).max()))
+    assert np.all(np.diff(d) >= -1e-12 * scale)
+    assert np.max(np.abs(d - d_ref)) < 1e-11 * scale
+    Uh = U.cpu().numpy()
+    assert np.max(np.abs(Uh.T @ Uh - np.eye(n))) < 1e-11
+    assert np.max(np.abs((Uh * d) @ Uh.T - K)) < 1e-11 * scale
+    assert U.t().is_contiguous()
+
+
 # ------------------------------------------------------------------ kinship
 def test_kinship_gemma_fixture_streaming():
     from jamma_amd.kinship import compute_kinship_streaming
