@@ -136,26 +136,31 @@ int jm_gemm_nt_f64(const double* A, int64_t lda, const double* B, int64_t ldb, i
 
 /*
  * Mixed-precision rotation (the same Z = U^T X as jm_rotate_f64) on bf16 MFMA
- * (v_mfma_f32_16x16x32_bf16) with FP32 accumulation:
- *   Z = lo^T X1 + mid^T X1 + hi^T X2 + hi^T X1,
- * UT = hi + mid + lo (three bfloat16 planes, 24 significant bits),
- * X1 = genotypes with missing -> bf16(mean), X2 = (mean - bf16(mean)) at missing.
+ * (v_mfma_f32_16x16x32_bf16) with FP32 accumulation. U^T is split into nplanes
+ * bfloat16 planes, each the bf16 rounding of the residual left by the planes
+ * before it; X1 = genotypes with missing -> bf16(mean), X2 = (mean - bf16(mean))
+ * at missing calls, 0 elsewhere:
+ *   nplanes = 2: UT = hi + lo (16 significant bits),
+ *                Z = lo^T X1 + hi^T X2 + hi^T X1            (3 bf16 products)
+ *   nplanes = 3: UT = hi + mid + lo (24 significant bits),
+ *                Z = lo^T X1 + mid^T X1 + hi^T X2 + hi^T X1 (4 bf16 products)
  * Both operands are pre-blocked so that every copy into LDS is one contiguous
  * kilobyte:
- *   jm_pack_ut_bf16x3: UT (float64, n x ld_u) -> planes (jm_packed_ut_bytes(n)
- *     bytes, 16-byte aligned), once per cohort;
+ *   jm_pack_ut_bf16x3: UT (float64, n x ld_u) -> planes
+ *     (jm_packed_ut_bytes(n, nplanes) bytes, 16-byte aligned), once per cohort;
  *   jm_pack_genotypes: SNP rows r(s) of g (col_idx as in jm_rotate_f64) and
  *     their means -> gp (jm_packed_geno_bytes(n, n_snps) bytes) and mm
  *     (n_snps x uint32: bf16(mean) | bf16 residual << 16), per SNP tile;
  *   jm_rotate_bf16x3: Z (float32, n x ldz) from planes, gp, mm.
  */
-int64_t jm_packed_ut_bytes(int64_t n);
+int64_t jm_packed_ut_bytes(int64_t n, int nplanes);
 int64_t jm_packed_geno_bytes(int64_t n, int64_t n_snps);
-int jm_pack_ut_bf16x3(const double* UT, int64_t ld_u, int64_t n, uint16_t* planes, void* stream);
+int jm_pack_ut_bf16x3(const double* UT, int64_t ld_u, int64_t n, int nplanes, uint16_t* planes,
+                      void* stream);
 int jm_pack_genotypes(const int8_t* g, int64_t ld_g, const int32_t* col_idx, const double* mean,
                       int64_t n, int64_t n_snps, uint8_t* gp, uint32_t* mm, void* stream);
-int jm_rotate_bf16x3(const uint16_t* planes, const uint8_t* gp, const uint32_t* mm, int64_t n,
-                     int64_t n_snps, float* Z, int64_t ldz, void* stream);
+int jm_rotate_bf16x3(const uint16_t* planes, int nplanes, const uint8_t* gp, const uint32_t* mm,
+                     int64_t n, int64_t n_snps, float* Z, int64_t ldz, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Per-SNP REML/MLE lambda search + Wald/LRT/Score                           */

@@ -74,13 +74,13 @@ _SIGS = {
                        c_void_p, c_int64, c_void_p], c_int),
     "jm_gemm_nt_f64": ([c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p,
                         c_int64, c_void_p], c_int),
-    "jm_packed_ut_bytes": ([c_int64], c_int64),
+    "jm_packed_ut_bytes": ([c_int64, c_int], c_int64),
     "jm_packed_geno_bytes": ([c_int64, c_int64], c_int64),
-    "jm_pack_ut_bf16x3": ([c_void_p, c_int64, c_int64, c_void_p, c_void_p], c_int),
+    "jm_pack_ut_bf16x3": ([c_void_p, c_int64, c_int64, c_int, c_void_p, c_void_p], c_int),
     "jm_pack_genotypes": ([c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_int64, c_void_p,
                            c_void_p, c_void_p], c_int),
-    "jm_rotate_bf16x3": ([c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_int64,
-                          c_void_p], c_int),
+    "jm_rotate_bf16x3": ([c_void_p, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p,
+                          c_int64, c_void_p], c_int),
     "jm_moments_work_bytes": ([c_void_p, c_int64, c_int], c_int64),
     "jm_spectrum_moments": ([c_void_p, c_int, c_int64, c_int64, c_void_p, c_int64, c_int, c_void_p,
                              c_void_p, c_void_p, c_int64, c_void_p], c_int),

@@ -215,7 +215,9 @@ __device__ __forceinline__ double bf16_to_f64(uint16_t h) {
 constexpr int V3_BM = 256, V3_BN = 256, V3_BK = 32;
 constexpr int V3_APLANE = V3_BM * V3_BK * 2;      // 16 KB per plane
 constexpr int V3_BBYTES = V3_BN * V3_BK;          // 8 KB
-constexpr int V3_STAGE = 3 * V3_APLANE + V3_BBYTES;  // 56 KB
+// LDS stage of NPL planes + genotypes: 40 KB (NPL = 2) or 56 KB (NPL = 3)
+template <int NPL>
+constexpr int v3_stage() { return NPL * V3_APLANE + V3_BBYTES; }
 
 typedef __attribute__((address_space(3))) void lds_void;
 
@@ -268,7 +270,7 @@ __device__ __forceinline__ void widen8p(uint64_t bytes, uint32_t mm, bf16x8_t& x
 constexpr int64_t UTP_BLOCK = 16384, GP_BLOCK = 8192;
 
 __global__ void k_pack_ut(const double* __restrict__ UT, int64_t ld_u, int64_t n, int64_t npan,
-                          int64_t nks, uint16_t* __restrict__ planes) {
+                          int64_t nks, int nplanes, uint16_t* __restrict__ planes) {
   const int64_t per_plane = npan * nks * (UTP_BLOCK / 2);  // bf16 elements per plane
   const int64_t chunks = npan * nks * 1024;                // 16-byte chunks per plane
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < chunks;
@@ -279,20 +281,18 @@ __global__ void k_pack_ut(const double* __restrict__ UT, int64_t ld_u, int64_t n
     const int c = cs ^ ((r >> 2) & 3);
     const int64_t P = blk / nks, sk = blk % nks;
     const int64_t i = P * 256 + r, k0 = sk * 32 + 8 * c;
-    uint16_t h[8], m[8], l[8];
+    uint16_t h[3][8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const double u = (i < n && k0 + e < n) ? UT[i * ld_u + k0 + e] : 0.0;
-      h[e] = f32_to_bf16_rne((float)u);
-      const double r1 = u - bf16_to_f64(h[e]);
-      m[e] = f32_to_bf16_rne((float)r1);
-      const double r2 = r1 - bf16_to_f64(m[e]);
-      l[e] = f32_to_bf16_rne((float)r2);
+      double u = (i < n && k0 + e < n) ? UT[i * ld_u + k0 + e] : 0.0;
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl) {  // successive residuals, each rounded to bf16
+        h[pl][e] = f32_to_bf16_rne((float)u);
+        u -= bf16_to_f64(h[pl][e]);
+      }
     }
     const int64_t off = q * 8;
-    memcpy(planes + off, h, 16);
-    memcpy(planes + per_plane + off, m, 16);
-    memcpy(planes + 2 * per_plane + off, l, 16);
+    for (int pl = 0; pl < nplanes; ++pl) memcpy(planes + pl * per_plane + off, h[pl], 16);
   }
 }
 
@@ -332,12 +332,17 @@ __global__ void k_pack_geno(const int8_t* __restrict__ g, int64_t ld_g,
 // grid of 64x128 wave tiles (128 accumulators per lane, two waves per SIMD),
 // both operands DMA-copied block by block into double-buffered LDS, one
 // barrier per 32-deep K step. Genotype fragments are widened to X1 / X2 with
-// widen8p; per K step: lo x X1, mid x X1, hi x X2, hi x X1 (small terms first).
+// widen8p. Per K step, smallest terms first:
+//   NPL = 2 (UT = hi + lo):        lo x X1, hi x X2, hi x X1   (3 products)
+//   NPL = 3 (UT = hi + mid + lo):  lo x X1, mid x X1, hi x X2, hi x X1
+template <int NPL>
 __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
     const uint16_t* __restrict__ planes, int64_t per_plane, const uint8_t* __restrict__ gp,
     const uint32_t* __restrict__ mmv, int64_t M, int64_t n_cols, int64_t nks,
     float* __restrict__ Z, int64_t ldz) {
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * V3_STAGE];
+  constexpr int STAGE = v3_stage<NPL>();
+  constexpr int ADMA = NPL * 2;  // 1 KB UT copies per wave per stage (NPL x 16 KB / 8 waves)
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * STAGE];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int nrb = (int)((M + V3_BM - 1) / V3_BM), ncb = (int)((n_cols + V3_BN - 1) / V3_BN);
@@ -350,15 +355,14 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
   const uint8_t* ap = (const uint8_t*)planes + (int64_t)rb * nks * UTP_BLOCK + lane * 16;
   const uint8_t* bp = gp + (int64_t)cb * nks * GP_BLOCK + lane * 16;
   const int64_t pstride = per_plane * 2;  // bytes per plane
-  auto issue = [&](int stage, int64_t sk) {
-    uint8_t* base = lds + stage * V3_STAGE;
+  auto issue = [&](uint8_t* base, int64_t sk) {
 #pragma unroll
-    for (int u = 0; u < 6; ++u) {
-      const int q = wave * 6 + u;  // 0..47: plane q >> 4, kilobyte q & 15
+    for (int u = 0; u < ADMA; ++u) {
+      const int q = wave * ADMA + u;  // plane q >> 4, kilobyte q & 15
       lds_dma16(ap + (q >> 4) * pstride + sk * UTP_BLOCK + (q & 15) * 1024,
                 base + (q >> 4) * V3_APLANE + (q & 15) * 1024);
     }
-    lds_dma16(bp + sk * GP_BLOCK + wave * 1024, base + 3 * V3_APLANE + wave * 1024);
+    lds_dma16(bp + sk * GP_BLOCK + wave * 1024, base + NPL * V3_APLANE + wave * 1024);
   };
   uint32_t mm[8];
 #pragma unroll
@@ -371,24 +375,19 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 8; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
-  issue(0, 0);
-  for (int kt = 0; kt < nks; ++kt) {
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of stage kt have landed
-    __syncthreads();                // ... and everyone's; stage kt-1 is free
-    if (kt + 1 < nks) issue((kt + 1) & 1, kt + 1);
-    const uint8_t* base = lds + (kt & 1) * V3_STAGE;
+  auto compute = [&](const uint8_t* base) {
     auto gbytes = [&](int tj) -> uint64_t {
       const int j = wc + 16 * tj + lr;
       const int c = (lk >> 1) ^ ((j >> 3) & 1);
       uint64_t v;
-      memcpy(&v, base + 3 * V3_APLANE + j * 32 + 16 * c + 8 * (lk & 1), 8);
+      memcpy(&v, base + NPL * V3_APLANE + j * 32 + 16 * c + 8 * (lk & 1), 8);
       return v;
     };
     bf16x8_t x1[8];
 #pragma unroll
     for (int tj = 0; tj < 8; ++tj) widen8p(gbytes(tj), mm[tj], x1[tj], nullptr);
 #pragma unroll 1
-    for (int pl = 2; pl >= 0; --pl) {
+    for (int pl = NPL - 1; pl >= 0; --pl) {
       bf16x8_t af[4];
 #pragma unroll
       for (int ti = 0; ti < 4; ++ti) {
@@ -412,6 +411,13 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
         for (int tj = 0; tj < 8; ++tj)
           acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ti], x1[tj], acc[ti][tj], 0, 0, 0);
     }
+  };
+  issue(lds, 0);
+  for (int kt = 0; kt < nks; ++kt) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of stage kt have landed
+    __syncthreads();                // ... and everyone's; stage kt-1 is free
+    if (kt + 1 < nks) issue(lds + ((kt + 1) & 1) * STAGE, kt + 1);
+    compute(lds + (kt & 1) * STAGE);
   }
 #pragma unroll
   for (int ti = 0; ti < 4; ++ti)
@@ -460,9 +466,9 @@ extern "C" int jm_gemm_nt_f64(const double* A, int64_t lda, const double* B, int
   return JM_OK;
 }
 
-extern "C" int64_t jm_packed_ut_bytes(int64_t n) {
-  if (n < 1) return 0;
-  return 3 * ((n + 255) / 256) * ((n + 31) / 32) * UTP_BLOCK;
+extern "C" int64_t jm_packed_ut_bytes(int64_t n, int nplanes) {
+  if (n < 1 || nplanes < 2 || nplanes > 3) return 0;
+  return nplanes * ((n + 255) / 256) * ((n + 31) / 32) * UTP_BLOCK;
 }
 
 extern "C" int64_t jm_packed_geno_bytes(int64_t n, int64_t n_snps) {
@@ -470,15 +476,16 @@ extern "C" int64_t jm_packed_geno_bytes(int64_t n, int64_t n_snps) {
   return ((n_snps + 255) / 256) * ((n + 31) / 32) * GP_BLOCK;
 }
 
-extern "C" int jm_pack_ut_bf16x3(const double* UT, int64_t ld_u, int64_t n, uint16_t* planes,
-                                 void* stream) {
+extern "C" int jm_pack_ut_bf16x3(const double* UT, int64_t ld_u, int64_t n, int nplanes,
+                                 uint16_t* planes, void* stream) {
   JM_CHECK_ARG(n >= 1 && ld_u >= n, "bad sizes");
+  JM_CHECK_ARG(nplanes == 2 || nplanes == 3, "nplanes must be 2 or 3");
   JM_CHECK_ARG(UT && planes && ((uintptr_t)planes % 16) == 0, "null or misaligned pointer");
   const int64_t npan = (n + 255) / 256, nks = (n + 31) / 32;
   const int64_t chunks = npan * nks * 1024;
   const unsigned grid = (unsigned)(chunks < 65536 ? (chunks + 255) / 256 : 256 * 64);
   hipLaunchKernelGGL(k_pack_ut, dim3(grid), dim3(256), 0, (hipStream_t)stream, UT, ld_u, n, npan,
-                     nks, planes);
+                     nks, nplanes, planes);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }
@@ -498,19 +505,25 @@ extern "C" int jm_pack_genotypes(const int8_t* g, int64_t ld_g, const int32_t* c
   return JM_OK;
 }
 
-extern "C" int jm_rotate_bf16x3(const uint16_t* planes, const uint8_t* gp, const uint32_t* mm,
-                                int64_t n, int64_t n_snps, float* Z, int64_t ldz, void* stream) {
+extern "C" int jm_rotate_bf16x3(const uint16_t* planes, int nplanes, const uint8_t* gp,
+                                const uint32_t* mm, int64_t n, int64_t n_snps, float* Z,
+                                int64_t ldz, void* stream) {
   JM_CHECK_ARG(n >= 1 && n_snps >= 0, "bad sizes");
+  JM_CHECK_ARG(nplanes == 2 || nplanes == 3, "nplanes must be 2 or 3");
   if (n_snps == 0) return JM_OK;
   JM_CHECK_ARG(planes && gp && mm && Z, "null pointer");
   JM_CHECK_ARG(ldz >= n_snps, "ldz < n_snps");
   JM_CHECK_ARG(((uintptr_t)planes % 16) == 0 && ((uintptr_t)gp % 16) == 0,
                "packed operands must be 16-byte aligned");
   const int64_t nks = (n + 31) / 32;
-  const int64_t per_plane = jm_packed_ut_bytes(n) / 6;  // bf16 elements per plane
+  const int64_t per_plane = jm_packed_ut_bytes(n, nplanes) / (2 * nplanes);  // bf16 per plane
   const int nrb = ceil_div(n, V3_BM), ncb = ceil_div(n_snps, V3_BN);
-  hipLaunchKernelGGL(k_rotate_bf16x3_packed, dim3(nrb * ncb), dim3(512), 0, (hipStream_t)stream,
-                     planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
+  if (nplanes == 2)
+    hipLaunchKernelGGL(k_rotate_bf16x3_packed<2>, dim3(nrb * ncb), dim3(512), 0,
+                       (hipStream_t)stream, planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
+  else
+    hipLaunchKernelGGL(k_rotate_bf16x3_packed<3>, dim3(nrb * ncb), dim3(512), 0,
+                       (hipStream_t)stream, planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

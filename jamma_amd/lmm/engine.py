@@ -2,7 +2,8 @@
 
 A ``Cohort`` keeps, in HBM, what every association request against one kinship
 shares: the eigenvalues and their compressed-spectrum plan, U^T in FP64 and
-(for the mixed path) its bf16 hi/mid/lo split. An ``AssocProblem`` adds one
+(for the mixed path) its split into bf16 planes (hi + lo by default, hi + mid + lo
+on request). An ``AssocProblem`` adds one
 phenotype and covariate set: the rotated covariates/phenotype, their spectrum
 moments, the null model (device) and the grid tables. ``associate`` then runs
 one SNP tile through rotation -> spectrum moments -> lambda search + tests.
@@ -62,9 +63,16 @@ def pair_tables(n_cvt: int) -> tuple[np.ndarray, np.ndarray]:
 class Cohort:
     """Eigendecomposition of one kinship, resident on the device."""
 
-    def __init__(self, eigenvalues, eigenvectors, precision: str = "mixed", device=None):
+    def __init__(self, eigenvalues, eigenvectors, precision: str = "mixed", device=None,
+                 bf16_planes: int = 2):
+        """bf16_planes: 2 = U^T as hi + lo (16 significant bits; the rotation issues 3
+        bf16 products per term: hi X1 + lo X1 + hi X2), 3 = hi + mid + lo (24 bits,
+        4 products)."""
         if precision not in PRECISIONS + ("both",):
             raise ValueError(f"precision must be one of {PRECISIONS + ('both',)}")
+        if bf16_planes not in (2, 3):
+            raise ValueError("bf16_planes must be 2 or 3")
+        self.bf16_planes = bf16_planes
         self.device = device or require_gpu()
         d = eigenvalues.detach().cpu().numpy() if isinstance(eigenvalues, torch.Tensor) \
             else np.asarray(eigenvalues, dtype=np.float64)
@@ -81,10 +89,11 @@ class Cohort:
         self.precision = precision
         self.split = None
         if precision in ("mixed", "both"):
-            nbytes = int(_lib.lib.jm_packed_ut_bytes(n))
+            nbytes = int(_lib.lib.jm_packed_ut_bytes(n, bf16_planes))
             planes = torch.empty((nbytes // 2,), dtype=torch.int16, device=self.device)
-            _lib.call("jm_pack_ut_bf16x3", ptr(self.UT), self.ld_u, n, ptr(planes), stream_ptr())
-            self.split = planes  # packed hi / mid / lo bf16 planes
+            _lib.call("jm_pack_ut_bf16x3", ptr(self.UT), self.ld_u, n, bf16_planes, ptr(planes),
+                      stream_ptr())
+            self.split = planes  # packed bf16 planes of U^T
         p = self.plan
         dev = self.device
         self.cl_start = to_dev(p.cl_start, torch.int32, dev)
@@ -158,8 +167,8 @@ class Cohort:
                           n_snps, ptr(gp), ptr(mm), st)
                 if timer:
                     timer[0].record()
-                _lib.call("jm_rotate_bf16x3", ptr(self.split), ptr(gp), ptr(mm), self.n, n_snps,
-                          ptr(Z), ldz, st)
+                _lib.call("jm_rotate_bf16x3", ptr(self.split), self.bf16_planes, ptr(gp), ptr(mm),
+                          self.n, n_snps, ptr(Z), ldz, st)
                 if timer:
                     timer[1].record()
             return Z, True, ldz
