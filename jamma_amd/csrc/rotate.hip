@@ -219,12 +219,6 @@ constexpr int V3_BBYTES = V3_BN * V3_BK;          // 8 KB
 template <int NPL>
 constexpr int v3_stage() { return NPL * V3_APLANE + V3_BBYTES; }
 
-typedef __attribute__((address_space(3))) void lds_void;
-
-__device__ __forceinline__ void lds_dma16(const void* gsrc, uint8_t* lds_dst) {
-  __builtin_amdgcn_global_load_lds(gsrc, (lds_void*)lds_dst, 16, 0, 0);
-}
-
 // Byte-permute form of widen8. The 2-bit genotype code indexes a 4-entry table
 // (0, 1, 2, missing -> x & 3 = 0, 1, 2, 3); v_perm_b32 picks the low and high
 // bytes of each bf16 from per-column tables:
@@ -330,28 +324,32 @@ __global__ void k_pack_geno(const int8_t* __restrict__ g, int64_t ld_g,
 
 // BF16x3 rotation on packed operands: 256x256 block tile, 8 waves in a 4 x 2
 // grid of 64x128 wave tiles (128 accumulators per lane, two waves per SIMD),
-// both operands DMA-copied block by block into double-buffered LDS, one
-// barrier per 32-deep K step. Genotype fragments are widened to X1 / X2 with
-// widen8p. Per K step, smallest terms first:
+// both operands copied block by block into NBUF LDS stages with
+// global_load_lds_dwordx4 (the copies of stages kt+1 .. kt+NBUF-1 in flight
+// while stage kt is multiplied), one barrier per 32-deep K step. Genotype
+// fragments are widened to X1 / X2 with widen8p. Per K step, smallest terms
+// first:
 //   NPL = 2 (UT = hi + lo):        lo x X1, hi x X2, hi x X1   (3 products)
 //   NPL = 3 (UT = hi + mid + lo):  lo x X1, mid x X1, hi x X2, hi x X1
-template <int NPL>
+template <int NPL, int NBUF>
 __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
     const uint16_t* __restrict__ planes, int64_t per_plane, const uint8_t* __restrict__ gp,
     const uint32_t* __restrict__ mmv, int64_t M, int64_t n_cols, int64_t nks,
     float* __restrict__ Z, int64_t ldz) {
   constexpr int STAGE = v3_stage<NPL>();
-  constexpr int ADMA = NPL * 2;  // 1 KB UT copies per wave per stage (NPL x 16 KB / 8 waves)
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * STAGE];
+  constexpr int ADMA = NPL * 2;    // 1 KB UT copies per wave per stage (NPL x 16 KB / 8 waves)
+  constexpr int DPS = ADMA + 1;    // copies per wave per stage (+ 1 KB of genotypes)
+  static_assert(NBUF >= 2 && NBUF * STAGE <= 160 * 1024, "LDS stages");
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[NBUF * STAGE];
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nrb = (int)((M + V3_BM - 1) / V3_BM), ncb = (int)((n_cols + V3_BN - 1) / V3_BN);
   int rb, cb;
   xcd_tile(blockIdx.x, nrb, ncb, rb, cb);
   const int64_t i0 = (int64_t)rb * V3_BM, s0 = (int64_t)cb * V3_BN;
   const int wr = (wave >> 1) * 64, wc = (wave & 1) * 128;
   const int lr = lane & 15, lk = lane >> 4;
-  // DMA sources: plane pl's block (rb, s) and genotype block (cb, s)
+  // copy sources: plane pl's block (rb, s) and genotype block (cb, s)
   const uint8_t* ap = (const uint8_t*)planes + (int64_t)rb * nks * UTP_BLOCK + lane * 16;
   const uint8_t* bp = gp + (int64_t)cb * nks * GP_BLOCK + lane * 16;
   const int64_t pstride = per_plane * 2;  // bytes per plane
@@ -359,10 +357,10 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
 #pragma unroll
     for (int u = 0; u < ADMA; ++u) {
       const int q = wave * ADMA + u;  // plane q >> 4, kilobyte q & 15
-      lds_dma16(ap + (q >> 4) * pstride + sk * UTP_BLOCK + (q & 15) * 1024,
-                base + (q >> 4) * V3_APLANE + (q & 15) * 1024);
+      lds_copy16(ap + (q >> 4) * pstride + sk * UTP_BLOCK + (q & 15) * 1024,
+                 base + (q >> 4) * V3_APLANE + (q & 15) * 1024);
     }
-    lds_dma16(bp + sk * GP_BLOCK + wave * 1024, base + NPL * V3_APLANE + wave * 1024);
+    lds_copy16(bp + sk * GP_BLOCK + wave * 1024, base + NPL * V3_APLANE + wave * 1024);
   };
   uint32_t mm[8];
 #pragma unroll
@@ -412,12 +410,24 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
           acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ti], x1[tj], acc[ti][tj], 0, 0, 0);
     }
   };
-  issue(lds, 0);
+#pragma unroll
+  for (int st = 0; st < NBUF - 1; ++st)
+    if (st < nks) issue(lds + st * STAGE, st);
+  int cur = 0;  // LDS stage of K step kt
   for (int kt = 0; kt < nks; ++kt) {
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of stage kt have landed
-    __syncthreads();                // ... and everyone's; stage kt-1 is free
-    if (kt + 1 < nks) issue(lds + ((kt + 1) & 1) * STAGE, kt + 1);
-    compute(lds + (kt & 1) * STAGE);
+    // In flight: this wave's copies of K steps kt .. min(kt + NBUF - 2, nks - 1), issued
+    // in that order and completed in order. Keep the later ones, wait for step kt.
+    if (kt + NBUF - 2 < nks)
+      wait_vmcnt<DPS * (NBUF - 2)>();
+    else
+      wait_vmcnt<0>();
+    __syncthreads();  // every wave's copies of step kt landed; step kt-1's stage is free
+    if (kt + NBUF - 1 < nks) {
+      const int nxt = cur == 0 ? NBUF - 1 : cur - 1;  // (kt + NBUF - 1) % NBUF = stage of kt-1
+      issue(lds + nxt * STAGE, kt + NBUF - 1);
+    }
+    compute(lds + cur * STAGE);
+    cur = cur + 1 == NBUF ? 0 : cur + 1;
   }
 #pragma unroll
   for (int ti = 0; ti < 4; ++ti)
@@ -518,11 +528,11 @@ extern "C" int jm_rotate_bf16x3(const uint16_t* planes, int nplanes, const uint8
   const int64_t nks = (n + 31) / 32;
   const int64_t per_plane = jm_packed_ut_bytes(n, nplanes) / (2 * nplanes);  // bf16 per plane
   const int nrb = ceil_div(n, V3_BM), ncb = ceil_div(n_snps, V3_BN);
-  if (nplanes == 2)
-    hipLaunchKernelGGL(k_rotate_bf16x3_packed<2>, dim3(nrb * ncb), dim3(512), 0,
+  if (nplanes == 2)  // 3 x 40 KB stages
+    hipLaunchKernelGGL((k_rotate_bf16x3_packed<2, 3>), dim3(nrb * ncb), dim3(512), 0,
                        (hipStream_t)stream, planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
-  else
-    hipLaunchKernelGGL(k_rotate_bf16x3_packed<3>, dim3(nrb * ncb), dim3(512), 0,
+  else  // 2 x 56 KB stages
+    hipLaunchKernelGGL((k_rotate_bf16x3_packed<3, 2>), dim3(nrb * ncb), dim3(512), 0,
                        (hipStream_t)stream, planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
   JM_CHECK_LAUNCH();
   return JM_OK;
