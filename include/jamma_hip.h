@@ -64,12 +64,13 @@ int jm_decode_bed(const uint8_t* bed, int64_t bytes_per_snp, const int32_t* snp_
                   int64_t n_snps, const int32_t* sample_idx, int64_t n_out, int8_t* g,
                   int64_t ld_g, int32_t* counts, void* stream);
 
-/* out (cols x ld_out) = transpose of in (rows x ld_in), int8, and/or out0, the
- * same transpose with missing calls (-1) written as 0. out holds the sample-major
- * genotype tile the imputation terms read, out0 the operand of the kinship
- * GEMM. Either output may be NULL; columns [rows, ld_out) are not written. */
+/* out (cols x ld_out) = transpose of in (rows x ld_in), int8, and/or out0
+ * (cols x ld_out0), the same transpose with missing calls (-1) written as 0.
+ * out holds the sample-major genotype tile the imputation terms read, out0 the
+ * operand of the kinship GEMM (possibly a column range of a larger staging
+ * buffer). Either output may be NULL; columns [rows, ld) are not written. */
 int jm_transpose_i8(const int8_t* in, int64_t rows, int64_t cols, int64_t ld_in, int8_t* out,
-                    int64_t ld_out, int8_t* out0, void* stream);
+                    int64_t ld_out, int8_t* out0, int64_t ld_out0, void* stream);
 
 /*
  * Replaces _accumulate_kinship (src/jamma/kinship/compute.py:46-57) inside the
@@ -78,11 +79,13 @@ int jm_transpose_i8(const int8_t* in, int64_t rows, int64_t cols, int64_t ld_in,
  * (src/jamma/kinship/missing.py:22-61), which this library applies as exact
  * corrections (see jm_kinship_finalize).
  *
- * S (n x n int32) += X0 X0^T over one tile of n_snps SNPs. gT0 is X0^T, the
- * sample-major tile with missing calls as 0 (jm_transpose_i8's out0), n x ld_gT
- * with ld_gT a multiple of 128 and columns [n_snps, ld_gT) zero. Exact int32
- * accumulation on int8 MFMA (v_mfma_i32_16x16x64_i8). Only the lower triangle
- * (i >= k) of S is guaranteed; jm_kinship_finalize mirrors it.
+ * S (n x n int32) += X0 X0^T over n_snps SNPs. gT0 is X0^T, the sample-major
+ * genotypes with missing calls as 0 (jm_transpose_i8's out0): round_up(n, 256)
+ * rows of ld_gT bytes, ld_gT a multiple of 128, columns [n_snps, round_up(n_snps,
+ * 128)) zero. Rows n .. round_up(n, 256) - 1 are read but never reach S (any
+ * value). Exact int32 accumulation on int8 MFMA (v_mfma_i32_16x16x64_i8); one
+ * call over many SNPs is faster than several over fewer. Only the lower
+ * triangle (i >= k) of S is guaranteed; jm_kinship_finalize mirrors it.
  */
 int jm_kinship_syrk_i8(const int8_t* gT0, int64_t ld_gT, int64_t n, int64_t n_snps, int32_t* S,
                        void* stream);

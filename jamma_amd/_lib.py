@@ -64,7 +64,7 @@ _SIGS = {
     "jm_decode_bed": ([c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64,
                        c_void_p, c_void_p], c_int),
     "jm_transpose_i8": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_void_p,
-                         c_void_p], c_int),
+                         c_int64, c_void_p], c_int),
     "jm_kinship_syrk_i8": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p], c_int),
     "jm_kinship_impute_terms": ([c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_void_p,
                                  c_void_p, c_void_p, c_int, c_void_p], c_int),

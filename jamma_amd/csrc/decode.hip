@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(256) k_decode_bed(const uint8_t* __restrict__ 
 __global__ void __launch_bounds__(256) k_transpose_i8(const int8_t* __restrict__ in, int64_t rows,
                                                       int64_t cols, int64_t ld_in,
                                                       int8_t* __restrict__ out, int64_t ld_out,
-                                                      int8_t* __restrict__ out0) {
+                                                      int8_t* __restrict__ out0, int64_t ld_out0) {
   __shared__ int8_t tile[64][65];
   const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(256) k_transpose_i8(const int8_t* __restrict__
     if (c < cols && r < rows) {
       const int8_t v = tile[tx][k];
       if (out) out[c * ld_out + r] = v;
-      if (out0) out0[c * ld_out + r] = v < 0 ? (int8_t)0 : v;  // missing -> 0
+      if (out0) out0[c * ld_out0 + r] = v < 0 ? (int8_t)0 : v;  // missing -> 0
     }
   }
 }
@@ -112,13 +112,16 @@ extern "C" int jm_decode_bed(const uint8_t* bed, int64_t bytes_per_snp, const in
 }
 
 extern "C" int jm_transpose_i8(const int8_t* in, int64_t rows, int64_t cols, int64_t ld_in,
-                               int8_t* out, int64_t ld_out, int8_t* out0, void* stream) {
-  JM_CHECK_ARG(rows >= 0 && cols >= 0 && ld_in >= cols && ld_out >= rows, "bad sizes");
+                               int8_t* out, int64_t ld_out, int8_t* out0, int64_t ld_out0,
+                               void* stream) {
+  JM_CHECK_ARG(rows >= 0 && cols >= 0 && ld_in >= cols, "bad sizes");
+  JM_CHECK_ARG((out == nullptr || ld_out >= rows) && (out0 == nullptr || ld_out0 >= rows),
+               "bad output leading dimension");
   if (rows == 0 || cols == 0) return JM_OK;
   JM_CHECK_ARG(in && (out || out0), "null pointer");
   const dim3 grid((unsigned)ceil_div(cols, 64), (unsigned)ceil_div(rows, 64));
   hipLaunchKernelGGL(k_transpose_i8, grid, dim3(256), 0, (hipStream_t)stream, in, rows, cols,
-                     ld_in, out, ld_out, out0);
+                     ld_in, out, ld_out, out0, ld_out0);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

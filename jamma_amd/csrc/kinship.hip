@@ -17,14 +17,29 @@ namespace jm {
 
 typedef int i4_t __attribute__((ext_vector_type(4)));
 
-constexpr int K_BM = 128, K_BK = 128;
-constexpr int K_LS = K_BK + 16;  // LDS row stride in bytes
-
-// missing (-1) -> 0 on packed bytes
-__device__ __forceinline__ uint32_t zero_neg_bytes(uint32_t w) {
-  const uint32_t neg = (w >> 7) & 0x01010101u;
-  return w & ~(neg * 0xFFu);
-}
+// ---------------------------------------------------------------------------
+// S += X0 X0^T on int8 MFMA
+// ---------------------------------------------------------------------------
+// 256x256 lower tiles (row panel >= column panel) of S, 4 waves of 128x128
+// (8 x 8 v_mfma_i32_16x16x64_i8 tiles, 256 int32 accumulators per lane in
+// AGPRs, one wave per SIMD). Both 256-row operand panels are copied 64 SNPs
+// deep per stage (2 x 16 KB) straight into LDS (global_load_lds_dwordx4, SGPR
+// base + per-lane offset), 4 stages, so the copies of the next 3 K steps are in
+// flight while one is multiplied; one barrier per step. LDS rows are 64 B with
+// 16-byte chunk c of row r at c ^ ((r >> 1) & 3): the 8 lanes of a
+// ds_read_b128 quarter (rows r..r+7, same chunk) then cover all 32 banks.
+//
+// Work split: the tiles go in groups of 4 row panels, column-major inside a
+// group, and each XCD takes a contiguous range of that order, so the 32 blocks
+// resident on one XCD work on a 4 x 8 patch and share its panels in L2. Whole
+// rounds of 256 tiles run as plain read-modify-write blocks; the tiles of a
+// last, partial round are split along K into parts that add into S with
+// integer atomics (exact, any order), so that round is as full as the others.
+constexpr int SY_B = 256, SY_BK = 64, SY_NBUF = 4;
+constexpr int SY_PANEL = SY_B * SY_BK;    // 16 KB per operand per stage
+constexpr int SY_STAGE = 2 * SY_PANEL;    // 32 KB
+constexpr int SY_CPO = SY_PANEL / 1024 / 4;  // 1 KB copies per operand per wave per stage (4)
+constexpr int SY_CPW = 2 * SY_CPO;        // copies per wave per stage (8)
 
 __device__ __forceinline__ void tri_block(int t, int& rb, int& cb) {
   int r = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
@@ -34,132 +49,78 @@ __device__ __forceinline__ void tri_block(int t, int& rb, int& cb) {
   cb = t - r * (r + 1) / 2;
 }
 
-// S += X0 X0^T for 128x128 tiles with row block >= column block.
-__global__ void __launch_bounds__(256) k_syrk_i8(const int8_t* __restrict__ gT, int64_t ld,
-                                                 int64_t n, int64_t K, int32_t* __restrict__ S) {
-  __shared__ __attribute__((aligned(16))) uint8_t As[K_BM * K_LS];
-  __shared__ __attribute__((aligned(16))) uint8_t Bs[K_BM * K_LS];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  int rb, cb;
-  tri_block(blockIdx.x, rb, cb);
-  const int64_t i0 = (int64_t)rb * K_BM, j0 = (int64_t)cb * K_BM;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-  const int lr = lane & 15, lk = lane >> 4;
-  i4_t acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = (i4_t){0, 0, 0, 0};
-
-  for (int64_t k0 = 0; k0 < K; k0 += K_BK) {
-    // 128 rows x 128 bytes per operand = 1024 x 16 B chunks; 4 per thread
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int idx = tid + 256 * u;
-      const int r = idx >> 3, c = (idx & 7) * 16;
-      uint4 wa = {0, 0, 0, 0}, wb = {0, 0, 0, 0};
-      const int64_t ia = i0 + r, ib = j0 + r;
-      if (k0 + c + 16 <= K) {
-        if (ia < n) wa = *reinterpret_cast<const uint4*>(gT + ia * ld + k0 + c);
-        if (ib < n) wb = *reinterpret_cast<const uint4*>(gT + ib * ld + k0 + c);
-      } else if (k0 + c < K) {
-        uint8_t ta[16], tb[16];
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-          const bool in = (k0 + c + e) < K;
-          ta[e] = (in && ia < n) ? (uint8_t)gT[ia * ld + k0 + c + e] : 0;
-          tb[e] = (in && ib < n) ? (uint8_t)gT[ib * ld + k0 + c + e] : 0;
-        }
-        memcpy(&wa, ta, 16);
-        memcpy(&wb, tb, 16);
-      }
-      wa.x = zero_neg_bytes(wa.x); wa.y = zero_neg_bytes(wa.y);
-      wa.z = zero_neg_bytes(wa.z); wa.w = zero_neg_bytes(wa.w);
-      wb.x = zero_neg_bytes(wb.x); wb.y = zero_neg_bytes(wb.y);
-      wb.z = zero_neg_bytes(wb.z); wb.w = zero_neg_bytes(wb.w);
-      *reinterpret_cast<uint4*>(&As[r * K_LS + c]) = wa;
-      *reinterpret_cast<uint4*>(&Bs[r * K_LS + c]) = wb;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < K_BK; kk += 64) {
-      i4_t af[4], bf[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        af[t] = *reinterpret_cast<const i4_t*>(&As[(wr + t * 16 + lr) * K_LS + kk + 16 * lk]);
-        bf[t] = *reinterpret_cast<const i4_t*>(&Bs[(wc + t * 16 + lr) * K_LS + kk + 16 * lk]);
-      }
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
-    }
-    __syncthreads();
+// Tile t of the group order: groups of 4 row panels (tiles before group g:
+// sum_{r < 4g} (r + 1) = 8g^2 + 2g), column-major inside a group (rows R0..R1
+// for columns <= R0, rows cb..R1 for the columns of the group's own triangle).
+__device__ __forceinline__ void group_tile(int t, int nb, int& rb, int& cb) {
+  int g = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.125f);
+  while (8 * (g + 1) * (g + 1) + 2 * (g + 1) <= t) ++g;
+  while (8 * g * g + 2 * g > t) --g;
+  const int R0 = 4 * g, R1 = min(4 * g + 3, nb - 1), h = R1 - R0 + 1;
+  int u = t - (8 * g * g + 2 * g);
+  if (u < (R0 + 1) * h) {
+    cb = u / h;
+    rb = R0 + u % h;
+    return;
   }
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t i = i0 + wr + a * 16 + 4 * lk + r;
-        const int64_t j = j0 + wc + b * 16 + lr;
-        if (i < n && j < n) S[i * n + j] += acc[a][b][r];
-      }
+  u -= (R0 + 1) * h;
+  cb = R0 + 1;
+  while (u >= R1 - cb + 1) {
+    u -= R1 - cb + 1;
+    ++cb;
+  }
+  rb = cb + u;
 }
 
-// S += X0 X0^T on 256x256 lower tiles (row block >= column block), for X0^T
-// with missing calls already 0 (jm_transpose_i8's out0). 4 waves, each a
-// 128x128 tile of 8x8 MFMA tiles (int32 accumulators in AGPRs). Both operand
-// panels (256 rows x 128 bytes each) are staged straight from global memory
-// into double-buffered LDS with global_load_lds_dwordx4, one barrier per
-// 128-deep K step; rows are XOR-swizzled in 16-byte chunks (chunk c of row r at
-// c ^ ((r >> 1) & 7): with 128-byte rows the 4-bank group of a chunk is
-// 8 (r & 1) + c, so this gives 16 distinct groups to the 16 lanes of a
-// ds_read_b128 phase). Lower tiles are enumerated XCD-major: the blocks resident on
-// one XCD sweep consecutive tiles of the same row panel, which then stays in
-// that XCD's L2.
-constexpr int S2_B = 256, S2_BK = 128;
-constexpr int S2_PANEL = S2_B * S2_BK;  // 32 KB per operand
-constexpr int S2_STAGE = 2 * S2_PANEL;  // 64 KB
+// Copy 16 bytes per lane from sbase + voff (SGPR base, 32-bit lane offset) to
+// lds_base + 16 * lane; ordering as for lds_copy16.
+__device__ __forceinline__ void lds_copy16_s(const void* sbase, uint32_t voff,
+                                             const uint8_t* lds_base) {
+  const uint32_t m0v = (uint32_t)(uintptr_t)(const lds_u8_t*)lds_base;
+  asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1"
+               :
+               : "v"(voff), "s"(sbase), "s"(m0v)
+               : "memory", "m0");
+}
 
-typedef __attribute__((address_space(3))) void lds_void_t;
-
-__global__ void __launch_bounds__(256) k_syrk_i8_v2(const int8_t* __restrict__ gT0, int64_t ld,
-                                                    int64_t n, int64_t K,
-                                                    int32_t* __restrict__ S) {
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * S2_STAGE];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nb = (int)((n + S2_B - 1) / S2_B);
-  const int total = nb * (nb + 1) / 2;
-  int t;
-  {  // XCD-major bijection of the lower tiles
+// ATOMIC = false: tile t = XCD-mapped block id < n_full, whole K, S += acc.
+// ATOMIC = true: block u -> tile n_full + u / split, K part u % split, atomicAdd.
+template <bool ATOMIC>
+__global__ void __launch_bounds__(256) k_syrk_i8(const int8_t* __restrict__ gT0, int64_t ld,
+                                                 int64_t n, int64_t K, int32_t* __restrict__ S,
+                                                 int n_full, int split) {
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[SY_NBUF * SY_STAGE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nb = (int)((n + SY_B - 1) / SY_B);
+  int t, part = 0;
+  if (!ATOMIC) {
     const int x = blockIdx.x % 8, j = blockIdx.x / 8;
-    const int base = total / 8, rem = total % 8;
+    const int base = n_full / 8, rem = n_full % 8;
     t = x * base + (x < rem ? x : rem) + j;
+  } else {
+    t = n_full + blockIdx.x / split;
+    part = blockIdx.x % split;
   }
   int rb, cb;
-  tri_block(t, rb, cb);
-  const int64_t i0 = (int64_t)rb * S2_B, j0 = (int64_t)cb * S2_B;
+  group_tile(t, nb, rb, cb);
+  const int nk_all = (int)((K + SY_BK - 1) / SY_BK);
+  const int kb0 = ATOMIC ? (int)((int64_t)nk_all * part / split) : 0;
+  const int nk = (ATOMIC ? (int)((int64_t)nk_all * (part + 1) / split) : nk_all) - kb0;
+  const int64_t i0 = (int64_t)rb * SY_B, j0 = (int64_t)cb * SY_B;
   const int wr = (wave >> 1) * 128, wc = (wave & 1) * 128;
   const int lr = lane & 15, lk = lane >> 4;
-  // DMA: per operand 32 instructions of 8 rows x 128 B; each wave issues 8 of each
-  const int d_r = lane >> 3, d_cs = lane & 7;
-  auto issue = [&](int stage, int64_t k0) {
-    uint8_t* base = lds + stage * S2_STAGE;
+  // a copy moves 16 rows x 64 B: lane -> row d_r, slot d_s; the chunk in the slot is
+  // d_s ^ ((row >> 1) & 3), which depends on the lane only (16-row copies)
+  const int d_r = lane >> 2, d_s = lane & 3;
+  const uint32_t voff = (uint32_t)(d_r * ld + 16 * (d_s ^ ((d_r >> 1) & 3)));
+  auto issue = [&](uint8_t* stage, int64_t k0) {
 #pragma unroll
     for (int op = 0; op < 2; ++op) {
       const int64_t r0 = op ? j0 : i0;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int q = wave * 8 + u;  // 0..31
-        const int row = 8 * q + d_r;
-        int64_t gr = r0 + row;
-        if (gr >= n) gr = n - 1;  // rows past the end load a valid row; never stored
-        const int c = d_cs ^ ((row >> 1) & 7);
-        __builtin_amdgcn_global_load_lds(gT0 + gr * ld + k0 + 16 * c,
-                                         (lds_void_t*)(base + op * S2_PANEL + q * 1024), 16, 0, 0);
+      for (int u = 0; u < SY_CPO; ++u) {
+        const int q = wave * SY_CPO + u;  // rows 16q .. 16q+15 of the panel
+        lds_copy16_s(gT0 + (r0 + 16 * q) * ld + k0, voff, stage + op * SY_PANEL + q * 1024);
       }
     }
   };
@@ -168,45 +129,59 @@ __global__ void __launch_bounds__(256) k_syrk_i8_v2(const int8_t* __restrict__ g
   for (int a = 0; a < 8; ++a)
 #pragma unroll
     for (int b = 0; b < 8; ++b) acc[a][b] = (i4_t){0, 0, 0, 0};
-  const int nk = (int)((K + S2_BK - 1) / S2_BK);
-  issue(0, 0);
+#pragma unroll
+  for (int st = 0; st < SY_NBUF - 1; ++st)
+    if (st < nk) issue(lds + st * SY_STAGE, (int64_t)(kb0 + st) * SY_BK);
+  int cur = 0;  // LDS stage of K step kt
   for (int kt = 0; kt < nk; ++kt) {
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (kt + 1 < nk) issue((kt + 1) & 1, (int64_t)(kt + 1) * S2_BK);
-    const uint8_t* base = lds + (kt & 1) * S2_STAGE;
-#pragma unroll
-    for (int kh = 0; kh < 2; ++kh) {  // two 64-deep MFMA steps per stage
-      i4_t af[8], bf[8];
-#pragma unroll
-      for (int ti = 0; ti < 8; ++ti) {
-        const int r = wr + 16 * ti + lr;
-        const int c = (4 * kh + lk) ^ ((r >> 1) & 7);
-        memcpy(&af[ti], base + r * S2_BK + 16 * c, 16);
-      }
-#pragma unroll
-      for (int tj = 0; tj < 8; ++tj) {
-        const int r = wc + 16 * tj + lr;
-        const int c = (4 * kh + lk) ^ ((r >> 1) & 7);
-        memcpy(&bf[tj], base + S2_PANEL + r * S2_BK + 16 * c, 16);
-      }
-#pragma unroll
-      for (int ti = 0; ti < 8; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < 8; ++tj)
-          acc[ti][tj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[ti], bf[tj], acc[ti][tj], 0, 0, 0);
+    // this wave's copies of steps kt .. kt+NBUF-2 may be in flight (in order);
+    // keep the later ones, wait for step kt
+    if (kt + SY_NBUF - 2 < nk)
+      wait_vmcnt<SY_CPW * (SY_NBUF - 2)>();
+    else
+      wait_vmcnt<0>();
+    __syncthreads();  // every wave's copies of step kt landed; step kt-1's stage is free
+    if (kt + SY_NBUF - 1 < nk) {
+      const int nxt = cur == 0 ? SY_NBUF - 1 : cur - 1;
+      issue(lds + nxt * SY_STAGE, (int64_t)(kb0 + kt + SY_NBUF - 1) * SY_BK);
     }
+    const uint8_t* base = lds + cur * SY_STAGE;
+    i4_t af[8], bf[8];
+#pragma unroll
+    for (int ti = 0; ti < 8; ++ti) {
+      const int r = wr + 16 * ti + lr;
+      memcpy(&af[ti], base + r * SY_BK + 16 * (lk ^ ((r >> 1) & 3)), 16);
+    }
+#pragma unroll
+    for (int tj = 0; tj < 8; ++tj) {
+      const int r = wc + 16 * tj + lr;
+      memcpy(&bf[tj], base + SY_PANEL + r * SY_BK + 16 * (lk ^ ((r >> 1) & 3)), 16);
+    }
+#pragma unroll
+    for (int ti = 0; ti < 8; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 8; ++tj)
+        acc[ti][tj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[ti], bf[tj], acc[ti][tj], 0, 0, 0);
+    cur = cur + 1 == SY_NBUF ? 0 : cur + 1;
   }
 #pragma unroll
-  for (int ti = 0; ti < 8; ++ti)
+  for (int ti = 0; ti < 8; ++ti) {
+    // one 16-row group at a time: bounds the read-modify-write loads in flight
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int tj = 0; tj < 8; ++tj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
         const int64_t j = j0 + wc + tj * 16 + lr;
-        if (i < n && j < n) S[i * n + j] += acc[ti][tj][r];
+        if (i < n && j < n) {
+          if (ATOMIC)
+            atomicAdd(&S[i * n + j], acc[ti][tj][r]);
+          else
+            S[i * n + j] += acc[ti][tj][r];
+        }
       }
+  }
 }
 
 // u[i] += sum_j (g_ij >= 0 ? g_ij mu_j : mu_j^2), one block per sample row.
@@ -353,13 +328,31 @@ extern "C" int jm_kinship_syrk_i8(const int8_t* gT0, int64_t ld_gT, int64_t n, i
   JM_CHECK_ARG(n >= 1 && n_snps >= 0, "bad sizes");
   if (n_snps == 0) return JM_OK;
   JM_CHECK_ARG(gT0 && S, "null pointer");
-  JM_CHECK_ARG(ld_gT >= n_snps && ld_gT % S2_BK == 0 && ((uintptr_t)gT0 % 16) == 0,
+  JM_CHECK_ARG(ld_gT >= n_snps && ld_gT % 128 == 0 && ((uintptr_t)gT0 % 16) == 0,
                "gT0 rows must be 16-byte aligned with ld_gT >= n_snps, a multiple of 128");
-  const int64_t nb = (n + S2_B - 1) / S2_B;
-  JM_CHECK_ARG(nb * (nb + 1) / 2 < (1LL << 31), "n too large");
-  hipLaunchKernelGGL(k_syrk_i8_v2, dim3((unsigned)(nb * (nb + 1) / 2)), dim3(256), 0,
-                     (hipStream_t)stream, gT0, ld_gT, n, n_snps, S);
-  JM_CHECK_LAUNCH();
+  JM_CHECK_ARG(16 * ld_gT < (1LL << 32), "ld_gT too large");
+  const int64_t nb = (n + SY_B - 1) / SY_B;
+  JM_CHECK_ARG(nb * (nb + 1) / 2 < (1LL << 28), "n too large");
+  const int total = (int)(nb * (nb + 1) / 2);
+  const int nk = (int)((n_snps + SY_BK - 1) / SY_BK);
+  // whole rounds of 256 tiles run unsplit; the tiles of a partial last round are
+  // split along K (at least 4 steps per part) to fill that round
+  const int tail = total % 256;
+  int split = tail ? 256 / tail : 1;
+  if (split > nk / 4) split = nk / 4;
+  if (split < 1) split = 1;
+  const int n_full = split > 1 ? total - tail : total;
+  hipStream_t st = (hipStream_t)stream;
+  if (n_full > 0) {
+    hipLaunchKernelGGL(k_syrk_i8<false>, dim3(n_full), dim3(256), 0, st, gT0, ld_gT, n, n_snps, S,
+                       n_full, 1);
+    JM_CHECK_LAUNCH();
+  }
+  if (total > n_full) {
+    hipLaunchKernelGGL(k_syrk_i8<true>, dim3((total - n_full) * split), dim3(256), 0, st, gT0,
+                       ld_gT, n, n_snps, S, n_full, split);
+    JM_CHECK_LAUNCH();
+  }
   return JM_OK;
 }
 

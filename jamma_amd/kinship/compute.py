@@ -27,16 +27,60 @@ logger = logging.getLogger("jamma_amd")
 
 
 class KinshipAccumulator:
-    """Device accumulators S (int32), u, C (float64) and s0 over SNP tiles."""
+    """Device accumulators S (int32), u, C (float64) and s0 over SNP tiles.
 
-    def __init__(self, n: int, device=None):
+    The int8 GEMM operand of incoming tiles is staged (sample-major, missing calls
+    0, rows padded to a multiple of 256) in one buffer of up to ``stage_bytes``;
+    jm_kinship_syrk_i8 runs once per full buffer and when ``S`` is read, so the
+    reference's 10,000-SNP batches cost one large launch instead of many small ones.
+    The imputation terms u and C are applied per tile."""
+
+    def __init__(self, n: int, device=None, stage_bytes: int = 1 << 31):
         self.device = device or require_gpu()
         self.n = n
-        self.S = torch.zeros((n, n), dtype=torch.int32, device=self.device)
+        self.n_pad = round_up(n, 256)
+        self._S = torch.zeros((n, n), dtype=torch.int32, device=self.device)
         self.u = torch.zeros((n,), dtype=torch.float64, device=self.device)
         self.C = None
         self.s0 = 0.0
         self.p = 0
+        self.stage_cols = int(min(1 << 17, max(128, (stage_bytes // self.n_pad) // 128 * 128)))
+        self._stage = None
+        self._fill = 0
+        self.syrk_timer = None  # optional (start, end) torch.cuda.Event pair around each SYRK
+
+    @property
+    def S(self) -> torch.Tensor:
+        self.flush()
+        return self._S
+
+    @S.setter
+    def S(self, value: torch.Tensor) -> None:
+        self.flush()
+        self._S = value
+
+    def _syrk(self, gT0: torch.Tensor, ld: int, n_snps: int) -> None:
+        if self.syrk_timer:
+            self.syrk_timer[0].record()
+        _lib.call("jm_kinship_syrk_i8", ptr(gT0), ld, self.n, n_snps, ptr(self._S), stream_ptr())
+        if self.syrk_timer:
+            self.syrk_timer[1].record()
+
+    def flush(self) -> None:
+        """Run the GEMM over the staged SNPs (S += X0 X0^T)."""
+        if self._fill == 0:
+            return
+        k, cols = self._fill, self.stage_cols
+        kp = min(round_up(k, 128), cols)
+        if kp > k:  # columns left from an earlier, longer fill must read as zero
+            self._stage[:, k:kp].zero_()
+        self._syrk(self._stage, cols, k)
+        self._fill = 0
+
+    def release(self) -> None:
+        """Flush and free the staging buffer."""
+        self.flush()
+        self._stage = None
 
     def add_tile(self, g: torch.Tensor, ld_g: int, n_snps: int, mu: np.ndarray,
                  has_missing: bool) -> None:
@@ -45,15 +89,26 @@ class KinshipAccumulator:
             return
         n = self.n
         ldt = round_up(n_snps, 128)
+        st = stream_ptr()
         gT = torch.empty((n, ldt), dtype=torch.int8, device=self.device)
-        gT0 = torch.empty((n, ldt), dtype=torch.int8, device=self.device)
         if ldt > n_snps:
             gT[:, n_snps:].zero_()
-            gT0[:, n_snps:].zero_()
-        st = stream_ptr()
-        _lib.call("jm_transpose_i8", ptr(g), n_snps, n, ld_g, ptr(gT), ldt, ptr(gT0), st)
-        _lib.call("jm_kinship_syrk_i8", ptr(gT0), ldt, n, n_snps, ptr(self.S), st)
-        del gT0
+        if n_snps > self.stage_cols:  # larger than the staging buffer: its own launch
+            self.flush()
+            gT0 = torch.zeros((self.n_pad, ldt), dtype=torch.int8, device=self.device)
+            _lib.call("jm_transpose_i8", ptr(g), n_snps, n, ld_g, ptr(gT), ldt, ptr(gT0), ldt, st)
+            self._syrk(gT0, ldt, n_snps)
+            del gT0
+        else:
+            if self._fill + n_snps > self.stage_cols:
+                self.flush()
+            if self._stage is None:  # padded rows stay zero
+                self._stage = torch.zeros((self.n_pad, self.stage_cols), dtype=torch.int8,
+                                          device=self.device)
+            dst = self._stage[:, self._fill:]
+            _lib.call("jm_transpose_i8", ptr(g), n_snps, n, ld_g, ptr(gT), ldt, ptr(dst),
+                      self.stage_cols, st)
+            self._fill += n_snps
         mu_d = to_dev(np.asarray(mu, dtype=np.float64), torch.float64, self.device)
         if has_missing and self.C is None:
             self.C = torch.zeros((n, n), dtype=torch.float64, device=self.device)
@@ -66,6 +121,9 @@ class KinshipAccumulator:
     def finalize(self, p: float | None = None, sub: "KinshipAccumulator | None" = None) -> torch.Tensor:
         """K = (S - u 1' - 1 u' + s0 + C + C') / p, optionally minus ``sub`` (LOCO)."""
         p = float(self.p if p is None else p)
+        self.release()
+        if sub is not None:
+            sub.release()
         K = torch.empty((self.n, self.n), dtype=torch.float64, device=self.device)
         _lib.call("jm_kinship_finalize", ptr(self.S), ptr(self.u), ptr(self.C), self.s0,
                   ptr(sub.S) if sub else None, ptr(sub.u) if sub else None,
@@ -209,7 +267,9 @@ def loco_kinship_streaming_device(bed_path: Path, chunk_size: int = 10_000,
                                        ksnps_indices)
     chr_f = chrom[idx]
     names = sorted(set(chr_f.tolist()))
-    accs = {c: KinshipAccumulator(bed.n_samples) for c in names}
+    # one staging budget shared by the per-chromosome accumulators
+    accs = {c: KinshipAccumulator(bed.n_samples, stage_bytes=(1 << 31) // len(names))
+            for c in names}
 
     def split(sel):
         cs = chrom[sel]
@@ -270,6 +330,8 @@ def compute_loco_kinship(genotypes: np.ndarray, chromosome_for_each_snp: np.ndar
     accs = {c: KinshipAccumulator(n) for c in names}
     for c in names:
         sel_c = idx[chrom[idx] == c]
+        if c != names[0]:
+            accs[names[names.index(c) - 1]].release()  # chromosomes run one after another
         for s in range(0, len(sel_c), batch_size):
             sel = sel_c[s:s + batch_size]
             tile = np.zeros((len(sel), ld), dtype=np.int8)

@@ -95,6 +95,39 @@ def test_kinship_int8_matches_oracle(n, p, miss, batch):
     assert np.array_equal(K, K.T)
 
 
+@pytest.mark.parametrize("n,tiles,stage_cols", [
+    (600, [300, 300, 700, 1200, 37], 512),  # flush when full, stale columns, oversized tiles
+    (5700, [3100], None),                   # 276 tiles: a full round + K-split tail round
+    (129, [70], None),                      # one tile, fewer K steps than a split needs
+])
+def test_syrk_int32_exact(n, tiles, stage_cols):
+    """S = X0 X0^T (missing calls as 0) is exact integer arithmetic: compare the lower
+    triangle with a float64 product of the same integers (exact below 2^53)."""
+    from jamma_amd.device import round_up, to_dev
+    from jamma_amd.kinship.compute import KinshipAccumulator
+
+    rng = np.random.default_rng(n)
+    p = sum(tiles)
+    g = rng.integers(0, 3, size=(p, n)).astype(np.int8)
+    g[rng.random(g.shape) < 0.01] = -1
+    stage = 1 << 31 if stage_cols is None else stage_cols * round_up(n, 256)
+    acc = KinshipAccumulator(n, stage_bytes=stage)
+    if stage_cols is not None:
+        assert acc.stage_cols == stage_cols
+    ld = round_up(n, 64)
+    s0 = 0
+    for t in tiles:
+        tile = np.zeros((t, ld), np.int8)
+        tile[:, :n] = g[s0:s0 + t]
+        acc.add_tile(to_dev(tile, torch.int8), ld, t, np.zeros(t), False)
+        s0 += t
+    S = acc.S.cpu().numpy()
+    x0 = np.where(g < 0, 0, g).astype(np.float64)
+    ref = (x0.T @ x0).astype(np.int64)
+    low = np.tril_indices(n)
+    assert np.array_equal(S[low], ref[low])
+
+
 def test_loco_kinship_matches_oracle():
     from jamma_amd.kinship import compute_loco_kinship
 
