@@ -261,6 +261,8 @@ typedef struct jm_lmm_problem {
   double logl_null;     /* null-model MLE log-likelihood (LRT) */
   const double* cl_weight;     /* C : cluster sizes (weight of log|1+lambda d_c| in
                                   logdet_h; equals logdet_mom of the m = 0 term) */
+  double* iab_shared;   /* n_index : shared pair sums at lambda = 0 (Iab), filled by
+                           jm_lmm_prepare */
 } jm_lmm_problem;
 
 /* Output columns of jm_lmm_assoc (n_snps x JM_NOUT float64). */

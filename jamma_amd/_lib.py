@@ -41,7 +41,7 @@ class LmmProblem(ctypes.Structure):
         ("grid_kappa", c_void_p), ("grid_shared", c_void_p), ("grid_logdet", c_void_p),
         ("null_kappa", c_void_p), ("null_shared", c_void_p),
         ("lambda_null", c_double), ("logl_null", c_double),
-        ("cl_weight", c_void_p),
+        ("cl_weight", c_void_p), ("iab_shared", c_void_p),
     ]
 
 

@@ -310,6 +310,16 @@ __global__ void k_lmm_prepare(jm_lmm_problem pr) {
     else
       pr.null_kappa[t] = kap;
   }
+  // shared pair sums at lambda = 0 (Iab: kappa_t = [m_t == 0])
+  for (int j = tid; j < ni; j += nthreads) {
+    double acc = 0.0;
+    if (pr.pair_row[j] < 0) {
+      const double* row = pr.shared_mom + (int64_t)j * T;
+      for (int t = 0; t < T; ++t)
+        if (pr.term_m[t] == 0) acc += row[t];
+    }
+    pr.iab_shared[j] = acc;
+  }
   // logdet per grid point + shared pair sums (one thread per (g, pair|logdet))
   for (int idx = tid; idx < (G + 1) * (ni + 1); idx += nthreads) {
     const int g = idx / (ni + 1), j = idx % (ni + 1);
@@ -423,6 +433,18 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx) {
 }
 
 constexpr int kMaxLevels = 24;  // n_cvt <= 20 -> levels 0..n_cvt
+
+// Address-space views: constant (wave-uniform reads become scalar loads) and global
+// (vector loads without the flat-address path).
+typedef const double __attribute__((address_space(4))) cdbl;
+typedef const int32_t __attribute__((address_space(4))) cint;
+typedef const double __attribute__((address_space(1))) gdbl;
+
+// k_lmm_assoc LDS per wave (doubles): two packed rows (ni + logdet slot each), the
+// Iab level logs, and ni + 1 int slots.
+__host__ __device__ constexpr int assoc_lds_per_wave(int ni_max) {
+  return 2 * (ni_max + 1) + kMaxLevels + (ni_max + 2) / 2;
+}
 constexpr int kXR = 8;          // per-SNP moment rows held in registers
 
 __device__ __forceinline__ void wave_sync() {
@@ -539,6 +561,136 @@ __device__ __forceinline__ int xpair_index(int r, int nc) {
   return ab_index(nc + 1, nc + 1, nc);
 }
 
+// ---------------------------------------------------------------------------
+// Grid stage of the REML and MLE searches
+// ---------------------------------------------------------------------------
+// _batch_grid_reml_with_iab / _batch_grid_mle (likelihood_jax.py:565-598,
+// 801-827) and the argmax that brackets the golden section (:539-547, NaN
+// first, first index on ties), one wavefront per SNP. Column k of a 64-column
+// pass is lane k: column 0 is Iab (lambda = 0, kappa_t = [m_t == 0]), column
+// k >= 1 grid point g = k - 1. A lane walks the T terms once per 8 moment rows:
+// kappa from the grid table (vector loads, the same table for every SNP) times
+// the SNP's moments (wave-uniform scalar loads). Row 0 of its Pab column is
+// completed with the shared pairs (table) in LDS and the recursion runs per
+// lane. The REML log-determinant term subtracts the Iab level logs in the
+// reference's interleaved order once column 0 has them. Output: the argmax
+// grid indices of REML and MLE, stored as doubles in the L_REMLE / L_MLE
+// columns of the SNP's output row (k_lmm_assoc reads and replaces them).
+template <bool MULTI>
+__global__ void __launch_bounds__(256) k_lmm_grid(jm_lmm_problem pr_one, int mode_one,
+                                                  const double* xmom, int64_t ldm, int64_t n_snps,
+                                                  double* out, const jm_lmm_problem* probs,
+                                                  const int32_t* modes, const int32_t* snp_prob,
+                                                  const double* const* snp_xmom, int ni_max) {
+  extern __shared__ double smem[];
+  const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
+  const int wpb = blockDim.x >> 6;
+  double* P = smem + (size_t)wib * (ni_max * 64 + kMaxLevels);  // column per lane
+  double* iab = P + ni_max * 64;                                  // Iab level logs
+  for (int64_t s = (int64_t)blockIdx.x * wpb + wib; s < n_snps; s += (int64_t)gridDim.x * wpb) {
+    const int pidx = MULTI ? snp_prob[s] : 0;
+    const int mode = MULTI ? modes[pidx] : mode_one;
+    if (mode == 3) continue;  // Score only: no search
+    const jm_lmm_problem& pr = MULTI ? probs[pidx] : pr_one;
+    const int nc = pr.n_cvt, T = pr.n_terms, n = pr.n, G = pr.n_grid;
+    const int ni = (nc + 3) * (nc + 2) / 2;
+    const int nrows = nc + 2;
+    // the SNP's moments and term powers are wave-uniform: constant address space,
+    // so they come through scalar loads; the grid kappa table through vector loads
+    const cdbl* xm = (const cdbl*)uniform_ptr(MULTI ? snp_xmom[s]
+                                                   : xmom + s * (int64_t)nrows * ldm);
+    const cint* term_m = (const cint*)pr.term_m;
+    const gdbl* gk = (const gdbl*)pr.grid_kappa;
+    double best_r = -INFINITY, best_m = -INFINITY;
+    int bi_r = 0x7fffffff, bi_m = 0x7fffffff;
+    for (int k0 = 0; k0 <= G; k0 += 64) {
+      const int k = k0 + lane;
+      const bool act = k <= G;
+      const bool is_iab = k == 0;
+      const int g = act && !is_iab ? k - 1 : 0;
+      for (int r0 = 0; r0 < nrows; r0 += 8) {
+        // rows past nrows repeat the last row (no branches in the term loop)
+        int64_t roff[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) roff[r] = (int64_t)min(r0 + r, nrows - 1) * ldm;
+        double acc[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) acc[r] = 0.0;
+        // kappa of terms t .. t+3, the next four in flight while these are used
+        auto kap4 = [&](int t0, double (&kv)[4]) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int t = min(t0 + u, T - 1);
+            const double kg = gk[(int64_t)t * G + g];
+            const double ki = term_m[t] == 0 ? 1.0 : 0.0;
+            kv[u] = (t0 + u < T) ? (is_iab ? ki : kg) : 0.0;
+          }
+        };
+        double kc[4], kn[4];
+        kap4(0, kn);
+        const int T4 = T & ~3;
+        for (int t0 = 0; t0 < T4; t0 += 4) {  // contiguous moment reads: scalar x8 loads
+#pragma unroll
+          for (int u = 0; u < 4; ++u) kc[u] = kn[u];
+          kap4(t0 + 4, kn);
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < 8; ++r) acc[r] = fma(kc[u], xm[roff[r] + t0 + u], acc[r]);
+        }
+        for (int u = 0; u < T - T4; ++u)  // kn holds terms T4 .. T - 1
+#pragma unroll
+          for (int r = 0; r < 8; ++r) acc[r] = fma(kn[u], xm[roff[r] + T4 + u], acc[r]);
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          if (r0 + r < nrows) P[xpair_index(r0 + r, nc) * 64 + lane] = acc[r];
+      }
+      for (int j = 0; j < ni; ++j)
+        if (pr.pair_row[j] < 0)
+          P[j * 64 + lane] = is_iab ? pr.iab_shared[j] : pr.grid_shared[(int64_t)g * ni + j];
+      wave_sync();
+      PabSummary o;
+      pab_recursion(P + lane, 64, nc, o, nullptr, nullptr);
+      // the level pivots stay on the column's diagonal: (p+1, p+1) is final after level p
+      if (k0 == 0 && lane == 0) {
+        for (int p = 0; p <= nc; ++p) {
+          const double v = P[ab_index(p + 1, p + 1, nc) * 64];
+          iab[p] = (v > 0.0) ? log(v) : 0.0;
+        }
+      }
+      wave_sync();
+      if (act && !is_iab) {
+        double hiw = 0.0;  // likelihood_jax.py:322-327 order
+        for (int p = 0; p <= nc; ++p) {
+          const double v = P[ab_index(p + 1, p + 1, nc) * 64 + lane];
+          hiw = hiw + ((v > 0.0) ? log(v) : 0.0);
+          hiw = hiw - iab[p];
+        }
+        o.hiw = hiw;
+        const double ldh = pr.grid_logdet[g];
+        const double fr = reml_value(o, nc, n, ldh);
+        const double fm = mle_value(o.pyy1, n, ldh);
+        if (bi_r == 0x7fffffff || better(fr, g, best_r, bi_r)) {
+          best_r = fr;
+          bi_r = g;
+        }
+        if (bi_m == 0x7fffffff || better(fm, g, best_m, bi_m)) {
+          best_m = fm;
+          bi_m = g;
+        }
+      }
+      wave_sync();  // the next pass reuses the columns
+    }
+    wave_argmax(best_r, bi_r);
+    wave_argmax(best_m, bi_m);
+    if (lane == 0) {
+      out[s * JM_NOUT + JM_OUT_L_REMLE] = (double)bi_r;
+      out[s * JM_NOUT + JM_OUT_L_MLE] = (double)bi_m;
+    }
+  }
+}
+
 // Packed index of the k-th pair (a <= b) over the covariate / phenotype indices
 // {1..c, c+2} (the pairs that do not involve the genotype), in packed order.
 __device__ __forceinline__ int shared_pair_index(int k, int nc) {
@@ -599,7 +751,7 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
-  const int per_wave = ni_max * 64 + kMaxLevels + ni_max + 1;
+  const int per_wave = assoc_lds_per_wave(ni_max);
   // srow_n > 0 (single problem): the block keeps the srow_n shared pair rows
   // (covariate / phenotype moments) in LDS, in slot order, ahead of the waves
   double* srows = smem;
@@ -612,15 +764,14 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
     }
     __syncthreads();
   }
-  double* P = smem + srow_len + (size_t)wib * per_wave;  // grid columns / two packed rows
-  double* iab_ld = P + ni_max * 64;           // kMaxLevels
-  int* slot = (int*)(iab_ld + kMaxLevels);    // memory-phase slots (<= ni_max + 1 ints)
+  double* P = smem + srow_len + (size_t)wib * per_wave;  // two packed rows
   double* PA = P;                              // row A: ni entries + logdet slot
   double* PB = P + ni_max + 1;                 // row B
+  double* iab_ld = P + 2 * (ni_max + 1);      // kMaxLevels
+  int* slot = (int*)(iab_ld + kMaxLevels);    // memory-phase slots (<= ni_max + 1 ints)
   const double phi = 0.6180339887498949;
   const int h = lane >> 5;  // half-wave: 0 -> REML / lambda A, 1 -> MLE / lambda B
   const int q = lane & 31;
-
   for (int64_t s = (int64_t)blockIdx.x * wpb + wib; s < n_snps; s += (int64_t)gridDim.x * wpb) {
     const int pidx = MULTI ? snp_prob[s] : 0;
     const jm_lmm_problem pr = MULTI ? probs[pidx] : pr_one;
@@ -661,10 +812,13 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
     wave_sync();
 
     double* os = out + s * JM_NOUT;
-    if (lane < JM_NOUT) os[lane] = NAN;  // columns a mode does not compute stay NaN
+    // grid argmax indices left by k_lmm_grid, then columns a mode does not compute are NaN
+    const int gi_r = do_golden ? (int)os[JM_OUT_L_REMLE] : 0;
+    const int gi_m = do_golden ? (int)os[JM_OUT_L_MLE] : 0;
+    wave_sync();
+    if (lane < JM_NOUT) os[lane] = NAN;
     int evals = 0;
-    double best_r = -INFINITY, best_m = -INFINITY;
-    int bi_r = 0x7fffffff, bi_m = 0x7fffffff;
+    int bi_r = 0, bi_m = 0;
     double aA = 0, bA = 0, cA = 0, dA = 0, fcA = 0, fdA = 0;
     double aB = 0, bB = 0, cB = 0, dB = 0, fcB = 0, fdB = 0;
     bool klA = false, klB = false;
@@ -893,55 +1047,11 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
 
       // ---- consume the stage ----
       if (stage == 0) {
-        // grid stage: lane g evaluates REML and MLE at lambda_g
+        // Iab done; the grid stage ran in k_lmm_grid
         if (do_golden) {
-          for (int g0 = 0; g0 < G; g0 += 64) {
-            const int g = g0 + lane;
-            const bool gv = g < G;
-            double acc[kXR];
-#pragma unroll
-            for (int r = 0; r < kXR; ++r) acc[r] = 0.0;
-            for (int t = 0; t < T; ++t) {
-              const double kg = gv ? pr.grid_kappa[(int64_t)t * G + g] : 0.0;
-#pragma unroll
-              for (int r = 0; r < kXR; ++r)
-                if (r < nrows) acc[r] = fma(kg, xm[(int64_t)r * ldm + t], acc[r]);
-            }
-            wave_sync();
-#pragma unroll
-            for (int r = 0; r < kXR; ++r)
-              if (r < nrows) P[xpair_index(r, nc) * 64 + lane] = acc[r];
-            for (int r = kXR; r < nrows; ++r) {
-              double a = 0.0;
-              for (int t = 0; t < T; ++t)
-                a = fma(gv ? pr.grid_kappa[(int64_t)t * G + g] : 0.0, xm[(int64_t)r * ldm + t], a);
-              P[xpair_index(r, nc) * 64 + lane] = a;
-            }
-            for (int u = nxm; u < n_mem; ++u) {
-              const int j = slot[u];
-              P[j * 64 + lane] = gv ? pr.grid_shared[(int64_t)g * ni + j] : 0.0;
-            }
-            wave_sync();
-            if (gv) {
-              PabSummary o;
-              pab_recursion(P + lane, 64, nc, o, iab_ld, nullptr);
-              const double ldh = pr.grid_logdet[g];
-              const double fr = reml_value(o, nc, n, ldh);
-              const double fm = mle_value(o.pyy1, n, ldh);
-              if (bi_r == 0x7fffffff || better(fr, g, best_r, bi_r)) {
-                best_r = fr;
-                bi_r = g;
-              }
-              if (bi_m == 0x7fffffff || better(fm, g, best_m, bi_m)) {
-                best_m = fm;
-                bi_m = g;
-              }
-            }
-            wave_sync();
-          }
+          bi_r = gi_r;
+          bi_m = gi_m;
           evals += G;
-          wave_argmax(best_r, bi_r);
-          wave_argmax(best_m, bi_m);
           const int iloA = bi_r > 0 ? bi_r - 1 : 0, ihiA = bi_r + 1 < G ? bi_r + 1 : G - 1;
           const int iloB = bi_m > 0 ? bi_m - 1 : 0, ihiB = bi_m + 1 < G ? bi_m + 1 : G - 1;
           aA = pr.grid_loglam[iloA];
@@ -1084,7 +1194,33 @@ static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* x
                         const double* const* snp_xmom, int T, int max_cvt, void* stream) {
   JM_CHECK_ARG(T >= 1, "n_terms=%d < 1", T);
   const int ni_max = (max_cvt + 3) * (max_cvt + 2) / 2;
-  const size_t per_wave = (size_t)(ni_max * 64 + kMaxLevels + ni_max + 1) * sizeof(double);
+  hipStream_t st = (hipStream_t)stream;
+  // grid stage (skips Score-only SNPs): one wave per SNP, a Pab column per lane in LDS
+  {
+    const size_t gw = (size_t)(ni_max * 64 + kMaxLevels) * sizeof(double);
+    int gwpb = (int)((64 * 1024) / gw);
+    if (gwpb > 4) gwpb = 4;
+    if (gwpb < 1) {
+      JM_CHECK_ARG(gw <= 160 * 1024, "problem too large for the LDS budget (n_cvt=%d)", max_cvt);
+      gwpb = 1;
+    }
+    int64_t gblocks = (n_snps + gwpb - 1) / gwpb;
+    if (gblocks > 256 * 32) gblocks = 256 * 32;
+    if (one) {
+      if (mode_one != 3)
+        hipLaunchKernelGGL(k_lmm_grid<false>, dim3((unsigned)gblocks), dim3(64 * gwpb), gw * gwpb,
+                           st, *one, mode_one, xmom, ldm, n_snps, out, nullptr, nullptr, nullptr,
+                           nullptr, ni_max);
+    } else {
+      jm_lmm_problem dummy;
+      memset(&dummy, 0, sizeof(dummy));
+      hipLaunchKernelGGL(k_lmm_grid<true>, dim3((unsigned)gblocks), dim3(64 * gwpb), gw * gwpb, st,
+                         dummy, 0, nullptr, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
+                         ni_max);
+    }
+    JM_CHECK_LAUNCH();
+  }
+  const size_t per_wave = (size_t)assoc_lds_per_wave(ni_max) * sizeof(double);
   // single problem: the shared pair rows go to LDS when they take <= 48 KB
   const int n_sh = (max_cvt + 1) * (max_cvt + 2) / 2;
   const size_t sh_bytes = (size_t)n_sh * T * sizeof(double);
@@ -1102,29 +1238,37 @@ static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* x
   if (blocks > max_blocks) blocks = max_blocks;
   const unsigned nb = (unsigned)blocks;
   const size_t lds = sh + per_wave * wpb;
-  hipStream_t st = (hipStream_t)stream;
   const int kt = (T + 63) / 64;  // > 8: blocks of 512 terms
+  // moment rows held in registers: the c + 2 rows of the widest problem, up to 8
+  const int xr = max_cvt + 2 <= 4 ? 4 : (max_cvt + 2 <= 6 ? 6 : 8);
+#define JM_LAUNCH_KT(KT_, XR_, MB_)                                                            \
+  launch_kt<KT_, XR_, MB_>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob,       \
+                           snp_xmom, ni_max, nb, wpb, lds, srow_n, st)
+#define JM_LAUNCH_XR(KT_)                  \
+  do {                                     \
+    if (xr == 4)                           \
+      JM_LAUNCH_KT(KT_, 4, false);         \
+    else if (xr == 6)                      \
+      JM_LAUNCH_KT(KT_, 6, false);         \
+    else                                   \
+      JM_LAUNCH_KT(KT_, 8, false);         \
+  } while (0)
   if (kt <= 1)
-    launch_kt<1, 8, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, srow_n, st);
+    JM_LAUNCH_XR(1);
   else if (kt <= 2)
-    launch_kt<2, 8, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, srow_n, st);
+    JM_LAUNCH_XR(2);
   else if (kt <= 3)
-    launch_kt<3, 8, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, srow_n, st);
+    JM_LAUNCH_XR(3);
   else if (kt <= 4)
-    launch_kt<4, 8, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, srow_n, st);
+    JM_LAUNCH_XR(4);
   else if (kt <= 6)
-    launch_kt<6, 4, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, srow_n, st);
+    JM_LAUNCH_KT(6, 4, false);
   else if (kt <= 8)
-    launch_kt<8, 4, false>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, srow_n, st);
+    JM_LAUNCH_KT(8, 4, false);
   else
-    launch_kt<8, 4, true>(one, mode_one, xmom, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                    ni_max, nb, wpb, lds, srow_n, st);
+    JM_LAUNCH_KT(8, 4, true);
+#undef JM_LAUNCH_XR
+#undef JM_LAUNCH_KT
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

@@ -240,6 +240,7 @@ class AssocProblem:
         self.grid_logdet = torch.empty((n_grid,), dtype=torch.float64, device=dev)
         self.null_kappa = torch.empty((T,), dtype=torch.float64, device=dev)
         self.null_shared = torch.empty((ni,), dtype=torch.float64, device=dev)
+        self.iab_shared = torch.empty((ni,), dtype=torch.float64, device=dev)
         pr = _lib.LmmProblem()
         pr.n, pr.n_cvt, pr.n_terms = n, c, T
         pr.n_clusters = len(cohort.plan.cl_size)
@@ -252,6 +253,7 @@ class AssocProblem:
         pr.grid_kappa, pr.grid_shared = ptr(self.grid_kappa), ptr(self.grid_shared)
         pr.grid_logdet = ptr(self.grid_logdet)
         pr.null_kappa, pr.null_shared = ptr(self.null_kappa), ptr(self.null_shared)
+        pr.iab_shared = ptr(self.iab_shared)
         self.prob = pr
         # null model MLE on the device (likelihood.py:571-607)
         nullout = torch.zeros(4, dtype=torch.float64, device=dev)
