@@ -20,14 +20,20 @@
 namespace jm {
 
 constexpr int kMomMaxTerms = 16;  // MAX_TERMS_PER_CLUSTER on the host side
-constexpr int kMomGroup = 6;      // partner rows accumulated per pass
+constexpr int kMomGroup = 3;      // partner rows accumulated per pass (48 accumulators)
 constexpr int kMomTerms = kMomMaxTerms;
+constexpr int kMomUnroll = 4;     // eigen-indices whose Z loads are in flight together
+
+typedef const double __attribute__((address_space(4))) mom_cdbl;
 
 // Partial moments of one cluster piece (grid.y) for 64 SNP columns (grid.x):
 //   work[s][b][seg_slot + m] = sum_{i in piece} Z[i][s] V[b][i] delta_i^m
 // (row b == nv: Z[i][s]^2). Pieces hold at most a fixed number of
 // eigen-indices, so the blocks carry balanced work however the spectrum
-// clusters. The summation order inside a piece is fixed.
+// clusters. The summation order inside a piece is fixed (i ascending).
+// Single problem: V and delta are the same for every lane and come through
+// scalar loads; the Z loads of kMomUnroll consecutive eigen-indices are issued
+// before their products are accumulated.
 template <typename ZT, bool MULTI>
 __global__ void __launch_bounds__(64) k_mom_seg(const ZT* __restrict__ Z, int64_t ldz,
                                                 int64_t n_snps, const double* __restrict__ V_one,
@@ -51,6 +57,8 @@ __global__ void __launch_bounds__(64) k_mom_seg(const ZT* __restrict__ Z, int64_
   const int i0 = pl.seg_start[seg], len = pl.seg_len[seg];
   const int M = pl.cl_terms[pl.seg_cluster[seg]];
   const int slot0 = pl.seg_slot[seg];
+  const mom_cdbl* dls = (const mom_cdbl*)pl.delta;
+  const mom_cdbl* Vs = (const mom_cdbl*)V_one;
   __shared__ double tr[kMomTerms * 65];  // term x column, padded
   for (int g0 = 0; g0 <= nvm; g0 += kMomGroup) {
     // rows of this group present in every column of the block (MULTI: nv
@@ -62,23 +70,38 @@ __global__ void __launch_bounds__(64) k_mom_seg(const ZT* __restrict__ Z, int64_
     for (int b = 0; b < kMomGroup; ++b)
 #pragma unroll
       for (int m = 0; m < kMomTerms; ++m) acc[b][m] = 0.0;
-    for (int i = i0; i < i0 + len; ++i) {
-      const double z = (double)Z[(int64_t)i * ldz + sc];
-      const double dl = pl.delta[i];
-      double prod[kMomGroup];
+    for (int ib = i0; ib < i0 + len; ib += kMomUnroll) {
+      double zu[kMomUnroll];
 #pragma unroll
-      for (int b = 0; b < kMomGroup; ++b) {
-        const int row = g0 + b;
-        prod[b] = 0.0;
-        if (b < ng && row <= nv) prod[b] = (row < nv) ? z * V[(int64_t)row * ldv + i] : z * z;
+      for (int u = 0; u < kMomUnroll; ++u) {
+        const int i = min(ib + u, i0 + len - 1);
+        zu[u] = (double)Z[(int64_t)i * ldz + sc];
       }
-      double pw = 1.0;
 #pragma unroll
-      for (int m = 0; m < kMomTerms; ++m) {
-        if (m < M) {
+      for (int u = 0; u < kMomUnroll; ++u) {
+        if (ib + u >= i0 + len) break;  // uniform
+        const int i = ib + u;
+        const double z = zu[u];
+        const double dl = MULTI ? pl.delta[i] : dls[i];
+        double prod[kMomGroup];
 #pragma unroll
-          for (int b = 0; b < kMomGroup; ++b) acc[b][m] += prod[b] * pw;
-          pw *= dl;
+        for (int b = 0; b < kMomGroup; ++b) {
+          const int row = g0 + b;
+          prod[b] = 0.0;
+          if (b < ng && row <= nv) {
+            const double vb = row < nv ? (MULTI ? V[(int64_t)row * ldv + i] : Vs[(int64_t)row * ldv + i])
+                                       : z;
+            prod[b] = z * vb;
+          }
+        }
+        double pw = 1.0;
+#pragma unroll
+        for (int m = 0; m < kMomTerms; ++m) {
+          if (m < M) {
+#pragma unroll
+            for (int b = 0; b < kMomGroup; ++b) acc[b][m] += prod[b] * pw;
+            pw *= dl;
+          }
         }
       }
     }
