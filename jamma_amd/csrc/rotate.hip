@@ -16,9 +16,10 @@
 //    and DMA-copied into double-buffered LDS.
 //
 // Block order is XCD-aware: consecutive hardware block ids go round-robin to
-// the 8 XCDs, so blocks b, b+8, b+16, ... share an XCD (and its L2). Tiles are
-// assigned so that the blocks of one XCD sweep the SNP columns of the same
-// row panel of UT, which then stays L2/MALL-resident while it is consumed.
+// the 8 XCDs, so blocks b, b+8, b+16, ... share an XCD (and its L2). Each XCD
+// takes a contiguous range of tiles; in the FP64 GEMM the range sweeps the SNP
+// columns of one UT row panel, in the BF16x3 rotation it covers 4 x 8 patches
+// of (row panel, SNP block), which share both operands in that XCD's L2.
 //
 // MFMA lane layouts (16x16 outputs): lane l supplies A[l%16][k-group l/16] and
 // B[k-group l/16][l%16]. The accumulator element r (0..3) of lane l holds
@@ -47,6 +48,22 @@ __device__ __forceinline__ void xcd_tile(int bid, int nrb, int ncb, int& rb, int
   const int t = off + j;
   rb = t / ncb;
   cb = t % ncb;
+}
+
+// The same XCD-contiguous ranges over a 2-D order: stripes of 4 row panels,
+// column-major inside a stripe, so the 32 blocks resident on one XCD cover a
+// 4 x 8 patch of (row panel, column block) and share 4 row panels and 8
+// column blocks in its L2 instead of 1 and 32 (bijective for any grid).
+__device__ __forceinline__ void xcd_tile_2d(int bid, int nrb, int ncb, int& rb, int& cb) {
+  const int total = nrb * ncb;
+  const int x = bid % 8, j = bid / 8;
+  const int base = total / 8, rem = total % 8;
+  const int t = x * base + (x < rem ? x : rem) + j;
+  const int stripe = t / (4 * ncb);
+  const int h = min(4, nrb - 4 * stripe);
+  const int u = t - stripe * 4 * ncb;
+  cb = u / h;
+  rb = 4 * stripe + u % h;
 }
 
 // ---------------------------------------------------------------------------
@@ -345,7 +362,7 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nrb = (int)((M + V3_BM - 1) / V3_BM), ncb = (int)((n_cols + V3_BN - 1) / V3_BN);
   int rb, cb;
-  xcd_tile(blockIdx.x, nrb, ncb, rb, cb);
+  xcd_tile_2d(blockIdx.x, nrb, ncb, rb, cb);
   const int64_t i0 = (int64_t)rb * V3_BM, s0 = (int64_t)cb * V3_BN;
   const int wr = (wave >> 1) * 64, wc = (wave & 1) * 128;
   const int lr = lane & 15, lk = lane >> 4;
@@ -528,8 +545,8 @@ extern "C" int jm_rotate_bf16x3(const uint16_t* planes, int nplanes, const uint8
   const int64_t nks = (n + 31) / 32;
   const int64_t per_plane = jm_packed_ut_bytes(n, nplanes) / (2 * nplanes);  // bf16 per plane
   const int nrb = ceil_div(n, V3_BM), ncb = ceil_div(n_snps, V3_BN);
-  if (nplanes == 2)  // 3 x 40 KB stages
-    hipLaunchKernelGGL((k_rotate_bf16x3_packed<2, 3>), dim3(nrb * ncb), dim3(512), 0,
+  if (nplanes == 2)  // 2 x 40 KB stages (measured faster than 3 at n = 10k)
+    hipLaunchKernelGGL((k_rotate_bf16x3_packed<2, 2>), dim3(nrb * ncb), dim3(512), 0,
                        (hipStream_t)stream, planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
   else  // 2 x 56 KB stages
     hipLaunchKernelGGL((k_rotate_bf16x3_packed<3, 2>), dim3(nrb * ncb), dim3(512), 0,
