@@ -440,6 +440,17 @@ typedef const double __attribute__((address_space(4))) cdbl;
 typedef const int32_t __attribute__((address_space(4))) cint;
 typedef const double __attribute__((address_space(1))) gdbl;
 
+// k_lmm_assoc LDS of the cohort constants (bytes): 64*KT terms x (2 doubles + 1 int)
+// and 32 clusters x 2 doubles, for the KT the launcher picks from T.
+static size_t assoc_lds_consts(int T) {
+  int kt = (T + 63) / 64;
+  if (kt > 8) kt = 8;
+  if (kt == 5) kt = 6;
+  if (kt == 7) kt = 8;
+  const int tb0 = 64 * kt;
+  return (size_t)(2 * tb0 + 64 + tb0 / 2) * sizeof(double);
+}
+
 // k_lmm_assoc LDS per wave (doubles): two packed rows (ni + logdet slot each), the
 // Iab level logs, and ni + 1 int slots.
 __host__ __device__ constexpr int assoc_lds_per_wave(int ni_max) {
@@ -754,8 +765,34 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
   const int per_wave = assoc_lds_per_wave(ni_max);
   // srow_n > 0 (single problem): the block keeps the srow_n shared pair rows
   // (covariate / phenotype moments) in LDS, in slot order, ahead of the waves
-  double* srows = smem;
+  // Cohort constants, staged once per block (all SNPs of a launch share the cohort):
+  // for the first 64*KT terms the cluster centre, logdet weight / power and power, for
+  // the first 32 clusters the centre and weight. Reading them from global memory in
+  // every evaluation (a dependent chain term -> cluster -> centre) took most of its time.
+  constexpr int TB0 = 64 * KT;
+  double* c_tdc = smem;                        // TB0
+  double* c_twl = c_tdc + TB0;                 // TB0
+  double* c_cdc = c_twl + TB0;                 // 32
+  double* c_cw = c_cdc + 32;                   // 32
+  int* c_tm = (int*)(c_cw + 32);               // TB0 ints
+  double* srows = c_cw + 32 + TB0 / 2;
   const size_t srow_len = MULTI ? 0 : (size_t)srow_n * pr_one.n_terms;
+  {
+    const int64_t s0 = (int64_t)blockIdx.x * wpb;
+    const jm_lmm_problem& p0 = MULTI ? probs[s0 < n_snps ? snp_prob[s0] : 0] : pr_one;
+    for (int t = threadIdx.x; t < TB0; t += blockDim.x) {
+      const int tc = t < p0.n_terms ? t : 0;
+      const int m = t < p0.n_terms ? p0.term_m[tc] : 0;
+      c_tm[t] = m;
+      c_tdc[t] = p0.cl_center[p0.term_cluster[tc]];
+      c_twl[t] = m > 0 ? p0.logdet_mom[tc] / (double)m : 0.0;
+    }
+    for (int c = threadIdx.x; c < 32; c += blockDim.x) {
+      c_cdc[c] = c < p0.n_clusters ? p0.cl_center[c] : 0.0;
+      c_cw[c] = c < p0.n_clusters ? p0.cl_weight[c] : 0.0;
+    }
+    __syncthreads();
+  }
   if (!MULTI && srow_n > 0) {
     const int T1 = pr_one.n_terms, nc1 = pr_one.n_cvt;
     for (size_t idx = threadIdx.x; idx < srow_len; idx += blockDim.x) {
@@ -764,7 +801,7 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
     }
     __syncthreads();
   }
-  double* P = smem + srow_len + (size_t)wib * per_wave;  // two packed rows
+  double* P = srows + srow_len + (size_t)wib * per_wave;  // two packed rows
   double* PA = P;                              // row A: ni entries + logdet slot
   double* PB = P + ni_max + 1;                 // row B
   double* iab_ld = P + 2 * (ni_max + 1);      // kMaxLevels
@@ -867,9 +904,10 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
           const int t = tb * TB + lane * KT + k;
           const bool act = t < T;
           const int tc = act ? t : 0;
-          const int mk = act ? pr.term_m[tc] : 0;
+          const int mk = tb == 0 ? c_tm[lane * KT + k] : (act ? pr.term_m[tc] : 0);
           if (k == 0 || mk == 0) {
-            const double dck = pr.cl_center[pr.term_cluster[tc]];
+            const double dck =
+                tb == 0 ? c_tdc[lane * KT + k] : pr.cl_center[pr.term_cluster[tc]];
             hA = 1.0 / fma(lamA, dck, 1.0);
             hB = 1.0 / fma(lamB, dck, 1.0);
             nzA = -lamA * hA;
@@ -888,7 +926,7 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
             kA[k] = act ? hA * pA : 0.0;
             kB[k] = act ? hB * pB : 0.0;
             if (mk > 0) {
-              const double wl = pr.logdet_mom[tc] / (double)mk;
+              const double wl = tb == 0 ? c_twl[lane * KT + k] : pr.logdet_mom[tc] / (double)mk;
               ldA -= pA * wl;
               ldB -= pB * wl;
             }
@@ -1029,8 +1067,8 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
         } else if (q >= first && c0 + q - first < n_cl) {
           const int c = c0 + q - first;
           kind = 3;
-          arg = fabs(fma(lam_h, pr.cl_center[c], 1.0));
-          w = pr.cl_weight[c];
+          arg = fabs(fma(lam_h, c < 32 ? c_cdc[c] : pr.cl_center[c], 1.0));
+          w = c < 32 ? c_cw[c] : pr.cl_weight[c];
         }
         const double lg = log(arg);
         if (kind == 1) contrib += lam0 ? lg : lg - iab_ld[q];
@@ -1237,7 +1275,7 @@ static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* x
   const int64_t max_blocks = 256 * 16;
   if (blocks > max_blocks) blocks = max_blocks;
   const unsigned nb = (unsigned)blocks;
-  const size_t lds = sh + per_wave * wpb;
+  const size_t lds = sh + per_wave * wpb + assoc_lds_consts(T);
   const int kt = (T + 63) / 64;  // > 8: blocks of 512 terms
   // moment rows held in registers: the c + 2 rows of the widest problem, up to 8
   const int xr = max_cvt + 2 <= 4 ? 4 : (max_cvt + 2 <= 6 ? 6 : 8);
