@@ -434,21 +434,30 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx) {
 
 constexpr int kMaxLevels = 24;  // n_cvt <= 20 -> levels 0..n_cvt
 
+// x (x + 1) (x + 2) / 6: the number of Pab recursion updates over the levels whose
+// triangle sides are 1 .. x.
+__host__ __device__ constexpr int tet(int x) { return x <= 0 ? 0 : x * (x + 1) * (x + 2) / 6; }
+
 // Address-space views: constant (wave-uniform reads become scalar loads) and global
 // (vector loads without the flat-address path).
 typedef const double __attribute__((address_space(4))) cdbl;
 typedef const int32_t __attribute__((address_space(4))) cint;
 typedef const double __attribute__((address_space(1))) gdbl;
 
-// k_lmm_assoc LDS of the cohort constants (bytes): 64*KT terms x (2 doubles + 1 int)
-// and 32 clusters x 2 doubles, for the KT the launcher picks from T.
-static size_t assoc_lds_consts(int T) {
+// KT (terms per lane in one block of terms) the launcher picks for T terms.
+static int assoc_kt(int T) {
   int kt = (T + 63) / 64;
   if (kt > 8) kt = 8;
   if (kt == 5) kt = 6;
   if (kt == 7) kt = 8;
-  const int tb0 = 64 * kt;
-  return (size_t)(2 * tb0 + 64 + tb0 / 2) * sizeof(double);
+  return kt;
+}
+
+// k_lmm_assoc block-level LDS ahead of the shared rows (bytes): 64*KT terms x (2 doubles
+// + 1 int), 32 clusters x 2 doubles and the recursion table of the widest problem.
+static size_t assoc_lds_consts(int T, int max_cvt) {
+  const int tb0 = 64 * assoc_kt(T);
+  return (size_t)(2 * tb0 + 64 + tb0 / 2 + (tet(max_cvt + 1) + 1) / 2) * sizeof(double);
 }
 
 // k_lmm_assoc LDS per wave (doubles): two packed rows (ni + logdet slot each), the
@@ -775,8 +784,15 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
   double* c_cdc = c_twl + TB0;                 // 32
   double* c_cw = c_cdc + 32;                   // 32
   int* c_tm = (int*)(c_cw + 32);               // TB0 ints
-  double* srows = c_cw + 32 + TB0 / 2;
-  const size_t srow_len = MULTI ? 0 : (size_t)srow_n * pr_one.n_terms;
+  // single problem: per recursion level p the packed LDS offsets (ab | ap << 8 | bp << 16)
+  // of the level's updates, level p starting at tet(nc + 1) - tet(nc + 2 - p)
+  int* rtab = c_tm + TB0;                      // tet(ni_lv) ints (+1 pad)
+  const int rtab_n = MULTI ? 0 : tet(pr_one.n_cvt + 1);
+  double* srows = c_cw + 32 + TB0 / 2 + (rtab_n + 1) / 2;
+  // shared rows: zero-padded to 64*KT terms (one block of terms) so that a lane reads its
+  // KT values without bounds checks; full T when T > 64*KT
+  const int sstride = MB ? pr_one.n_terms : TB0;
+  const size_t srow_len = MULTI ? 0 : (size_t)srow_n * sstride;
   {
     const int64_t s0 = (int64_t)blockIdx.x * wpb;
     const jm_lmm_problem& p0 = MULTI ? probs[s0 < n_snps ? snp_prob[s0] : 0] : pr_one;
@@ -793,11 +809,24 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
     }
     __syncthreads();
   }
-  if (!MULTI && srow_n > 0) {
-    const int T1 = pr_one.n_terms, nc1 = pr_one.n_cvt;
-    for (size_t idx = threadIdx.x; idx < srow_len; idx += blockDim.x) {
-      const int k = (int)(idx / T1), t = (int)(idx % T1);
-      srows[idx] = pr_one.shared_mom[(int64_t)shared_pair_index(k, nc1) * T1 + t];
+  if (!MULTI) {
+    const int T1 = pr_one.n_terms, nc1 = pr_one.n_cvt, m = nc1 + 2;
+    for (int k = 0; k < srow_n; ++k) {
+      const int j = shared_pair_index(k, nc1);
+      for (int t = threadIdx.x; t < sstride; t += blockDim.x)
+        srows[(size_t)k * sstride + t] = t < T1 ? pr_one.shared_mom[(int64_t)j * T1 + t] : 0.0;
+    }
+    for (int e = threadIdx.x; e < rtab_n; e += blockDim.x) {
+      int p = 1;
+      while (tet(m - 1) - tet(m - p - 1) <= e) ++p;  // level of entry e
+      int rem = e - (tet(m - 1) - tet(m - p)), ap = 0;
+      const int Lw = m - p;
+      while (rem >= Lw - ap) {
+        rem -= Lw - ap;
+        ++ap;
+      }
+      const int a = p + 1 + ap, b = a + rem;
+      rtab[e] = ab_index(a, b, nc1) | (ab_index(a, p, nc1) << 8) | (ab_index(b, p, nc1) << 16);
     }
     __syncthreads();
   }
@@ -972,6 +1001,36 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
         const int e = e0 + (qv >> 1);
         if ((lane & 7) == 0 && e < nrows && e < XR) ((qv & 1) ? PB : PA)[xpair_index(e, nc)] = r;
       }
+      if (!MULTI && !MB && srow_len && nxm == 0) {
+        // single problem, one block of terms, every shared row in LDS (zero-padded)
+        for (int u0 = 0; u0 <= n_mem; u0 += 4) {
+          double v[8];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int u = u0 + e;
+            v[2 * e] = 0.0;
+            v[2 * e + 1] = 0.0;
+            if (u < n_mem) {
+              const double* row = srows + (size_t)u * TB0 + lane * KT;
+#pragma unroll
+              for (int k = 0; k < KT; ++k) {
+                const double x = row[k];
+                v[2 * e] = fma(kA0[k], x, v[2 * e]);
+                v[2 * e + 1] = fma(kB0[k], x, v[2 * e + 1]);
+              }
+            } else if (u == n_mem) {
+              v[2 * e] = ldA0;
+              v[2 * e + 1] = ldB0;
+            }
+          }
+          const double r = reduce8(v, lane);
+          const int u = u0 + (qv >> 1);
+          if ((lane & 7) == 0 && u <= n_mem) {
+            double* dst = (qv & 1) ? PB : PA;
+            dst[u < n_mem ? slot[u] : ni] = r;  // slot ni: m >= 1 part of logdet_h
+          }
+        }
+      } else
       for (int u0 = 0; u0 <= n_mem; u0 += 4) {
         double v[8];
 #pragma unroll
@@ -993,7 +1052,7 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
           for (int e = 0; e < 4; ++e) {
             const int u = u0 + e;
             if (u < n_mem && u >= nxm && srow_len) {  // shared row from LDS
-              const double* row = srows + (size_t)(u - nxm) * T;
+              const double* row = srows + (size_t)(u - nxm) * sstride;
 #pragma unroll
               for (int k = 0; k < KT; ++k) {
                 const int t = tb * TB + lane * KT + k;
@@ -1035,15 +1094,26 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
         const double inv = (d != 0.0) ? 1.0 / d : 0.0;
         const int Lw = nc + 2 - p;
         const int np = Lw * (Lw + 1) / 2;
+        const int lv0 = MULTI ? 0 : tet(nc + 1) - tet(nc + 2 - p);  // this level's entries
         for (int idx = q; idx < np; idx += 32) {
-          int ap = 0, rem = idx;
-          while (rem >= Lw - ap) {
-            rem -= Lw - ap;
-            ++ap;
+          int iab, ia, ib;
+          if (!MULTI) {
+            const int e = rtab[lv0 + idx];
+            iab = e & 255;
+            ia = (e >> 8) & 255;
+            ib = e >> 16;
+          } else {
+            int ap = 0, rem = idx;
+            while (rem >= Lw - ap) {
+              rem -= Lw - ap;
+              ++ap;
+            }
+            const int a = p + 1 + ap, b = a + rem;
+            iab = ab_index(a, b, nc);
+            ia = ab_index(a, p, nc);
+            ib = ab_index(b, p, nc);
           }
-          const int a = p + 1 + ap, b = a + rem;
-          const double pa = Ph[ab_index(a, p, nc)], pb = Ph[ab_index(b, p, nc)];
-          const int iab = ab_index(a, b, nc);
+          const double pa = Ph[ia], pb = Ph[ib];
           Ph[iab] = Ph[iab] - pa * pb * inv;
         }
         wave_sync();
@@ -1261,7 +1331,8 @@ static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* x
   const size_t per_wave = (size_t)assoc_lds_per_wave(ni_max) * sizeof(double);
   // single problem: the shared pair rows go to LDS when they take <= 48 KB
   const int n_sh = (max_cvt + 1) * (max_cvt + 2) / 2;
-  const size_t sh_bytes = (size_t)n_sh * T * sizeof(double);
+  const int sstride = T > 512 ? T : 64 * assoc_kt(T);  // the kernel's padded row length
+  const size_t sh_bytes = (size_t)n_sh * sstride * sizeof(double);
   const int srow_n = (one && sh_bytes <= 48 * 1024) ? n_sh : 0;
   const size_t sh = srow_n ? sh_bytes : 0;
   int wpb = (int)((80 * 1024 - sh) / per_wave);
@@ -1275,7 +1346,7 @@ static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* x
   const int64_t max_blocks = 256 * 16;
   if (blocks > max_blocks) blocks = max_blocks;
   const unsigned nb = (unsigned)blocks;
-  const size_t lds = sh + per_wave * wpb + assoc_lds_consts(T);
+  const size_t lds = sh + per_wave * wpb + assoc_lds_consts(T, max_cvt);
   const int kt = (T + 63) / 64;  // > 8: blocks of 512 terms
   // moment rows held in registers: the c + 2 rows of the widest problem, up to 8
   const int xr = max_cvt + 2 <= 4 ? 4 : (max_cvt + 2 <= 6 ? 6 : 8);
