@@ -895,12 +895,16 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
       // ---- the lambdas of this stage ----
       const bool lam0 = stage == 0;
       double lamA = 0.0, lamB = 0.0;
+      // exponents of the two lambdas; half-wave h takes the exponential of its own, so a
+      // stage waits for one exp instead of two in a row
+      double xA = 0.0, xB = 0.0;
+      bool take_exp = true;
       if (stage == 1) {
-        lamA = exp(cA);
-        lamB = exp(cB);
+        xA = cA;
+        xB = cB;
       } else if (stage == 2) {
-        lamA = exp(dA);
-        lamB = exp(dB);
+        xA = dA;
+        xB = dB;
       } else if (stage >= 3 && stage < st_final) {
         klA = fcA > fdA;
         klB = fcB > fdB;
@@ -910,15 +914,23 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
         const double ncB = nbB - phi * (nbB - naB), ndB = naB + phi * (nbB - naB);
         aA = naA; bA = nbA; cA = ncA; dA = ndA;
         aB = naB; bB = nbB; cB = ncB; dB = ndB;
-        lamA = exp(klA ? cA : dA);
-        lamB = exp(klB ? cB : dB);
+        xA = klA ? cA : dA;
+        xB = klB ? cB : dB;
       } else if (stage == st_final) {
-        lam_reml = exp((aA + bA) / 2.0);
-        lam_mle = exp((aB + bB) / 2.0);
-        lamA = lam_reml;
-        lamB = lam_mle;
-      } else if (stage == st_score) {
-        lamA = lamB = pr.lambda_null;
+        xA = (aA + bA) / 2.0;
+        xB = (aB + bB) / 2.0;
+      } else {
+        take_exp = false;
+        if (stage == st_score) lamA = lamB = pr.lambda_null;
+      }
+      if (take_exp) {
+        const double e = exp(h ? xB : xA);
+        lamA = bcast(e, 0);
+        lamB = bcast(e, 32);
+        if (stage == st_final) {
+          lam_reml = lamA;
+          lam_mle = lamB;
+        }
       }
 
       // ---- 1. kappa and the m >= 1 logdet part, per block of 64*KT terms ----
