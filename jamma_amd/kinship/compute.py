@@ -47,7 +47,8 @@ class KinshipAccumulator:
         self.stage_cols = int(min(1 << 17, max(128, (stage_bytes // self.n_pad) // 128 * 128)))
         self._stage = None
         self._fill = 0
-        self.syrk_timer = None  # optional (start, end) torch.cuda.Event pair around each SYRK
+        # optional list: each SYRK launch appends its (start, end) torch.cuda.Event pair
+        self.syrk_timer = None
 
     @property
     def S(self) -> torch.Tensor:
@@ -60,11 +61,14 @@ class KinshipAccumulator:
         self._S = value
 
     def _syrk(self, gT0: torch.Tensor, ld: int, n_snps: int) -> None:
-        if self.syrk_timer:
-            self.syrk_timer[0].record()
+        ev = None
+        if self.syrk_timer is not None:
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            ev[0].record()
         _lib.call("jm_kinship_syrk_i8", ptr(gT0), ld, self.n, n_snps, ptr(self._S), stream_ptr())
-        if self.syrk_timer:
-            self.syrk_timer[1].record()
+        if ev is not None:
+            ev[1].record()
+            self.syrk_timer.append(ev)
 
     def flush(self) -> None:
         """Run the GEMM over the staged SNPs (S += X0 X0^T)."""

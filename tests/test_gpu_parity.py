@@ -56,7 +56,7 @@ def test_decode_bed_exact():
 # ------------------------------------------------------------------ eigendecomposition
 @pytest.mark.parametrize("n", [1, 7, 300, 1031])
 def test_eigh_device_matches_lapack(n):
-    """rocsolver_dsyevd on the device vs LAPACK dsyevd (numpy): same ascending
+    """rocSOLVER dsytrd + dorgtr + dstedc on the device vs LAPACK dsyevd (numpy): same ascending
     spectrum, U diag(d) U^T reproduces K, U orthonormal."""
     from jamma_amd.lmm.eigen import eigh_device
 
