@@ -317,6 +317,53 @@ int jm_lmm_assoc_multi(const jm_lmm_problem* probs, const int32_t* modes, int n_
                        int64_t ldm, int64_t n_snps, double* out, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Large-cohort eigendecomposition (n*n > 2^31 - 1). Together with rocBLAS'   */
+/* 64-bit level-2 routines and rocSOLVER dstedc on the halves, these replace  */
+/* np.linalg.eigh (lmm/eigen.py:94) where the library's 32-bit tridiagonal    */
+/* solvers stop. All matrices column-major, all offsets 64-bit.              */
+/* ------------------------------------------------------------------------ */
+
+/* C (m x n, ldc) = alpha op(A) op(B) + beta C, op(X) = X or X^T (trans_* != 0);
+ * FP64 MFMA. lower_only != 0 skips the tiles with every element above the
+ * diagonal (i < j); elements above the diagonal in the tiles it does compute are
+ * written too. */
+int jm_dgemm(int trans_a, int trans_b, int64_t m, int64_t n, int64_t k, double alpha,
+             const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+             int64_t ldc, int lower_only, void* stream);
+
+/* Householder reduction step (dlatrd): *e = *a, then *a = 1. */
+int jm_tridiag_take_unit(double* a, double* e, void* stream);
+
+/* End of a dlatrd column step on w, v (len elements): w *= tau,
+ * alpha = -tau/2 (w . v), w += alpha v (tau read from device memory). */
+int jm_tridiag_finish(int64_t len, const double* tau, double* w, const double* v, void* stream);
+
+/* Rank-one update D + rho w w^T, D = diag(dl) strictly ascending, rho > 0, no w_i
+ * = 0 (the deflated problem of a divide-and-conquer merge; K eigen-pairs). Root j
+ * = dorg[j] + tau[j] (lam[j] holds the sum) with dorg[j] the pole dl[j] or
+ * dl[j+1] nearer to it; zhat = Gu-Eisenstat weights (exact for the computed
+ * roots); inv_norm[j] = 1 / || zhat / (dl - lambda_j) ||. w2 = w^2, w2sum = sum
+ * w2. Device arrays of K doubles. */
+int jm_secular_solve(int64_t K, const double* dl, const double* w, const double* w2, double rho,
+                     double w2sum, double* dorg, double* tau, double* lam, double* zhat,
+                     double* inv_norm, void* stream);
+
+/* Writes the rank-one update's eigenvectors into the merge matrix M (row-major,
+ * leading dimension ldm, zero elsewhere): M[row[i]][col[j]] = zhat_i /
+ * (dl_i - lambda_j) * inv_norm[j] for i, j < K, and M[row_defl[t]][col_defl[t]]
+ * = 1 for the n_defl deflated pairs. */
+int jm_secular_vectors(int64_t K, const double* dl, const double* zhat, const double* dorg,
+                       const double* tau, const double* inv_norm, const int64_t* row,
+                       const int64_t* col, int64_t n_defl, const int64_t* row_defl,
+                       const int64_t* col_defl, double* M, int64_t ldm, void* stream);
+
+/* M (row-major, n_cols columns, ldm) <- G_0 G_1 ... G_{r-1} M, where G_t rotates
+ * rows p = pq[2t], q = pq[2t+1] by (c, s) = (cs[2t], cs[2t+1]):
+ * M[p] <- c M[p] - s M[q], M[q] <- s M[p] + c M[q]. */
+int jm_apply_rotations(int64_t r, const int64_t* pq, const double* cs, int64_t n_cols, double* M,
+                       int64_t ldm, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Host-callable special functions (the device code compiled for the CPU),   */
 /* exported so the CPU test-suite can check them against SciPy.              */
 /* ------------------------------------------------------------------------ */

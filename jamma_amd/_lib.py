@@ -92,6 +92,17 @@ _SIGS = {
     "jm_lmm_prepare": ([POINTER(LmmProblem), c_void_p], c_int),
     "jm_lmm_assoc": ([POINTER(LmmProblem), c_int, c_void_p, c_int64, c_int64, c_void_p, c_void_p],
                      c_int),
+    "jm_dgemm": ([c_int, c_int, c_int64, c_int64, c_int64, c_double, c_void_p, c_int64, c_void_p,
+                  c_int64, c_double, c_void_p, c_int64, c_int, c_void_p], c_int),
+    "jm_tridiag_take_unit": ([c_void_p, c_void_p, c_void_p], c_int),
+    "jm_tridiag_finish": ([c_int64, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "jm_secular_solve": ([c_int64, c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p,
+                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "jm_secular_vectors": ([c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p],
+                           c_int),
+    "jm_apply_rotations": ([c_int64, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p],
+                           c_int),
     "jm_host_betainc": ([c_double, c_double, c_double], c_double),
     "jm_host_chi2_sf1": ([c_double], c_double),
 }

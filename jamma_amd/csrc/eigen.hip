@@ -1,0 +1,420 @@
+// Device kernels of the large-cohort symmetric eigendecomposition.
+//
+// Replaces np.linalg.eigh (src/jamma/lmm/eigen.py:94) for kinship matrices
+// whose n*n element count exceeds 2^31 - 1 (n > 46,340), where the library
+// tridiagonal solvers, which take 32-bit sizes, no longer apply. Every offset
+// here is 64-bit. The host side (jamma_amd/lmm/eigen.py) runs:
+//
+//  1. blocked Householder reduction to tridiagonal form (lower, LAPACK dsytrd /
+//     dlatrd order of operations): the per-column matrix-vector products go to
+//     rocBLAS' 64-bit level-2 routines, the rank-2k update of the trailing
+//     matrix to k_dgemm, and the end of each column step (scale by tau,
+//     alpha = -tau/2 w.v, w += alpha v) to k_latrd_finish;
+//  2. Cuppen divide and conquer on the tridiagonal matrix: tear it into halves
+//     (recursively) until a half fits the library solver, then merge each pair
+//     through the rank-one update D + rho z z^T (Bunch-Nielsen-Sorensen):
+//     deflation on the host, the secular equation's roots (k_secular_roots),
+//     the Gu-Eisenstat (Loewner) recomputation of z (k_secular_zhat) and the
+//     normalised eigenvectors (k_secular_norms, k_secular_fill), the Givens
+//     rotations of deflated pairs (k_apply_rotations), then
+//     Q = blockdiag(Q1, Q2) M with k_dgemm;
+//  3. back-transformation Q <- H_1 ... H_{n-1} Q in blocks of reflectors
+//     (compact WY, I - V T V^T) with k_dgemm.
+//
+// k_dgemm is an FP64 MFMA GEMM (v_mfma_f64_16x16x4f64) in column-major BLAS
+// semantics: C = alpha op(A) op(B) + beta C. 128x64 block tile (i x j), 4 waves
+// of 64x32, 16-deep K steps staged through LDS in [index][k] order whatever the
+// operand's storage order, so all four transpose combinations share one inner
+// loop. The MFMA's B slot takes the i side, so the 16 lanes of an output group
+// store 16 consecutive rows of the column-major C.
+
+#include "jm_common.h"
+
+namespace jm {
+
+typedef double d4e_t __attribute__((ext_vector_type(4)));
+
+constexpr int DG_TI = 128, DG_TJ = 64, DG_BK = 16, DG_LS = DG_BK + 1;
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) k_dgemm(int64_t m, int64_t n, int64_t k, double alpha,
+                                               const double* __restrict__ A, int64_t lda,
+                                               const double* __restrict__ B, int64_t ldb,
+                                               double beta, double* __restrict__ C, int64_t ldc,
+                                               int lower_only, int64_t nti) {
+  __shared__ double As[DG_TI * DG_LS];  // op(A)(i, kk) at [i][kk]
+  __shared__ double Bs[DG_TJ * DG_LS];  // op(B)(kk, j) at [j][kk]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t bid = blockIdx.x;
+  const int64_t i0 = (bid % nti) * DG_TI, j0 = (bid / nti) * DG_TJ;
+  if (lower_only && i0 + DG_TI <= j0) return;  // every element of the tile has i < j
+  const int wi = (wave & 1) * 64, wj = (wave >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  d4e_t acc[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = (d4e_t){0.0, 0.0, 0.0, 0.0};
+
+  for (int64_t k0 = 0; k0 < k; k0 += DG_BK) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {  // op(A) tile: 128 x 16
+      const int idx = tid + 256 * u;
+      int r, c;
+      if (!TA) {  // A(i, kk) at A[i + kk * lda]: consecutive lanes take consecutive i
+        r = idx & 127;
+        c = idx >> 7;
+      } else {    // op(A)(i, kk) = A(kk, i) at A[kk + i * lda]: consecutive lanes take kk
+        c = idx & 15;
+        r = idx >> 4;
+      }
+      const int64_t i = i0 + r, kk = k0 + c;
+      double v = 0.0;
+      if (i < m && kk < k) v = TA ? A[kk + i * lda] : A[i + kk * lda];
+      As[r * DG_LS + c] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {  // op(B) tile: 16 x 64
+      const int idx = tid + 256 * u;
+      int r, c;
+      if (!TB) {  // B(kk, j) at B[kk + j * ldb]
+        c = idx & 15;
+        r = idx >> 4;
+      } else {    // op(B)(kk, j) = B(j, kk) at B[j + kk * ldb]
+        r = idx & 63;
+        c = idx >> 6;
+      }
+      const int64_t j = j0 + r, kk = k0 + c;
+      double v = 0.0;
+      if (j < n && kk < k) v = TB ? B[j + kk * ldb] : B[kk + j * ldb];
+      Bs[r * DG_LS + c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < DG_BK; kk += 4) {
+      double bi[4], aj[2];
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) bi[ti] = As[(wi + ti * 16 + lr) * DG_LS + kk + lk];
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj) aj[tj] = Bs[(wj + tj * 16 + lr) * DG_LS + kk + lk];
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+          acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(aj[tj], bi[ti], acc[ti][tj], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // accumulator element r of lane l: D[(l / 16) + 4 r][l % 16] = C(i = l % 16, j = l / 16 + 4 r)
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + wi + ti * 16 + lr;
+        const int64_t j = j0 + wj + tj * 16 + lk + 4 * r;
+        if (i < m && j < n) {
+          double v = alpha * acc[ti][tj][r];
+          if (beta != 0.0) v += beta * C[i + j * ldc];
+          C[i + j * ldc] = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Tridiagonal reduction helpers
+// ---------------------------------------------------------------------------
+// e = a; a = 1 (the reflector's unit entry replaces the subdiagonal element).
+__global__ void k_take_unit(double* a, double* e) {
+  if (threadIdx.x == 0) {
+    *e = *a;
+    *a = 1.0;
+  }
+}
+
+// One block: w *= tau; alpha = -tau/2 (w . v); w += alpha v   (len elements)
+__global__ void __launch_bounds__(1024) k_latrd_finish(int64_t len, const double* __restrict__ tau,
+                                                       double* __restrict__ w,
+                                                       const double* __restrict__ v) {
+  __shared__ double part[16];
+  const double t = *tau;
+  double s = 0.0;
+  for (int64_t q = threadIdx.x; q < len; q += blockDim.x) {
+    const double x = t * w[q];
+    w[q] = x;
+    s += x * v[q];
+  }
+  s = wave_sum(s);
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) part[wave] = s;
+  __syncthreads();
+  double tot = 0.0;
+  for (int q = 0; q < nw; ++q) tot += part[q];
+  const double alpha = -0.5 * t * tot;
+  for (int64_t q = threadIdx.x; q < len; q += blockDim.x) w[q] += alpha * v[q];
+}
+
+// ---------------------------------------------------------------------------
+// Rank-one merge: eigen-pairs of D + rho w w^T (D = diag(dl) ascending, rho > 0,
+// no w_i = 0 and no two equal dl after deflation).
+// ---------------------------------------------------------------------------
+// Root j lies in (dl[j], dl[j+1]) (j < K-1) or in (dl[K-1], dl[K-1] + rho |w|^2].
+// It is stored as origin pole o_j (dorg[j] = dl[o_j]) plus offset tau_j, with
+// the pole nearer to the root as origin, so that every difference
+// dl[i] - lambda_j = (dl[i] - dl[o_j]) - tau_j keeps its relative accuracy.
+// The secular function g(lambda) = 1/rho + sum_i w_i^2 / (dl[i] - lambda) is
+// increasing between poles. With lambda = dl[o] + s t (s = +1 from the left
+// pole, -1 from the right one), h(t) = s g(dl[o] + s t) increases from -inf at
+// t = 0+; bisection runs over the bit patterns of t in (0, T], which orders
+// positive doubles, so it finds t to the last bit in at most 64 evaluations
+// whatever its magnitude.
+__device__ __forceinline__ double secular_h(const double* __restrict__ dl,
+                                            const double* __restrict__ w2, int64_t K, double inv_rho,
+                                            double dorg, double s, double t) {
+  double g = inv_rho;
+  for (int64_t i = 0; i < K; ++i) g += w2[i] / ((dl[i] - dorg) - s * t);
+  return s * g;
+}
+
+__global__ void __launch_bounds__(256) k_secular_roots(int64_t K, const double* __restrict__ dl,
+                                                       const double* __restrict__ w2, double rho,
+                                                       double w2sum, double* __restrict__ dorg,
+                                                       double* __restrict__ tau,
+                                                       double* __restrict__ lam) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= K) return;
+  const double inv_rho = 1.0 / rho;
+  double o, s, T;
+  if (j < K - 1) {
+    const double gap = dl[j + 1] - dl[j];
+    const double mid = 0.5 * gap;
+    if (secular_h(dl, w2, K, inv_rho, dl[j], 1.0, mid) >= 0.0) {
+      o = dl[j];
+      s = 1.0;
+      T = mid;
+    } else {
+      o = dl[j + 1];
+      s = -1.0;
+      T = gap - mid;
+    }
+  } else {
+    o = dl[j];
+    s = 1.0;
+    T = rho * w2sum;
+  }
+  uint64_t lo = 0, hi = (uint64_t)__double_as_longlong(T);
+  while (hi - lo > 1) {
+    const uint64_t md = lo + (hi - lo) / 2;
+    if (secular_h(dl, w2, K, inv_rho, o, s, __longlong_as_double((long long)md)) >= 0.0)
+      hi = md;
+    else
+      lo = md;
+  }
+  const double t = __longlong_as_double((long long)hi);
+  dorg[j] = o;
+  tau[j] = s * t;
+  lam[j] = o + s * t;
+}
+
+// Gu-Eisenstat: zhat_i^2 = (lambda_i - dl_i)/rho * prod_{j != i} (lambda_j - dl_i)/(dl_j - dl_i),
+// the weights for which the computed roots are exact; sign from w_i. The product
+// is renormalised every 32 factors (frexp) so that it cannot over- or underflow.
+__global__ void __launch_bounds__(256) k_secular_zhat(int64_t K, const double* __restrict__ dl,
+                                                      const double* __restrict__ w, double rho,
+                                                      const double* __restrict__ dorg,
+                                                      const double* __restrict__ tau,
+                                                      double* __restrict__ zhat) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K) return;
+  const double di = dl[i];
+  double p = 1.0;
+  int e2 = 0;
+  for (int64_t j = 0; j < K; ++j) {
+    const double num = (dorg[j] - di) + tau[j];  // lambda_j - dl_i
+    p *= (j == i) ? num : num / (dl[j] - di);
+    if ((j & 31) == 31) {
+      int e;
+      p = frexp(p, &e);
+      e2 += e;
+    }
+  }
+  p = fabs(p) / rho;
+  int e;
+  p = frexp(p, &e);
+  e2 += e;
+  if (e2 & 1) {
+    p *= 2.0;
+    e2 -= 1;
+  }
+  const double z = ldexp(sqrt(p), e2 / 2);
+  zhat[i] = w[i] < 0.0 ? -z : z;
+}
+
+// inv_norm[j] = 1 / || (zhat_i / (dl_i - lambda_j))_i ||
+__global__ void __launch_bounds__(256) k_secular_norms(int64_t K, const double* __restrict__ dl,
+                                                       const double* __restrict__ zhat,
+                                                       const double* __restrict__ dorg,
+                                                       const double* __restrict__ tau,
+                                                       double* __restrict__ inv_norm) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= K) return;
+  const double o = dorg[j], t = tau[j];
+  double s = 0.0;
+  for (int64_t i = 0; i < K; ++i) {
+    const double v = zhat[i] / ((dl[i] - o) - t);
+    s += v * v;
+  }
+  inv_norm[j] = 1.0 / sqrt(s);
+}
+
+// M[row[i] * ldm + col[j]] = zhat_i / (dl_i - lambda_j) * inv_norm[j]   (i, j < K)
+__global__ void __launch_bounds__(256) k_secular_fill(int64_t K, const double* __restrict__ dl,
+                                                      const double* __restrict__ zhat,
+                                                      const double* __restrict__ dorg,
+                                                      const double* __restrict__ tau,
+                                                      const double* __restrict__ inv_norm,
+                                                      const int64_t* __restrict__ row,
+                                                      const int64_t* __restrict__ col,
+                                                      double* __restrict__ M, int64_t ldm,
+                                                      int64_t njb) {
+  const int64_t i = (int64_t)blockIdx.x / njb;
+  const int64_t j = ((int64_t)blockIdx.x % njb) * blockDim.x + threadIdx.x;
+  if (i >= K || j >= K) return;
+  const double v = zhat[i] / ((dl[i] - dorg[j]) - tau[j]) * inv_norm[j];
+  M[row[i] * ldm + col[j]] = v;
+}
+
+// M[row[t] * ldm + col[t]] = 1   (deflated eigen-pairs keep their basis vector)
+__global__ void k_scatter_ones(int64_t cnt, const int64_t* __restrict__ row,
+                               const int64_t* __restrict__ col, double* __restrict__ M,
+                               int64_t ldm) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < cnt) M[row[t] * ldm + col[t]] = 1.0;
+}
+
+// Left-multiply M (row-major, n_cols columns) by G_0 G_1 ... G_{r-1}: applied
+// last to first, G_t mixes rows p_t and q_t:
+//   M[p, :] <- c M[p, :] - s M[q, :],   M[q, :] <- s M[p, :] + c M[q, :].
+// One thread per column; the rotation sequence is read in the same order by
+// every lane.
+__global__ void __launch_bounds__(256) k_apply_rotations(int64_t r, const int64_t* __restrict__ pq,
+                                                         const double* __restrict__ cs,
+                                                         int64_t n_cols, double* __restrict__ M,
+                                                         int64_t ldm) {
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n_cols) return;
+  for (int64_t t = r - 1; t >= 0; --t) {
+    const int64_t p = pq[2 * t], q = pq[2 * t + 1];
+    const double c = cs[2 * t], s = cs[2 * t + 1];
+    double* mp = M + p * ldm + col;
+    double* mq = M + q * ldm + col;
+    const double x = *mp, y = *mq;
+    *mp = c * x - s * y;
+    *mq = s * x + c * y;
+  }
+}
+
+}  // namespace jm
+
+using namespace jm;
+
+extern "C" int jm_dgemm(int trans_a, int trans_b, int64_t m, int64_t n, int64_t k, double alpha,
+                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                        double* C, int64_t ldc, int lower_only, void* stream) {
+  JM_CHECK_ARG(m >= 0 && n >= 0 && k >= 0, "bad sizes");
+  if (m == 0 || n == 0) return JM_OK;
+  JM_CHECK_ARG(C && (k == 0 || (A && B)), "null pointer");
+  JM_CHECK_ARG(lda >= (trans_a ? k : m) && lda >= 1, "lda too small");
+  JM_CHECK_ARG(ldb >= (trans_b ? n : k) && ldb >= 1, "ldb too small");
+  JM_CHECK_ARG(ldc >= m, "ldc < m");
+  const int64_t nti = (m + DG_TI - 1) / DG_TI, ntj = (n + DG_TJ - 1) / DG_TJ;
+  JM_CHECK_ARG(nti * ntj < (int64_t)1 << 31, "grid too large");
+  const dim3 grid((unsigned)(nti * ntj)), block(256);
+  hipStream_t st = (hipStream_t)stream;
+  const int lo = lower_only ? 1 : 0;
+  if (!trans_a && !trans_b)
+    hipLaunchKernelGGL((k_dgemm<false, false>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc, lo, nti);
+  else if (!trans_a && trans_b)
+    hipLaunchKernelGGL((k_dgemm<false, true>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc, lo, nti);
+  else if (trans_a && !trans_b)
+    hipLaunchKernelGGL((k_dgemm<true, false>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc, lo, nti);
+  else
+    hipLaunchKernelGGL((k_dgemm<true, true>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc, lo, nti);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int jm_tridiag_take_unit(double* a, double* e, void* stream) {
+  JM_CHECK_ARG(a && e, "null pointer");
+  hipLaunchKernelGGL(k_take_unit, dim3(1), dim3(64), 0, (hipStream_t)stream, a, e);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int jm_tridiag_finish(int64_t len, const double* tau, double* w, const double* v,
+                                 void* stream) {
+  JM_CHECK_ARG(len >= 0, "bad length");
+  if (len == 0) return JM_OK;
+  JM_CHECK_ARG(tau && w && v, "null pointer");
+  hipLaunchKernelGGL(k_latrd_finish, dim3(1), dim3(1024), 0, (hipStream_t)stream, len, tau, w, v);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int jm_secular_solve(int64_t K, const double* dl, const double* w, const double* w2,
+                                double rho, double w2sum, double* dorg, double* tau, double* lam,
+                                double* zhat, double* inv_norm, void* stream) {
+  JM_CHECK_ARG(K >= 0 && rho > 0.0, "bad sizes or rho <= 0");
+  if (K == 0) return JM_OK;
+  JM_CHECK_ARG(dl && w && w2 && dorg && tau && lam && zhat && inv_norm, "null pointer");
+  const dim3 grid((unsigned)((K + 255) / 256)), block(256);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_secular_roots, grid, block, 0, st, K, dl, w2, rho, w2sum, dorg, tau, lam);
+  JM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_secular_zhat, grid, block, 0, st, K, dl, w, rho, dorg, tau, zhat);
+  JM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_secular_norms, grid, block, 0, st, K, dl, zhat, dorg, tau, inv_norm);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int jm_secular_vectors(int64_t K, const double* dl, const double* zhat,
+                                  const double* dorg, const double* tau, const double* inv_norm,
+                                  const int64_t* row, const int64_t* col, int64_t n_defl,
+                                  const int64_t* row_defl, const int64_t* col_defl, double* M,
+                                  int64_t ldm, void* stream) {
+  JM_CHECK_ARG(K >= 0 && n_defl >= 0 && ldm >= K + n_defl, "bad sizes");
+  JM_CHECK_ARG(M, "null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (K > 0) {
+    JM_CHECK_ARG(dl && zhat && dorg && tau && inv_norm && row && col, "null pointer");
+    const int64_t njb = (K + 255) / 256;
+    JM_CHECK_ARG(K * njb < (int64_t)1 << 31, "K too large");
+    hipLaunchKernelGGL(k_secular_fill, dim3((unsigned)(K * njb)), dim3(256), 0, st, K, dl, zhat,
+                       dorg, tau, inv_norm, row, col, M, ldm, njb);
+    JM_CHECK_LAUNCH();
+  }
+  if (n_defl > 0) {
+    JM_CHECK_ARG(row_defl && col_defl, "null pointer");
+    hipLaunchKernelGGL(k_scatter_ones, dim3((unsigned)((n_defl + 255) / 256)), dim3(256), 0, st,
+                       n_defl, row_defl, col_defl, M, ldm);
+    JM_CHECK_LAUNCH();
+  }
+  return JM_OK;
+}
+
+extern "C" int jm_apply_rotations(int64_t r, const int64_t* pq, const double* cs, int64_t n_cols,
+                                  double* M, int64_t ldm, void* stream) {
+  JM_CHECK_ARG(r >= 0 && n_cols >= 0 && ldm >= n_cols, "bad sizes");
+  if (r == 0 || n_cols == 0) return JM_OK;
+  JM_CHECK_ARG(pq && cs && M, "null pointer");
+  hipLaunchKernelGGL(k_apply_rotations, dim3((unsigned)((n_cols + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, r, pq, cs, n_cols, M, ldm);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}

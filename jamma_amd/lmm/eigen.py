@@ -10,8 +10,22 @@ on the tridiagonal matrix applied to that factor (dstedc). They are called direc
 rather than through torch.linalg.eigh: the hipSOLVER layer torch goes through sizes
 its workspace in 32-bit integers and fails for cohorts of ~30k samples, while
 rocSOLVER manages its own workspace through the rocBLAS handle. Taking the steps one
-by one lets a caller report progress between them (the reduction alone takes about
-a minute at n = 50,000).
+by one lets a caller report progress between them.
+
+Those routines take 32-bit sizes and have no 64-bit variant, so they serve
+n <= 46,340 (n * n <= 2^31 - 1). Larger cohorts (the reference reaches them with
+ILP64 LAPACK, src/jamma/lmm/eigen.py:9,39) go through _eigh_large, which keeps
+every offset 64-bit:
+  1. blocked Householder tridiagonalisation (LAPACK dsytrd/dlatrd, lower) with
+     rocBLAS' 64-bit symv/gemv, rocSOLVER's 64-bit dlarfg, and this library's
+     FP64 MFMA GEMM (jm_dgemm) for the rank-2k trailing updates;
+  2. Cuppen divide and conquer on the tridiagonal matrix: halves are torn off
+     until each fits rocSOLVER's dstedc, and pairs are merged through the
+     rank-one update D + rho z z^T (deflation as LAPACK dlaed2; secular roots,
+     Gu-Eisenstat weights and eigenvectors on the device, jm_secular_*), then
+     Q = blockdiag(Q1, Q2) M with jm_dgemm;
+  3. back-transformation by the reflectors in compact WY blocks (I - V T V^T),
+     T from the V^T v products the reduction already forms (LAPACK dlarft).
 """
 
 from __future__ import annotations
@@ -23,10 +37,17 @@ import warnings
 import numpy as np
 import torch
 
-from jamma_amd.device import require_gpu, to_dev
+from jamma_amd import _lib
+from jamma_amd.device import ptr, require_gpu, stream_ptr, to_dev
 
-_RB_EVECT_ORIGINAL = 211  # rocblas_evect_original
-_RB_FILL_LOWER = 122      # rocblas_fill_lower
+_RB_EVECT_ORIGINAL = 211     # rocblas_evect_original
+_RB_EVECT_TRIDIAGONAL = 212  # rocblas_evect_tridiagonal
+_RB_FILL_LOWER = 122         # rocblas_fill_lower
+_RB_OP_N, _RB_OP_T = 111, 112
+_RB_POINTER_HOST, _RB_POINTER_DEVICE = 0, 1
+_LIB_MAX_N = 46_340          # largest n with n * n <= 2^31 - 1
+_LARGE_PANEL = 64            # reflectors per panel / per compact-WY block
+_LARGE_LEAF = 30_000         # largest tridiagonal block handed to rocSOLVER dstedc
 _SOLVER = {}
 
 
@@ -48,6 +69,14 @@ def _solver(device: torch.device):
     rs.rocsolver_dorgtr.argtypes = [vp, ci, ci, vp, ci, vp]
     rs.rocsolver_dstedc.argtypes = [vp, ci, ci, vp, vp, vp, ci, vp]
     for f in (rs.rocsolver_dsytrd, rs.rocsolver_dorgtr, rs.rocsolver_dstedc):
+        f.restype = ci
+    i64 = ctypes.c_int64
+    rb.rocblas_set_pointer_mode.argtypes = [vp, ci]
+    rb.rocblas_dsymv_64.argtypes = [vp, ci, i64, vp, vp, i64, vp, i64, vp, vp, i64]
+    rb.rocblas_dgemv_64.argtypes = [vp, ci, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64]
+    rs.rocsolver_dlarfg_64.argtypes = [vp, i64, vp, vp, i64, vp]
+    for f in (rb.rocblas_set_pointer_mode, rb.rocblas_dsymv_64, rb.rocblas_dgemv_64,
+              rs.rocsolver_dlarfg_64):
         f.restype = ci
     h = ctypes.c_void_p()
     with torch.cuda.device(idx):
@@ -99,6 +128,287 @@ def _dsyevd(Kd: torch.Tensor, progress=None):
     return D, A.t()
 
 
+def _ok(name, st):
+    if st != 0:
+        raise RuntimeError(f"{name} failed with status {st}")
+
+
+def _tridiagonalize(A: torch.Tensor, nb: int, progress=None):
+    """Householder reduction of the symmetric device matrix A (n x n, float64; as a
+    column-major matrix it is the same) to tridiagonal form, LAPACK dsytrd/dlatrd
+    with UPLO = 'L' and panels of nb columns.
+
+    A is overwritten: column c (row c of the tensor) holds reflector c's vector
+    below the diagonal, with its unit entry at row c + 1 stored as 1.
+    Returns (d, e, tau) on the device and, per panel, the nb x nb matrix whose
+    strictly upper part is V^T V of the panel's reflectors (the dlatrd products
+    W(0:i, i) = V(:, 0:i)^T v_i), for the compact-WY factors.
+    """
+    n = A.shape[0]
+    dev = A.device
+    rb, rs, h = _solver(dev)
+    f64 = torch.float64
+    e = torch.zeros(max(n - 1, 1), dtype=f64, device=dev)
+    tau = torch.zeros(max(n - 1, 1), dtype=f64, device=dev)
+    Wt = torch.zeros((nb, n), dtype=f64, device=dev)  # column-major W (n x nb, ld n)
+    consts = torch.tensor([1.0, 0.0, -1.0], dtype=f64, device=dev)
+    one, zero, mone = (consts.data_ptr() + 8 * k for k in range(3))
+    base, wb, ld = A.data_ptr(), Wt.data_ptr(), n
+    st = stream_ptr()
+
+    def pa(r, c):
+        return base + 8 * (c * ld + r)
+
+    def pw(r, c):
+        return wb + 8 * (c * ld + r)
+
+    def gemv(trans, m, k, alpha, a, lda, x, incx, beta, y):
+        _ok("rocblas_dgemv_64", rb.rocblas_dgemv_64(h, trans, m, k, alpha, a, lda, x, incx, beta,
+                                                    y, 1))
+
+    gram = []
+    _ok("rocblas_set_stream", rb.rocblas_set_stream(h, ctypes.c_void_p(st)))
+    _ok("rocblas_set_pointer_mode", rb.rocblas_set_pointer_mode(h, _RB_POINTER_DEVICE))
+    try:
+        next_report = 8192
+        for p0 in range(0, n, nb):
+            Np = n - p0
+            nbp = min(nb, Np)
+            for i in range(nbp):
+                gc = p0 + i
+                if i > 0:  # apply the panel's earlier reflectors to column i
+                    gemv(_RB_OP_N, Np - i, i, mone, pa(gc, p0), ld, pw(i, 0), ld, one, pa(gc, gc))
+                    gemv(_RB_OP_N, Np - i, i, mone, pw(i, 0), ld, pa(gc, p0), ld, one, pa(gc, gc))
+                if i == Np - 1:
+                    continue
+                L = Np - i - 1
+                v = pa(gc + 1, gc)
+                _ok("rocsolver_dlarfg_64", rs.rocsolver_dlarfg_64(
+                    h, L, v, pa(p0 + min(i + 2, Np - 1), gc), 1, tau.data_ptr() + 8 * gc))
+                _lib.call("jm_tridiag_take_unit", v, e.data_ptr() + 8 * gc, st)
+                w = pw(i + 1, i)
+                _ok("rocblas_dsymv_64", rb.rocblas_dsymv_64(h, _RB_FILL_LOWER, L, one,
+                                                            pa(gc + 1, gc + 1), ld, v, 1, zero,
+                                                            w, 1))
+                if i > 0:
+                    gemv(_RB_OP_T, L, i, one, pw(i + 1, 0), ld, v, 1, zero, pw(0, i))
+                    gemv(_RB_OP_N, L, i, mone, pa(gc + 1, p0), ld, pw(0, i), 1, one, w)
+                    gemv(_RB_OP_T, L, i, one, pa(gc + 1, p0), ld, v, 1, zero, pw(0, i))
+                    gemv(_RB_OP_N, L, i, mone, pw(i + 1, 0), ld, pw(0, i), 1, one, w)
+                _lib.call("jm_tridiag_finish", L, tau.data_ptr() + 8 * gc, w, v, st)
+            gram.append(Wt[:nbp, :nbp].t().clone())  # W(0:nbp, 0:nbp), row-major
+            if Np > nbp:  # trailing update A22 -= V W^T + W V^T (lower triangle)
+                N2 = Np - nbp
+                _lib.call("jm_dgemm", 0, 1, N2, N2, nbp, -1.0, pa(p0 + nbp, p0), ld,
+                          pw(nbp, 0), ld, 1.0, pa(p0 + nbp, p0 + nbp), ld, 1, st)
+                _lib.call("jm_dgemm", 0, 1, N2, N2, nbp, -1.0, pw(nbp, 0), ld,
+                          pa(p0 + nbp, p0), ld, 1.0, pa(p0 + nbp, p0 + nbp), ld, 1, st)
+            if progress is not None and p0 + nbp >= next_report and Np > nbp:
+                torch.cuda.current_stream().synchronize()
+                progress(f"tridiagonal reduction: {p0 + nbp} / {n} columns")
+                next_report += 8192
+    finally:
+        rb.rocblas_set_pointer_mode(h, _RB_POINTER_HOST)
+    d = A.diagonal().clone()
+    return d, e[: max(n - 1, 0)], tau, gram
+
+
+def _stedc(d: np.ndarray, e: np.ndarray, device):
+    """rocSOLVER dstedc on a tridiagonal block (n * n <= 2^31 - 1): (w ascending host,
+    Q device (n, n) whose row j is eigenvector j)."""
+    n = len(d)
+    rb, rs, h = _solver(device)
+    D = torch.as_tensor(np.ascontiguousarray(d), dtype=torch.float64).to(device)
+    E = torch.zeros(max(n, 1), dtype=torch.float64, device=device)
+    if n > 1:
+        E[: n - 1] = torch.as_tensor(np.ascontiguousarray(e), dtype=torch.float64).to(device)
+    C = torch.empty((n, n), dtype=torch.float64, device=device)
+    info = torch.zeros(1, dtype=torch.int32, device=device)
+    _ok("rocblas_set_stream", rb.rocblas_set_stream(h, ctypes.c_void_p(stream_ptr())))
+    _ok("rocsolver_dstedc", rs.rocsolver_dstedc(h, _RB_EVECT_TRIDIAGONAL, n, D.data_ptr(),
+                                                E.data_ptr(), C.data_ptr(), n, info.data_ptr()))
+    code = int(info.item())
+    if code != 0:
+        raise RuntimeError(f"rocsolver_dstedc did not converge: info={code} (n={n})")
+    return D.cpu().numpy(), C
+
+
+def _rank_one_merge(dc: np.ndarray, z: np.ndarray, rho: float, device):
+    """Eigen-pairs of diag(dc) + rho z z^T (rho >= 0, |z| = sqrt 2 as torn): (w ascending
+    host, M device (n, n) row-major with M[a, c] = component a of eigenvector c).
+
+    Deflation follows LAPACK dlaed2: z is scaled to unit norm (rho doubled); a pair
+    deflates when rho |z_k| <= tol, or by a Givens rotation when two poles are closer
+    than tol allows (|t c s| <= tol), tol = 8 eps max(max|d|, max|z|).
+    """
+    n = len(dc)
+    f64 = torch.float64
+    z = z / np.sqrt(2.0)
+    rho = 2.0 * rho
+    order = np.argsort(dc, kind="stable")
+    ds = dc[order].astype(np.float64).tolist()
+    zs = z[order].astype(np.float64).tolist()
+    tol = 8.0 * (np.finfo(np.float64).eps / 2) * max(float(np.max(np.abs(dc))),
+                                                      float(np.max(np.abs(z))))
+    M = torch.zeros((n, n), dtype=f64, device=device)
+    nd, defl_val, defl_row, rots = [], [], [], []
+    if rho * float(np.max(np.abs(z))) <= tol:
+        defl_val, defl_row = ds, order.tolist()
+    else:
+        pj = -1
+        for k in range(n):
+            if rho * abs(zs[k]) <= tol:
+                defl_val.append(ds[k])
+                defl_row.append(int(order[k]))
+                continue
+            if pj < 0:
+                pj = k
+                continue
+            s_, c_ = zs[pj], zs[k]
+            tau = float(np.hypot(c_, s_))
+            t = ds[k] - ds[pj]
+            c_, s_ = c_ / tau, -s_ / tau
+            if abs(t * c_ * s_) <= tol:
+                zs[k], zs[pj] = tau, 0.0
+                rots.append((int(order[pj]), int(order[k]), c_, s_))
+                dp = ds[pj] * c_ * c_ + ds[k] * s_ * s_
+                ds[k] = ds[pj] * s_ * s_ + ds[k] * c_ * c_
+                defl_val.append(dp)
+                defl_row.append(int(order[pj]))
+            else:
+                nd.append(pj)
+            pj = k
+        if pj >= 0:
+            nd.append(pj)
+    K = len(nd)
+    st = stream_ptr()
+    lam = np.zeros(0)
+    if K:
+        dl = torch.as_tensor([ds[k] for k in nd], dtype=f64).to(device)
+        wh = np.array([zs[k] for k in nd])
+        w = torch.as_tensor(wh, dtype=f64).to(device)
+        w2 = w * w
+        work = torch.empty((5, K), dtype=f64, device=device)  # dorg, tau, lam, zhat, inv_norm
+        _lib.call("jm_secular_solve", K, ptr(dl), ptr(w), ptr(w2), rho, float(np.sum(wh * wh)),
+                  ptr(work[0]), ptr(work[1]), ptr(work[2]), ptr(work[3]), ptr(work[4]), st)
+        lam = work[2].cpu().numpy()
+    vals = np.concatenate([lam, np.asarray(defl_val, dtype=np.float64)])
+    perm = np.argsort(vals, kind="stable")
+    col = np.empty(n, dtype=np.int64)
+    col[perm] = np.arange(n)
+    rows_nd = torch.as_tensor(order[nd].astype(np.int64) if K else np.zeros(0, np.int64)).to(device)
+    cols_nd = torch.as_tensor(col[:K]).to(device)
+    rows_d = torch.as_tensor(np.asarray(defl_row, dtype=np.int64)).to(device)
+    cols_d = torch.as_tensor(col[K:]).to(device)
+    if K:
+        _lib.call("jm_secular_vectors", K, ptr(dl), ptr(work[3]), ptr(work[0]), ptr(work[1]),
+                  ptr(work[4]), ptr(rows_nd), ptr(cols_nd), n - K, ptr(rows_d), ptr(cols_d),
+                  ptr(M), n, st)
+    elif n:
+        _lib.call("jm_secular_vectors", 0, None, None, None, None, None, None, None, n,
+                  ptr(rows_d), ptr(cols_d), ptr(M), n, st)
+    if rots:
+        pq = torch.as_tensor(np.array([(p, q) for p, q, _, _ in rots], dtype=np.int64)).to(device)
+        cs = torch.as_tensor(np.array([(c, s) for _, _, c, s in rots], dtype=np.float64)).to(device)
+        _lib.call("jm_apply_rotations", len(rots), ptr(pq), ptr(cs), n, ptr(M), n, st)
+    torch.cuda.current_stream().synchronize()  # host arrays above stay alive until here
+    return vals[perm], M
+
+
+def _tri_eig(d: np.ndarray, e: np.ndarray, device, leaf: int):
+    """Eigen-pairs of the symmetric tridiagonal (d, e): (w ascending host, Q device (n, n)
+    whose row j is eigenvector j). Cuppen tear at the middle until blocks fit dstedc."""
+    n = len(d)
+    if n <= leaf:
+        return _stedc(d, e, device)
+    m = n // 2
+    beta = float(e[m - 1])
+    rho, sgn = abs(beta), (1.0 if beta >= 0 else -1.0)
+    d1, d2 = d[:m].copy(), d[m:].copy()
+    d1[-1] -= rho
+    d2[0] -= rho
+    w1, Q1 = _tri_eig(d1, e[: m - 1], device, leaf)
+    w2, Q2 = _tri_eig(d2, e[m:], device, leaf)
+    # z = Q^T (e_{m-1} + sgn e_m): the last row of Q1 and the first row of Q2
+    z = np.concatenate([Q1[:, m - 1].cpu().numpy(), sgn * Q2[:, 0].cpu().numpy()])
+    w, M = _rank_one_merge(np.concatenate([w1, w2]), z, rho, device)
+    Q = torch.empty((n, n), dtype=torch.float64, device=device)  # column-major Q
+    st = stream_ptr()
+    # Q(0:m, :) = Q1 M(0:m, :) and Q(m:, :) = Q2 M(m:, :); M row-major = M^T column-major
+    _lib.call("jm_dgemm", 0, 1, m, n, m, 1.0, ptr(Q1), m, ptr(M), n, 0.0, ptr(Q), n, 0, st)
+    _lib.call("jm_dgemm", 0, 1, n - m, n, n - m, 1.0, ptr(Q2), n - m, M.data_ptr() + 8 * m * n,
+              n, 0.0, Q.data_ptr() + 8 * m, n, 0, st)
+    torch.cuda.current_stream().synchronize()
+    del Q1, Q2, M
+    return w, Q
+
+
+def _compact_wy_t(tau: np.ndarray, G: np.ndarray) -> np.ndarray:
+    """Upper-triangular T with H_0 ... H_{k-1} = I - V T V^T (LAPACK dlarft, forward,
+    columnwise): T[i, i] = tau_i, T[:i, i] = -tau_i T[:i, :i] (V^T v_i)[:i], where the
+    strictly upper part of G holds V^T V."""
+    k = len(tau)
+    T = np.zeros((k, k))
+    for i in range(k):
+        T[i, i] = tau[i]
+        if i:
+            T[:i, i] = -tau[i] * (T[:i, :i] @ G[:i, i])
+    return T
+
+
+def _back_transform(A: torch.Tensor, tau: torch.Tensor, gram, Q: torch.Tensor, nb: int):
+    """Q (column-major, n x n) <- H_0 H_1 ... H_{n-2} Q, the reflectors of
+    _tridiagonalize applied in blocks of nb from the last block to the first."""
+    n = A.shape[0]
+    dev = A.device
+    st = stream_ptr()
+    tau_h = tau.cpu().numpy()
+    blocks = []
+    for b, p0 in enumerate(range(0, n, nb)):
+        r = min(nb, n - 1 - p0)  # reflectors of this panel (the last column has none)
+        if r > 0:
+            T = _compact_wy_t(tau_h[p0:p0 + r], gram[b].cpu().numpy()[:r, :r])
+            blocks.append((p0, r, T))
+    Y = torch.empty((nb, n), dtype=torch.float64, device=dev)   # column-major (r x n, ld r)
+    Y2 = torch.empty((nb, n), dtype=torch.float64, device=dev)
+    for p0, r, T in reversed(blocks):
+        L = n - 1 - p0
+        Vt = A[p0:p0 + r, p0 + 1:].clone()  # column-major V (L x r, ld L)
+        Vt[:, :r] = torch.triu(Vt[:, :r], 1) + torch.eye(r, dtype=torch.float64, device=dev)
+        Td = torch.as_tensor(np.ascontiguousarray(T.T)).to(dev)  # column-major T
+        QR = Q.data_ptr() + 8 * (p0 + 1)
+        _lib.call("jm_dgemm", 1, 0, r, n, L, 1.0, ptr(Vt), L, QR, n, 0.0, ptr(Y), r, 0, st)
+        _lib.call("jm_dgemm", 0, 0, r, n, r, 1.0, ptr(Td), r, ptr(Y), r, 0.0, ptr(Y2), r, 0, st)
+        _lib.call("jm_dgemm", 0, 0, L, n, r, -1.0, ptr(Vt), L, ptr(Y2), r, 1.0, QR, n, 0, st)
+        del Vt, Td
+
+
+def _eigh_large(Kd: torch.Tensor, progress=None, nb: int | None = None,
+                leaf: int | None = None):
+    """(d ascending device, U columns) of a symmetric device float64 matrix with 64-bit
+    offsets throughout (for n beyond the library solvers' 32-bit limit). nb: panel
+    width; leaf: largest tridiagonal block solved by dstedc."""
+    nb = _LARGE_PANEL if nb is None else nb
+    leaf = _LARGE_LEAF if leaf is None else leaf
+    if nb < 1 or leaf < 1 or leaf > _LIB_MAX_N:
+        raise ValueError("panel width and leaf size must be in [1, 46340]")
+    A = Kd.clone(memory_format=torch.contiguous_format)
+    d, e, tau, gram = _tridiagonalize(A, nb, progress)
+    torch.cuda.current_stream().synchronize()
+    if progress is not None:
+        progress("tridiagonal reduction")
+    w, Q = _tri_eig(d.cpu().numpy(), e.cpu().numpy(), A.device, leaf)
+    if progress is not None:
+        progress("tridiagonal divide and conquer")
+    _back_transform(A, tau, gram, Q, nb)
+    torch.cuda.current_stream().synchronize()
+    if progress is not None:
+        progress("back-transformation")
+    del A
+    # Q column-major: tensor row j is eigenvector j, so U = Q^T as a tensor
+    return torch.as_tensor(w, dtype=torch.float64).to(Kd.device), Q.t()
+
+
 def eigh_device(K, threshold: float = 1e-10, progress=None):
     """-> (d host float64 ascending thresholded, U device float64 (n x n)).
 
@@ -111,7 +421,10 @@ def eigh_device(K, threshold: float = 1e-10, progress=None):
         raise ValueError(f"Kinship matrix must be square, got shape {tuple(Kd.shape)}")
     if Kd.shape[0] == 0:
         raise ValueError("Kinship matrix is empty")
-    d, U = _dsyevd(Kd, progress)
+    if Kd.shape[0] <= _LIB_MAX_N:
+        d, U = _dsyevd(Kd, progress)
+    else:
+        d, U = _eigh_large(Kd, progress)
     del Kd
     d = d.cpu().numpy()
     n_neg = int(np.sum(d < -threshold))

@@ -1,0 +1,151 @@
+"""The large-cohort eigendecomposition (64-bit offsets, n beyond 46,340) at small n.
+
+The path is forced at small sizes with small panels and tear thresholds, so that
+every stage runs several times: blocked Householder panels with trailing updates,
+several levels of divide and conquer, both kinds of deflation (small weights and
+Givens rotations of coincident poles) and the compact-WY back-transformation.
+Checks against LAPACK (numpy): the spectrum, the residual K U - U diag(d), and
+U^T U = I. A structural mistake in any stage (a wrong reflector, a missed
+rotation, a transposed merge) breaks the residual or the orthogonality by many
+orders of magnitude.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from tests.util import balding_nichols
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(K, d, U, tol=1e-11):
+    K = np.asarray(K)
+    d = d.cpu().numpy() if isinstance(d, torch.Tensor) else np.asarray(d)
+    U = U.cpu().numpy()
+    n = K.shape[0]
+    scale = max(1.0, float(np.max(np.abs(np.linalg.eigvalsh(K)))))
+    ref = np.linalg.eigvalsh(K)
+    assert np.all(np.diff(d) >= 0)
+    assert np.max(np.abs(d - ref)) <= tol * scale, np.max(np.abs(d - ref))
+    assert np.max(np.abs(K @ U - U * d[None, :])) <= tol * scale
+    assert np.max(np.abs(U.T @ U - np.eye(n))) <= 1e-12 * max(1.0, n / 100)
+
+
+def _kinship(n, p, seed):
+    G = balding_nichols(n, p, seed=seed)
+    mu = np.nanmean(G, axis=0)
+    X = np.where(np.isnan(G), mu, G) - mu
+    return X @ X.T / p
+
+
+@pytest.mark.parametrize("n,nb,leaf", [(2, 4, 1), (37, 8, 8), (300, 32, 50), (1031, 64, 200)])
+def test_large_path_matches_lapack(n, nb, leaf):
+    from jamma_amd.lmm.eigen import _eigh_large
+
+    K = _kinship(n, 2 * n + 50, seed=n)
+    d, U = _eigh_large(torch.as_tensor(K, device="cuda"), nb=nb, leaf=leaf)
+    _check(K, d, U)
+
+
+def test_large_path_random_symmetric():
+    from jamma_amd.lmm.eigen import _eigh_large
+
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(517, 517))
+    K = (X + X.T) / 2
+    d, U = _eigh_large(torch.as_tensor(K, device="cuda"), nb=24, leaf=70)
+    _check(K, d, U)
+
+
+def test_large_path_repeated_and_zero_eigenvalues():
+    """Three identical blocks (every eigenvalue threefold) and a rank-20 kinship of
+    300 samples (280 zero eigenvalues): the deflation branches."""
+    from jamma_amd.lmm.eigen import _eigh_large
+
+    rng = np.random.default_rng(4)
+    B = rng.normal(size=(100, 100))
+    K1 = np.kron(np.eye(3), B @ B.T / 100)
+    d, U = _eigh_large(torch.as_tensor(K1, device="cuda"), nb=16, leaf=40)
+    _check(K1, d, U)
+    X = rng.normal(size=(300, 20))
+    K2 = X @ X.T / 20
+    d, U = _eigh_large(torch.as_tensor(K2, device="cuda"), nb=32, leaf=60)
+    _check(K2, d, U)
+
+
+def test_tridiagonal_merge_with_coincident_poles():
+    """T2 = J T1 J (mirror image): the two halves have the same spectrum, so the
+    merge deflates by Givens rotations; Wilkinson's W+ has close pairs."""
+    from jamma_amd.lmm.eigen import _tri_eig
+
+    rng = np.random.default_rng(5)
+    m = 120
+    d1, e1 = rng.normal(size=m), rng.uniform(0.1, 1.0, m - 1)
+    cases = [(np.concatenate([d1, d1[::-1]]), np.concatenate([e1, [0.5], e1[::-1]]), 120),
+             (np.abs(np.arange(-100, 101)).astype(float), np.ones(200), 30)]
+    for d, e, leaf in cases:
+        T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+        w, Q = _tri_eig(d, e, torch.device("cuda"), leaf)
+        Qh = Q.cpu().numpy().T  # rows of Q are the eigenvectors
+        n = len(d)
+        assert np.max(np.abs(w - np.linalg.eigvalsh(T))) <= 1e-12 * np.max(np.abs(w))
+        assert np.max(np.abs(T @ Qh - Qh * w[None, :])) <= 1e-12 * np.max(np.abs(w))
+        assert np.max(np.abs(Qh.T @ Qh - np.eye(n))) <= 1e-12
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_dgemm_transposes(ta, tb):
+    from jamma_amd import _lib
+    from jamma_amd.device import stream_ptr
+
+    rng = np.random.default_rng(10 * ta + tb)
+    m, n, k = 203, 77, 45
+    A = rng.normal(size=(k, m) if ta else (m, k))
+    B = rng.normal(size=(n, k) if tb else (k, n))
+    C0 = rng.normal(size=(m, n))
+    ref = -0.5 * ((A.T if ta else A) @ (B.T if tb else B)) + 2.0 * C0
+    # column-major device buffers: a (r x c) matrix is the row-major tensor of its transpose
+    Ad = torch.as_tensor(np.ascontiguousarray(A.T), device="cuda")
+    Bd = torch.as_tensor(np.ascontiguousarray(B.T), device="cuda")
+    Cd = torch.as_tensor(np.ascontiguousarray(C0.T), device="cuda")
+    _lib.call("jm_dgemm", ta, tb, m, n, k, -0.5, Ad.data_ptr(), A.shape[0], Bd.data_ptr(),
+              B.shape[0], 2.0, Cd.data_ptr(), m, 0, stream_ptr())
+    got = Cd.cpu().numpy().T
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_dgemm_lower_only_leaves_upper_tiles():
+    from jamma_amd import _lib
+    from jamma_amd.device import stream_ptr
+
+    rng = np.random.default_rng(11)
+    n, k = 300, 16
+    A, B = rng.normal(size=(n, k)), rng.normal(size=(n, k))
+    Cd = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    _lib.call("jm_dgemm", 0, 1, n, n, k, 1.0, torch.as_tensor(A.T.copy(), device="cuda").data_ptr(),
+              n, torch.as_tensor(B.T.copy(), device="cuda").data_ptr(), n, 0.0, Cd.data_ptr(), n,
+              1, stream_ptr())
+    got = Cd.cpu().numpy().T  # column-major C
+    ref = A @ B.T
+    low = np.tril(np.ones((n, n), bool))
+    assert np.max(np.abs(got[low] - ref[low])) <= 1e-12 * np.max(np.abs(ref))
+    # tiles entirely above the diagonal (i < j for every element) are untouched
+    i_tile, j_tile = np.arange(n)[:, None] // 128, np.arange(n)[None, :] // 64
+    untouched = (i_tile + 1) * 128 <= j_tile * 64
+    assert untouched.any() and np.all(got[untouched] == 0.0)
+
+
+def test_eigh_device_routes_large_cohorts(monkeypatch):
+    """eigh_device sends n above the library limit through the 64-bit path; the
+    thresholding and warnings are the same."""
+    import jamma_amd.lmm.eigen as eg
+
+    K = _kinship(260, 120, seed=7)  # rank-deficient: p < n
+    monkeypatch.setattr(eg, "_LIB_MAX_N", 100)
+    monkeypatch.setattr(eg, "_LARGE_LEAF", 64)
+    with pytest.warns(UserWarning, match="close to zero"):
+        d, U = eg.eigh_device(K)
+    assert int(np.sum(d == 0.0)) >= 260 - 119
+    Un = U.cpu().numpy()
+    assert np.max(np.abs(K @ Un - Un * d[None, :])) <= 1e-11 * np.max(d)

@@ -234,3 +234,28 @@ def test_segment_plan_pieces_fold_to_cluster_moments(seg_max):
         sl = slice(pl.cl_start[c], pl.cl_start[c] + pl.cl_size[c])
         direct = np.sum(z[sl] * pl.idx_delta[sl] ** m)
         assert abs(folded - direct) <= 1e-12 * max(1.0, np.sum(np.abs(z[sl] * pl.idx_delta[sl] ** m)))
+
+
+def test_compact_wy_factor_reproduces_reflector_product():
+    """T from the V^T V products (LAPACK dlarft, forward columnwise) gives
+    H_0 H_1 ... H_{k-1} = I - V T V^T, including a trivial reflector (tau = 0)."""
+    from jamma_amd.lmm.eigen import _compact_wy_t
+
+    rng = np.random.default_rng(2)
+    L, k = 40, 9
+    V = np.tril(rng.normal(size=(L, k)), -1) + np.eye(L, k)  # unit lower trapezoidal
+    tau = 2.0 / np.sum(V * V, axis=0)  # orthogonal reflectors
+    tau[4] = 0.0
+    H = np.eye(L)
+    for i in range(k):
+        H = H @ (np.eye(L) - tau[i] * np.outer(V[:, i], V[:, i]))
+    T = _compact_wy_t(tau, V.T @ V)
+    assert np.allclose(np.triu(T), T)
+    assert np.max(np.abs(np.eye(L) - V @ T @ V.T - H)) < 1e-13
+
+
+def test_library_eigen_limit_is_the_32_bit_bound():
+    from jamma_amd.lmm import eigen as eg
+
+    assert eg._LIB_MAX_N ** 2 <= 2**31 - 1 < (eg._LIB_MAX_N + 1) ** 2
+    assert eg._LARGE_LEAF <= eg._LIB_MAX_N
