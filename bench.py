@@ -78,8 +78,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=2048,
                     help="SNPs of the CPU baseline sample (about 15 s of host work at n=10k)")
-    ap.add_argument("--planes", type=int, default=2, choices=[2, 3],
-                    help="bf16 planes of U^T on the mixed path (2: 3 MFMA products, 3: 4)")
+    ap.add_argument("--planes", type=int, default=None, choices=[2, 3],
+                    help="bf16 planes of U^T on the mixed path (2: 3 MFMA products, 3: 4; "
+                         "default: the cohort's choice by sample size)")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 path (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-serving", action="store_true",
@@ -184,6 +185,7 @@ def main() -> int:
 
     t0 = time.perf_counter()
     cohort = Cohort(d, U, precision="both", bf16_planes=args.planes)
+    args.planes = cohort.bf16_planes
     prob = AssocProblem(cohort, y, W)
     torch.cuda.synchronize()
     log(f"cohort + problem setup {time.perf_counter() - t0:.2f}s: T={cohort.n_terms} terms "
@@ -244,9 +246,13 @@ def main() -> int:
         tests = {1: (("p_wald", 2),), 2: (("p_lrt", 7),), 3: (("p_score", 8),)}.get(
             args.mode, (("p_wald", 2), ("p_lrt", 7), ("p_score", 8)))
         for name, col in tests:
-            dl = np.abs(np.log10(a[:, col]) - np.log10(b[:, col]))
-            agree = all(np.array_equal(a[:, col] < t, b[:, col] < t) for t in (0.05, 0.01, 1e-3, 5e-8))
-            parity[name] = {"max_abs_dlog10p": float(np.nanmax(dl)), "significance_agreement": agree}
+            with np.errstate(divide="ignore", invalid="ignore"):  # p = 0 (underflow) in both
+                dl = np.abs(np.log10(a[:, col]) - np.log10(b[:, col]))
+            flips = sum(int(np.sum((a[:, col] < t) != (b[:, col] < t)))
+                        for t in (0.05, 0.01, 1e-3, 5e-8))
+            parity[name] = {"max_abs_dlog10p": float(np.nanmax(dl)),
+                            "significance_agreement": flips == 0, "significance_flips": flips,
+                            "p_zero_in_both": int(np.sum((a[:, col] == 0) & (b[:, col] == 0)))}
         parity["beta_sign_agreement"] = bool(np.array_equal(np.sign(a[:, 0]), np.sign(b[:, 0])))
         parity["n_sig_5e-8_wald"] = int(np.sum(b[:, 2] < 5e-8))
 

@@ -60,24 +60,32 @@ def pair_tables(n_cvt: int) -> tuple[np.ndarray, np.ndarray]:
     return pair_row, shared_src
 
 
+TWO_PLANE_MAX_N = 16384  # largest cohort rotated with the two-plane split by default
+
+
 class Cohort:
     """Eigendecomposition of one kinship, resident on the device."""
 
     def __init__(self, eigenvalues, eigenvectors, precision: str = "mixed", device=None,
-                 bf16_planes: int = 2):
+                 bf16_planes: int | None = None):
         """bf16_planes: 2 = U^T as hi + lo (16 significant bits; the rotation issues 3
         bf16 products per term: hi X1 + lo X1 + hi X2), 3 = hi + mid + lo (24 bits,
-        4 products)."""
+        4 products). None picks 2 up to TWO_PLANE_MAX_N samples and 3 above: the
+        relative error of a rotated value is about 2^-17 with two planes, and
+        |delta log10 p| grows with -log10 p, which reaches ~60 in cohorts of 50,000
+        (there two planes measured 1.5e-4)."""
         if precision not in PRECISIONS + ("both",):
             raise ValueError(f"precision must be one of {PRECISIONS + ('both',)}")
-        if bf16_planes not in (2, 3):
-            raise ValueError("bf16_planes must be 2 or 3")
-        self.bf16_planes = bf16_planes
-        self.device = device or require_gpu()
         d = eigenvalues.detach().cpu().numpy() if isinstance(eigenvalues, torch.Tensor) \
             else np.asarray(eigenvalues, dtype=np.float64)
         d = np.ascontiguousarray(d, dtype=np.float64)
         self.n = n = d.shape[0]
+        if bf16_planes is None:
+            bf16_planes = 2 if n <= TWO_PLANE_MAX_N else 3
+        if bf16_planes not in (2, 3):
+            raise ValueError("bf16_planes must be 2 or 3")
+        self.bf16_planes = bf16_planes
+        self.device = device or require_gpu()
         self.d = d
         self.plan: SpectrumPlan = plan_spectrum(d)
         U = to_dev(eigenvectors, torch.float64, self.device)
