@@ -331,6 +331,15 @@ int jm_dgemm(int trans_a, int trans_b, int64_t m, int64_t n, int64_t k, double a
              const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
              int64_t ldc, int lower_only, void* stream);
 
+/* y = A x for a symmetric A (N x N, lda) stored in its lower triangle, column-
+ * major: every stored element is read once (64 x 64 tiles; each off-diagonal
+ * tile serves A_IJ x_J and A_IJ^T x_I). work: jm_symv_work_doubles(N) doubles of
+ * device scratch (partial sums, summed in a fixed order: deterministic). The
+ * reduction's symv (dlatrd, dsymv), with 64-bit offsets. */
+int64_t jm_symv_work_doubles(int64_t N);
+int jm_symv_lower(int64_t N, const double* A, int64_t lda, const double* x, double* y,
+                  double* work, int64_t work_doubles, void* stream);
+
 /* Householder reduction step (dlatrd): *e = *a, then *a = 1. */
 int jm_tridiag_take_unit(double* a, double* e, void* stream);
 

@@ -6,10 +6,12 @@
 // here is 64-bit. The host side (jamma_amd/lmm/eigen.py) runs:
 //
 //  1. blocked Householder reduction to tridiagonal form (lower, LAPACK dsytrd /
-//     dlatrd order of operations): the per-column matrix-vector products go to
-//     rocBLAS' 64-bit level-2 routines, the rank-2k update of the trailing
-//     matrix to k_dgemm, and the end of each column step (scale by tau,
-//     alpha = -tau/2 w.v, w += alpha v) to k_latrd_finish;
+//     dlatrd order of operations): the symmetric matrix-vector product of each
+//     column step to k_symv_lower / k_symv_reduce (the lower triangle read
+//     once), the panel's small matrix-vector products to rocBLAS' 64-bit gemv,
+//     the rank-2k update of the trailing matrix to k_dgemm, and the end of each
+//     column step (scale by tau, alpha = -tau/2 w.v, w += alpha v) to
+//     k_latrd_finish;
 //  2. Cuppen divide and conquer on the tridiagonal matrix: tear it into halves
 //     (recursively) until a half fits the library solver, then merge each pair
 //     through the rank-one update D + rho z z^T (Bunch-Nielsen-Sorensen):
@@ -153,6 +155,106 @@ __global__ void __launch_bounds__(1024) k_latrd_finish(int64_t len, const double
   for (int q = 0; q < nw; ++q) tot += part[q];
   const double alpha = -0.5 * t * tot;
   for (int64_t q = threadIdx.x; q < len; q += blockDim.x) w[q] += alpha * v[q];
+}
+
+// y = A x for a symmetric A stored in its lower triangle (column-major, lda),
+// reading every stored element once. 64 x 64 tiles; block (I, c) takes tiles
+// J in [c chunk, min((c + 1) chunk, I + 1)) of tile row I, wave g the tile
+// columns 16 g .. 16 g + 15, lane r the tile row r (so each load instruction
+// reads 64 consecutive doubles of a column):
+//   off-diagonal tile (J < I): y_I += A_IJ x_J, summed over the block's tiles
+//     and written to Y1[I][c]; A_IJ^T x_I (16 columns per wave, reduced across
+//     the lanes by halving exchanges) written to P2[I][64 J + ...];
+//   diagonal tile: the symmetric tile from its lower half, into y_I only.
+// k_symv_reduce: y_i = sum_c Y1[R][c][r] + sum_{I > R} P2[I][i] (i = 64 R + r),
+// in a fixed order (deterministic).
+constexpr int SV_T = 64;
+
+__device__ __forceinline__ double reduce16_lanes(double (&v)[16], int lane) {
+  // after the exchanges lane L holds the 64-lane sum of v[(L >> 2) & 15]
+#pragma unroll
+  for (int h = 8, m = 32; h >= 1; h >>= 1, m >>= 1) {
+    const bool up = (lane & m) != 0;
+#pragma unroll
+    for (int k = 0; k < h; ++k) {
+      const double keep = up ? v[k + h] : v[k];
+      const double send = up ? v[k] : v[k + h];
+      v[k] = keep + shfl_xor_d(send, m);
+    }
+  }
+  double t = v[0];
+  t += shfl_xor_d(t, 2);
+  t += shfl_xor_d(t, 1);
+  return t;
+}
+
+__global__ void __launch_bounds__(256) k_symv_lower(int64_t N, const double* __restrict__ A,
+                                                    int64_t lda, const double* __restrict__ x,
+                                                    int chunk, int nchmax,
+                                                    double* __restrict__ P2,
+                                                    double* __restrict__ Y1) {
+  __shared__ double red[4][64];
+  const int64_t I = blockIdx.x;
+  const int c = blockIdx.y;
+  const int64_t J0 = (int64_t)c * chunk;
+  if (J0 > I) return;
+  const int64_t J1 = (J0 + chunk < I + 1) ? J0 + chunk : I + 1;
+  const int lane = threadIdx.x & 63;
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int64_t row = I * SV_T + lane;
+  const bool rok = row < N;
+  const double xi = rok ? x[row] : 0.0;
+  // wave-uniform column pointers + the lane's row offset: one SGPR base per load
+  const double* __restrict__ Ablk = A + I * SV_T;
+  double yacc = 0.0;
+  const int64_t Joff = J1 < I ? J1 : I;  // off-diagonal tiles of this block: [J0, Joff)
+  for (int64_t J = J0; J < Joff; ++J) {
+    const int64_t col0 = J * SV_T + 16 * g;  // < N: tile J < I lies left of row tile I
+    double a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const char* colp = (const char*)(Ablk + (col0 + k) * lda);  // wave-uniform
+      a[k] = rok ? *(const double*)(colp + (uint32_t)(lane * 8)) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      yacc += a[k] * x[col0 + k];
+      a[k] *= xi;
+    }
+    const double t = reduce16_lanes(a, lane);
+    if ((lane & 3) == 0) P2[I * N + col0 + ((lane >> 2) & 15)] = t;
+  }
+  if (J1 == I + 1) {  // the diagonal tile, symmetric from its lower half
+    const int64_t col0 = I * SV_T + 16 * g;
+#pragma unroll 4
+    for (int k = 0; k < 16; ++k) {
+      const int64_t col = col0 + k;
+      if (col < N && rok) {
+        const double v = (lane >= 16 * g + k) ? A[row + col * lda] : A[col + row * lda];
+        yacc += v * x[col];
+      }
+    }
+  }
+  red[g][lane] = yacc;
+  __syncthreads();
+  if (g == 0)
+    Y1[(I * nchmax + c) * SV_T + lane] = (red[0][lane] + red[1][lane]) +
+                                         (red[2][lane] + red[3][lane]);
+}
+
+__global__ void __launch_bounds__(256) k_symv_reduce(int64_t N, int64_t nt, int chunk, int nchmax,
+                                                     const double* __restrict__ P2,
+                                                     const double* __restrict__ Y1,
+                                                     double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t R = i / SV_T;
+  const int r = (int)(i % SV_T);
+  const int nch = (int)(R / chunk) + 1;
+  double s = 0.0;
+  for (int c = 0; c < nch; ++c) s += Y1[(R * nchmax + c) * SV_T + r];
+  for (int64_t I = R + 1; I < nt; ++I) s += P2[I * N + i];
+  y[i] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -345,6 +447,43 @@ extern "C" int jm_dgemm(int trans_a, int trans_b, int64_t m, int64_t n, int64_t 
   else
     hipLaunchKernelGGL((k_dgemm<true, true>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
                        beta, C, ldc, lo, nti);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+static void symv_shape(int64_t N, int64_t& nt, int& chunk, int& nchmax) {
+  nt = (N + SV_T - 1) / SV_T;
+  const int64_t tiles = nt * (nt + 1) / 2;
+  int64_t ch = tiles / 2048;  // aim at a few thousand blocks
+  ch = ch < 4 ? 4 : (ch > 64 ? 64 : ch);
+  chunk = (int)ch;
+  nchmax = (int)((nt + chunk - 1) / chunk);
+}
+
+extern "C" int64_t jm_symv_work_doubles(int64_t N) {
+  if (N < 1) return 0;
+  const int64_t nt = (N + SV_T - 1) / SV_T;
+  return nt * N + nt * (nt / 4 + 1) * SV_T;  // P2 + Y1 for any chunk >= 4
+}
+
+extern "C" int jm_symv_lower(int64_t N, const double* A, int64_t lda, const double* x, double* y,
+                             double* work, int64_t work_doubles, void* stream) {
+  JM_CHECK_ARG(N >= 0 && lda >= N, "bad sizes");
+  if (N == 0) return JM_OK;
+  JM_CHECK_ARG(A && x && y && work, "null pointer");
+  JM_CHECK_ARG(work_doubles >= jm_symv_work_doubles(N), "workspace too small");
+  int64_t nt;
+  int chunk, nchmax;
+  symv_shape(N, nt, chunk, nchmax);
+  JM_CHECK_ARG(nt < 65536 * 32768LL, "N too large");
+  double* P2 = work;
+  double* Y1 = work + nt * N;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_symv_lower, dim3((unsigned)nt, (unsigned)nchmax), dim3(256), 0, st, N, A,
+                     lda, x, chunk, nchmax, P2, Y1);
+  JM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_symv_reduce, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, nt,
+                     chunk, nchmax, P2, Y1, y);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

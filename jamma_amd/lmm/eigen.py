@@ -17,8 +17,9 @@ n <= 46,340 (n * n <= 2^31 - 1). Larger cohorts (the reference reaches them with
 ILP64 LAPACK, src/jamma/lmm/eigen.py:9,39) go through _eigh_large, which keeps
 every offset 64-bit:
   1. blocked Householder tridiagonalisation (LAPACK dsytrd/dlatrd, lower) with
-     rocBLAS' 64-bit symv/gemv, rocSOLVER's 64-bit dlarfg, and this library's
-     FP64 MFMA GEMM (jm_dgemm) for the rank-2k trailing updates;
+     this library's lower-triangle symv (jm_symv_lower, each stored element read
+     once per column step), rocBLAS' 64-bit gemv, rocSOLVER's 64-bit dlarfg, and
+     this library's FP64 MFMA GEMM (jm_dgemm) for the rank-2k trailing updates;
   2. Cuppen divide and conquer on the tridiagonal matrix: halves are torn off
      until each fits rocSOLVER's dstedc, and pairs are merged through the
      rank-one update D + rho z z^T (deflation as LAPACK dlaed2; secular roots,
@@ -72,10 +73,9 @@ def _solver(device: torch.device):
         f.restype = ci
     i64 = ctypes.c_int64
     rb.rocblas_set_pointer_mode.argtypes = [vp, ci]
-    rb.rocblas_dsymv_64.argtypes = [vp, ci, i64, vp, vp, i64, vp, i64, vp, vp, i64]
     rb.rocblas_dgemv_64.argtypes = [vp, ci, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64]
     rs.rocsolver_dlarfg_64.argtypes = [vp, i64, vp, vp, i64, vp]
-    for f in (rb.rocblas_set_pointer_mode, rb.rocblas_dsymv_64, rb.rocblas_dgemv_64,
+    for f in (rb.rocblas_set_pointer_mode, rb.rocblas_dgemv_64,
               rs.rocsolver_dlarfg_64):
         f.restype = ci
     h = ctypes.c_void_p()
@@ -151,6 +151,8 @@ def _tridiagonalize(A: torch.Tensor, nb: int, progress=None):
     e = torch.zeros(max(n - 1, 1), dtype=f64, device=dev)
     tau = torch.zeros(max(n - 1, 1), dtype=f64, device=dev)
     Wt = torch.zeros((nb, n), dtype=f64, device=dev)  # column-major W (n x nb, ld n)
+    nwork = int(_lib.lib.jm_symv_work_doubles(n))
+    symv_work = torch.empty(max(nwork, 1), dtype=f64, device=dev)
     consts = torch.tensor([1.0, 0.0, -1.0], dtype=f64, device=dev)
     one, zero, mone = (consts.data_ptr() + 8 * k for k in range(3))
     base, wb, ld = A.data_ptr(), Wt.data_ptr(), n
@@ -187,9 +189,8 @@ def _tridiagonalize(A: torch.Tensor, nb: int, progress=None):
                     h, L, v, pa(p0 + min(i + 2, Np - 1), gc), 1, tau.data_ptr() + 8 * gc))
                 _lib.call("jm_tridiag_take_unit", v, e.data_ptr() + 8 * gc, st)
                 w = pw(i + 1, i)
-                _ok("rocblas_dsymv_64", rb.rocblas_dsymv_64(h, _RB_FILL_LOWER, L, one,
-                                                            pa(gc + 1, gc + 1), ld, v, 1, zero,
-                                                            w, 1))
+                _lib.call("jm_symv_lower", L, pa(gc + 1, gc + 1), ld, v, w,
+                          symv_work.data_ptr(), nwork, st)
                 if i > 0:
                     gemv(_RB_OP_T, L, i, one, pw(i + 1, 0), ld, v, 1, zero, pw(0, i))
                     gemv(_RB_OP_N, L, i, mone, pa(gc + 1, p0), ld, pw(0, i), 1, one, w)

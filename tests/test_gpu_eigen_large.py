@@ -151,3 +151,26 @@ def test_eigh_device_routes_large_cohorts(monkeypatch):
     assert int(np.sum(d == 0.0)) == int(np.sum(np.abs(ref) < 1e-10)) >= 260 - 120
     Un = U.cpu().numpy()
     assert np.max(np.abs(K @ Un - Un * d[None, :])) <= 1e-11 * np.max(d)
+
+
+@pytest.mark.parametrize("N,lda", [(1, 1), (63, 70), (64, 64), (65, 100), (1000, 1003), (4097, 4100)])
+def test_symv_lower_reads_the_lower_triangle(N, lda):
+    """y = A x from the lower triangle only: the strictly upper part holds garbage."""
+    from jamma_amd import _lib
+    from jamma_amd.device import stream_ptr
+
+    rng = np.random.default_rng(N)
+    S = rng.normal(size=(N, N))
+    S = S + S.T
+    stored = np.full((lda, N), np.nan)  # column-major N x N with leading dimension lda
+    stored[:N, :] = np.where(np.tril(np.ones((N, N), bool)), S, 1e300)
+    x = rng.normal(size=N)
+    Ad = torch.as_tensor(np.ascontiguousarray(stored.T), device="cuda")
+    xd = torch.as_tensor(x, device="cuda")
+    yd = torch.full((N,), np.nan, dtype=torch.float64, device="cuda")
+    nw = int(_lib.lib.jm_symv_work_doubles(N))
+    wk = torch.full((nw,), np.nan, dtype=torch.float64, device="cuda")
+    _lib.call("jm_symv_lower", N, Ad.data_ptr(), lda, xd.data_ptr(), yd.data_ptr(), wk.data_ptr(),
+              nw, stream_ptr())
+    ref = S @ x
+    assert np.max(np.abs(yd.cpu().numpy() - ref)) <= 1e-12 * np.max(np.abs(ref)) * np.sqrt(N)
