@@ -86,10 +86,12 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving session that adds request p50/p99 to the line")
     ap.add_argument("--workload", default="precision",
-                    choices=["precision", "scale", "serving", "copy"],
+                    choices=["precision", "scale", "loco", "serving", "copy"],
                     help="precision: the headline config; scale: n=50,000 x 200,000 SNPs, 5 "
                          "subpopulations (Fst 0.02), 0.5%% missing, Wald test in 10,000-SNP "
-                         "tiles; serving: resident cohorts + Poisson request stream (p50/p99)")
+                         "tiles; loco: n=20,000 x 300,000 SNPs on 19 chromosomes, 8 "
+                         "correlated phenotypes, leave-one-chromosome-out kinships, -lmm 4; "
+                         "serving: resident cohorts + Poisson request stream (p50/p99)")
     ap.add_argument("--pops", type=int, default=3, help="Balding-Nichols subpopulations")
     ap.add_argument("--fst", type=float, default=0.05)
     ap.add_argument("--miss", type=float, default=0.01, help="missing-call rate")
@@ -108,6 +110,8 @@ def main() -> int:
         return serving(args)
     if args.workload == "copy":
         return copy_calibration()
+    if args.workload == "loco":
+        return loco(args)
     if args.workload == "scale":
         args.n, args.p, args.pops, args.fst, args.miss = 50_000, 200_000, 5, 0.02, 0.005
         args.tile, args.kin_tile, args.mode = 10_000, 10_000, 1
@@ -361,6 +365,140 @@ def hbm_traffic(kernel: str, n: int, tile: int):
                 int(f[1]) == n and int(f[2]) == tile:
             return float(f[3]) + float(f[4])
     return None
+
+
+def loco(args) -> int:
+    """Leave-one-chromosome-out, several phenotypes: n = 20,000 samples x 300,000 SNPs on
+    19 chromosomes (length of chromosome c proportional to c^-0.5), 3 Balding-Nichols
+    subpopulations (Fst 0.05, 1% missing), 8 phenotypes (h2 = 0.5, 200 causal SNPs,
+    genetic correlation 0.3), intercept + 3 Gaussian covariates, -lmm 4.
+
+    Mirrors run_lmm_loco_multi on device-resident genotypes: per-chromosome int8 partial
+    Grams, K_c = (S_full - S_c) / (p - p_c), one eigendecomposition per chromosome shared by
+    the 8 phenotypes, one rotation per SNP tile shared by the 8 phenotypes. The timed job
+    (mixed precision) is kinship + 19 eigendecompositions + association; every SNP x
+    phenotype is then recomputed on the FP64 device path (untimed) for parity."""
+    import numpy as np
+    import torch
+
+    from jamma_amd.device import require_gpu, round_up, to_dev
+    from jamma_amd.io.plink import stats_from_counts
+    from jamma_amd.kinship.compute import KinshipAccumulator
+    from jamma_amd.lmm.eigen import eigh_device
+    from jamma_amd.lmm.engine import AssocProblem, Cohort
+
+    dev = require_gpu()
+    n, p, n_chr, n_ph, tile = 20_000, 300_000, 19, 8, args.tile
+    ld = round_up(n, 64)
+    w = np.arange(1, n_chr + 1) ** -0.5
+    sizes = np.floor(p * w / w.sum()).astype(int)
+    sizes[0] += p - sizes.sum()
+    chrom = np.repeat(np.arange(n_chr), sizes)
+    g = gen_cohort(n, p, 3, 0.05, 0.01, args.seed, ld)
+    counts = counts_of(g, n)
+    mean32, miss, poly = stats_from_counts(counts, np.float32)
+    maf = np.minimum(mean32 / 2, 1 - mean32 / 2)
+    keep = (maf >= 0.01) & (miss / n <= 0.05) & poly
+    mu64, _, _ = stats_from_counts(counts, np.float64)
+    log(f"loco: generated {n} x {p} genotypes, {int(keep.sum())} pass the filters, "
+        f"chromosome sizes {sizes.tolist()}")
+    rng = np.random.default_rng(args.seed)
+    idx_all = np.where(keep)[0]
+    causal = np.sort(rng.choice(idx_all, 200, replace=False))
+    gc = g[torch.as_tensor(causal, device=dev)][:, :n].to(torch.float64)
+    mc = to_dev(mean32[causal].astype(np.float64), torch.float64)
+    gc = torch.where(gc < 0, mc[:, None], gc).cpu().numpy()
+    b0 = rng.normal(size=200)
+    G = np.empty((n, n_ph))
+    for k in range(n_ph):  # genetic correlation 0.3 between every pair of traits
+        gv = (np.sqrt(0.3) * b0 + np.sqrt(0.7) * rng.normal(size=200)) @ gc
+        G[:, k] = (gv - gv.mean()) / gv.std()
+    W = np.column_stack([np.ones(n), rng.normal(size=(n, 3))])
+    Y = np.sqrt(0.5) * G + np.sqrt(0.5) * rng.normal(size=(n, n_ph)) + \
+        (W[:, 1:] @ rng.normal(0, 0.3, (3, n_ph)))
+    means = to_dev(mean32.astype(np.float64), torch.float64)
+
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    # per-chromosome int8 partial Grams (one staging budget shared by the 19)
+    accs = [KinshipAccumulator(n, stage_bytes=(1 << 31) // n_chr) for _ in range(n_chr)]
+    for c in range(n_chr):
+        sel = idx_all[chrom[idx_all] == c]
+        for s in range(0, len(sel), 16384):
+            ss = torch.as_tensor(sel[s:s + 16384], device=dev)
+            accs[c].add_tile(g[ss].contiguous(), ld, len(ss), mu64[sel[s:s + 16384]], True)
+    full = KinshipAccumulator(n)
+    for a in accs:
+        full.S += a.S
+        full.u += a.u
+        full.C = a.C.clone() if full.C is None else full.C + a.C
+        full.s0 += a.s0
+        full.p += a.p
+    torch.cuda.synchronize()
+    t_kin = time.perf_counter() - t_all
+    log(f"loco: partial Grams of {n_chr} chromosomes + total in {t_kin:.2f}s")
+    t_eig = t_rot = t_assoc = 0.0
+    out = {"mixed": [], "f64": []}
+    for c in range(n_chr):
+        t0 = time.perf_counter()
+        K = full.finalize(full.p - accs[c].p, sub=accs[c])
+        accs[c] = None  # this chromosome's partial Gram is no longer needed
+        d, U = eigh_device(K)
+        del K
+        torch.cuda.synchronize()
+        t_eig += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cohort = Cohort(d, U, precision="both")
+        del U
+        probs = [AssocProblem(cohort, Y[:, k], W) for k in range(n_ph)]
+        sel = idx_all[chrom[idx_all] == c]
+        res = {"mixed": [], "f64": []}
+        for prec in ("mixed", "f64"):
+            if prec == "f64":
+                torch.cuda.synchronize()
+                t_assoc += time.perf_counter() - t0
+            for s in range(0, len(sel), tile):
+                ss = torch.as_tensor(sel[s:s + tile], device=dev)
+                gt = g[ss].contiguous()
+                Z, f32, ldz = cohort.rotate_genotypes(gt, ld, means[ss], len(ss), prec)
+                res[prec].append(torch.stack(
+                    [pr.assoc_moments(cohort.moments(Z, f32, ldz, len(ss), pr.V, pr.n_cvt + 1),
+                                      len(ss), 4)[: len(ss)] for pr in probs], 1).cpu())
+                del Z, gt
+        for prec in res:
+            out[prec].append(torch.cat(res[prec]))
+        log(f"loco: chromosome {c + 1}: {len(sel)} SNPs x {n_ph} phenotypes, T={cohort.n_terms}, "
+            f"{cohort.bf16_planes} bf16 planes")
+        del cohort, probs
+    total = time.perf_counter() - t_all
+    a = torch.cat(out["mixed"]).numpy()  # (SNPs, phenotypes, outputs)
+    b = torch.cat(out["f64"]).numpy()
+    tests = a.shape[0] * n_ph
+    parity = {}
+    for name, col in (("p_wald", 2), ("p_lrt", 7), ("p_score", 8)):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            dl = np.abs(np.log10(a[:, :, col]) - np.log10(b[:, :, col]))
+        flips = sum(int(np.sum((a[:, :, col] < t) != (b[:, :, col] < t)))
+                    for t in (0.05, 0.01, 1e-3, 5e-8))
+        parity[name] = {"max_abs_dlog10p": float(np.nanmax(dl)), "significance_flips": flips}
+    parity["beta_sign_agreement"] = bool(np.array_equal(np.sign(a[:, :, 0]), np.sign(b[:, :, 0])))
+    parity["n_sig_5e-8_wald"] = int(np.sum(b[:, :, 2] < 5e-8))
+    timed = t_kin + t_eig + t_assoc
+    line = {
+        "metric": "LOCO SNP x phenotype tests/s (kinships + eigendecompositions + association)",
+        "value": tests / timed, "unit": "tests/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "int8 kinship, f64 eigen, bf16x3 rotation + f64 lambda/tests",
+        "data": "synthetic (Balding-Nichols genotypes generated on device)",
+        "config": {"workload": f"loco: n={n} x {p} SNPs ({a.shape[0]} pass filters) on {n_chr} "
+                               f"chromosomes (length ~ c^-0.5), {n_ph} phenotypes (genetic corr "
+                               f"0.3, h2=0.5), intercept + 3 covariates, -lmm 4",
+                   "n": n, "p": p, "chromosomes": n_chr, "phenotypes": n_ph, "tile_snps": tile},
+        "seconds": {"kinship": t_kin, "eigendecompositions": t_eig, "association_mixed": t_assoc,
+                    "timed_total": timed, "wall_including_f64_parity": total},
+        "parity_mixed_vs_f64": parity,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 def copy_calibration() -> int:

@@ -340,6 +340,15 @@ int64_t jm_symv_work_doubles(int64_t N);
 int jm_symv_lower(int64_t N, const double* A, int64_t lda, const double* x, double* y,
                   double* work, int64_t work_doubles, void* stream);
 
+/* jm_dgemm with K split over `splits` block rows (1..64) whose partial products
+ * go to work (splits * m * n doubles, column-major m x n each) and are then summed
+ * in a fixed order: C = alpha sum + beta C. For outputs too small to fill the
+ * device with one block per tile (V^T Q in the back-transformation). */
+int jm_dgemm_splitk(int trans_a, int trans_b, int64_t m, int64_t n, int64_t k, double alpha,
+                    const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                    double* C, int64_t ldc, int splits, double* work, int64_t work_doubles,
+                    void* stream);
+
 /* Householder reduction step (dlatrd): *e = *a, then *a = 1. */
 int jm_tridiag_take_unit(double* a, double* e, void* stream);
 

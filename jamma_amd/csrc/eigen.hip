@@ -43,11 +43,16 @@ __global__ void __launch_bounds__(256) k_dgemm(int64_t m, int64_t n, int64_t k, 
                                                const double* __restrict__ A, int64_t lda,
                                                const double* __restrict__ B, int64_t ldb,
                                                double beta, double* __restrict__ C, int64_t ldc,
-                                               int lower_only, int64_t nti) {
+                                               int lower_only, int64_t nti, int64_t kchunk,
+                                               int64_t cstride) {
   __shared__ double As[DG_TI * DG_LS];  // op(A)(i, kk) at [i][kk]
   __shared__ double Bs[DG_TJ * DG_LS];  // op(B)(kk, j) at [j][kk]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t bid = blockIdx.x;
+  // split K: block row blockIdx.y takes k in [kbeg, kend) and writes C + blockIdx.y * cstride
+  const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
+  const int64_t kend = (kbeg + kchunk < k) ? kbeg + kchunk : k;
+  C += (int64_t)blockIdx.y * cstride;
   const int64_t i0 = (bid % nti) * DG_TI, j0 = (bid / nti) * DG_TJ;
   if (lower_only && i0 + DG_TI <= j0) return;  // every element of the tile has i < j
   const int wi = (wave & 1) * 64, wj = (wave >> 1) * 32;
@@ -58,7 +63,7 @@ __global__ void __launch_bounds__(256) k_dgemm(int64_t m, int64_t n, int64_t k, 
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = (d4e_t){0.0, 0.0, 0.0, 0.0};
 
-  for (int64_t k0 = 0; k0 < k; k0 += DG_BK) {
+  for (int64_t k0 = kbeg; k0 < kend; k0 += DG_BK) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) {  // op(A) tile: 128 x 16
       const int idx = tid + 256 * u;
@@ -72,7 +77,7 @@ __global__ void __launch_bounds__(256) k_dgemm(int64_t m, int64_t n, int64_t k, 
       }
       const int64_t i = i0 + r, kk = k0 + c;
       double v = 0.0;
-      if (i < m && kk < k) v = TA ? A[kk + i * lda] : A[i + kk * lda];
+      if (i < m && kk < kend) v = TA ? A[kk + i * lda] : A[i + kk * lda];
       As[r * DG_LS + c] = v;
     }
 #pragma unroll
@@ -88,7 +93,7 @@ __global__ void __launch_bounds__(256) k_dgemm(int64_t m, int64_t n, int64_t k, 
       }
       const int64_t j = j0 + r, kk = k0 + c;
       double v = 0.0;
-      if (j < n && kk < k) v = TB ? B[j + kk * ldb] : B[kk + j * ldb];
+      if (j < n && kk < kend) v = TB ? B[j + kk * ldb] : B[kk + j * ldb];
       Bs[r * DG_LS + c] = v;
     }
     __syncthreads();
@@ -122,6 +127,21 @@ __global__ void __launch_bounds__(256) k_dgemm(int64_t m, int64_t n, int64_t k, 
           C[i + j * ldc] = v;
         }
       }
+}
+
+// C = alpha sum_s W_s + beta C, W_s = W + s m n (column-major m x n, ld m), s in a
+// fixed order.
+__global__ void __launch_bounds__(256) k_splitk_sum(int64_t m, int64_t n, int splits, double alpha,
+                                                    const double* __restrict__ W, double beta,
+                                                    double* __restrict__ C, int64_t ldc) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m * n) return;
+  const int64_t i = q % m, j = q / m;
+  double s = 0.0;
+  for (int t = 0; t < splits; ++t) s += W[t * m * n + q];
+  double v = alpha * s;
+  if (beta != 0.0) v += beta * C[i + j * ldc];
+  C[i + j * ldc] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -158,15 +178,17 @@ __global__ void __launch_bounds__(1024) k_latrd_finish(int64_t len, const double
 }
 
 // y = A x for a symmetric A stored in its lower triangle (column-major, lda),
-// reading every stored element once. 64 x 64 tiles; block (I, c) takes tiles
-// J in [c chunk, min((c + 1) chunk, I + 1)) of tile row I, wave g the tile
-// columns 16 g .. 16 g + 15, lane r the tile row r (so each load instruction
-// reads 64 consecutive doubles of a column):
-//   off-diagonal tile (J < I): y_I += A_IJ x_J, summed over the block's tiles
-//     and written to Y1[I][c]; A_IJ^T x_I (16 columns per wave, reduced across
-//     the lanes by halving exchanges) written to P2[I][64 J + ...];
-//   diagonal tile: the symmetric tile from its lower half, into y_I only.
-// k_symv_reduce: y_i = sum_c Y1[R][c][r] + sum_{I > R} P2[I][i] (i = 64 R + r),
+// reading every stored element once. 64 x 64 tiles; block (J, c) takes column
+// tile J and the row tiles I in [J + c chunk, min(J + (c + 1) chunk, nt)), so
+// each wave streams its 16 columns (16 g .. 16 g + 15 of the tile) downwards,
+// lane r taking row r of each tile (one load = 64 consecutive doubles):
+//   y_I += A_IJ x_J: summed over the 4 waves' columns in LDS, written to
+//     P[I][J] (64 values; the diagonal tile, symmetric from its lower half,
+//     goes there too);
+//   y_J += A_IJ^T x_I (I > J): 16 column sums per lane accumulated over the
+//     block's tiles, reduced across the lanes once at the end (halving
+//     exchanges) and written to Y1[J][c].
+// k_symv_reduce: y_i = sum_{J <= R} P[R][J][r] + sum_c Y1[R][c][r] (i = 64 R + r)
 // in a fixed order (deterministic).
 constexpr int SV_T = 64;
 
@@ -191,70 +213,87 @@ __device__ __forceinline__ double reduce16_lanes(double (&v)[16], int lane) {
 __global__ void __launch_bounds__(256) k_symv_lower(int64_t N, const double* __restrict__ A,
                                                     int64_t lda, const double* __restrict__ x,
                                                     int chunk, int nchmax,
-                                                    double* __restrict__ P2,
+                                                    double* __restrict__ P,
                                                     double* __restrict__ Y1) {
-  __shared__ double red[4][64];
-  const int64_t I = blockIdx.x;
+  __shared__ double red[2][4][64];
+  const int64_t nt = (N + SV_T - 1) / SV_T;
+  const int64_t J = blockIdx.x;
   const int c = blockIdx.y;
-  const int64_t J0 = (int64_t)c * chunk;
-  if (J0 > I) return;
-  const int64_t J1 = (J0 + chunk < I + 1) ? J0 + chunk : I + 1;
+  const int64_t I0 = J + (int64_t)c * chunk;
+  if (I0 >= nt) return;
+  const int64_t I1 = (I0 + chunk < nt) ? I0 + chunk : nt;
   const int lane = threadIdx.x & 63;
   const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const int64_t row = I * SV_T + lane;
-  const bool rok = row < N;
-  const double xi = rok ? x[row] : 0.0;
-  // wave-uniform column pointers + the lane's row offset: one SGPR base per load
-  const double* __restrict__ Ablk = A + I * SV_T;
-  double yacc = 0.0;
-  const int64_t Joff = J1 < I ? J1 : I;  // off-diagonal tiles of this block: [J0, Joff)
-  for (int64_t J = J0; J < Joff; ++J) {
-    const int64_t col0 = J * SV_T + 16 * g;  // < N: tile J < I lies left of row tile I
-    double a[16];
+  const int64_t col0 = J * SV_T + 16 * g;
+  double xj[16], t[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const char* colp = (const char*)(Ablk + (col0 + k) * lda);  // wave-uniform
-      a[k] = rok ? *(const double*)(colp + (uint32_t)(lane * 8)) : 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      yacc += a[k] * x[col0 + k];
-      a[k] *= xi;
-    }
-    const double t = reduce16_lanes(a, lane);
-    if ((lane & 3) == 0) P2[I * N + col0 + ((lane >> 2) & 15)] = t;
+  for (int k = 0; k < 16; ++k) {
+    xj[k] = (col0 + k < N) ? x[col0 + k] : 0.0;
+    t[k] = 0.0;
   }
-  if (J1 == I + 1) {  // the diagonal tile, symmetric from its lower half
-    const int64_t col0 = I * SV_T + 16 * g;
+  int par = 0;
+  for (int64_t I = I0; I < I1; ++I) {
+    const int64_t row = I * SV_T + lane;
+    const bool rok = row < N;
+    double yr = 0.0;
+    if (I > J) {  // every column of the tile is < N
+      const double xi = rok ? x[row] : 0.0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const double a = rok ? A[row + (col0 + k) * lda] : 0.0;
+        yr += a * xj[k];
+        t[k] += a * xi;
+      }
+    } else {  // diagonal tile
 #pragma unroll 4
-    for (int k = 0; k < 16; ++k) {
-      const int64_t col = col0 + k;
-      if (col < N && rok) {
-        const double v = (lane >= 16 * g + k) ? A[row + col * lda] : A[col + row * lda];
-        yacc += v * x[col];
+      for (int k = 0; k < 16; ++k) {
+        const int64_t col = col0 + k;
+        if (rok && col < N)
+          yr += ((lane >= 16 * g + k) ? A[row + col * lda] : A[col + row * lda]) * xj[k];
       }
     }
+    red[par][g][lane] = yr;
+    __syncthreads();
+    if (g == 0)
+      P[(I * nt + J) * SV_T + lane] = (red[par][0][lane] + red[par][1][lane]) +
+                                      (red[par][2][lane] + red[par][3][lane]);
+    par ^= 1;
   }
-  red[g][lane] = yacc;
-  __syncthreads();
-  if (g == 0)
-    Y1[(I * nchmax + c) * SV_T + lane] = (red[0][lane] + red[1][lane]) +
-                                         (red[2][lane] + red[3][lane]);
+  const double tot = reduce16_lanes(t, lane);
+  if ((lane & 3) == 0) Y1[(J * nchmax + c) * SV_T + 16 * g + ((lane >> 2) & 15)] = tot;
 }
 
-__global__ void __launch_bounds__(256) k_symv_reduce(int64_t N, int64_t nt, int chunk, int nchmax,
-                                                     const double* __restrict__ P2,
-                                                     const double* __restrict__ Y1,
-                                                     double* __restrict__ y) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const int64_t R = i / SV_T;
-  const int r = (int)(i % SV_T);
-  const int nch = (int)(R / chunk) + 1;
+// One block per row tile R (64 rows, one per lane); wave w sums P[R][J] for
+// J = w, w + 16, ... <= R; the 16 wave sums and the Y1 chunks of column tile R
+// are then added in a fixed order.
+__global__ void __launch_bounds__(1024) k_symv_reduce(int64_t N, int64_t nt, int chunk,
+                                                      int nchmax,
+                                                      const double* __restrict__ P,
+                                                      const double* __restrict__ Y1,
+                                                      double* __restrict__ y) {
+  __shared__ double part[16][64];
+  const int64_t R = blockIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const double* pr = P + R * nt * SV_T + lane;
   double s = 0.0;
-  for (int c = 0; c < nch; ++c) s += Y1[(R * nchmax + c) * SV_T + r];
-  for (int64_t I = R + 1; I < nt; ++I) s += P2[I * N + i];
-  y[i] = s;
+  int64_t J = w;
+  for (; J + 48 <= R; J += 64) {  // four independent loads in flight
+    const double a0 = pr[J * SV_T], a1 = pr[(J + 16) * SV_T];
+    const double a2 = pr[(J + 32) * SV_T], a3 = pr[(J + 48) * SV_T];
+    s += (a0 + a1) + (a2 + a3);
+  }
+  for (; J <= R; J += 16) s += pr[J * SV_T];
+  part[w][lane] = s;
+  __syncthreads();
+  const int64_t i = R * SV_T + lane;
+  if (w == 0 && i < N) {
+    double tt = 0.0;
+    const int nch = (int)((nt - R + chunk - 1) / chunk);
+    for (int c = 0; c < nch; ++c) tt += Y1[(R * nchmax + c) * SV_T + lane];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tt += part[q][lane];
+    y[i] = tt;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -421,6 +460,29 @@ __global__ void __launch_bounds__(256) k_apply_rotations(int64_t r, const int64_
 
 using namespace jm;
 
+static int launch_dgemm(int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
+                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                        double* C, int64_t ldc, int lo, int splits, int64_t kchunk,
+                        int64_t cstride, hipStream_t st) {
+  const int64_t nti = (m + DG_TI - 1) / DG_TI, ntj = (n + DG_TJ - 1) / DG_TJ;
+  JM_CHECK_ARG(nti * ntj < (int64_t)1 << 31, "grid too large");
+  const dim3 grid((unsigned)(nti * ntj), (unsigned)splits), block(256);
+  if (!ta && !tb)
+    hipLaunchKernelGGL((k_dgemm<false, false>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc, lo, nti, kchunk, cstride);
+  else if (!ta && tb)
+    hipLaunchKernelGGL((k_dgemm<false, true>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc, lo, nti, kchunk, cstride);
+  else if (ta && !tb)
+    hipLaunchKernelGGL((k_dgemm<true, false>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc, lo, nti, kchunk, cstride);
+  else
+    hipLaunchKernelGGL((k_dgemm<true, true>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc, lo, nti, kchunk, cstride);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
 extern "C" int jm_dgemm(int trans_a, int trans_b, int64_t m, int64_t n, int64_t k, double alpha,
                         const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                         double* C, int64_t ldc, int lower_only, void* stream) {
@@ -430,23 +492,29 @@ extern "C" int jm_dgemm(int trans_a, int trans_b, int64_t m, int64_t n, int64_t 
   JM_CHECK_ARG(lda >= (trans_a ? k : m) && lda >= 1, "lda too small");
   JM_CHECK_ARG(ldb >= (trans_b ? n : k) && ldb >= 1, "ldb too small");
   JM_CHECK_ARG(ldc >= m, "ldc < m");
-  const int64_t nti = (m + DG_TI - 1) / DG_TI, ntj = (n + DG_TJ - 1) / DG_TJ;
-  JM_CHECK_ARG(nti * ntj < (int64_t)1 << 31, "grid too large");
-  const dim3 grid((unsigned)(nti * ntj)), block(256);
+  return launch_dgemm(trans_a, trans_b, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                      lower_only ? 1 : 0, 1, k, 0, (hipStream_t)stream);
+}
+
+extern "C" int jm_dgemm_splitk(int trans_a, int trans_b, int64_t m, int64_t n, int64_t k,
+                               double alpha, const double* A, int64_t lda, const double* B,
+                               int64_t ldb, double beta, double* C, int64_t ldc, int splits,
+                               double* work, int64_t work_doubles, void* stream) {
+  JM_CHECK_ARG(m >= 0 && n >= 0 && k >= 0 && splits >= 1 && splits <= 64, "bad sizes");
+  if (m == 0 || n == 0) return JM_OK;
+  JM_CHECK_ARG(C && work && (k == 0 || (A && B)), "null pointer");
+  JM_CHECK_ARG(lda >= (trans_a ? k : m) && lda >= 1, "lda too small");
+  JM_CHECK_ARG(ldb >= (trans_b ? n : k) && ldb >= 1, "ldb too small");
+  JM_CHECK_ARG(ldc >= m, "ldc < m");
+  JM_CHECK_ARG(work_doubles >= (int64_t)splits * m * n, "workspace too small");
+  // K chunks in whole LDS steps; splits past the end of K write zeros
+  const int64_t kchunk = ((k + splits - 1) / splits + DG_BK - 1) / DG_BK * DG_BK;
   hipStream_t st = (hipStream_t)stream;
-  const int lo = lower_only ? 1 : 0;
-  if (!trans_a && !trans_b)
-    hipLaunchKernelGGL((k_dgemm<false, false>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, lo, nti);
-  else if (!trans_a && trans_b)
-    hipLaunchKernelGGL((k_dgemm<false, true>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, lo, nti);
-  else if (trans_a && !trans_b)
-    hipLaunchKernelGGL((k_dgemm<true, false>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, lo, nti);
-  else
-    hipLaunchKernelGGL((k_dgemm<true, true>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, lo, nti);
+  const int rc = launch_dgemm(trans_a, trans_b, m, n, k, 1.0, A, lda, B, ldb, 0.0, work, m, 0,
+                              splits, kchunk > 0 ? kchunk : DG_BK, m * n, st);
+  if (rc != JM_OK) return rc;
+  hipLaunchKernelGGL(k_splitk_sum, dim3((unsigned)((m * n + 255) / 256)), dim3(256), 0, st, m, n,
+                     splits, alpha, work, beta, C, ldc);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }
@@ -463,7 +531,7 @@ static void symv_shape(int64_t N, int64_t& nt, int& chunk, int& nchmax) {
 extern "C" int64_t jm_symv_work_doubles(int64_t N) {
   if (N < 1) return 0;
   const int64_t nt = (N + SV_T - 1) / SV_T;
-  return nt * N + nt * (nt / 4 + 1) * SV_T;  // P2 + Y1 for any chunk >= 4
+  return nt * nt * SV_T + nt * (nt / 4 + 1) * SV_T;  // P + Y1 for any chunk >= 4
 }
 
 extern "C" int jm_symv_lower(int64_t N, const double* A, int64_t lda, const double* x, double* y,
@@ -477,13 +545,13 @@ extern "C" int jm_symv_lower(int64_t N, const double* A, int64_t lda, const doub
   symv_shape(N, nt, chunk, nchmax);
   JM_CHECK_ARG(nt < 65536 * 32768LL, "N too large");
   double* P2 = work;
-  double* Y1 = work + nt * N;
+  double* Y1 = work + nt * nt * SV_T;
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_symv_lower, dim3((unsigned)nt, (unsigned)nchmax), dim3(256), 0, st, N, A,
                      lda, x, chunk, nchmax, P2, Y1);
   JM_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_symv_reduce, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, nt,
-                     chunk, nchmax, P2, Y1, y);
+  hipLaunchKernelGGL(k_symv_reduce, dim3((unsigned)nt), dim3(1024), 0, st, N, nt, chunk, nchmax,
+                     P2, Y1, y);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

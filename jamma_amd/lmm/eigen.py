@@ -372,13 +372,19 @@ def _back_transform(A: torch.Tensor, tau: torch.Tensor, gram, Q: torch.Tensor, n
             blocks.append((p0, r, T))
     Y = torch.empty((nb, n), dtype=torch.float64, device=dev)   # column-major (r x n, ld r)
     Y2 = torch.empty((nb, n), dtype=torch.float64, device=dev)
+    max_split = 16
+    work = torch.empty(max_split * nb * n, dtype=torch.float64, device=dev)
+    tiles = -(-n // 64)  # output tiles of V^T Q (one tile row)
     for p0, r, T in reversed(blocks):
         L = n - 1 - p0
         Vt = A[p0:p0 + r, p0 + 1:].clone()  # column-major V (L x r, ld L)
         Vt[:, :r] = torch.triu(Vt[:, :r], 1) + torch.eye(r, dtype=torch.float64, device=dev)
         Td = torch.as_tensor(np.ascontiguousarray(T.T)).to(dev)  # column-major T
         QR = Q.data_ptr() + 8 * (p0 + 1)
-        _lib.call("jm_dgemm", 1, 0, r, n, L, 1.0, ptr(Vt), L, QR, n, 0.0, ptr(Y), r, 0, st)
+        # V^T Q_R has one row of output tiles and a long K: split K to fill the device
+        splits = int(min(max_split, max(1, -(-4096 // tiles)), max(1, L // 1024)))
+        _lib.call("jm_dgemm_splitk", 1, 0, r, n, L, 1.0, ptr(Vt), L, QR, n, 0.0, ptr(Y), r,
+                  splits, ptr(work), work.numel(), st)
         _lib.call("jm_dgemm", 0, 0, r, n, r, 1.0, ptr(Td), r, ptr(Y), r, 0.0, ptr(Y2), r, 0, st)
         _lib.call("jm_dgemm", 0, 0, L, n, r, -1.0, ptr(Vt), L, ptr(Y2), r, 1.0, QR, n, 0, st)
         del Vt, Td

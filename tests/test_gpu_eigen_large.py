@@ -174,3 +174,24 @@ def test_symv_lower_reads_the_lower_triangle(N, lda):
               nw, stream_ptr())
     ref = S @ x
     assert np.max(np.abs(yd.cpu().numpy() - ref)) <= 1e-12 * np.max(np.abs(ref)) * np.sqrt(N)
+
+
+@pytest.mark.parametrize("splits", [1, 3, 16])
+def test_dgemm_splitk_matches(splits):
+    from jamma_amd import _lib
+    from jamma_amd.device import stream_ptr
+
+    rng = np.random.default_rng(splits)
+    m, n, k = 64, 300, 2000
+    A = rng.normal(size=(k, m))  # op(A) = A^T
+    B = rng.normal(size=(k, n))
+    C0 = rng.normal(size=(m, n))
+    ref = 1.5 * (A.T @ B) - C0
+    Ad = torch.as_tensor(np.ascontiguousarray(A.T), device="cuda")
+    Bd = torch.as_tensor(np.ascontiguousarray(B.T), device="cuda")
+    Cd = torch.as_tensor(np.ascontiguousarray(C0.T), device="cuda")
+    wk = torch.full((splits * m * n,), np.nan, dtype=torch.float64, device="cuda")
+    _lib.call("jm_dgemm_splitk", 1, 0, m, n, k, 1.5, Ad.data_ptr(), k, Bd.data_ptr(), k, -1.0,
+              Cd.data_ptr(), m, splits, wk.data_ptr(), wk.numel(), stream_ptr())
+    got = Cd.cpu().numpy().T
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref)) * np.sqrt(k)
