@@ -24,10 +24,12 @@
 //     (compact WY, I - V T V^T) with k_dgemm.
 //
 // k_dgemm is an FP64 MFMA GEMM (v_mfma_f64_16x16x4f64) in column-major BLAS
-// semantics: C = alpha op(A) op(B) + beta C. 128x64 block tile (i x j), 4 waves
-// of 64x32, 16-deep K steps staged through LDS in [index][k] order whatever the
+// semantics: C = alpha op(A) op(B) + beta C. 64x64 block tile (i x j), 4 waves
+// of 32x32, 32-deep K steps staged through LDS in [index][k] order whatever the
 // operand's storage order, so all four transpose combinations share one inner
-// loop. The MFMA's B slot takes the i side, so the 16 lanes of an output group
+// loop; the next step's global loads are issued into registers before the
+// current step's MFMAs (one FP64 MFMA occupies a SIMD for 64 cycles, so the 32
+// per wave and step cover the load latency). The MFMA's B slot takes the i side, so the 16 lanes of an output group
 // store 16 consecutive rows of the column-major C.
 
 #include "jm_common.h"
@@ -36,7 +38,49 @@ namespace jm {
 
 typedef double d4e_t __attribute__((ext_vector_type(4)));
 
-constexpr int DG_TI = 128, DG_TJ = 64, DG_BK = 16, DG_LS = DG_BK + 1;
+constexpr int DG_TI = 64, DG_TJ = 64, DG_BK = 32, DG_LS = DG_BK + 1;
+
+// Global -> register loads of one K step: op(A) tile 64 x 32 and op(B) tile
+// 32 x 64, 8 doubles each per thread, with consecutive lanes on the operand's
+// contiguous dimension.
+template <bool TA, bool TB>
+__device__ __forceinline__ void dgemm_fetch(const double* __restrict__ A, int64_t lda,
+                                            const double* __restrict__ B, int64_t ldb,
+                                            int64_t m, int64_t n, int64_t i0, int64_t j0,
+                                            int64_t k0, int64_t kend, int tid, double (&ra)[8],
+                                            double (&rb)[8]) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int idx = tid + 256 * u;
+    const int r = TA ? (idx >> 5) : (idx & 63), c = TA ? (idx & 31) : (idx >> 6);
+    const int64_t i = i0 + r, kk = k0 + c;
+    ra[u] = (i < m && kk < kend) ? (TA ? A[kk + i * lda] : A[i + kk * lda]) : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int idx = tid + 256 * u;
+    const int r = TB ? (idx & 63) : (idx >> 5), c = TB ? (idx >> 6) : (idx & 31);
+    const int64_t j = j0 + r, kk = k0 + c;
+    rb[u] = (j < n && kk < kend) ? (TB ? B[j + kk * ldb] : B[kk + j * ldb]) : 0.0;
+  }
+}
+
+template <bool TA, bool TB>
+__device__ __forceinline__ void dgemm_stash(double* As, double* Bs, int tid, const double (&ra)[8],
+                                            const double (&rb)[8]) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int idx = tid + 256 * u;
+    const int r = TA ? (idx >> 5) : (idx & 63), c = TA ? (idx & 31) : (idx >> 6);
+    As[r * DG_LS + c] = ra[u];
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int idx = tid + 256 * u;
+    const int r = TB ? (idx & 63) : (idx >> 5), c = TB ? (idx >> 6) : (idx & 31);
+    Bs[r * DG_LS + c] = rb[u];
+  }
+}
 
 template <bool TA, bool TB>
 __global__ void __launch_bounds__(256) k_dgemm(int64_t m, int64_t n, int64_t k, double alpha,
@@ -55,66 +99,38 @@ __global__ void __launch_bounds__(256) k_dgemm(int64_t m, int64_t n, int64_t k, 
   C += (int64_t)blockIdx.y * cstride;
   const int64_t i0 = (bid % nti) * DG_TI, j0 = (bid / nti) * DG_TJ;
   if (lower_only && i0 + DG_TI <= j0) return;  // every element of the tile has i < j
-  const int wi = (wave & 1) * 64, wj = (wave >> 1) * 32;
+  const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
-  d4e_t acc[4][2];
+  d4e_t acc[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = (d4e_t){0.0, 0.0, 0.0, 0.0};
-
+  double ra[8], rb[8];
+  if (kbeg < kend) dgemm_fetch<TA, TB>(A, lda, B, ldb, m, n, i0, j0, kbeg, kend, tid, ra, rb);
   for (int64_t k0 = kbeg; k0 < kend; k0 += DG_BK) {
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {  // op(A) tile: 128 x 16
-      const int idx = tid + 256 * u;
-      int r, c;
-      if (!TA) {  // A(i, kk) at A[i + kk * lda]: consecutive lanes take consecutive i
-        r = idx & 127;
-        c = idx >> 7;
-      } else {    // op(A)(i, kk) = A(kk, i) at A[kk + i * lda]: consecutive lanes take kk
-        c = idx & 15;
-        r = idx >> 4;
-      }
-      const int64_t i = i0 + r, kk = k0 + c;
-      double v = 0.0;
-      if (i < m && kk < kend) v = TA ? A[kk + i * lda] : A[i + kk * lda];
-      As[r * DG_LS + c] = v;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {  // op(B) tile: 16 x 64
-      const int idx = tid + 256 * u;
-      int r, c;
-      if (!TB) {  // B(kk, j) at B[kk + j * ldb]
-        c = idx & 15;
-        r = idx >> 4;
-      } else {    // op(B)(kk, j) = B(j, kk) at B[j + kk * ldb]
-        r = idx & 63;
-        c = idx >> 6;
-      }
-      const int64_t j = j0 + r, kk = k0 + c;
-      double v = 0.0;
-      if (j < n && kk < kend) v = TB ? B[j + kk * ldb] : B[kk + j * ldb];
-      Bs[r * DG_LS + c] = v;
-    }
+    dgemm_stash<TA, TB>(As, Bs, tid, ra, rb);
     __syncthreads();
+    if (k0 + DG_BK < kend)  // the next step's loads fly while this step multiplies
+      dgemm_fetch<TA, TB>(A, lda, B, ldb, m, n, i0, j0, k0 + DG_BK, kend, tid, ra, rb);
 #pragma unroll
     for (int kk = 0; kk < DG_BK; kk += 4) {
-      double bi[4], aj[2];
+      double bi[2], aj[2];
 #pragma unroll
-      for (int ti = 0; ti < 4; ++ti) bi[ti] = As[(wi + ti * 16 + lr) * DG_LS + kk + lk];
+      for (int ti = 0; ti < 2; ++ti) bi[ti] = As[(wi + ti * 16 + lr) * DG_LS + kk + lk];
 #pragma unroll
       for (int tj = 0; tj < 2; ++tj) aj[tj] = Bs[(wj + tj * 16 + lr) * DG_LS + kk + lk];
 #pragma unroll
       for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-        for (int ti = 0; ti < 4; ++ti)
+        for (int ti = 0; ti < 2; ++ti)
           acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(aj[tj], bi[ti], acc[ti][tj], 0, 0, 0);
     }
     __syncthreads();
   }
   // accumulator element r of lane l: D[(l / 16) + 4 r][l % 16] = C(i = l % 16, j = l / 16 + 4 r)
 #pragma unroll
-  for (int ti = 0; ti < 4; ++ti)
+  for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll

@@ -132,8 +132,8 @@ def test_dgemm_lower_only_leaves_upper_tiles():
     low = np.tril(np.ones((n, n), bool))
     assert np.max(np.abs(got[low] - ref[low])) <= 1e-12 * np.max(np.abs(ref))
     # tiles entirely above the diagonal (i < j for every element) are untouched
-    i_tile, j_tile = np.arange(n)[:, None] // 128, np.arange(n)[None, :] // 64
-    untouched = (i_tile + 1) * 128 <= j_tile * 64
+    i_tile, j_tile = np.arange(n)[:, None] // 64, np.arange(n)[None, :] // 64
+    untouched = (i_tile + 1) * 64 <= j_tile * 64
     assert untouched.any() and np.all(got[untouched] == 0.0)
 
 
