@@ -12,10 +12,11 @@ its workspace in 32-bit integers and fails for cohorts of ~30k samples, while
 rocSOLVER manages its own workspace through the rocBLAS handle. Taking the steps one
 by one lets a caller report progress between them.
 
-Those routines take 32-bit sizes and have no 64-bit variant, so they serve
-n <= 46,340 (n * n <= 2^31 - 1). Larger cohorts (the reference reaches them with
-ILP64 LAPACK, src/jamma/lmm/eigen.py:9,39) go through _eigh_large, which keeps
-every offset 64-bit:
+Those routines take 32-bit sizes and have no 64-bit variant, so they could serve
+only n <= 46,340 (n * n <= 2^31 - 1); the reference reaches larger cohorts with
+ILP64 LAPACK (src/jamma/lmm/eigen.py:9,39). They are used below 8,192 samples.
+From 8,192 on (where it is also faster) the eigendecomposition is _eigh_large,
+which keeps every offset 64-bit:
   1. blocked Householder tridiagonalisation (LAPACK dsytrd/dlatrd, lower) with
      this library's lower-triangle symv (jm_symv_lower, each stored element read
      once per column step), rocBLAS' 64-bit gemv, rocSOLVER's 64-bit dlarfg, and
@@ -47,6 +48,10 @@ _RB_FILL_LOWER = 122         # rocblas_fill_lower
 _RB_OP_N, _RB_OP_T = 111, 112
 _RB_POINTER_HOST, _RB_POINTER_DEVICE = 0, 1
 _LIB_MAX_N = 46_340          # largest n with n * n <= 2^31 - 1
+# From this size on the 64-bit path is also the faster one (one MI355X, kinship-like
+# matrices: n = 8,192 0.85 s vs rocSOLVER 0.93 s; 20,000 5.5 s vs 7.6 s; 30,000 15.5 s
+# vs 22.8 s; eigenvalues equal to 1e-15, residuals and orthogonality alike).
+_OWN_MIN_N = 8192
 _LARGE_PANEL = 64            # reflectors per panel / per compact-WY block
 _LARGE_LEAF = 30_000         # largest tridiagonal block handed to rocSOLVER dstedc
 _SOLVER = {}
@@ -94,6 +99,8 @@ def _dsyevd(Kd: torch.Tensor, progress=None):
     the step's work has finished on the device.
     """
     n = Kd.shape[0]
+    if n > _LIB_MAX_N:
+        raise ValueError(f"rocSOLVER's 32-bit routines cannot take n = {n}")
     rb, rs, h = _solver(Kd.device)
     A = Kd.clone(memory_format=torch.contiguous_format)  # overwritten with eigenvectors
     D = torch.empty(n, dtype=torch.float64, device=Kd.device)
@@ -428,7 +435,7 @@ def eigh_device(K, threshold: float = 1e-10, progress=None):
         raise ValueError(f"Kinship matrix must be square, got shape {tuple(Kd.shape)}")
     if Kd.shape[0] == 0:
         raise ValueError("Kinship matrix is empty")
-    if Kd.shape[0] <= _LIB_MAX_N:
+    if Kd.shape[0] < min(_OWN_MIN_N, _LIB_MAX_N + 1):
         d, U = _dsyevd(Kd, progress)
     else:
         d, U = _eigh_large(Kd, progress)

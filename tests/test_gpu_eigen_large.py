@@ -138,12 +138,12 @@ def test_dgemm_lower_only_leaves_upper_tiles():
 
 
 def test_eigh_device_routes_large_cohorts(monkeypatch):
-    """eigh_device sends n above the library limit through the 64-bit path; the
-    thresholding and warnings are the same."""
+    """eigh_device sends large cohorts through the 64-bit path; the thresholding and
+    warnings are the same."""
     import jamma_amd.lmm.eigen as eg
 
     K = _kinship(260, 120, seed=7)  # rank-deficient: p < n
-    monkeypatch.setattr(eg, "_LIB_MAX_N", 100)
+    monkeypatch.setattr(eg, "_OWN_MIN_N", 100)
     monkeypatch.setattr(eg, "_LARGE_LEAF", 64)
     with pytest.warns(UserWarning, match="close to zero"):
         d, U = eg.eigh_device(K)
