@@ -344,10 +344,10 @@ __global__ void k_pack_geno(const int8_t* __restrict__ g, int64_t ld_g,
 // both operands copied block by block into NBUF LDS stages with
 // global_load_lds_dwordx4 (the copies of stages kt+1 .. kt+NBUF-1 in flight
 // while stage kt is multiplied), one barrier per 32-deep K step. Genotype
-// fragments are widened to X1 / X2 with widen8p. Per K step, smallest terms
-// first:
-//   NPL = 2 (UT = hi + lo):        lo x X1, hi x X2, hi x X1   (3 products)
-//   NPL = 3 (UT = hi + mid + lo):  lo x X1, mid x X1, hi x X2, hi x X1
+// fragments are widened to X1 and X2 in one pass per column block (the two share
+// the code-to-byte selectors and the LDS read). Per K step:
+//   NPL = 2 (UT = hi + lo):        hi x X2, hi x X1, lo x X1   (3 products)
+//   NPL = 3 (UT = hi + mid + lo):  hi x X2, hi x X1, mid x X1, lo x X1
 template <int NPL, int NBUF>
 __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
     const uint16_t* __restrict__ planes, int64_t per_plane, const uint8_t* __restrict__ gp,
@@ -399,10 +399,8 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
       return v;
     };
     bf16x8_t x1[8];
-#pragma unroll
-    for (int tj = 0; tj < 8; ++tj) widen8p(gbytes(tj), mm[tj], x1[tj], nullptr);
 #pragma unroll 1
-    for (int pl = NPL - 1; pl >= 0; --pl) {
+    for (int pl = 0; pl < NPL; ++pl) {
       bf16x8_t af[4];
 #pragma unroll
       for (int ti = 0; ti < 4; ++ti) {
@@ -410,11 +408,11 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
         const int c = lk ^ ((r >> 2) & 3);
         memcpy(&af[ti], base + pl * V3_APLANE + r * 64 + 16 * c, 16);
       }
-      if (pl == 0) {
+      if (pl == 0) {  // hi plane first: X1 and X2 of a column block from one widening
 #pragma unroll
         for (int tj = 0; tj < 8; ++tj) {
-          bf16x8_t x2, unused;
-          widen8p(gbytes(tj), mm[tj], unused, &x2);
+          bf16x8_t x2;
+          widen8p(gbytes(tj), mm[tj], x1[tj], &x2);
 #pragma unroll
           for (int ti = 0; ti < 4; ++ti)
             acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ti], x2, acc[ti][tj], 0, 0, 0);
