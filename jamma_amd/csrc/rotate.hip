@@ -35,6 +35,7 @@ namespace jm {
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 
 // XCD-aware mapping of a linear block id onto (row block, column block).
 // Hardware block bid runs on XCD bid % 8; XCD x receives the contiguous tile
@@ -222,46 +223,41 @@ __device__ __forceinline__ double bf16_to_f64(uint16_t h) {
 // BF16x3 rotation
 // ---------------------------------------------------------------------------
 // Per 32-deep K step a 256x256 block stages the three UT planes (3 x 256 x 32
-// bf16 = 48 KB) and the int8 genotypes (256 x 32 B = 8 KB) from global memory
-// into LDS with global_load_lds_dwordx4 (no register staging), double
-// buffered. LDS rows are XOR-swizzled in 16-byte chunks so that the
-// ds_read_b128 (A) and ds_read_b64 (B) fragment reads of 16 lanes hit 64
-// distinct banks: A chunk c of row r sits at c ^ ((r >> 2) & 3) (64-byte rows:
-// the 4-bank group of a chunk is 4 (r & 3) + c), genotype chunk c of column j
-// at c ^ ((j >> 3) & 1).
+// bf16 = 48 KB) and the genotype selectors (256 x 64 B = 16 KB) from global
+// memory into LDS with global_load_lds_dwordx4 (no register staging), double
+// buffered. LDS rows are 64 bytes, XOR-swizzled in 16-byte chunks so that the
+// ds_read_b128 fragment reads of 16 lanes hit 64 distinct banks: chunk c of
+// row r (A) or column j (genotypes) sits at c ^ ((r >> 2) & 3) (the 4-bank
+// group of a chunk is 4 (r & 3) + c).
 constexpr int V3_BM = 256, V3_BN = 256, V3_BK = 32;
 constexpr int V3_APLANE = V3_BM * V3_BK * 2;      // 16 KB per plane
-constexpr int V3_BBYTES = V3_BN * V3_BK;          // 8 KB
-// LDS stage of NPL planes + genotypes: 40 KB (NPL = 2) or 56 KB (NPL = 3)
+constexpr int V3_BBYTES = V3_BN * V3_BK * 2;      // 16 KB: two selector bytes per genotype
+// LDS stage of NPL planes + genotypes: 48 KB (NPL = 2) or 64 KB (NPL = 3)
 template <int NPL>
 constexpr int v3_stage() { return NPL * V3_APLANE + V3_BBYTES; }
 
-// Byte-permute form of widen8. The 2-bit genotype code indexes a 4-entry table
-// (0, 1, 2, missing -> x & 3 = 0, 1, 2, 3); v_perm_b32 picks the low and high
-// bytes of each bf16 from per-column tables:
+// Byte-permute widening of 8 genotypes to bf16. The 2-bit genotype code c (0, 1,
+// 2, missing = 3) indexes 4-entry byte tables; v_perm_b32 picks the low byte of
+// each bf16 from table byte c and the high byte from table byte c + 4, so the
+// packed tile stores, per genotype, the selector pair (c, c + 4) and one dword
+// holds the selectors of two genotypes. Per-column tables:
 //   X1: lo bytes {00, 80, 00, lo(m1)}, hi bytes {00, 3F, 40, hi(m1)}  (0, 1.0, 2.0, m1)
 //   X2: lo bytes {00, 00, 00, lo(m2)}, hi bytes {00, 00, 00, hi(m2)}
-// mm = m1 | m2 << 16. About 18 VALU per 8 genotypes instead of ~70.
-__device__ __forceinline__ void widen8p(uint64_t bytes, uint32_t mm, bf16x8_t& x1, bf16x8_t* x2) {
-  const uint32_t t1lo = ((mm & 0xFFu) << 24) | 0x8000u;
-  const uint32_t t1hi = (((mm >> 8) & 0xFFu) << 24) | 0x403F00u;
+// mm = m1 | m2 << 16. 4 table + 8 permute VALU per 8 genotypes for X1 and X2.
+__device__ __forceinline__ void widen8s(const uint32_t (&sel)[4], uint32_t mm, bf16x8_t& x1,
+                                        bf16x8_t& x2) {
+  const uint32_t t1lo = __builtin_amdgcn_perm(mm, 0x00008000u, 0x04020100u);  // lo(m1) in byte 3
+  const uint32_t t1hi = __builtin_amdgcn_perm(mm, 0x00403F00u, 0x05020100u);  // hi(m1) in byte 3
+  const uint32_t t2lo = __builtin_amdgcn_perm(mm, 0u, 0x060C0C0Cu);           // lo(m2) in byte 3
+  const uint32_t t2hi = mm & 0xFF000000u;                                     // hi(m2) in byte 3
   uint32_t w1[4], w2[4];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const uint32_t idx = (uint32_t)(bytes >> (32 * h)) & 0x03030303u;
-    const uint32_t selA = __builtin_amdgcn_perm(idx, idx, 0x01010000u) + 0x04000400u;
-    const uint32_t selB = __builtin_amdgcn_perm(idx, idx, 0x03030202u) + 0x04000400u;
-    w1[2 * h] = __builtin_amdgcn_perm(t1hi, t1lo, selA);
-    w1[2 * h + 1] = __builtin_amdgcn_perm(t1hi, t1lo, selB);
-    if (x2) {
-      const uint32_t t2lo = ((mm >> 16) & 0xFFu) << 24;
-      const uint32_t t2hi = ((mm >> 24) & 0xFFu) << 24;
-      w2[2 * h] = __builtin_amdgcn_perm(t2hi, t2lo, selA);
-      w2[2 * h + 1] = __builtin_amdgcn_perm(t2hi, t2lo, selB);
-    }
+  for (int q = 0; q < 4; ++q) {
+    w1[q] = __builtin_amdgcn_perm(t1hi, t1lo, sel[q]);
+    w2[q] = __builtin_amdgcn_perm(t2hi, t2lo, sel[q]);
   }
   memcpy(&x1, w1, 16);
-  if (x2) memcpy(x2, w2, 16);
+  memcpy(&x2, w2, 16);
 }
 
 // ---------------------------------------------------------------------------
@@ -273,12 +269,13 @@ __device__ __forceinline__ void widen8p(uint64_t bytes, uint32_t mm, bf16x8_t& x
 //   UT planes: plane-major, then panel P (256 rows), then K step s (32
 //     columns): a 16 KB block of 256 rows x 64 B, row r's 16-byte chunk c at
 //     r * 64 + 16 * (c ^ ((r >> 2) & 3)). Rows >= n and columns >= n are 0.
-//   genotypes: column block B (256 SNPs), then K step s (32 samples): an 8 KB
-//     block of 256 SNPs x 32 B, SNP j's chunk c at j * 32 + 16 * (c ^ ((j >> 3)
-//     & 1)). The 2-bit codes are those of the decoded tile (0/1/2, missing
-//     0xFF); SNPs >= n_snps and samples >= n are 0.
+//   genotypes: column block B (256 SNPs), then K step s (32 samples): a 16 KB
+//     block of 256 SNPs x 64 B, SNP j's chunk c (samples 8c .. 8c + 7, two
+//     selector bytes (code, code + 4) each; code = decoded value & 3, missing
+//     0xFF -> 3) at j * 64 + 16 * (c ^ ((j >> 2) & 3)). SNPs >= n_snps and
+//     samples >= n hold code 0.
 // The swizzles are those of the LDS stage, so a block copies verbatim.
-constexpr int64_t UTP_BLOCK = 16384, GP_BLOCK = 8192;
+constexpr int64_t UTP_BLOCK = 16384, GP_BLOCK = 16384;
 
 __global__ void k_pack_ut(const double* __restrict__ UT, int64_t ld_u, int64_t n, int64_t npan,
                           int64_t nks, int nplanes, uint16_t* __restrict__ planes) {
@@ -312,28 +309,35 @@ __global__ void k_pack_geno(const int8_t* __restrict__ g, int64_t ld_g,
                             int64_t n, int64_t n_snps, int64_t nks, uint8_t* __restrict__ gp,
                             uint32_t* __restrict__ mm) {
   const int64_t nblk = (n_snps + 255) / 256;
-  const int64_t chunks = nblk * nks * 512;  // 16-byte chunks
+  const int64_t chunks = nblk * nks * 1024;  // 16-byte chunks
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < chunks;
        q += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t blk = q >> 9;
-    const int pos = (int)(q & 511);
-    const int j = pos >> 1, cs = pos & 1;
-    const int c = cs ^ ((j >> 3) & 1);
+    const int64_t blk = q >> 10;
+    const int pos = (int)(q & 1023);
+    const int j = pos >> 2, cs = pos & 3;
+    const int c = cs ^ ((j >> 2) & 3);
     const int64_t B = blk / nks, sk = blk % nks;
-    const int64_t snp = B * 256 + j, k0 = sk * 32 + 16 * c;
+    const int64_t snp = B * 256 + j, k0 = sk * 32 + 8 * c;
     uint8_t v[16];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      v[2 * e] = 0;
+      v[2 * e + 1] = 4;
+    }
     if (snp < n_snps) {
       const int8_t* row = g + (col ? (int64_t)col[snp] : snp) * ld_g;
 #pragma unroll
-      for (int e = 0; e < 16; ++e) v[e] = (k0 + e < n) ? (uint8_t)row[k0 + e] : (uint8_t)0;
+      for (int e = 0; e < 8; ++e)
+        if (k0 + e < n) {
+          const uint8_t code = (uint8_t)row[k0 + e] & 3u;
+          v[2 * e] = code;
+          v[2 * e + 1] = (uint8_t)(code + 4);
+        }
       if (sk == 0 && c == 0) {
         const double mu = mean[col ? (int64_t)col[snp] : snp];
         const uint16_t h1 = f32_to_bf16_rne((float)mu);
         mm[snp] = (uint32_t)h1 | ((uint32_t)f32_to_bf16_rne((float)(mu - bf16_to_f64(h1))) << 16);
       }
-    } else {
-#pragma unroll
-      for (int e = 0; e < 16; ++e) v[e] = 0;
     }
     memcpy(gp + q * 16, v, 16);
   }
@@ -355,7 +359,7 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
     float* __restrict__ Z, int64_t ldz) {
   constexpr int STAGE = v3_stage<NPL>();
   constexpr int ADMA = NPL * 2;    // 1 KB UT copies per wave per stage (NPL x 16 KB / 8 waves)
-  constexpr int DPS = ADMA + 1;    // copies per wave per stage (+ 1 KB of genotypes)
+  constexpr int DPS = ADMA + 2;    // copies per wave per stage (+ 2 KB of genotype selectors)
   static_assert(NBUF >= 2 && NBUF * STAGE <= 160 * 1024, "LDS stages");
   __shared__ __attribute__((aligned(1024))) uint8_t lds[NBUF * STAGE];
   const int tid = threadIdx.x;
@@ -377,7 +381,10 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
       lds_copy16(ap + (q >> 4) * pstride + sk * UTP_BLOCK + (q & 15) * 1024,
                  base + (q >> 4) * V3_APLANE + (q & 15) * 1024);
     }
-    lds_copy16(bp + sk * GP_BLOCK + wave * 1024, base + NPL * V3_APLANE + wave * 1024);
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      lds_copy16(bp + sk * GP_BLOCK + (2 * wave + u) * 1024,
+                 base + NPL * V3_APLANE + (2 * wave + u) * 1024);
   };
   uint32_t mm[8];
 #pragma unroll
@@ -391,12 +398,15 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
 #pragma unroll
     for (int b = 0; b < 8; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
   auto compute = [&](const uint8_t* base) {
-    auto gbytes = [&](int tj) -> uint64_t {
+    auto gsel = [&](int tj, uint32_t (&sel)[4]) {  // selectors of 8 samples of column j
       const int j = wc + 16 * tj + lr;
-      const int c = (lk >> 1) ^ ((j >> 3) & 1);
-      uint64_t v;
-      memcpy(&v, base + NPL * V3_APLANE + j * 32 + 16 * c + 8 * (lk & 1), 8);
-      return v;
+      const int c = lk ^ ((j >> 2) & 3);
+      u32x4_t v;  // one 16-byte LDS read
+      memcpy(&v, base + NPL * V3_APLANE + j * 64 + 16 * c, 16);
+      sel[0] = v[0];
+      sel[1] = v[1];
+      sel[2] = v[2];
+      sel[3] = v[3];
     };
     bf16x8_t x1[8];
 #pragma unroll 1
@@ -411,8 +421,10 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
       if (pl == 0) {  // hi plane first: X1 and X2 of a column block from one widening
 #pragma unroll
         for (int tj = 0; tj < 8; ++tj) {
+          uint32_t sel[4];
+          gsel(tj, sel);
           bf16x8_t x2;
-          widen8p(gbytes(tj), mm[tj], x1[tj], &x2);
+          widen8s(sel, mm[tj], x1[tj], x2);
 #pragma unroll
           for (int ti = 0; ti < 4; ++ti)
             acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ti], x2, acc[ti][tj], 0, 0, 0);
@@ -543,10 +555,10 @@ extern "C" int jm_rotate_bf16x3(const uint16_t* planes, int nplanes, const uint8
   const int64_t nks = (n + 31) / 32;
   const int64_t per_plane = jm_packed_ut_bytes(n, nplanes) / (2 * nplanes);  // bf16 per plane
   const int nrb = ceil_div(n, V3_BM), ncb = ceil_div(n_snps, V3_BN);
-  if (nplanes == 2)  // 2 x 40 KB stages (measured faster than 3 at n = 10k)
+  if (nplanes == 2)  // 2 x 48 KB stages (measured faster than 3 at n = 10k)
     hipLaunchKernelGGL((k_rotate_bf16x3_packed<2, 2>), dim3(nrb * ncb), dim3(512), 0,
                        (hipStream_t)stream, planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
-  else  // 2 x 56 KB stages
+  else  // 2 x 64 KB stages
     hipLaunchKernelGGL((k_rotate_bf16x3_packed<3, 2>), dim3(nrb * ncb), dim3(512), 0,
                        (hipStream_t)stream, planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
   JM_CHECK_LAUNCH();
