@@ -349,8 +349,26 @@ int jm_dgemm_splitk(int trans_a, int trans_b, int64_t m, int64_t n, int64_t k, d
                     double* C, int64_t ldc, int splits, double* work, int64_t work_doubles,
                     void* stream);
 
-/* Householder reduction step (dlatrd): *e = *a, then *a = 1. */
-int jm_tridiag_take_unit(double* a, double* e, void* stream);
+/* Householder reflector of a dlatrd column step (LAPACK dlarfg on alpha and the
+ * len - 1 elements of x, tau and x written as dlarfg does), then *e = beta and
+ * *alpha = 1 (the reflector's unit entry). One block; deterministic. */
+int jm_tridiag_larfg(int64_t len, double* alpha, double* x, double* tau, double* e, void* stream);
+
+/* dlatrd's update of column i by the panel's earlier i reflectors:
+ * y[r] -= sum_c a0[r + c lda] w0[c ldw] + w0[r + c ldw] a0[c lda] for r < len
+ * (the vectors are the panels' first rows; y does not overlap them). */
+int jm_tridiag_panel_update(int64_t len, int i, const double* a0, int64_t lda, const double* w0,
+                            int64_t ldw, double* y, void* stream);
+
+/* dlatrd's corrections after the symmetric product w = A22 v (len rows, i <= 64
+ * panel columns ap (lda) and wp (ldw)): t1 = wp^T v, t2 = ap^T v,
+ * w -= ap t1 + wp t2, t2 stored to gram_col (i values: the panel's V^T v). work:
+ * jm_tridiag_panel_work_doubles(len) doubles; block partial sums added in a
+ * fixed order. */
+int64_t jm_tridiag_panel_work_doubles(int64_t N);
+int jm_tridiag_panel_step(int64_t len, int i, const double* ap, int64_t lda, const double* wp,
+                          int64_t ldw, const double* v, double* w, double* gram_col, double* work,
+                          int64_t work_doubles, void* stream);
 
 /* End of a dlatrd column step on w, v (len elements): w *= tau,
  * alpha = -tau/2 (w . v), w += alpha v (tau read from device memory). */

@@ -100,7 +100,12 @@ _SIGS = {
     "jm_dgemm_splitk": ([c_int, c_int, c_int64, c_int64, c_int64, c_double, c_void_p, c_int64,
                          c_void_p, c_int64, c_double, c_void_p, c_int64, c_int, c_void_p, c_int64,
                          c_void_p], c_int),
-    "jm_tridiag_take_unit": ([c_void_p, c_void_p, c_void_p], c_int),
+    "jm_tridiag_larfg": ([c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "jm_tridiag_panel_update": ([c_int64, c_int, c_void_p, c_int64, c_void_p, c_int64, c_void_p,
+                                 c_void_p], c_int),
+    "jm_tridiag_panel_work_doubles": ([c_int64], c_int64),
+    "jm_tridiag_panel_step": ([c_int64, c_int, c_void_p, c_int64, c_void_p, c_int64, c_void_p,
+                               c_void_p, c_void_p, c_void_p, c_int64, c_void_p], c_int),
     "jm_tridiag_finish": ([c_int64, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "jm_secular_solve": ([c_int64, c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p,
                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p], c_int),

@@ -6,12 +6,13 @@
 // here is 64-bit. The host side (jamma_amd/lmm/eigen.py) runs:
 //
 //  1. blocked Householder reduction to tridiagonal form (lower, LAPACK dsytrd /
-//     dlatrd order of operations): the symmetric matrix-vector product of each
-//     column step to k_symv_lower / k_symv_reduce (the lower triangle read
-//     once), the panel's small matrix-vector products to rocBLAS' 64-bit gemv,
-//     the rank-2k update of the trailing matrix to k_dgemm, and the end of each
-//     column step (scale by tau, alpha = -tau/2 w.v, w += alpha v) to
-//     k_latrd_finish;
+//     dlatrd order of operations): per column the update by the panel's earlier
+//     reflectors (k_panel_update), the reflector (k_larfg_unit), the symmetric
+//     matrix-vector product (k_symv_lower / k_symv_reduce, the lower triangle
+//     read once), the panel's corrections (k_panel_dots / k_panel_apply) and the
+//     end of the step (scale by tau, alpha = -tau/2 w.v, w += alpha v:
+//     k_latrd_finish); per panel the rank-2k update of the trailing matrix
+//     (k_dgemm);
 //  2. Cuppen divide and conquer on the tridiagonal matrix: tear it into halves
 //     (recursively) until a half fits the library solver, then merge each pair
 //     through the rank-one update D + rho z z^T (Bunch-Nielsen-Sorensen):
@@ -163,11 +164,129 @@ __global__ void __launch_bounds__(256) k_splitk_sum(int64_t m, int64_t n, int sp
 // ---------------------------------------------------------------------------
 // Tridiagonal reduction helpers
 // ---------------------------------------------------------------------------
-// e = a; a = 1 (the reflector's unit entry replaces the subdiagonal element).
-__global__ void k_take_unit(double* a, double* e) {
+// dlatrd's column update with the panel's earlier reflectors, one kernel for the
+// two matrix-vector products: y[r] -= sum_{c < i} a0[r + c lda] xw[c] + w0[r + c ldw] xa[c]
+// (r < len), where xw[c] = w0[c ldw] and xa[c] = a0[c lda] are the panels' first rows
+// (A_loc(i, c) and W_loc(i, c)); y is column i from row i (it does not overlap them).
+__global__ void __launch_bounds__(256) k_panel_update(int64_t len, int i,
+                                                      const double* __restrict__ a0, int64_t lda,
+                                                      const double* __restrict__ w0, int64_t ldw,
+                                                      double* __restrict__ y) {
+  __shared__ double xs[2][64];
+  if (threadIdx.x < (unsigned)i) {
+    xs[0][threadIdx.x] = w0[(int64_t)threadIdx.x * ldw];
+    xs[1][threadIdx.x] = a0[(int64_t)threadIdx.x * lda];
+  }
+  __syncthreads();
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < len;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < i; ++c) s += a0[r + c * lda] * xs[0][c] + w0[r + c * ldw] * xs[1][c];
+    y[r] -= s;
+  }
+}
+
+// Partial column dot products of the two panels with v: block (b, q) sums column q
+// (q < i: Wp's column q, else Ap's column q - i) times v over rows
+// [b PD_ROWS, (b + 1) PD_ROWS), 4 rows per thread, into part[b][q].
+constexpr int PD_ROWS = 1024;  // rows per block
+__global__ void __launch_bounds__(256) k_panel_dots(int64_t len, int i,
+                                                    const double* __restrict__ ap, int64_t lda,
+                                                    const double* __restrict__ wp, int64_t ldw,
+                                                    const double* __restrict__ v,
+                                                    double* __restrict__ part) {
+  __shared__ double red[4];
+  const int q = blockIdx.y;
+  const double* col = q < i ? wp + (int64_t)q * ldw : ap + (int64_t)(q - i) * lda;
+  const int64_t r0 = (int64_t)blockIdx.x * PD_ROWS;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < PD_ROWS / 256; ++k) {
+    const int64_t r = r0 + threadIdx.x + 256 * k;
+    if (r < len) s += col[r] * v[r];
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    part[(int64_t)blockIdx.x * 128 + q] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// t1 = sum of the W-panel partials, t2 = sum of the A-panel partials (block order);
+// block 0 stores t2 = Ap^T v (the panel's V^T v_i, kept for the compact-WY factor);
+// w[r] -= sum_c Ap(r, c) t1[c] + Wp(r, c) t2[c].
+__global__ void __launch_bounds__(256) k_panel_apply(int64_t len, int i, int nblk,
+                                                     const double* __restrict__ ap, int64_t lda,
+                                                     const double* __restrict__ wp, int64_t ldw,
+                                                     const double* __restrict__ part,
+                                                     double* __restrict__ gram_col,
+                                                     double* __restrict__ w) {
+  __shared__ double ts[128];
+  if (threadIdx.x < (unsigned)(2 * i)) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * 128 + threadIdx.x];
+    ts[threadIdx.x] = s;
+    if (blockIdx.x == 0 && threadIdx.x >= (unsigned)i) gram_col[threadIdx.x - i] = s;
+  }
+  __syncthreads();
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < len;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < i; ++c) s += ap[r + c * lda] * ts[c] + wp[r + c * ldw] * ts[i + c];
+    w[r] -= s;
+  }
+}
+
+// Householder reflector (LAPACK dlarfg) on (alpha, x[0 .. len-2]) in one block, and
+// the step that follows it: e = beta, alpha's slot = 1 (the reflector's unit entry).
+// ||x|| is formed as scale * sqrt(sum (x / scale)^2), scale = max |x|.
+__global__ void __launch_bounds__(1024) k_larfg_unit(int64_t len, double* __restrict__ alpha_p,
+                                                     double* __restrict__ x,
+                                                     double* __restrict__ tau,
+                                                     double* __restrict__ e) {
+  __shared__ double part[16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int64_t m = len - 1;
+  double mx = 0.0;
+  for (int64_t q = threadIdx.x; q < m; q += blockDim.x) mx = fmax(mx, fabs(x[q]));
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, shfl_xor_d(mx, o));
+  if (lane == 0) part[wv] = mx;
+  __syncthreads();
+  double scale = 0.0;
+  for (int q = 0; q < nw; ++q) scale = fmax(scale, part[q]);
+  __syncthreads();
+  double ss = 0.0;
+  if (scale > 0.0) {
+    const double inv = 1.0 / scale;
+    for (int64_t q = threadIdx.x; q < m; q += blockDim.x) {
+      const double t = x[q] * inv;
+      ss += t * t;
+    }
+  }
+  ss = wave_sum(ss);
+  if (lane == 0) part[wv] = ss;
+  __syncthreads();
+  double tot = 0.0;
+  for (int q = 0; q < nw; ++q) tot += part[q];
+  const double alpha = *alpha_p;
+  const double xnorm = scale * sqrt(tot);
+  __syncthreads();
+  if (xnorm == 0.0) {  // H = I
+    if (threadIdx.x == 0) {
+      *tau = 0.0;
+      *e = alpha;
+      *alpha_p = 1.0;
+    }
+    return;
+  }
+  const double beta = -copysign(hypot(alpha, xnorm), alpha);
+  const double f = 1.0 / (alpha - beta);
+  for (int64_t q = threadIdx.x; q < m; q += blockDim.x) x[q] *= f;
   if (threadIdx.x == 0) {
-    *e = *a;
-    *a = 1.0;
+    *tau = (beta - alpha) / beta;
+    *e = beta;
+    *alpha_p = 1.0;
   }
 }
 
@@ -231,7 +350,7 @@ __global__ void __launch_bounds__(256) k_symv_lower(int64_t N, const double* __r
                                                     int chunk, int nchmax,
                                                     double* __restrict__ P,
                                                     double* __restrict__ Y1) {
-  __shared__ double red[2][4][64];
+  __shared__ double red[4][4][64];  // [2 parities x 2 tiles][wave][lane]
   const int64_t nt = (N + SV_T - 1) / SV_T;
   const int64_t J = blockIdx.x;
   const int c = blockIdx.y;
@@ -247,32 +366,55 @@ __global__ void __launch_bounds__(256) k_symv_lower(int64_t N, const double* __r
     xj[k] = (col0 + k < N) ? x[col0 + k] : 0.0;
     t[k] = 0.0;
   }
-  int par = 0;
-  for (int64_t I = I0; I < I1; ++I) {
-    const int64_t row = I * SV_T + lane;
-    const bool rok = row < N;
+  // the diagonal tile (first tile of chunk 0): symmetric from its lower half
+  if (I0 == J) {
+    const int64_t row = J * SV_T + lane;
     double yr = 0.0;
-    if (I > J) {  // every column of the tile is < N
-      const double xi = rok ? x[row] : 0.0;
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        const double a = rok ? A[row + (col0 + k) * lda] : 0.0;
-        yr += a * xj[k];
-        t[k] += a * xi;
-      }
-    } else {  // diagonal tile
 #pragma unroll 4
-      for (int k = 0; k < 16; ++k) {
-        const int64_t col = col0 + k;
-        if (rok && col < N)
-          yr += ((lane >= 16 * g + k) ? A[row + col * lda] : A[col + row * lda]) * xj[k];
-      }
+    for (int k = 0; k < 16; ++k) {
+      const int64_t col = col0 + k;
+      if (row < N && col < N)
+        yr += ((lane >= 16 * g + k) ? A[row + col * lda] : A[col + row * lda]) * xj[k];
     }
-    red[par][g][lane] = yr;
+    red[0][g][lane] = yr;
     __syncthreads();
     if (g == 0)
-      P[(I * nt + J) * SV_T + lane] = (red[par][0][lane] + red[par][1][lane]) +
-                                      (red[par][2][lane] + red[par][3][lane]);
+      P[(J * nt + J) * SV_T + lane] = (red[0][0][lane] + red[0][1][lane]) +
+                                      (red[0][2][lane] + red[0][3][lane]);
+    __syncthreads();
+  }
+  // off-diagonal tiles, two per step: 32 loads in flight per lane, one barrier per pair
+  // (the LDS sums alternate between two halves of red, so a step's writes cannot meet
+  // the previous step's reads)
+  int par = 0;
+  for (int64_t I = (I0 == J ? I0 + 1 : I0); I < I1; I += 2) {
+    const bool two = I + 1 < I1;
+    const int64_t row0 = I * SV_T + lane, row1 = row0 + SV_T;
+    const bool rok0 = row0 < N, rok1 = two && row1 < N;
+    double a0[16], a1[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {  // every column of these tiles is < N (I > J)
+      a0[k] = rok0 ? A[row0 + (col0 + k) * lda] : 0.0;
+      a1[k] = rok1 ? A[row1 + (col0 + k) * lda] : 0.0;
+    }
+    const double xi0 = rok0 ? x[row0] : 0.0, xi1 = rok1 ? x[row1] : 0.0;
+    double yr0 = 0.0, yr1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      yr0 += a0[k] * xj[k];
+      yr1 += a1[k] * xj[k];
+      t[k] += a0[k] * xi0 + a1[k] * xi1;
+    }
+    double (*r0)[64] = red[2 * par], (*r1)[64] = red[2 * par + 1];
+    r0[g][lane] = yr0;
+    r1[g][lane] = yr1;
+    __syncthreads();
+    if (g == 0) {
+      P[(I * nt + J) * SV_T + lane] = (r0[0][lane] + r0[1][lane]) + (r0[2][lane] + r0[3][lane]);
+      if (two)
+        P[((I + 1) * nt + J) * SV_T + lane] = (r1[0][lane] + r1[1][lane]) +
+                                               (r1[2][lane] + r1[3][lane]);
+    }
     par ^= 1;
   }
   const double tot = reduce16_lanes(t, lane);
@@ -572,9 +714,48 @@ extern "C" int jm_symv_lower(int64_t N, const double* A, int64_t lda, const doub
   return JM_OK;
 }
 
-extern "C" int jm_tridiag_take_unit(double* a, double* e, void* stream) {
-  JM_CHECK_ARG(a && e, "null pointer");
-  hipLaunchKernelGGL(k_take_unit, dim3(1), dim3(64), 0, (hipStream_t)stream, a, e);
+extern "C" int jm_tridiag_larfg(int64_t len, double* alpha, double* x, double* tau, double* e,
+                                void* stream) {
+  JM_CHECK_ARG(len >= 1, "bad length");
+  JM_CHECK_ARG(alpha && tau && e && (len == 1 || x), "null pointer");
+  hipLaunchKernelGGL(k_larfg_unit, dim3(1), dim3(1024), 0, (hipStream_t)stream, len, alpha, x, tau,
+                     e);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int64_t jm_tridiag_panel_work_doubles(int64_t N) {
+  return ((N + PD_ROWS - 1) / PD_ROWS + 1) * 128;
+}
+
+extern "C" int jm_tridiag_panel_update(int64_t len, int i, const double* a0, int64_t lda,
+                                       const double* w0, int64_t ldw, double* y, void* stream) {
+  JM_CHECK_ARG(len >= 0 && i >= 0 && i <= 64, "bad sizes");
+  if (len == 0 || i == 0) return JM_OK;
+  JM_CHECK_ARG(a0 && w0 && y, "null pointer");
+  const unsigned grid = (unsigned)((len + 255) / 256 < 1024 ? (len + 255) / 256 : 1024);
+  hipLaunchKernelGGL(k_panel_update, dim3(grid), dim3(256), 0, (hipStream_t)stream, len, i, a0,
+                     lda, w0, ldw, y);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int jm_tridiag_panel_step(int64_t len, int i, const double* ap, int64_t lda,
+                                     const double* wp, int64_t ldw, const double* v, double* w,
+                                     double* gram_col, double* work, int64_t work_doubles,
+                                     void* stream) {
+  JM_CHECK_ARG(len >= 0 && i >= 0 && i <= 64, "bad sizes");
+  if (len == 0 || i == 0) return JM_OK;
+  JM_CHECK_ARG(ap && wp && v && w && gram_col && work, "null pointer");
+  const int64_t nblk = (len + PD_ROWS - 1) / PD_ROWS;
+  JM_CHECK_ARG(work_doubles >= nblk * 128, "workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_panel_dots, dim3((unsigned)nblk, (unsigned)(2 * i)), dim3(256), 0, st, len,
+                     i, ap, lda, wp, ldw, v, work);
+  JM_CHECK_LAUNCH();
+  const unsigned grid = (unsigned)((len + 255) / 256 < 1024 ? (len + 255) / 256 : 1024);
+  hipLaunchKernelGGL(k_panel_apply, dim3(grid), dim3(256), 0, st, len, i, (int)nblk, ap, lda, wp,
+                     ldw, work, gram_col, w);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

@@ -17,10 +17,11 @@ only n <= 46,340 (n * n <= 2^31 - 1); the reference reaches larger cohorts with
 ILP64 LAPACK (src/jamma/lmm/eigen.py:9,39). They are used below 8,192 samples.
 From 8,192 on (where it is also faster) the eigendecomposition is _eigh_large,
 which keeps every offset 64-bit:
-  1. blocked Householder tridiagonalisation (LAPACK dsytrd/dlatrd, lower) with
-     this library's lower-triangle symv (jm_symv_lower, each stored element read
-     once per column step), rocBLAS' 64-bit gemv, rocSOLVER's 64-bit dlarfg, and
-     this library's FP64 MFMA GEMM (jm_dgemm) for the rank-2k trailing updates;
+  1. blocked Householder tridiagonalisation (LAPACK dsytrd/dlatrd, lower), every
+     step a kernel of this library: the lower-triangle symv (jm_symv_lower, each
+     stored element read once per column step), the reflector (jm_tridiag_larfg),
+     the panel's matrix-vector products (jm_tridiag_panel_update / _step), and the
+     FP64 MFMA GEMM (jm_dgemm) for the rank-2k trailing updates;
   2. Cuppen divide and conquer on the tridiagonal matrix: halves are torn off
      until each fits rocSOLVER's dstedc, and pairs are merged through the
      rank-one update D + rho z z^T (deflation as LAPACK dlaed2; secular roots,
@@ -45,8 +46,6 @@ from jamma_amd.device import ptr, require_gpu, stream_ptr, to_dev
 _RB_EVECT_ORIGINAL = 211     # rocblas_evect_original
 _RB_EVECT_TRIDIAGONAL = 212  # rocblas_evect_tridiagonal
 _RB_FILL_LOWER = 122         # rocblas_fill_lower
-_RB_OP_N, _RB_OP_T = 111, 112
-_RB_POINTER_HOST, _RB_POINTER_DEVICE = 0, 1
 _LIB_MAX_N = 46_340          # largest n with n * n <= 2^31 - 1
 # From this size on the 64-bit path is also the faster one (one MI355X, kinship-like
 # matrices: n = 8,192 0.85 s vs rocSOLVER 0.93 s; 20,000 5.5 s vs 7.6 s; 30,000 15.5 s
@@ -75,13 +74,6 @@ def _solver(device: torch.device):
     rs.rocsolver_dorgtr.argtypes = [vp, ci, ci, vp, ci, vp]
     rs.rocsolver_dstedc.argtypes = [vp, ci, ci, vp, vp, vp, ci, vp]
     for f in (rs.rocsolver_dsytrd, rs.rocsolver_dorgtr, rs.rocsolver_dstedc):
-        f.restype = ci
-    i64 = ctypes.c_int64
-    rb.rocblas_set_pointer_mode.argtypes = [vp, ci]
-    rb.rocblas_dgemv_64.argtypes = [vp, ci, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64]
-    rs.rocsolver_dlarfg_64.argtypes = [vp, i64, vp, vp, i64, vp]
-    for f in (rb.rocblas_set_pointer_mode, rb.rocblas_dgemv_64,
-              rs.rocsolver_dlarfg_64):
         f.restype = ci
     h = ctypes.c_void_p()
     with torch.cuda.device(idx):
@@ -153,15 +145,14 @@ def _tridiagonalize(A: torch.Tensor, nb: int, progress=None):
     """
     n = A.shape[0]
     dev = A.device
-    rb, rs, h = _solver(dev)
     f64 = torch.float64
     e = torch.zeros(max(n - 1, 1), dtype=f64, device=dev)
     tau = torch.zeros(max(n - 1, 1), dtype=f64, device=dev)
     Wt = torch.zeros((nb, n), dtype=f64, device=dev)  # column-major W (n x nb, ld n)
     nwork = int(_lib.lib.jm_symv_work_doubles(n))
     symv_work = torch.empty(max(nwork, 1), dtype=f64, device=dev)
-    consts = torch.tensor([1.0, 0.0, -1.0], dtype=f64, device=dev)
-    one, zero, mone = (consts.data_ptr() + 8 * k for k in range(3))
+    npw = int(_lib.lib.jm_tridiag_panel_work_doubles(n))
+    panel_work = torch.empty(npw, dtype=f64, device=dev)
     base, wb, ld = A.data_ptr(), Wt.data_ptr(), n
     st = stream_ptr()
 
@@ -171,52 +162,40 @@ def _tridiagonalize(A: torch.Tensor, nb: int, progress=None):
     def pw(r, c):
         return wb + 8 * (c * ld + r)
 
-    def gemv(trans, m, k, alpha, a, lda, x, incx, beta, y):
-        _ok("rocblas_dgemv_64", rb.rocblas_dgemv_64(h, trans, m, k, alpha, a, lda, x, incx, beta,
-                                                    y, 1))
-
     gram = []
-    _ok("rocblas_set_stream", rb.rocblas_set_stream(h, ctypes.c_void_p(st)))
-    _ok("rocblas_set_pointer_mode", rb.rocblas_set_pointer_mode(h, _RB_POINTER_DEVICE))
-    try:
-        next_report = 8192
-        for p0 in range(0, n, nb):
-            Np = n - p0
-            nbp = min(nb, Np)
-            for i in range(nbp):
-                gc = p0 + i
-                if i > 0:  # apply the panel's earlier reflectors to column i
-                    gemv(_RB_OP_N, Np - i, i, mone, pa(gc, p0), ld, pw(i, 0), ld, one, pa(gc, gc))
-                    gemv(_RB_OP_N, Np - i, i, mone, pw(i, 0), ld, pa(gc, p0), ld, one, pa(gc, gc))
-                if i == Np - 1:
-                    continue
-                L = Np - i - 1
-                v = pa(gc + 1, gc)
-                _ok("rocsolver_dlarfg_64", rs.rocsolver_dlarfg_64(
-                    h, L, v, pa(p0 + min(i + 2, Np - 1), gc), 1, tau.data_ptr() + 8 * gc))
-                _lib.call("jm_tridiag_take_unit", v, e.data_ptr() + 8 * gc, st)
-                w = pw(i + 1, i)
-                _lib.call("jm_symv_lower", L, pa(gc + 1, gc + 1), ld, v, w,
-                          symv_work.data_ptr(), nwork, st)
-                if i > 0:
-                    gemv(_RB_OP_T, L, i, one, pw(i + 1, 0), ld, v, 1, zero, pw(0, i))
-                    gemv(_RB_OP_N, L, i, mone, pa(gc + 1, p0), ld, pw(0, i), 1, one, w)
-                    gemv(_RB_OP_T, L, i, one, pa(gc + 1, p0), ld, v, 1, zero, pw(0, i))
-                    gemv(_RB_OP_N, L, i, mone, pw(i + 1, 0), ld, pw(0, i), 1, one, w)
-                _lib.call("jm_tridiag_finish", L, tau.data_ptr() + 8 * gc, w, v, st)
-            gram.append(Wt[:nbp, :nbp].t().clone())  # W(0:nbp, 0:nbp), row-major
-            if Np > nbp:  # trailing update A22 -= V W^T + W V^T (lower triangle)
-                N2 = Np - nbp
-                _lib.call("jm_dgemm", 0, 1, N2, N2, nbp, -1.0, pa(p0 + nbp, p0), ld,
-                          pw(nbp, 0), ld, 1.0, pa(p0 + nbp, p0 + nbp), ld, 1, st)
-                _lib.call("jm_dgemm", 0, 1, N2, N2, nbp, -1.0, pw(nbp, 0), ld,
-                          pa(p0 + nbp, p0), ld, 1.0, pa(p0 + nbp, p0 + nbp), ld, 1, st)
-            if progress is not None and p0 + nbp >= next_report and Np > nbp:
-                torch.cuda.current_stream().synchronize()
-                progress(f"tridiagonal reduction: {p0 + nbp} / {n} columns")
-                next_report += 8192
-    finally:
-        rb.rocblas_set_pointer_mode(h, _RB_POINTER_HOST)
+    next_report = 8192
+    for p0 in range(0, n, nb):
+        Np = n - p0
+        nbp = min(nb, Np)
+        for i in range(nbp):
+            gc = p0 + i
+            if i > 0:  # apply the panel's earlier reflectors to column i
+                _lib.call("jm_tridiag_panel_update", Np - i, i, pa(gc, p0), ld, pw(i, 0), ld,
+                          pa(gc, gc), st)
+            if i == Np - 1:
+                continue
+            L = Np - i - 1
+            v = pa(gc + 1, gc)
+            _lib.call("jm_tridiag_larfg", L, v, pa(gc + 2, gc) if L > 1 else None,
+                      tau.data_ptr() + 8 * gc, e.data_ptr() + 8 * gc, st)
+            w = pw(i + 1, i)
+            _lib.call("jm_symv_lower", L, pa(gc + 1, gc + 1), ld, v, w, symv_work.data_ptr(),
+                      nwork, st)
+            if i > 0:  # w -= V (W^T v) + W (V^T v); V^T v kept in W(0:i, i)
+                _lib.call("jm_tridiag_panel_step", L, i, pa(gc + 1, p0), ld, pw(i + 1, 0), ld,
+                          v, w, pw(0, i), panel_work.data_ptr(), npw, st)
+            _lib.call("jm_tridiag_finish", L, tau.data_ptr() + 8 * gc, w, v, st)
+        gram.append(Wt[:nbp, :nbp].t().clone())  # W(0:nbp, 0:nbp), row-major
+        if Np > nbp:  # trailing update A22 -= V W^T + W V^T (lower triangle)
+            N2 = Np - nbp
+            _lib.call("jm_dgemm", 0, 1, N2, N2, nbp, -1.0, pa(p0 + nbp, p0), ld,
+                      pw(nbp, 0), ld, 1.0, pa(p0 + nbp, p0 + nbp), ld, 1, st)
+            _lib.call("jm_dgemm", 0, 1, N2, N2, nbp, -1.0, pw(nbp, 0), ld,
+                      pa(p0 + nbp, p0), ld, 1.0, pa(p0 + nbp, p0 + nbp), ld, 1, st)
+        if progress is not None and p0 + nbp >= next_report and Np > nbp:
+            torch.cuda.current_stream().synchronize()
+            progress(f"tridiagonal reduction: {p0 + nbp} / {n} columns")
+            next_report += 8192
     d = A.diagonal().clone()
     return d, e[: max(n - 1, 0)], tau, gram
 
