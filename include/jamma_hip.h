@@ -340,6 +340,13 @@ int64_t jm_symv_work_doubles(int64_t N);
 int jm_symv_lower(int64_t N, const double* A, int64_t lda, const double* x, double* y,
                   double* work, int64_t work_doubles, void* stream);
 
+/* Two products in one pass: C = alpha (op(A1) op(B1) + op(A2) op(B2)) + beta C, both
+ * pairs of the same shapes and leading dimensions (the rank-2k trailing update
+ * A22 -= V W^T + W V^T reads and writes C once). */
+int jm_dgemm2(int trans_a, int trans_b, int64_t m, int64_t n, int64_t k, double alpha,
+              const double* A1, const double* B1, const double* A2, const double* B2, int64_t lda,
+              int64_t ldb, double beta, double* C, int64_t ldc, int lower_only, void* stream);
+
 /* jm_dgemm with K split over `splits` block rows (1..64) whose partial products
  * go to work (splits * m * n doubles, column-major m x n each) and are then summed
  * in a fixed order: C = alpha sum + beta C. For outputs too small to fill the

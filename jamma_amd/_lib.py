@@ -97,6 +97,9 @@ _SIGS = {
     "jm_symv_work_doubles": ([c_int64], c_int64),
     "jm_symv_lower": ([c_int64, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
                        c_void_p], c_int),
+    "jm_dgemm2": ([c_int, c_int, c_int64, c_int64, c_int64, c_double, c_void_p, c_void_p,
+                   c_void_p, c_void_p, c_int64, c_int64, c_double, c_void_p, c_int64, c_int,
+                   c_void_p], c_int),
     "jm_dgemm_splitk": ([c_int, c_int, c_int64, c_int64, c_int64, c_double, c_void_p, c_int64,
                          c_void_p, c_int64, c_double, c_void_p, c_int64, c_int, c_void_p, c_int64,
                          c_void_p], c_int),

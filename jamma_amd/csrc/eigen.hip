@@ -44,25 +44,35 @@ constexpr int DG_TI = 64, DG_TJ = 64, DG_BK = 32, DG_LS = DG_BK + 1;
 // Global -> register loads of one K step: op(A) tile 64 x 32 and op(B) tile
 // 32 x 64, 8 doubles each per thread, with consecutive lanes on the operand's
 // contiguous dimension.
+// K indices from k1 on belong to the second operand pair (A2, B2) at kk - k1
+// (k1 >= k: one pair).
 template <bool TA, bool TB>
-__device__ __forceinline__ void dgemm_fetch(const double* __restrict__ A, int64_t lda,
-                                            const double* __restrict__ B, int64_t ldb,
-                                            int64_t m, int64_t n, int64_t i0, int64_t j0,
-                                            int64_t k0, int64_t kend, int tid, double (&ra)[8],
-                                            double (&rb)[8]) {
+__device__ __forceinline__ void dgemm_fetch(const double* __restrict__ A,
+                                            const double* __restrict__ A2, int64_t lda,
+                                            const double* __restrict__ B,
+                                            const double* __restrict__ B2, int64_t ldb,
+                                            int64_t k1, int64_t m, int64_t n, int64_t i0,
+                                            int64_t j0, int64_t k0, int64_t kend, int tid,
+                                            double (&ra)[8], double (&rb)[8]) {
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     const int idx = tid + 256 * u;
     const int r = TA ? (idx >> 5) : (idx & 63), c = TA ? (idx & 31) : (idx >> 6);
     const int64_t i = i0 + r, kk = k0 + c;
-    ra[u] = (i < m && kk < kend) ? (TA ? A[kk + i * lda] : A[i + kk * lda]) : 0.0;
+    const bool two = kk >= k1;
+    const double* Ab = two ? A2 : A;
+    const int64_t kq = two ? kk - k1 : kk;
+    ra[u] = (i < m && kk < kend) ? (TA ? Ab[kq + i * lda] : Ab[i + kq * lda]) : 0.0;
   }
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     const int idx = tid + 256 * u;
     const int r = TB ? (idx & 63) : (idx >> 5), c = TB ? (idx >> 6) : (idx & 31);
     const int64_t j = j0 + r, kk = k0 + c;
-    rb[u] = (j < n && kk < kend) ? (TB ? B[j + kk * ldb] : B[kk + j * ldb]) : 0.0;
+    const bool two = kk >= k1;
+    const double* Bb = two ? B2 : B;
+    const int64_t kq = two ? kk - k1 : kk;
+    rb[u] = (j < n && kk < kend) ? (TB ? Bb[j + kq * ldb] : Bb[kq + j * ldb]) : 0.0;
   }
 }
 
@@ -89,7 +99,8 @@ __global__ void __launch_bounds__(256) k_dgemm(int64_t m, int64_t n, int64_t k, 
                                                const double* __restrict__ B, int64_t ldb,
                                                double beta, double* __restrict__ C, int64_t ldc,
                                                int lower_only, int64_t nti, int64_t kchunk,
-                                               int64_t cstride) {
+                                               int64_t cstride, const double* __restrict__ A2,
+                                               const double* __restrict__ B2, int64_t k1) {
   __shared__ double As[DG_TI * DG_LS];  // op(A)(i, kk) at [i][kk]
   __shared__ double Bs[DG_TJ * DG_LS];  // op(B)(kk, j) at [j][kk]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -108,12 +119,14 @@ __global__ void __launch_bounds__(256) k_dgemm(int64_t m, int64_t n, int64_t k, 
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = (d4e_t){0.0, 0.0, 0.0, 0.0};
   double ra[8], rb[8];
-  if (kbeg < kend) dgemm_fetch<TA, TB>(A, lda, B, ldb, m, n, i0, j0, kbeg, kend, tid, ra, rb);
+  if (kbeg < kend)
+    dgemm_fetch<TA, TB>(A, A2, lda, B, B2, ldb, k1, m, n, i0, j0, kbeg, kend, tid, ra, rb);
   for (int64_t k0 = kbeg; k0 < kend; k0 += DG_BK) {
     dgemm_stash<TA, TB>(As, Bs, tid, ra, rb);
     __syncthreads();
     if (k0 + DG_BK < kend)  // the next step's loads fly while this step multiplies
-      dgemm_fetch<TA, TB>(A, lda, B, ldb, m, n, i0, j0, k0 + DG_BK, kend, tid, ra, rb);
+      dgemm_fetch<TA, TB>(A, A2, lda, B, B2, ldb, k1, m, n, i0, j0, k0 + DG_BK, kend, tid, ra,
+                          rb);
 #pragma unroll
     for (int kk = 0; kk < DG_BK; kk += 4) {
       double bi[2], aj[2];
@@ -621,22 +634,23 @@ using namespace jm;
 static int launch_dgemm(int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha,
                         const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                         double* C, int64_t ldc, int lo, int splits, int64_t kchunk,
-                        int64_t cstride, hipStream_t st) {
+                        int64_t cstride, hipStream_t st, const double* A2 = nullptr,
+                        const double* B2 = nullptr, int64_t k1 = INT64_MAX) {
   const int64_t nti = (m + DG_TI - 1) / DG_TI, ntj = (n + DG_TJ - 1) / DG_TJ;
   JM_CHECK_ARG(nti * ntj < (int64_t)1 << 31, "grid too large");
   const dim3 grid((unsigned)(nti * ntj), (unsigned)splits), block(256);
   if (!ta && !tb)
     hipLaunchKernelGGL((k_dgemm<false, false>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, lo, nti, kchunk, cstride);
+                       beta, C, ldc, lo, nti, kchunk, cstride, A2, B2, k1);
   else if (!ta && tb)
     hipLaunchKernelGGL((k_dgemm<false, true>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, lo, nti, kchunk, cstride);
+                       beta, C, ldc, lo, nti, kchunk, cstride, A2, B2, k1);
   else if (ta && !tb)
     hipLaunchKernelGGL((k_dgemm<true, false>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, lo, nti, kchunk, cstride);
+                       beta, C, ldc, lo, nti, kchunk, cstride, A2, B2, k1);
   else
     hipLaunchKernelGGL((k_dgemm<true, true>), grid, block, 0, st, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, lo, nti, kchunk, cstride);
+                       beta, C, ldc, lo, nti, kchunk, cstride, A2, B2, k1);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }
@@ -652,6 +666,20 @@ extern "C" int jm_dgemm(int trans_a, int trans_b, int64_t m, int64_t n, int64_t 
   JM_CHECK_ARG(ldc >= m, "ldc < m");
   return launch_dgemm(trans_a, trans_b, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
                       lower_only ? 1 : 0, 1, k, 0, (hipStream_t)stream);
+}
+
+extern "C" int jm_dgemm2(int trans_a, int trans_b, int64_t m, int64_t n, int64_t k, double alpha,
+                         const double* A1, const double* B1, const double* A2, const double* B2,
+                         int64_t lda, int64_t ldb, double beta, double* C, int64_t ldc,
+                         int lower_only, void* stream) {
+  JM_CHECK_ARG(m >= 0 && n >= 0 && k >= 0, "bad sizes");
+  if (m == 0 || n == 0) return JM_OK;
+  JM_CHECK_ARG(C && (k == 0 || (A1 && B1 && A2 && B2)), "null pointer");
+  JM_CHECK_ARG(lda >= (trans_a ? k : m) && lda >= 1, "lda too small");
+  JM_CHECK_ARG(ldb >= (trans_b ? n : k) && ldb >= 1, "ldb too small");
+  JM_CHECK_ARG(ldc >= m, "ldc < m");
+  return launch_dgemm(trans_a, trans_b, m, n, 2 * k, alpha, A1, lda, B1, ldb, beta, C, ldc,
+                      lower_only ? 1 : 0, 1, 2 * k, 0, (hipStream_t)stream, A2, B2, k);
 }
 
 extern "C" int jm_dgemm_splitk(int trans_a, int trans_b, int64_t m, int64_t n, int64_t k,

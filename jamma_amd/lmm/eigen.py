@@ -188,10 +188,9 @@ def _tridiagonalize(A: torch.Tensor, nb: int, progress=None):
         gram.append(Wt[:nbp, :nbp].t().clone())  # W(0:nbp, 0:nbp), row-major
         if Np > nbp:  # trailing update A22 -= V W^T + W V^T (lower triangle)
             N2 = Np - nbp
-            _lib.call("jm_dgemm", 0, 1, N2, N2, nbp, -1.0, pa(p0 + nbp, p0), ld,
-                      pw(nbp, 0), ld, 1.0, pa(p0 + nbp, p0 + nbp), ld, 1, st)
-            _lib.call("jm_dgemm", 0, 1, N2, N2, nbp, -1.0, pw(nbp, 0), ld,
-                      pa(p0 + nbp, p0), ld, 1.0, pa(p0 + nbp, p0 + nbp), ld, 1, st)
+            _lib.call("jm_dgemm2", 0, 1, N2, N2, nbp, -1.0, pa(p0 + nbp, p0), pw(nbp, 0),
+                      pw(nbp, 0), pa(p0 + nbp, p0), ld, ld, 1.0, pa(p0 + nbp, p0 + nbp), ld, 1,
+                      st)
         if progress is not None and p0 + nbp >= next_report and Np > nbp:
             torch.cuda.current_stream().synchronize()
             progress(f"tridiagonal reduction: {p0 + nbp} / {n} columns")

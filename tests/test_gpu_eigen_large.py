@@ -195,3 +195,21 @@ def test_dgemm_splitk_matches(splits):
               Cd.data_ptr(), m, splits, wk.data_ptr(), wk.numel(), stream_ptr())
     got = Cd.cpu().numpy().T
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref)) * np.sqrt(k)
+
+
+def test_dgemm2_two_pairs_in_one_pass():
+    from jamma_amd import _lib
+    from jamma_amd.device import stream_ptr
+
+    rng = np.random.default_rng(12)
+    n, k = 333, 64
+    A, W = rng.normal(size=(n, k)), rng.normal(size=(n, k))
+    C0 = rng.normal(size=(n, n))
+    ref = C0 - (A @ W.T + W @ A.T)
+    Ad = torch.as_tensor(A.T.copy(), device="cuda")
+    Wd = torch.as_tensor(W.T.copy(), device="cuda")
+    Cd = torch.as_tensor(C0.T.copy(), device="cuda")
+    _lib.call("jm_dgemm2", 0, 1, n, n, k, -1.0, Ad.data_ptr(), Wd.data_ptr(), Wd.data_ptr(),
+              Ad.data_ptr(), n, n, 1.0, Cd.data_ptr(), n, 0, stream_ptr())
+    got = Cd.cpu().numpy().T
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
