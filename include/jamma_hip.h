@@ -367,7 +367,7 @@ int jm_tridiag_larfg(int64_t len, double* alpha, double* x, double* tau, double*
 int jm_tridiag_panel_update(int64_t len, int i, const double* a0, int64_t lda, const double* w0,
                             int64_t ldw, double* y, void* stream);
 
-/* dlatrd's corrections after the symmetric product w = A22 v (len rows, i <= 64
+/* dlatrd's corrections after the symmetric product w = A22 v (len rows, i < 128
  * panel columns ap (lda) and wp (ldw)): t1 = wp^T v, t2 = ap^T v,
  * w -= ap t1 + wp t2, t2 stored to gram_col (i values: the panel's V^T v). work:
  * jm_tridiag_panel_work_doubles(len) doubles; block partial sums added in a

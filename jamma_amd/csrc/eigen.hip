@@ -177,6 +177,8 @@ __global__ void __launch_bounds__(256) k_splitk_sum(int64_t m, int64_t n, int sp
 // ---------------------------------------------------------------------------
 // Tridiagonal reduction helpers
 // ---------------------------------------------------------------------------
+constexpr int PANEL_MAX = 128;  // reflectors per panel (panel column index i < PANEL_MAX)
+
 // dlatrd's column update with the panel's earlier reflectors, one kernel for the
 // two matrix-vector products: y[r] -= sum_{c < i} a0[r + c lda] xw[c] + w0[r + c ldw] xa[c]
 // (r < len), where xw[c] = w0[c ldw] and xa[c] = a0[c lda] are the panels' first rows
@@ -185,7 +187,7 @@ __global__ void __launch_bounds__(256) k_panel_update(int64_t len, int i,
                                                       const double* __restrict__ a0, int64_t lda,
                                                       const double* __restrict__ w0, int64_t ldw,
                                                       double* __restrict__ y) {
-  __shared__ double xs[2][64];
+  __shared__ double xs[2][PANEL_MAX];
   if (threadIdx.x < (unsigned)i) {
     xs[0][threadIdx.x] = w0[(int64_t)threadIdx.x * ldw];
     xs[1][threadIdx.x] = a0[(int64_t)threadIdx.x * lda];
@@ -193,9 +195,13 @@ __global__ void __launch_bounds__(256) k_panel_update(int64_t len, int i,
   __syncthreads();
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < len;
        r += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int c = 0; c < i; ++c) s += a0[r + c * lda] * xs[0][c] + w0[r + c * ldw] * xs[1][c];
-    y[r] -= s;
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll 8
+    for (int c = 0; c < i; ++c) {
+      s0 += a0[r + c * lda] * xs[0][c];
+      s1 += w0[r + c * ldw] * xs[1][c];
+    }
+    y[r] -= s0 + s1;
   }
 }
 
@@ -222,7 +228,7 @@ __global__ void __launch_bounds__(256) k_panel_dots(int64_t len, int i,
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0)
-    part[(int64_t)blockIdx.x * 128 + q] = (red[0] + red[1]) + (red[2] + red[3]);
+    part[(int64_t)blockIdx.x * 2 * PANEL_MAX + q] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // t1 = sum of the W-panel partials, t2 = sum of the A-panel partials (block order);
@@ -234,19 +240,23 @@ __global__ void __launch_bounds__(256) k_panel_apply(int64_t len, int i, int nbl
                                                      const double* __restrict__ part,
                                                      double* __restrict__ gram_col,
                                                      double* __restrict__ w) {
-  __shared__ double ts[128];
+  __shared__ double ts[2 * PANEL_MAX];
   if (threadIdx.x < (unsigned)(2 * i)) {
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * 128 + threadIdx.x];
+    for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * 2 * PANEL_MAX + threadIdx.x];
     ts[threadIdx.x] = s;
     if (blockIdx.x == 0 && threadIdx.x >= (unsigned)i) gram_col[threadIdx.x - i] = s;
   }
   __syncthreads();
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < len;
        r += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int c = 0; c < i; ++c) s += ap[r + c * lda] * ts[c] + wp[r + c * ldw] * ts[i + c];
-    w[r] -= s;
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll 8
+    for (int c = 0; c < i; ++c) {
+      s0 += ap[r + c * lda] * ts[c];
+      s1 += wp[r + c * ldw] * ts[i + c];
+    }
+    w[r] -= s0 + s1;
   }
 }
 
@@ -753,12 +763,12 @@ extern "C" int jm_tridiag_larfg(int64_t len, double* alpha, double* x, double* t
 }
 
 extern "C" int64_t jm_tridiag_panel_work_doubles(int64_t N) {
-  return ((N + PD_ROWS - 1) / PD_ROWS + 1) * 128;
+  return ((N + PD_ROWS - 1) / PD_ROWS + 1) * 2 * PANEL_MAX;
 }
 
 extern "C" int jm_tridiag_panel_update(int64_t len, int i, const double* a0, int64_t lda,
                                        const double* w0, int64_t ldw, double* y, void* stream) {
-  JM_CHECK_ARG(len >= 0 && i >= 0 && i <= 64, "bad sizes");
+  JM_CHECK_ARG(len >= 0 && i >= 0 && i < PANEL_MAX, "bad sizes");
   if (len == 0 || i == 0) return JM_OK;
   JM_CHECK_ARG(a0 && w0 && y, "null pointer");
   const unsigned grid = (unsigned)((len + 255) / 256 < 1024 ? (len + 255) / 256 : 1024);
@@ -772,11 +782,11 @@ extern "C" int jm_tridiag_panel_step(int64_t len, int i, const double* ap, int64
                                      const double* wp, int64_t ldw, const double* v, double* w,
                                      double* gram_col, double* work, int64_t work_doubles,
                                      void* stream) {
-  JM_CHECK_ARG(len >= 0 && i >= 0 && i <= 64, "bad sizes");
+  JM_CHECK_ARG(len >= 0 && i >= 0 && i < PANEL_MAX, "bad sizes");
   if (len == 0 || i == 0) return JM_OK;
   JM_CHECK_ARG(ap && wp && v && w && gram_col && work, "null pointer");
   const int64_t nblk = (len + PD_ROWS - 1) / PD_ROWS;
-  JM_CHECK_ARG(work_doubles >= nblk * 128, "workspace too small");
+  JM_CHECK_ARG(work_doubles >= nblk * 2 * PANEL_MAX, "workspace too small");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_panel_dots, dim3((unsigned)nblk, (unsigned)(2 * i)), dim3(256), 0, st, len,
                      i, ap, lda, wp, ldw, v, work);

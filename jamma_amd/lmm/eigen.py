@@ -51,7 +51,7 @@ _LIB_MAX_N = 46_340          # largest n with n * n <= 2^31 - 1
 # matrices: n = 8,192 0.65 s vs rocSOLVER 0.93 s; 20,000 4.8 s vs 7.6 s; 30,000 14.3 s
 # vs 22.9 s; eigenvalues equal to 1e-15, residuals and orthogonality alike).
 _OWN_MIN_N = 8192
-_LARGE_PANEL = 64            # reflectors per panel / per compact-WY block
+_LARGE_PANEL = 128           # reflectors per panel / per compact-WY block
 _LARGE_LEAF = 30_000         # largest tridiagonal block handed to rocSOLVER dstedc
 _SOLVER = {}
 

@@ -39,7 +39,8 @@ def _kinship(n, p, seed):
     return X @ X.T / p
 
 
-@pytest.mark.parametrize("n,nb,leaf", [(2, 4, 1), (37, 8, 8), (300, 32, 50), (1031, 64, 200)])
+@pytest.mark.parametrize("n,nb,leaf", [(2, 4, 1), (37, 8, 8), (300, 32, 50), (1031, 64, 200),
+                                       (1500, 128, 400)])
 def test_large_path_matches_lapack(n, nb, leaf):
     from jamma_amd.lmm.eigen import _eigh_large
 
