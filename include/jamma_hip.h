@@ -151,7 +151,8 @@ int jm_gemm_nt_f64(const double* A, int64_t lda, const double* B, int64_t ldb, i
  * kilobyte:
  *   jm_pack_ut_bf16x3: UT (float64, n x ld_u) -> planes
  *     (jm_packed_ut_bytes(n, nplanes) bytes, 16-byte aligned), once per cohort;
- *   jm_pack_genotypes: SNP rows r(s) of g (col_idx as in jm_rotate_f64) and
+ *   jm_pack_genotypes: SNP rows r(s) of g (col_idx as in jm_rotate_f64; rows
+ *     8-byte aligned: g and ld_g multiples of 8) and
  *     their means -> gp (jm_packed_geno_bytes(n, n_snps) bytes) and mm
  *     (n_snps x uint32: bf16(mean) | bf16 residual << 16), per SNP tile;
  *   jm_rotate_bf16x3: Z (float32, n x ldz) from planes, gp, mm.

@@ -318,28 +318,31 @@ __global__ void k_pack_geno(const int8_t* __restrict__ g, int64_t ld_g,
     const int c = cs ^ ((j >> 2) & 3);
     const int64_t B = blk / nks, sk = blk % nks;
     const int64_t snp = B * 256 + j, k0 = sk * 32 + 8 * c;
-    uint8_t v[16];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      v[2 * e] = 0;
-      v[2 * e + 1] = 4;
-    }
+    // selector pairs (code, code + 4) of 8 genotypes; padding reads as code 0
+    uint64_t codes = 0;  // byte e = code of sample k0 + e
     if (snp < n_snps) {
       const int8_t* row = g + (col ? (int64_t)col[snp] : snp) * ld_g;
+      if (k0 + 8 <= n) {  // rows are 16-byte aligned and k0 % 8 == 0: one 8-byte load
+        codes = *reinterpret_cast<const uint64_t*>(row + k0) & 0x0303030303030303ull;
+      } else {
 #pragma unroll
-      for (int e = 0; e < 8; ++e)
-        if (k0 + e < n) {
-          const uint8_t code = (uint8_t)row[k0 + e] & 3u;
-          v[2 * e] = code;
-          v[2 * e + 1] = (uint8_t)(code + 4);
-        }
+        for (int e = 0; e < 8; ++e)
+          if (k0 + e < n) codes |= (uint64_t)((uint8_t)row[k0 + e] & 3u) << (8 * e);
+      }
       if (sk == 0 && c == 0) {
         const double mu = mean[col ? (int64_t)col[snp] : snp];
         const uint16_t h1 = f32_to_bf16_rne((float)mu);
         mm[snp] = (uint32_t)h1 | ((uint32_t)f32_to_bf16_rne((float)(mu - bf16_to_f64(h1))) << 16);
       }
     }
-    memcpy(gp + q * 16, v, 16);
+    uint32_t w[4];  // dword h: bytes (c_2h, c_2h + 4, c_2h+1, c_2h+1 + 4)
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const uint32_t two = (uint32_t)(codes >> (16 * h)) & 0x0303u;  // codes of samples 2h, 2h+1
+      const uint32_t spread = (two & 0x3u) | ((two & 0x300u) << 8);  // bytes 0 and 2
+      w[h] = spread | ((spread + 0x00040004u) << 8);
+    }
+    memcpy(gp + q * 16, w, 16);
   }
 }
 
@@ -533,6 +536,7 @@ extern "C" int jm_pack_genotypes(const int8_t* g, int64_t ld_g, const int32_t* c
   JM_CHECK_ARG(n >= 1 && n_snps >= 0 && ld_g >= n, "bad sizes");
   if (n_snps == 0) return JM_OK;
   JM_CHECK_ARG(g && mean && gp && mm && ((uintptr_t)gp % 16) == 0, "null or misaligned pointer");
+  JM_CHECK_ARG(ld_g % 8 == 0 && ((uintptr_t)g % 8) == 0, "genotype rows must be 8-byte aligned");
   const int64_t nks = (n + 31) / 32;
   const int64_t chunks = ((n_snps + 255) / 256) * nks * 512;
   const unsigned grid = (unsigned)(chunks < 65536 ? (chunks + 255) / 256 : 256 * 64);
