@@ -219,53 +219,65 @@ def _stedc(d: np.ndarray, e: np.ndarray, device):
     return D.cpu().numpy(), C
 
 
-def _rank_one_merge(dc: np.ndarray, z: np.ndarray, rho: float, device):
-    """Eigen-pairs of diag(dc) + rho z z^T (rho >= 0, |z| = sqrt 2 as torn): (w ascending
-    host, M device (n, n) row-major with M[a, c] = component a of eigenvector c).
+def _deflate(dc: np.ndarray, z: np.ndarray, rho: float):
+    """Deflation of diag(dc) + rho z z^T as LAPACK dlaed2 (host, O(n)).
 
-    Deflation follows LAPACK dlaed2: z is scaled to unit norm (rho doubled); a pair
-    deflates when rho |z_k| <= tol, or by a Givens rotation when two poles are closer
-    than tol allows (|t c s| <= tol), tol = 8 eps max(max|d|, max|z|).
+    z is scaled to unit norm and rho doubled (|z| = sqrt 2 as torn); a pole deflates
+    when rho |z_k| <= tol, or, when it is within tol of the previous candidate pole
+    (|t c s| <= tol), by a Givens rotation (c, s) of the two basis vectors that zeroes
+    its weight; tol = 8 eps max(max|d|, max|z|). Returns (order, ds, zs, nd, defl_val,
+    defl_row, rots, rho): the sort order of dc, the updated sorted poles and weights,
+    the sorted positions that stay in the secular equation, the deflated eigenvalues
+    and their basis rows, and the rotations (row p, row q, c, s) in the order applied.
     """
-    n = len(dc)
-    f64 = torch.float64
     z = z / np.sqrt(2.0)
     rho = 2.0 * rho
     order = np.argsort(dc, kind="stable")
     ds = dc[order].astype(np.float64).tolist()
     zs = z[order].astype(np.float64).tolist()
+    n = len(ds)
     tol = 8.0 * (np.finfo(np.float64).eps / 2) * max(float(np.max(np.abs(dc))),
                                                       float(np.max(np.abs(z))))
-    M = torch.zeros((n, n), dtype=f64, device=device)
     nd, defl_val, defl_row, rots = [], [], [], []
     if rho * float(np.max(np.abs(z))) <= tol:
-        defl_val, defl_row = ds, order.tolist()
-    else:
-        pj = -1
-        for k in range(n):
-            if rho * abs(zs[k]) <= tol:
-                defl_val.append(ds[k])
-                defl_row.append(int(order[k]))
-                continue
-            if pj < 0:
-                pj = k
-                continue
-            s_, c_ = zs[pj], zs[k]
-            tau = float(np.hypot(c_, s_))
-            t = ds[k] - ds[pj]
-            c_, s_ = c_ / tau, -s_ / tau
-            if abs(t * c_ * s_) <= tol:
-                zs[k], zs[pj] = tau, 0.0
-                rots.append((int(order[pj]), int(order[k]), c_, s_))
-                dp = ds[pj] * c_ * c_ + ds[k] * s_ * s_
-                ds[k] = ds[pj] * s_ * s_ + ds[k] * c_ * c_
-                defl_val.append(dp)
-                defl_row.append(int(order[pj]))
-            else:
-                nd.append(pj)
+        return order, ds, zs, nd, list(ds), order.tolist(), rots, rho
+    pj = -1
+    for k in range(n):
+        if rho * abs(zs[k]) <= tol:
+            defl_val.append(ds[k])
+            defl_row.append(int(order[k]))
+            continue
+        if pj < 0:
             pj = k
-        if pj >= 0:
+            continue
+        s_, c_ = zs[pj], zs[k]
+        tau = float(np.hypot(c_, s_))
+        t = ds[k] - ds[pj]
+        c_, s_ = c_ / tau, -s_ / tau
+        if abs(t * c_ * s_) <= tol:
+            zs[k], zs[pj] = tau, 0.0
+            rots.append((int(order[pj]), int(order[k]), c_, s_))
+            dp = ds[pj] * c_ * c_ + ds[k] * s_ * s_
+            ds[k] = ds[pj] * s_ * s_ + ds[k] * c_ * c_
+            defl_val.append(dp)
+            defl_row.append(int(order[pj]))
+        else:
             nd.append(pj)
+        pj = k
+    if pj >= 0:
+        nd.append(pj)
+    return order, ds, zs, nd, defl_val, defl_row, rots, rho
+
+
+def _rank_one_merge(dc: np.ndarray, z: np.ndarray, rho: float, device):
+    """Eigen-pairs of diag(dc) + rho z z^T (rho >= 0, |z| = sqrt 2 as torn): (w ascending
+    host, M device (n, n) row-major with M[a, c] = component a of eigenvector c).
+    Deflation on the host (_deflate), the secular equation and the eigenvectors of the
+    rest on the device, then the deflation rotations applied to M's rows."""
+    n = len(dc)
+    f64 = torch.float64
+    order, ds, zs, nd, defl_val, defl_row, rots, rho = _deflate(dc, z, rho)
+    M = torch.zeros((n, n), dtype=f64, device=device)
     K = len(nd)
     st = stream_ptr()
     lam = np.zeros(0)

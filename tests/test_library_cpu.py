@@ -259,3 +259,54 @@ def test_library_eigen_limit_is_the_32_bit_bound():
 
     assert eg._LIB_MAX_N ** 2 <= 2**31 - 1 < (eg._LIB_MAX_N + 1) ** 2
     assert eg._LARGE_LEAF <= eg._LIB_MAX_N
+
+
+def _merge_case(kind, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "coincident":  # every pole twice (mirror-image halves): Givens deflation
+        d1 = np.sort(rng.normal(size=40))
+        dc = np.concatenate([d1, d1])
+    elif kind == "tiny-weights":
+        dc = np.concatenate([np.sort(rng.normal(size=30)), np.sort(rng.normal(size=30))])
+    else:
+        dc = np.concatenate([np.sort(rng.normal(size=25)), np.sort(rng.normal(size=35))])
+    z = rng.normal(size=len(dc))
+    if kind == "tiny-weights":
+        z[rng.choice(len(z), 12, replace=False)] = 1e-19
+    z *= np.sqrt(2.0) / np.linalg.norm(z)  # torn weights have |z| = sqrt 2
+    return dc, z, 0.7
+
+
+@pytest.mark.parametrize("kind", ["generic", "coincident", "tiny-weights"])
+def test_merge_deflation_keeps_the_spectrum(kind):
+    """dlaed2's deflation: the deflated values plus the spectrum of the reduced secular
+    problem are the spectrum of D + rho z z^T, and in the rotated basis the deflated
+    directions decouple (the convention jm_apply_rotations applies to the merge matrix)."""
+    from jamma_amd.lmm.eigen import _deflate
+
+    dc, z, rho = _merge_case(kind, 3)
+    Mfull = np.diag(dc) + rho * np.outer(z, z)
+    order, ds, zs, nd, dval, drow, rots, rho2 = _deflate(dc, z, rho)
+    assert len(nd) + len(dval) == len(dc)
+    if kind == "coincident":
+        assert len(rots) > 0
+    if kind == "tiny-weights":
+        assert len(dval) >= 12
+    dl = np.array([ds[k] for k in nd])
+    w = np.array([zs[k] for k in nd])
+    assert np.all(np.diff(dl) > 0)  # strictly ascending poles for the secular solver
+    red = np.linalg.eigvalsh(np.diag(dl) + rho2 * np.outer(w, w)) if len(nd) else np.zeros(0)
+    got = np.sort(np.concatenate([red, dval]))
+    ref = np.linalg.eigvalsh(Mfull)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+    # rotated basis: Q = G_1 G_2 ... (columns p, q of each G mixed by (c, s))
+    Q = np.eye(len(dc))
+    for p, q, c, s in rots:
+        qp, qq = Q[:, p].copy(), Q[:, q].copy()
+        Q[:, p], Q[:, q] = c * qp + s * qq, -s * qp + c * qq
+    R = Q.T @ Mfull @ Q
+    for v, r in zip(dval, drow):
+        off = np.abs(R[r]).copy()
+        off[r] = 0.0
+        assert abs(R[r, r] - v) <= 1e-13 * np.max(np.abs(ref))
+        assert np.max(off) <= 1e-13 * np.max(np.abs(ref))
