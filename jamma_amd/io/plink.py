@@ -10,6 +10,7 @@ plus per-SNP genotype counts).
 
 from __future__ import annotations
 
+import warnings
 from pathlib import Path
 from typing import Any
 
@@ -101,7 +102,9 @@ class BedFile:
     def window_to_device(self, start: int, end: int) -> torch.Tensor:
         """Raw bytes of SNPs [start, end) as a device uint8 tensor."""
         lo, hi = start * self.bytes_per_snp, end * self.bytes_per_snp
-        host = torch.from_numpy(np.ascontiguousarray(self._mm[lo:hi]))
+        with warnings.catch_warnings():  # the read-only memory map is only read (copied)
+            warnings.simplefilter("ignore", UserWarning)
+            host = torch.from_numpy(np.ascontiguousarray(self._mm[lo:hi]))
         return host.to(require_gpu(), non_blocking=False)
 
 
