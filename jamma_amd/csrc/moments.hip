@@ -26,14 +26,116 @@ constexpr int kMomUnroll = 4;     // eigen-indices whose Z loads are in flight t
 
 typedef const double __attribute__((address_space(4))) mom_cdbl;
 
-// Partial moments of one cluster piece (grid.y) for 64 SNP columns (grid.x):
-//   work[s][b][seg_slot + m] = sum_{i in piece} Z[i][s] V[b][i] delta_i^m
-// (row b == nv: Z[i][s]^2). Pieces hold at most a fixed number of
-// eigen-indices, so the blocks carry balanced work however the spectrum
-// clusters. The summation order inside a piece is fixed (i ascending).
-// Single problem: V and delta are the same for every lane and come through
-// scalar loads; the Z loads of kMomUnroll consecutive eigen-indices are issued
-// before their products are accumulated.
+// Partial moments of one cluster piece for 64 SNP columns, with the piece's
+// term count M a compile-time constant so that the inner loop is straight-line
+// code:
+//   work[s][b][slot0 + m] = sum_{i in piece} Z[i][s] V[b][i] delta_i^m
+// (row b == nv: Z[i][s]^2). The summation order is fixed (i ascending, one
+// fused multiply-add per (i, b, m)), so every M gives bitwise the same sums
+// as a loop over m would. Single problem: V and delta are the same for every
+// lane and come through scalar loads, issued for kMomUnroll consecutive
+// eigen-indices together with their Z loads.
+template <typename ZT, bool MULTI, int M>
+__device__ __forceinline__ void mom_piece(const ZT* __restrict__ Z, int64_t ldz, int64_t n_snps,
+                                          int64_t sc, const double* V, int64_t ldv, int nv,
+                                          int nvm, const double* __restrict__ delta, int i0,
+                                          int len, int slot0, int n_slots, double* tr,
+                                          double* __restrict__ work) {
+  const mom_cdbl* dls = (const mom_cdbl*)delta;
+  const mom_cdbl* Vs = (const mom_cdbl*)V;
+  const int iend = i0 + len;
+  for (int g0 = 0; g0 <= nvm; g0 += kMomGroup) {
+    // rows of this group present in every column of the block (MULTI: nv
+    // differs per column; missing rows are written as 0 and never read)
+    int ng = nvm + 1 - g0;
+    ng = ng > kMomGroup ? kMomGroup : ng;
+    double acc[kMomGroup][M];
+#pragma unroll
+    for (int b = 0; b < kMomGroup; ++b)
+#pragma unroll
+      for (int m = 0; m < M; ++m) acc[b][m] = 0.0;
+    // one eigen-index: prod_b = Z V_b (or Z^2), acc[b][m] += prod_b delta^m
+    auto step = [&](double z, double dl, const double (&vb)[kMomGroup]) {
+      double prod[kMomGroup];
+#pragma unroll
+      for (int b = 0; b < kMomGroup; ++b) {
+        const int row = g0 + b;
+        prod[b] = (b < ng && row <= nv) ? z * (row < nv ? vb[b] : z) : 0.0;
+      }
+      double pw = 1.0;
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+#pragma unroll
+        for (int b = 0; b < kMomGroup; ++b) acc[b][m] += prod[b] * pw;
+        pw *= dl;
+      }
+    };
+    auto load_v = [&](int row, int i) -> double {
+      if (MULTI) return row < nv ? V[(int64_t)row * ldv + i] : 0.0;
+      return row < nv ? Vs[(int64_t)row * ldv + i] : 0.0;
+    };
+    // Z of the next kMomUnroll eigen-indices is loaded while the current ones
+    // are accumulated
+    int ib = i0;
+    ZT zc[kMomUnroll];
+    if (ib + kMomUnroll <= iend) {
+#pragma unroll
+      for (int u = 0; u < kMomUnroll; ++u) zc[u] = Z[(int64_t)(ib + u) * ldz + sc];
+    }
+    for (; ib + kMomUnroll <= iend; ib += kMomUnroll) {
+      const bool more = ib + 2 * kMomUnroll <= iend;
+      ZT zn[kMomUnroll];
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < kMomUnroll; ++u)
+          zn[u] = Z[(int64_t)(ib + kMomUnroll + u) * ldz + sc];
+      }
+      double zu[kMomUnroll], dl[kMomUnroll], vv[kMomUnroll][kMomGroup];
+#pragma unroll
+      for (int u = 0; u < kMomUnroll; ++u) zu[u] = (double)zc[u];
+#pragma unroll
+      for (int u = 0; u < kMomUnroll; ++u) {
+        dl[u] = MULTI ? delta[ib + u] : dls[ib + u];
+#pragma unroll
+        for (int b = 0; b < kMomGroup; ++b) vv[u][b] = b < ng ? load_v(g0 + b, ib + u) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kMomUnroll; ++u) step(zu[u], dl[u], vv[u]);
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < kMomUnroll; ++u) zc[u] = zn[u];
+      }
+    }
+    for (; ib < iend; ++ib) {
+      double vv[kMomGroup];
+#pragma unroll
+      for (int b = 0; b < kMomGroup; ++b) vv[b] = b < ng ? load_v(g0 + b, ib) : 0.0;
+      step((double)Z[(int64_t)ib * ldz + sc], MULTI ? delta[ib] : dls[ib], vv);
+    }
+    // transpose through LDS, one row at a time, so that the M consecutive slots
+    // of a (column, row) are written by consecutive lanes
+#pragma unroll
+    for (int b = 0; b < kMomGroup; ++b) {
+      if (b < ng) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) tr[m * 65 + threadIdx.x] = acc[b][m];
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * M; e += 64) {
+          const int col = e / M, m = e % M;
+          const int64_t sc2 = (int64_t)blockIdx.x * 64 + col;
+          if (sc2 < n_snps)
+            work[(sc2 * (nvm + 1) + g0 + b) * n_slots + slot0 + m] = tr[m * 65 + col];
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// One block per (64 SNP columns, cluster piece): grid (ceil(n_snps / 64), n_seg).
+// Pieces hold at most a fixed number of eigen-indices, so the blocks carry
+// balanced work however the spectrum clusters; the piece's term count
+// (1..kMomMaxTerms, fixed by the plan) selects the instantiation.
 template <typename ZT, bool MULTI>
 __global__ void __launch_bounds__(64) k_mom_seg(const ZT* __restrict__ Z, int64_t ldz,
                                                 int64_t n_snps, const double* __restrict__ V_one,
@@ -41,8 +143,7 @@ __global__ void __launch_bounds__(64) k_mom_seg(const ZT* __restrict__ Z, int64_
                                                 const jm_mom_col* __restrict__ cols, int nv_max,
                                                 jm_spectrum_plan pl, double* __restrict__ work) {
   const int64_t s = (int64_t)blockIdx.x * 64 + threadIdx.x;
-  const bool active = s < n_snps;
-  const int64_t sc = active ? s : 0;
+  const int64_t sc = s < n_snps ? s : 0;
   const int seg = blockIdx.y;
   const double* V = V_one;
   int64_t ldv = ldv_one;
@@ -57,72 +158,20 @@ __global__ void __launch_bounds__(64) k_mom_seg(const ZT* __restrict__ Z, int64_
   const int i0 = pl.seg_start[seg], len = pl.seg_len[seg];
   const int M = pl.cl_terms[pl.seg_cluster[seg]];
   const int slot0 = pl.seg_slot[seg];
-  const mom_cdbl* dls = (const mom_cdbl*)pl.delta;
-  const mom_cdbl* Vs = (const mom_cdbl*)V_one;
   __shared__ double tr[kMomTerms * 65];  // term x column, padded
-  for (int g0 = 0; g0 <= nvm; g0 += kMomGroup) {
-    // rows of this group present in every column of the block (MULTI: nv
-    // differs per column; missing rows are written as 0 and never read)
-    int ng = nvm + 1 - g0;
-    ng = ng > kMomGroup ? kMomGroup : ng;
-    double acc[kMomGroup][kMomTerms];
-#pragma unroll
-    for (int b = 0; b < kMomGroup; ++b)
-#pragma unroll
-      for (int m = 0; m < kMomTerms; ++m) acc[b][m] = 0.0;
-    for (int ib = i0; ib < i0 + len; ib += kMomUnroll) {
-      double zu[kMomUnroll];
-#pragma unroll
-      for (int u = 0; u < kMomUnroll; ++u) {
-        const int i = min(ib + u, i0 + len - 1);
-        zu[u] = (double)Z[(int64_t)i * ldz + sc];
-      }
-#pragma unroll
-      for (int u = 0; u < kMomUnroll; ++u) {
-        if (ib + u >= i0 + len) break;  // uniform
-        const int i = ib + u;
-        const double z = zu[u];
-        const double dl = MULTI ? pl.delta[i] : dls[i];
-        double prod[kMomGroup];
-#pragma unroll
-        for (int b = 0; b < kMomGroup; ++b) {
-          const int row = g0 + b;
-          prod[b] = 0.0;
-          if (b < ng && row <= nv) {
-            const double vb = row < nv ? (MULTI ? V[(int64_t)row * ldv + i] : Vs[(int64_t)row * ldv + i])
-                                       : z;
-            prod[b] = z * vb;
-          }
-        }
-        double pw = 1.0;
-#pragma unroll
-        for (int m = 0; m < kMomTerms; ++m) {
-          if (m < M) {
-#pragma unroll
-            for (int b = 0; b < kMomGroup; ++b) acc[b][m] += prod[b] * pw;
-            pw *= dl;
-          }
-        }
-      }
-    }
-    // transpose through LDS, one row at a time, so that the M consecutive slots
-    // of a (column, row) are written by consecutive lanes
-#pragma unroll
-    for (int b = 0; b < kMomGroup; ++b) {
-      if (b < ng) {
-#pragma unroll
-        for (int m = 0; m < kMomTerms; ++m) tr[m * 65 + threadIdx.x] = acc[b][m];
-        __syncthreads();
-        for (int e = threadIdx.x; e < 64 * M; e += 64) {
-          const int col = e / M, m = e % M;
-          const int64_t sc2 = (int64_t)blockIdx.x * 64 + col;
-          if (sc2 < n_snps)
-            work[(sc2 * (nvm + 1) + g0 + b) * pl.n_slots + slot0 + m] = tr[m * 65 + col];
-        }
-        __syncthreads();
-      }
-    }
+#define JM_MOM_CASE(k)                                                                       \
+  case k:                                                                                    \
+    mom_piece<ZT, MULTI, k>(Z, ldz, n_snps, sc, V, ldv, nv, nvm, pl.delta, i0, len, slot0,   \
+                            pl.n_slots, tr, work);                                           \
+    break;
+  switch (M) {
+    JM_MOM_CASE(1) JM_MOM_CASE(2) JM_MOM_CASE(3) JM_MOM_CASE(4)
+    JM_MOM_CASE(5) JM_MOM_CASE(6) JM_MOM_CASE(7) JM_MOM_CASE(8)
+    JM_MOM_CASE(9) JM_MOM_CASE(10) JM_MOM_CASE(11) JM_MOM_CASE(12)
+    JM_MOM_CASE(13) JM_MOM_CASE(14) JM_MOM_CASE(15) JM_MOM_CASE(16)
+    default: break;  // the plan caps terms per cluster at kMomMaxTerms
   }
+#undef JM_MOM_CASE
 }
 
 // mom[s][b][t] = sum over the pieces p of the cluster c of term t (in order)
