@@ -27,6 +27,7 @@
 #include "jm_common.h"
 
 #include <cmath>
+#include <cstdlib>
 
 namespace jm {
 
@@ -1246,6 +1247,418 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
   }
 }
 
+// ---------------------------------------------------------------------------
+// Association search with 16 lanes per SNP (single problem, T <= 16 * KT)
+// ---------------------------------------------------------------------------
+// The same evaluations, stages and outputs as k_lmm_assoc, with a SNP's work
+// spread over a 16-lane group (four SNPs per wavefront) instead of a whole
+// wavefront: the serial parts of an evaluation (reductions, the Pab
+// recursion, the logs) are then shared by four SNPs. Lane j of a group owns
+// the KT contiguous terms j*KT .. j*KT+KT-1. An evaluation:
+//   1. h_c = 1 / (1 + lambda d_c) for both lambdas, clusters spread over the
+//      group's lanes, into the group's LDS table;
+//   2. kappa_t = h_c (-lambda h_c)^m and the m >= 1 logdet partials per lane;
+//   3. dot products with the SNP's moment rows (global memory), the logdet
+//      partials and the shared pair rows (block LDS), 8 rows x 2 lambdas at a
+//      time, reduce-scattered over the group into the two packed Pab rows;
+//   4. the Pab recursion, lanes 0-7 on row A (REML), lanes 8-15 on row B (MLE);
+//   5. the logs (pivots, P_yy, clusters) over the 8 lanes of each half.
+// Every SNP of a launch runs the same stage sequence, so the four groups of a
+// wavefront stay in lockstep; a group past n_snps repeats the last SNP and
+// writes nothing.
+constexpr int kGL = 16;  // lanes per SNP
+
+// Reduce-scatter of 16 per-lane values over a 16-lane group: on return lane j
+// holds the group sum of value j & 15 (partners from xpartner<3..0>).
+__device__ __forceinline__ double reduce16(double (&v)[16], int lane) {
+  {
+    const bool up = lane & 8;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const double send = up ? v[i] : v[i + 8];
+      const double keep = up ? v[i + 8] : v[i];
+      v[i] = keep + xpartner<3>(send, lane);
+    }
+  }
+  {
+    const bool up = lane & 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double send = up ? v[i] : v[i + 4];
+      const double keep = up ? v[i + 4] : v[i];
+      v[i] = keep + xpartner<2>(send, lane);
+    }
+  }
+  {
+    const bool up = lane & 2;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const double send = up ? v[i] : v[i + 2];
+      const double keep = up ? v[i + 2] : v[i];
+      v[i] = keep + xpartner<1>(send, lane);
+    }
+  }
+  {
+    const bool up = lane & 1;
+    const double send = up ? v[0] : v[1];
+    const double keep = up ? v[1] : v[0];
+    v[0] = keep + xpartner<0>(send, lane);
+  }
+  return v[0];
+}
+
+// Sum over the 8 lanes of a half-group; every lane of the half gets the same bits.
+__device__ __forceinline__ double oct_sum(double v, int lane) {
+  v += xpartner<0>(v, lane);
+  v += xpartner<1>(v, lane);
+  v += xpartner<2>(v, lane);
+  return v;
+}
+
+// LDS of k_lmm_assoc16 (bytes): block constants, the shared pair rows and 16
+// per-group areas (two packed rows with their logdet slot, Iab level logs and
+// the h table of both lambdas), then the int tables.
+__host__ __device__ constexpr int assoc16_group_doubles(int ni, int C) {
+  return 2 * (ni + 1) + kMaxLevels + 2 * C;
+}
+static size_t assoc16_lds(int KT, int nc, int C) {
+  const int TB = kGL * KT;
+  const int ni = (nc + 3) * (nc + 2) / 2;
+  const int n_sh = (nc + 1) * (nc + 2) / 2;
+  const int nrows = nc + 2;
+  const int NR = nrows + 1 + n_sh;
+  return (size_t)(TB + 2 * C + (size_t)n_sh * TB + 16 * (size_t)assoc16_group_doubles(ni, C)) *
+             sizeof(double) +
+         (size_t)(TB + tet(nc + 1) + NR) * sizeof(int);
+}
+
+template <int KT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_lmm_assoc16(jm_lmm_problem pr, int mode,
+                                                     const double* __restrict__ xmom, int64_t ldm,
+                                                     int64_t n_snps, double* __restrict__ out) {
+  extern __shared__ double smem[];
+  constexpr int TB = kGL * KT;
+  const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
+  const int grp = lane >> 4;
+  const int j = lane & 15;
+  const int h = j >> 3;  // 0: row A (REML, lambda A), 1: row B (MLE, lambda B)
+  const int q8 = j & 7;
+  const int nc = pr.n_cvt, T = pr.n_terms, n = pr.n, G = pr.n_grid, C = pr.n_clusters;
+  const int ni = (nc + 3) * (nc + 2) / 2;
+  const int nrows = nc + 2;  // per-SNP moment rows: W_1..W_c, y, xx
+  const int n_sh = (nc + 1) * (nc + 2) / 2;
+  const int NR = nrows + 1 + n_sh;  // dot-product rows: SNP rows, logdet, shared rows
+  const int per_grp = assoc16_group_doubles(ni, C);
+  double* c_twl = smem;                       // TB: logdet weight / power (0: m == 0, t >= T)
+  double* c_cdc = c_twl + TB;                 // C cluster centres
+  double* c_cw = c_cdc + C;                   // C cluster weights
+  double* srows = c_cw + C;                   // n_sh x TB shared pair rows, zero-padded
+  double* groups = srows + (size_t)n_sh * TB;  // 16 x per_grp
+  int* c_tinf = (int*)(groups + 16 * per_grp);  // TB: m | cluster << 8
+  int* rtab = c_tinf + TB;                    // tet(nc + 1) recursion updates
+  int* vslot = rtab + tet(nc + 1);            // NR: packed-row slot of each dot-product row
+  for (int t = threadIdx.x; t < TB; t += blockDim.x) {
+    const bool act = t < T;
+    const int m = act ? pr.term_m[t] : 0;
+    const int cl = act ? pr.term_cluster[t] : 0;
+    c_tinf[t] = m | (cl << 8);
+    c_twl[t] = (act && m > 0) ? pr.logdet_mom[t] / (double)m : 0.0;
+  }
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    c_cdc[c] = pr.cl_center[c];
+    c_cw[c] = pr.cl_weight[c];
+  }
+  for (int k = 0; k < n_sh; ++k) {
+    const int jj = shared_pair_index(k, nc);
+    for (int t = threadIdx.x; t < TB; t += blockDim.x)
+      srows[(size_t)k * TB + t] = t < T ? pr.shared_mom[(int64_t)jj * T + t] : 0.0;
+  }
+  {
+    const int m = nc + 2;
+    for (int e = threadIdx.x; e < tet(nc + 1); e += blockDim.x) {
+      int p = 1;
+      while (tet(m - 1) - tet(m - p - 1) <= e) ++p;  // level of entry e
+      int rem = e - (tet(m - 1) - tet(m - p)), ap = 0;
+      const int Lw = m - p;
+      while (rem >= Lw - ap) {
+        rem -= Lw - ap;
+        ++ap;
+      }
+      const int a = p + 1 + ap, b = a + rem;
+      rtab[e] = ab_index(a, b, nc) | (ab_index(a, p, nc) << 8) | (ab_index(b, p, nc) << 16);
+    }
+  }
+  for (int r = threadIdx.x; r < NR; r += blockDim.x)
+    vslot[r] = r < nrows ? xpair_index(r, nc) : (r == nrows ? ni : shared_pair_index(r - nrows - 1, nc));
+  __syncthreads();
+
+  double* PA = groups + (size_t)(wib * 4 + grp) * per_grp;  // ni entries + logdet slot
+  double* PB = PA + ni + 1;
+  double* iab_ld = PB + ni + 1;  // kMaxLevels
+  double* H = iab_ld + kMaxLevels;  // 2 C: (h_A, h_B) per cluster
+  double* Ph = h ? PB : PA;
+  int tinf[KT];
+#pragma unroll
+  for (int k = 0; k < KT; ++k) tinf[k] = c_tinf[j * KT + k];
+  const double phi = 0.6180339887498949;
+  const bool do_reml = (mode == 1 || mode == 4);
+  const bool do_mle = (mode == 2 || mode == 4);
+  const bool do_score = (mode == 3 || mode == 4);
+  const bool do_golden = do_reml || do_mle;
+  const double df = (double)(n - nc - 1);
+  const double nn = (double)n;
+  const double cr = 0.5 * df * (log(df) - log(2.0 * M_PI) - 1.0);
+  const double cm = 0.5 * nn * (log(nn) - log(2.0 * M_PI) - 1.0);
+  const int st_final = 3 + pr.n_iter, st_score = 4 + pr.n_iter;
+  const int64_t n_quads = (n_snps + 3) / 4;
+  for (int64_t wq = (int64_t)blockIdx.x * 4 + wib; wq < n_quads; wq += (int64_t)gridDim.x * 4) {
+    const int64_t s_raw = wq * 4 + grp;
+    const bool active = s_raw < n_snps;
+    const int64_t s = active ? s_raw : n_snps - 1;
+    const double* xm = xmom + s * (int64_t)nrows * ldm;
+    double* os = out + s * JM_NOUT;
+    // grid argmax indices left by k_lmm_grid, then columns a mode does not compute are NaN
+    const int gi_r = do_golden ? (int)os[JM_OUT_L_REMLE] : 0;
+    const int gi_m = do_golden ? (int)os[JM_OUT_L_MLE] : 0;
+    wave_sync();
+    if (active && j < JM_NOUT) os[j] = NAN;
+    int evals = 0;
+    double aA = 0, bA = 0, cA = 0, dA = 0, fcA = 0, fdA = 0;
+    double aB = 0, bB = 0, cB = 0, dB = 0, fcB = 0, fdB = 0;
+    bool klA = false, klB = false;
+    double lam_reml = NAN, lam_mle = NAN;
+    for (int stage = 0;;) {
+      // ---- the lambdas of this stage ----
+      const bool lam0 = stage == 0;
+      double lamA = 0.0, lamB = 0.0, xA = 0.0, xB = 0.0;
+      bool take_exp = true;
+      if (stage == 1) {
+        xA = cA;
+        xB = cB;
+      } else if (stage == 2) {
+        xA = dA;
+        xB = dB;
+      } else if (stage >= 3 && stage < st_final) {
+        klA = fcA > fdA;
+        klB = fcB > fdB;
+        const double naA = klA ? aA : cA, nbA = klA ? dA : bA;
+        const double naB = klB ? aB : cB, nbB = klB ? dB : bB;
+        const double ncA = nbA - phi * (nbA - naA), ndA = naA + phi * (nbA - naA);
+        const double ncB = nbB - phi * (nbB - naB), ndB = naB + phi * (nbB - naB);
+        aA = naA; bA = nbA; cA = ncA; dA = ndA;
+        aB = naB; bB = nbB; cB = ncB; dB = ndB;
+        xA = klA ? cA : dA;
+        xB = klB ? cB : dB;
+      } else if (stage == st_final) {
+        xA = (aA + bA) / 2.0;
+        xB = (aB + bB) / 2.0;
+      } else {
+        take_exp = false;
+        if (stage == st_score) lamA = lamB = pr.lambda_null;
+      }
+      if (take_exp) {
+        // each half takes the exponential of its own lambda; lane 15 - j is in the other half
+        const double e = exp(h ? xB : xA);
+        const double o = xpartner<3>(e, lane);
+        lamA = h ? o : e;
+        lamB = h ? e : o;
+        if (stage == st_final) {
+          lam_reml = lamA;
+          lam_mle = lamB;
+        }
+      }
+      // ---- 1. h table ----
+      for (int c = j; c < C; c += kGL) {
+        const double dc = c_cdc[c];
+        H[2 * c] = 1.0 / fma(lamA, dc, 1.0);
+        H[2 * c + 1] = 1.0 / fma(lamB, dc, 1.0);
+      }
+      wave_sync();
+      // ---- 2. kappa and the m >= 1 logdet partials ----
+      double kA[KT], kB[KT];
+      double ldA = 0.0, ldB = 0.0;
+      {
+        double pA = 1.0, pB = 1.0;
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+          const int t = j * KT + k;
+          const int m = tinf[k] & 255, cl = tinf[k] >> 8;
+          const double hA = H[2 * cl], hB = H[2 * cl + 1];
+          const double nzA = -lamA * hA, nzB = -lamB * hB;
+          if (k == 0) {
+            pA = ipow16(nzA, m);
+            pB = ipow16(nzB, m);
+          } else {
+            pA = m == 0 ? 1.0 : pA * nzA;
+            pB = m == 0 ? 1.0 : pB * nzB;
+          }
+          const bool act = t < T;
+          kA[k] = act ? hA * pA : 0.0;
+          kB[k] = act ? hB * pB : 0.0;
+          const double wl = c_twl[t];
+          ldA -= pA * wl;
+          ldB -= pB * wl;
+        }
+      }
+      // ---- 3. dot products, 8 rows x 2 lambdas per reduce-scatter ----
+      for (int b0 = 0; b0 < NR; b0 += 8) {
+        double v[16];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int r = b0 + e;
+          double sa = 0.0, sb = 0.0;
+          if (r < nrows) {
+            const double* row = xm + (int64_t)r * ldm + j * KT;
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+              const double x = (j * KT + k < T) ? row[k] : 0.0;
+              sa = fma(kA[k], x, sa);
+              sb = fma(kB[k], x, sb);
+            }
+          } else if (r == nrows) {
+            sa = ldA;
+            sb = ldB;
+          } else if (r < NR) {
+            const double* row = srows + (size_t)(r - nrows - 1) * TB + j * KT;
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+              const double x = row[k];
+              sa = fma(kA[k], x, sa);
+              sb = fma(kB[k], x, sb);
+            }
+          }
+          v[2 * e] = sa;
+          v[2 * e + 1] = sb;
+        }
+        const double r = reduce16(v, lane);
+        const int row = b0 + (j >> 1);
+        if (row < NR) ((j & 1) ? PB : PA)[vslot[row]] = r;
+      }
+      wave_sync();
+      // ---- 4. Pab recursion: half A on PA, half B on PB ----
+      double pyy_lvl = 0.0;
+      for (int p = 1; p <= nc + 1; ++p) {
+        const double d = Ph[ab_index(p, p, nc)];
+        if (p == nc + 1) pyy_lvl = Ph[ab_index(nc + 2, nc + 2, nc)];
+        const double inv = (d != 0.0) ? 1.0 / d : 0.0;
+        const int Lw = nc + 2 - p;
+        const int np = Lw * (Lw + 1) / 2;
+        const int lv0 = tet(nc + 1) - tet(nc + 2 - p);
+        for (int idx = q8; idx < np; idx += 8) {
+          const int e = rtab[lv0 + idx];
+          const int iab = e & 255, ia = (e >> 8) & 255, ib = e >> 16;
+          const double pa = Ph[ia], pb = Ph[ib];
+          Ph[iab] = Ph[iab] - pa * pb * inv;
+        }
+        wave_sync();
+      }
+      // ---- 5. logs: pivots (A), P_yy, clusters ----
+      const double pyy1 = Ph[ab_index(nc + 2, nc + 2, nc)];
+      const double lam_h = h ? lamB : lamA;
+      const int n_piv = h ? 0 : nc + 1;
+      const int n_cl = lam0 ? 0 : C;
+      const int n_items = n_piv + 1 + n_cl;
+      const double dfh = h ? nn : df;
+      double u = 0.0;  // -0.5 x (this lane's share of S) - 0.5 dfh log P_yy
+      for (int it = q8; it < n_items; it += 8) {
+        if (it < n_piv) {
+          const double piv = Ph[ab_index(it + 1, it + 1, nc)];
+          const double lg = log(piv > 0.0 ? piv : 1.0);
+          if (lam0) iab_ld[it] = lg;
+          u -= 0.5 * (lam0 ? lg : lg - iab_ld[it]);
+        } else if (it == n_piv) {
+          u -= 0.5 * dfh * log(clamp_pyy(pyy1));
+        } else {
+          const int c = it - n_piv - 1;
+          u -= 0.5 * (c_cw[c] * log(fabs(fma(lam_h, c_cdc[c], 1.0))));
+        }
+      }
+      const double fh = (h ? cm : cr) - 0.5 * Ph[ni] + oct_sum(u, lane);
+      const double fo = xpartner<3>(fh, lane);
+      const double fA = h ? fo : fh, fB = h ? fh : fo;
+      const double po = xpartner<3>(pyy_lvl, lane);
+      const double pyyA = h ? po : pyy_lvl;
+      const bool writer = active && j == 0;
+
+      // ---- consume the stage ----
+      if (stage == 0) {
+        // Iab done; the grid stage ran in k_lmm_grid
+        if (do_golden) {
+          evals += G;
+          const int iloA = gi_r > 0 ? gi_r - 1 : 0, ihiA = gi_r + 1 < G ? gi_r + 1 : G - 1;
+          const int iloB = gi_m > 0 ? gi_m - 1 : 0, ihiB = gi_m + 1 < G ? gi_m + 1 : G - 1;
+          aA = pr.grid_loglam[iloA];
+          bA = pr.grid_loglam[ihiA];
+          aB = pr.grid_loglam[iloB];
+          bB = pr.grid_loglam[ihiB];
+          cA = bA - phi * (bA - aA);
+          dA = aA + phi * (bA - aA);
+          cB = bB - phi * (bB - aB);
+          dB = aB + phi * (bB - aB);
+          stage = 1;
+        } else {
+          stage = st_score;
+        }
+      } else if (stage == 1) {
+        fcA = fA;
+        fcB = fB;
+        stage = 2;
+      } else if (stage == 2) {
+        fdA = fA;
+        fdB = fB;
+        stage = 3;
+      } else if (stage < st_final) {
+        const double nfcA = klA ? fA : fdA, nfdA = klA ? fcA : fA;
+        const double nfcB = klB ? fB : fdB, nfdB = klB ? fcB : fB;
+        fcA = nfcA;
+        fdA = nfdA;
+        fcB = nfcB;
+        fdB = nfdB;
+        ++stage;
+      } else if (stage == st_final) {
+        // Wald at the REML optimum (likelihood_jax.py:601-675), LRT at the MLE optimum
+        evals += 2 * (pr.n_iter + 3);
+        if (writer && do_reml) {
+          const double pxx = PA[ab_index(nc + 1, nc + 1, nc)];
+          const double pxy = PA[ab_index(nc + 1, nc + 2, nc)];
+          const double pxyy = clamp_pyy(PA[ab_index(nc + 2, nc + 2, nc)]);
+          double beta, se, tau;
+          beta_se(df, pxx, pxy, pxyy, beta, se, tau);
+          os[JM_OUT_BETA] = beta;
+          os[JM_OUT_SE] = se;
+          os[JM_OUT_P_WALD] = (pxx > 0.0) ? f_pvalue(df, (pyyA - pxyy) * tau) : NAN;
+          os[JM_OUT_L_REMLE] = lam_reml;
+          os[JM_OUT_LOGL_H1] = fA;
+        }
+        if (writer && do_mle) {
+          os[JM_OUT_L_MLE] = lam_mle;
+          os[JM_OUT_LOGL_MLE] = fB;
+          os[JM_OUT_P_LRT] = chi2_sf1(fmax(2.0 * (fB - pr.logl_null), 0.0));
+        }
+        if (!do_score) break;
+        stage = st_score;
+      } else {  // score at the null lambda (likelihood_jax.py:706-773)
+        evals += 1;
+        if (writer) {
+          const double pxx = PA[ab_index(nc + 1, nc + 1, nc)];
+          const double pxy = PA[ab_index(nc + 1, nc + 2, nc)];
+          const double pxyy = clamp_pyy(PA[ab_index(nc + 2, nc + 2, nc)]);
+          double sb, sse, tau;
+          beta_se(df, pxx, pxy, pxyy, sb, sse, tau);
+          os[JM_OUT_BETA_SCORE] = sb;
+          os[JM_OUT_SE_SCORE] = sse;
+          os[JM_OUT_P_SCORE] = (pxx > 0.0) ? f_pvalue(df, nn * (pxy * pxy) / (pyyA * pxx)) : NAN;
+        }
+        break;
+      }
+      wave_sync();
+    }
+    if (active && j == 0) os[JM_OUT_EVALS] = (double)evals;
+    wave_sync();
+  }
+}
+
 }  // namespace jm
 
 using namespace jm;
@@ -1339,6 +1752,33 @@ static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* x
                          ni_max);
     }
     JM_CHECK_LAUNCH();
+  }
+  // single problem with at most 256 terms: 16 lanes per SNP when the LDS fits
+  // (JM_LMM_WAVE_PER_SNP=1 keeps the wavefront-per-SNP kernel, for comparisons)
+  if (one) {
+    const char* env = getenv("JM_LMM_WAVE_PER_SNP");
+    const bool wave = env && env[0] == '1';
+    const int kt16 = T <= 64 ? 4 : (T <= 128 ? 8 : (T <= 192 ? 12 : (T <= 256 ? 16 : 0)));
+    const size_t lds16 = kt16 ? assoc16_lds(kt16, one->n_cvt, one->n_clusters) : 0;
+    if (!wave && kt16 && one->n_clusters >= 1 && lds16 <= 64 * 1024) {
+      const int64_t quads = (n_snps + 3) / 4;
+      int64_t b16 = (quads + 3) / 4;
+      if (b16 > 256 * 8) b16 = 256 * 8;
+#define JM_LAUNCH16(KT_)                                                                         \
+  hipLaunchKernelGGL(k_lmm_assoc16<KT_>, dim3((unsigned)b16), dim3(256), lds16, st, *one, mode_one, \
+                     xmom, ldm, n_snps, out)
+      if (kt16 == 4)
+        JM_LAUNCH16(4);
+      else if (kt16 == 8)
+        JM_LAUNCH16(8);
+      else if (kt16 == 12)
+        JM_LAUNCH16(12);
+      else
+        JM_LAUNCH16(16);
+#undef JM_LAUNCH16
+      JM_CHECK_LAUNCH();
+      return JM_OK;
+    }
   }
   const size_t per_wave = (size_t)assoc_lds_per_wave(ni_max) * sizeof(double);
   // single problem: the shared pair rows go to LDS when they take <= 48 KB
