@@ -23,6 +23,25 @@ constexpr int kMomMaxTerms = 16;  // MAX_TERMS_PER_CLUSTER on the host side
 constexpr int kMomGroup = 3;      // partner rows accumulated per pass (48 accumulators)
 constexpr int kMomTerms = kMomMaxTerms;
 constexpr int kMomUnroll = 4;     // eigen-indices whose Z loads are in flight together
+constexpr int kMomWaves = 4;      // row groups of one block, one wavefront each
+
+// The LDS transpose areas are private to a wavefront: a wave-level barrier
+// with an LDS fence orders its writes and reads.
+__device__ __forceinline__ void wave_barrier_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Wavefronts per k_mom_seg block: one per group of kMomGroup moment rows, at most
+// kMomWaves (the waves then take every kMomWaves-th group). The waves of a block
+// read the same Z columns at about the same time, so Z comes from HBM once.
+static inline int mom_waves(int nvm) {
+  const int groups = (nvm + 1 + kMomGroup - 1) / kMomGroup;
+  return groups < kMomWaves ? groups : kMomWaves;
+}
+static inline size_t mom_lds(int nvm) {
+  return (size_t)mom_waves(nvm) * kMomTerms * 65 * sizeof(double);
+}
 
 typedef const double __attribute__((address_space(4))) mom_cdbl;
 
@@ -39,12 +58,14 @@ template <typename ZT, bool MULTI, int M>
 __device__ __forceinline__ void mom_piece(const ZT* __restrict__ Z, int64_t ldz, int64_t n_snps,
                                           int64_t sc, const double* V, int64_t ldv, int nv,
                                           int nvm, const double* __restrict__ delta, int i0,
-                                          int len, int slot0, int n_slots, double* tr,
+                                          int len, int slot0, int n_slots, int g_first,
+                                          int g_step, double* tr,
                                           double* __restrict__ work) {
   const mom_cdbl* dls = (const mom_cdbl*)delta;
   const mom_cdbl* Vs = (const mom_cdbl*)V;
   const int iend = i0 + len;
-  for (int g0 = 0; g0 <= nvm; g0 += kMomGroup) {
+  const int lane = threadIdx.x & 63;
+  for (int g0 = g_first; g0 <= nvm; g0 += g_step) {
     // rows of this group present in every column of the block (MULTI: nv
     // differs per column; missing rows are written as 0 and never read)
     int ng = nvm + 1 - g0;
@@ -118,15 +139,15 @@ __device__ __forceinline__ void mom_piece(const ZT* __restrict__ Z, int64_t ldz,
     for (int b = 0; b < kMomGroup; ++b) {
       if (b < ng) {
 #pragma unroll
-        for (int m = 0; m < M; ++m) tr[m * 65 + threadIdx.x] = acc[b][m];
-        __syncthreads();
-        for (int e = threadIdx.x; e < 64 * M; e += 64) {
+        for (int m = 0; m < M; ++m) tr[m * 65 + lane] = acc[b][m];
+        wave_barrier_lds();
+        for (int e = lane; e < 64 * M; e += 64) {
           const int col = e / M, m = e % M;
           const int64_t sc2 = (int64_t)blockIdx.x * 64 + col;
           if (sc2 < n_snps)
             work[(sc2 * (nvm + 1) + g0 + b) * n_slots + slot0 + m] = tr[m * 65 + col];
         }
-        __syncthreads();
+        wave_barrier_lds();
       }
     }
   }
@@ -137,12 +158,13 @@ __device__ __forceinline__ void mom_piece(const ZT* __restrict__ Z, int64_t ldz,
 // balanced work however the spectrum clusters; the piece's term count
 // (1..kMomMaxTerms, fixed by the plan) selects the instantiation.
 template <typename ZT, bool MULTI>
-__global__ void __launch_bounds__(64) k_mom_seg(const ZT* __restrict__ Z, int64_t ldz,
+__global__ void __launch_bounds__(64 * kMomWaves) __attribute__((amdgpu_waves_per_eu(3))) k_mom_seg(const ZT* __restrict__ Z, int64_t ldz,
                                                 int64_t n_snps, const double* __restrict__ V_one,
                                                 int64_t ldv_one, int nv_one,
                                                 const jm_mom_col* __restrict__ cols, int nv_max,
                                                 jm_spectrum_plan pl, double* __restrict__ work) {
-  const int64_t s = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  const int64_t s = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const int wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
   const int64_t sc = s < n_snps ? s : 0;
   const int seg = blockIdx.y;
   const double* V = V_one;
@@ -158,11 +180,12 @@ __global__ void __launch_bounds__(64) k_mom_seg(const ZT* __restrict__ Z, int64_
   const int i0 = pl.seg_start[seg], len = pl.seg_len[seg];
   const int M = pl.cl_terms[pl.seg_cluster[seg]];
   const int slot0 = pl.seg_slot[seg];
-  __shared__ double tr[kMomTerms * 65];  // term x column, padded
+  extern __shared__ double tr_all[];  // per wave: term x column, padded
+  double* tr = tr_all + wave * kMomTerms * 65;
 #define JM_MOM_CASE(k)                                                                       \
   case k:                                                                                    \
     mom_piece<ZT, MULTI, k>(Z, ldz, n_snps, sc, V, ldv, nv, nvm, pl.delta, i0, len, slot0,   \
-                            pl.n_slots, tr, work);                                           \
+                            pl.n_slots, wave * kMomGroup, n_waves * kMomGroup, tr, work);    \
     break;
   switch (M) {
     JM_MOM_CASE(1) JM_MOM_CASE(2) JM_MOM_CASE(3) JM_MOM_CASE(4)
@@ -244,10 +267,10 @@ extern "C" int jm_spectrum_moments(const void* Z, int z_is_f32, int64_t ldz, int
   const dim3 grid((unsigned)((n_snps + 63) / 64), (unsigned)pl.n_seg);
   hipStream_t st = (hipStream_t)stream;
   if (z_is_f32)
-    hipLaunchKernelGGL((k_mom_seg<float, false>), grid, dim3(64), 0, st, (const float*)Z, ldz,
+    hipLaunchKernelGGL((k_mom_seg<float, false>), grid, dim3(64 * mom_waves(nv)), mom_lds(nv), st, (const float*)Z, ldz,
                        n_snps, V, ldv, nv, nullptr, 0, pl, work);
   else
-    hipLaunchKernelGGL((k_mom_seg<double, false>), grid, dim3(64), 0, st, (const double*)Z, ldz,
+    hipLaunchKernelGGL((k_mom_seg<double, false>), grid, dim3(64 * mom_waves(nv)), mom_lds(nv), st, (const double*)Z, ldz,
                        n_snps, V, ldv, nv, nullptr, 0, pl, work);
   JM_CHECK_LAUNCH();
   hipLaunchKernelGGL((k_mom_fold<false>), dim3(2048), dim3(256), 0, st, n_snps, nv, nullptr, 0, pl,
@@ -270,10 +293,10 @@ extern "C" int jm_spectrum_moments_multi(const void* Z, int z_is_f32, int64_t ld
   const dim3 grid((unsigned)((n_snps + 63) / 64), (unsigned)pl.n_seg);
   hipStream_t st = (hipStream_t)stream;
   if (z_is_f32)
-    hipLaunchKernelGGL((k_mom_seg<float, true>), grid, dim3(64), 0, st, (const float*)Z, ldz,
+    hipLaunchKernelGGL((k_mom_seg<float, true>), grid, dim3(64 * mom_waves(nv_max)), mom_lds(nv_max), st, (const float*)Z, ldz,
                        n_snps, nullptr, 0, 0, cols, nv_max, pl, work);
   else
-    hipLaunchKernelGGL((k_mom_seg<double, true>), grid, dim3(64), 0, st, (const double*)Z, ldz,
+    hipLaunchKernelGGL((k_mom_seg<double, true>), grid, dim3(64 * mom_waves(nv_max)), mom_lds(nv_max), st, (const double*)Z, ldz,
                        n_snps, nullptr, 0, 0, cols, nv_max, pl, work);
   JM_CHECK_LAUNCH();
   hipLaunchKernelGGL((k_mom_fold<true>), dim3(2048), dim3(256), 0, st, n_snps, 0, cols, nv_max,

@@ -60,10 +60,10 @@ class SpectrumPlan:
 
 def segment_plan(pl: "SpectrumPlan", seg_max: int | None = None) -> "SpectrumPlan":
     """Cut every cluster into near-equal pieces of at most seg_max eigen-indices
-    (default max(64, min(512, ceil(n / 64)))); fills the seg_* fields in place."""
+    (default max(64, min(512, ceil(n / 20)))); fills the seg_* fields in place."""
     n = pl.n
     if seg_max is None:
-        seg_max = max(64, min(512, -(-n // 64)))
+        seg_max = max(64, min(512, -(-n // 20)))
     starts, lens, clus, seg0, nseg = [], [], [], [], []
     for c in range(len(pl.cl_size)):
         size, st = int(pl.cl_size[c]), int(pl.cl_start[c])
