@@ -299,8 +299,10 @@ int jm_lmm_prepare(const jm_lmm_problem* prob, void* stream);
  * (+_mle), batch_calc_wald_stats, batch_calc_score_stats and
  * calc_lrt_pvalue_jax (likelihood_jax.py:231-883; call sites
  * runner_streaming.py:471-554). mode: 1 Wald, 2 LRT, 3 Score, 4 all.
- * One wavefront per SNP; the reference's 50-point log grid + golden-section
- * schedule is kept exactly.
+ * 16 lanes per SNP when the problem has at most 256 spectrum terms and its
+ * tables fit 64 KB of LDS, else one wavefront per SNP (the environment
+ * variable JM_LMM_WAVE_PER_SNP=1 forces the latter); the reference's 50-point
+ * log grid + golden-section schedule is kept exactly.
  */
 int jm_lmm_assoc(const jm_lmm_problem* prob, int mode, const double* xmom, int64_t ldm,
                  int64_t n_snps, double* out, void* stream);
