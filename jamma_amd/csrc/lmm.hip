@@ -590,7 +590,7 @@ __device__ __forceinline__ int xpair_index(int r, int nc) {
 // first, first index on ties), one wavefront per SNP. Column k of a 64-column
 // pass is lane k: column 0 is Iab (lambda = 0, kappa_t = [m_t == 0]), column
 // k >= 1 grid point g = k - 1. A lane walks the T terms once per 8 moment rows:
-// kappa from the grid table (vector loads, the same table for every SNP) times
+// kappa at its grid point (a reciprocal per cluster, a multiply per term) times
 // the SNP's moments (wave-uniform scalar loads). Row 0 of its Pab column is
 // completed with the shared pairs (table) in LDS and the recursion runs per
 // lane. The REML log-determinant term subtracts the Iab level logs in the
@@ -617,12 +617,13 @@ __global__ void __launch_bounds__(256) k_lmm_grid(jm_lmm_problem pr_one, int mod
     const int nc = pr.n_cvt, T = pr.n_terms, n = pr.n, G = pr.n_grid;
     const int ni = (nc + 3) * (nc + 2) / 2;
     const int nrows = nc + 2;
-    // the SNP's moments and term powers are wave-uniform: constant address space,
-    // so they come through scalar loads; the grid kappa table through vector loads
+    // the SNP's moments and the term data are wave-uniform: constant address space,
+    // so they come through scalar loads
     const cdbl* xm = (const cdbl*)uniform_ptr(MULTI ? snp_xmom[s]
                                                    : xmom + s * (int64_t)nrows * ldm);
     const cint* term_m = (const cint*)pr.term_m;
-    const gdbl* gk = (const gdbl*)pr.grid_kappa;
+    const cint* term_cl = (const cint*)pr.term_cluster;
+    const cdbl* cl_center = (const cdbl*)pr.cl_center;
     double best_r = -INFINITY, best_m = -INFINITY;
     int bi_r = 0x7fffffff, bi_m = 0x7fffffff;
     for (int k0 = 0; k0 <= G; k0 += 64) {
@@ -630,6 +631,8 @@ __global__ void __launch_bounds__(256) k_lmm_grid(jm_lmm_problem pr_one, int mod
       const bool act = k <= G;
       const bool is_iab = k == 0;
       const int g = act && !is_iab ? k - 1 : 0;
+      // the lane's lambda: exactly the value the grid tables were built with
+      const double lam_g = is_iab ? 0.0 : exp(pr.grid_loglam[g]);
       for (int r0 = 0; r0 < nrows; r0 += 8) {
         // rows past nrows repeat the last row (no branches in the term loop)
         int64_t roff[8];
@@ -638,14 +641,29 @@ __global__ void __launch_bounds__(256) k_lmm_grid(jm_lmm_problem pr_one, int mod
         double acc[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) acc[r] = 0.0;
-        // kappa of terms t .. t+3, the next four in flight while these are used
+        // kappa of terms t .. t+3 at the lane's lambda, walking the terms in order: a
+        // cluster's terms are contiguous with m = 0, 1, 2, ..., so h_c = 1/(1 + lambda d_c)
+        // is formed once per cluster (a uniform branch) and (lambda h_c)^m by one multiply
+        // per term, the operations term_coeffs uses for the prepared tables (lambda = 0
+        // gives the Iab column's [m == 0], up to the sign of zeros)
+        double ch = 1.0, cz = 0.0, czm = 1.0;
         auto kap4 = [&](int t0, double (&kv)[4]) {
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            const int t = min(t0 + u, T - 1);
-            const double kg = gk[(int64_t)t * G + g];
-            const double ki = term_m[t] == 0 ? 1.0 : 0.0;
-            kv[u] = (t0 + u < T) ? (is_iab ? ki : kg) : 0.0;
+            const int t = t0 + u;
+            double kt = 0.0;
+            if (t < T) {
+              const int m = term_m[t];
+              if (m == 0) {
+                ch = 1.0 / fma(lam_g, cl_center[term_cl[t]], 1.0);
+                cz = lam_g * ch;
+                czm = 1.0;
+              } else {
+                czm *= cz;
+              }
+              kt = (m & 1) ? -ch * czm : ch * czm;
+            }
+            kv[u] = kt;
           }
         };
         double kc[4], kn[4];
