@@ -164,7 +164,9 @@ __global__ void __launch_bounds__(64 * kMomWaves) __attribute__((amdgpu_waves_pe
                                                 const jm_mom_col* __restrict__ cols, int nv_max,
                                                 jm_spectrum_plan pl, double* __restrict__ work) {
   const int64_t s = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
-  const int wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+  // wave-uniform (readfirstlane): the rows of the wave's group, and with them the V
+  // addresses, stay uniform, so V comes through scalar loads
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), n_waves = blockDim.x >> 6;
   const int64_t sc = s < n_snps ? s : 0;
   const int seg = blockIdx.y;
   const double* V = V_one;
