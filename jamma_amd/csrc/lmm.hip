@@ -1357,7 +1357,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   extern __shared__ double smem[];
   constexpr int TB = kGL * KT;
   const int lane = threadIdx.x & 63;
-  const int wib = threadIdx.x >> 6;
+  const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar loop control
   const int grp = lane >> 4;
   const int j = lane & 15;
   const int h = j >> 3;  // 0: row A (REML, lambda A), 1: row B (MLE, lambda B)
