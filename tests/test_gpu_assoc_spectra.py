@@ -5,7 +5,8 @@ that compress into many clusters: 16 terms per lane with more clusters than lane
 in a group, 12 terms per lane, and SNP counts that leave idle lane groups. The
 16-lanes-per-SNP kernel and the wavefront-per-SNP kernel (JM_LMM_WAVE_PER_SNP=1)
 run on the same device moments and are checked against the oracle's chunk compute
-(runner_streaming.py:471-554) on the same rotated inputs.
+(runner_streaming.py:471-554) on the same rotated inputs. The moments kernel is checked
+against its defining sums with more partner rows than a block has wavefronts.
 """
 
 import numpy as np
@@ -62,3 +63,36 @@ def test_search_kernels_match_oracle(monkeypatch, rng_d, ncov, p, t_range, seed)
             np.testing.assert_allclose(-np.log10(out[:, col]), -np.log10(ref[k]), atol=1e-5,
                                        err_msg=k)
         np.testing.assert_allclose(out[:, 4], ref["logl_H1"], rtol=1e-9)
+
+
+@pytest.mark.parametrize("nv,z_f32", [(14, False), (5, True), (1, False)])
+def test_moments_many_partner_rows(nv, z_f32):
+    """The moments kernel against its definition, with more row groups than a block
+    has wavefronts (nv = 14: 15 rows in 5 groups of 3 on 4 waves), a SNP count that
+    leaves a partial 64-column block, FP32 and FP64 Z:
+      mom[s][b][t] = sum_{i in c(t)} Z[i][s] V[b][i] delta_i^m(t)   (b < nv)
+      mom[s][nv][t] = sum_{i in c(t)} Z[i][s]^2 delta_i^m(t)."""
+    from jamma_amd.lmm.engine import Cohort
+
+    rng = np.random.default_rng(nv)
+    n, p = 300, 45
+    d = np.sort(rng.uniform(0.05, 3.0, n))
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    co = Cohort(d, torch.as_tensor(Q, device="cuda"), precision="f64")
+    pl = co.plan
+    Zh = rng.normal(size=(n, p))
+    if z_f32:
+        Zh = Zh.astype(np.float32).astype(np.float64)
+    V = rng.normal(size=(nv, n))
+    Zd = torch.as_tensor(Zh.astype(np.float32) if z_f32 else Zh, device="cuda")
+    got = co.moments(Zd, z_f32, p, p, torch.as_tensor(V, device="cuda"), nv).cpu().numpy()
+    rows = np.concatenate([V[None, :, :] * Zh.T[:, None, :], (Zh.T ** 2)[:, None, :]], axis=1)
+    ref = np.zeros((p, nv + 1, pl.n_terms))
+    scale = np.zeros_like(ref)
+    for c in range(len(pl.cl_size)):
+        sl = slice(pl.cl_start[c], pl.cl_start[c] + pl.cl_size[c])
+        for m in range(pl.cl_terms[c]):
+            w = pl.idx_delta[sl] ** m
+            ref[:, :, pl.cl_toff[c] + m] = rows[:, :, sl] @ w
+            scale[:, :, pl.cl_toff[c] + m] = np.abs(rows[:, :, sl]) @ np.abs(w)
+    assert np.all(np.abs(got - ref) <= 1e-13 * scale + 1e-300)
