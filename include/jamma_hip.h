@@ -418,6 +418,11 @@ int jm_apply_rotations(int64_t r, const int64_t* pq, const double* cs, int64_t n
 double jm_host_betainc(double a, double b, double x);
 /* chi-square(1) survival function (jax.scipy.stats.chi2.sf(x, 1)). */
 double jm_host_chi2_sf1(double x);
+/* Wald / Score p-value from an F(1, df) statistic with the reference's guards
+ * (likelihood_jax.py:668-673): 1 for f_stat <= 0, NaN for a NaN statistic. */
+double jm_host_f_pvalue(double df, double f_stat);
+/* LRT p-value, calc_lrt_pvalue_jax (likelihood_jax.py:231-251). */
+double jm_host_lrt_pvalue(double logl_h1, double logl_h0);
 
 #ifdef __cplusplus
 }

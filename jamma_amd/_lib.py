@@ -119,6 +119,8 @@ _SIGS = {
                            c_int),
     "jm_host_betainc": ([c_double, c_double, c_double], c_double),
     "jm_host_chi2_sf1": ([c_double], c_double),
+    "jm_host_f_pvalue": ([c_double, c_double], c_double),
+    "jm_host_lrt_pvalue": ([c_double, c_double], c_double),
 }
 
 EXPORTED = tuple(_SIGS)

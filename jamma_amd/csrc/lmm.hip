@@ -99,13 +99,23 @@ __host__ __device__ __attribute__((noinline)) double chi2_sf1(double x) {
   return erfc(sqrt(0.5 * x));
 }
 
-// F(1, df) upper tail through betainc, exactly as likelihood_jax.py:668-673.
+// F(1, df) upper tail through betainc, exactly as likelihood_jax.py:668-673. A NaN
+// statistic gives NaN, as jnp.maximum / jnp.clip propagate it (C's fmax / fmin would
+// drop it and return p ~ 1).
 __host__ __device__ static double f_pvalue(double df, double f_stat) {
+  if (f_stat != f_stat) return NAN;  // NaN (host and device)
   double z = df / (df + fmax(f_stat, 1e-10));
   z = fmin(fmax(z, 0.0), 1.0);
   double p = betainc_reg(0.5 * df, 0.5, z);
   if (f_stat <= 0.0) p = 1.0;
   return p;
+}
+
+// LRT p-value, calc_lrt_pvalue_jax (likelihood_jax.py:231-251): chi2_1 tail of
+// max(2 (logl_H1 - logl_H0), 0) with jnp.maximum's NaN propagation.
+__host__ __device__ static double lrt_pvalue(double logl_h1, double logl_h0) {
+  const double stat = 2.0 * (logl_h1 - logl_h0);
+  return chi2_sf1(stat < 0.0 ? 0.0 : stat);
 }
 
 // GEMMA packed upper-triangle index (likelihood.py:25-43), 1-based a, b.
@@ -1239,7 +1249,7 @@ __global__ void __launch_bounds__(256) k_lmm_assoc(jm_lmm_problem pr_one, int mo
         if (lane == 0 && do_mle) {
           os[JM_OUT_L_MLE] = lam_mle;
           os[JM_OUT_LOGL_MLE] = fB;
-          os[JM_OUT_P_LRT] = chi2_sf1(fmax(2.0 * (fB - pr.logl_null), 0.0));
+          os[JM_OUT_P_LRT] = lrt_pvalue(fB, pr.logl_null);
         }
         if (!do_score) break;
         stage = st_score;
@@ -1652,7 +1662,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         if (writer && do_mle) {
           os[JM_OUT_L_MLE] = lam_mle;
           os[JM_OUT_LOGL_MLE] = fB;
-          os[JM_OUT_P_LRT] = chi2_sf1(fmax(2.0 * (fB - pr.logl_null), 0.0));
+          os[JM_OUT_P_LRT] = lrt_pvalue(fB, pr.logl_null);
         }
         if (!do_score) break;
         stage = st_score;
@@ -1883,3 +1893,7 @@ extern "C" int jm_lmm_assoc_multi(const jm_lmm_problem* probs, const int32_t* mo
 
 extern "C" double jm_host_betainc(double a, double b, double x) { return betainc_reg(a, b, x); }
 extern "C" double jm_host_chi2_sf1(double x) { return chi2_sf1(x); }
+extern "C" double jm_host_f_pvalue(double df, double f_stat) { return f_pvalue(df, f_stat); }
+extern "C" double jm_host_lrt_pvalue(double logl_h1, double logl_h0) {
+  return lrt_pvalue(logl_h1, logl_h0);
+}

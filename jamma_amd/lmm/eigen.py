@@ -413,6 +413,13 @@ def _eigh_large(Kd: torch.Tensor, progress=None, nb: int | None = None,
     return torch.as_tensor(w, dtype=torch.float64).to(Kd.device), Q.t()
 
 
+def _select_solver(n: int):
+    """The solver eigh_device uses for an n x n matrix: rocSOLVER below _OWN_MIN_N, the
+    64-bit path from there on. rocSOLVER's 32-bit routines are never chosen above
+    _LIB_MAX_N, whatever _OWN_MIN_N is."""
+    return _dsyevd if n < min(_OWN_MIN_N, _LIB_MAX_N + 1) else _eigh_large
+
+
 def eigh_device(K, threshold: float = 1e-10, progress=None):
     """-> (d host float64 ascending thresholded, U device float64 (n x n)).
 
@@ -425,10 +432,7 @@ def eigh_device(K, threshold: float = 1e-10, progress=None):
         raise ValueError(f"Kinship matrix must be square, got shape {tuple(Kd.shape)}")
     if Kd.shape[0] == 0:
         raise ValueError("Kinship matrix is empty")
-    if Kd.shape[0] < min(_OWN_MIN_N, _LIB_MAX_N + 1):
-        d, U = _dsyevd(Kd, progress)
-    else:
-        d, U = _eigh_large(Kd, progress)
+    d, U = _select_solver(Kd.shape[0])(Kd, progress)
     del Kd
     d = d.cpu().numpy()
     n_neg = int(np.sum(d < -threshold))

@@ -292,6 +292,41 @@ class AssocProblem:
         return self.assoc_moments(xmom, n_snps, mode)
 
 
+def assoc_multi(problems, modes, snp_prob, snp_xmom, out: torch.Tensor, stream=None) -> None:
+    """jm_lmm_assoc_multi: SNP s is tested under problems[snp_prob[s]] (AssocProblems of
+    one cohort) in mode modes[snp_prob[s]], from the moments block at device address
+    snp_xmom[s] ((n_cvt+2) x T float64 rows); its results go to row s of out."""
+    if not problems:
+        raise ValueError("no problems")
+    co = problems[0].cohort
+    if any(p.cohort is not co for p in problems):
+        raise ValueError("all problems of one launch must share the cohort")
+    snp_prob = np.ascontiguousarray(snp_prob, dtype=np.int32)
+    snp_xmom = np.ascontiguousarray(snp_xmom, dtype=np.uint64)
+    B = snp_prob.shape[0]
+    if snp_xmom.shape != (B,) or out.shape[0] < B or out.shape[1] != _lib.JM_NOUT:
+        raise ValueError("snp_prob / snp_xmom / out shapes disagree")
+    if B and (snp_prob.min() < 0 or snp_prob.max() >= len(problems)):
+        raise ValueError("snp_prob out of range")
+    if B == 0:
+        return
+    T = co.n_terms
+    probs = (_lib.LmmProblem * len(problems))(*[p.prob for p in problems])
+    dev = co.device
+    probs_dev = to_dev(np.frombuffer(probs, dtype=np.uint8), torch.uint8, dev)
+    modes_dev = to_dev(np.asarray(modes, dtype=np.int32), torch.int32, dev)
+    sp = to_dev(snp_prob, torch.int32, dev)
+    sx = to_dev(snp_xmom.view(np.int64), torch.int64, dev)
+    max_cvt = max(p.n_cvt for p in problems)
+    _lib.call("jm_lmm_assoc_multi", ptr(probs_dev), ptr(modes_dev), len(problems), max_cvt, T,
+              ptr(sp), ptr(sx), T, B, ptr(out), stream_ptr(stream))
+    if stream is not None:
+        # the launch reads these device copies on `stream`: the caching allocator must not
+        # hand their memory out again before that stream has passed the launch
+        for t in (probs_dev, modes_dev, sp, sx):
+            t.record_stream(stream)
+
+
 def ctypes_ref(pr):
     import ctypes
     return ctypes.byref(pr)

@@ -7,8 +7,9 @@
   Fst 0.05, 1 % missing, 3 covariates + intercept, -lmm 4): the mixed path against the FP64
   device path on 4,096 SNPs (|delta log10 p| <= 1e-4, significance and sign agreement), and the
   FP64 device path against the oracle on 256 of them. The oracle takes the device
-  eigendecomposition (checked against LAPACK in test_gpu_parity.py) because a host
-  eigendecomposition at n = 10,000 does not fit a test's time.
+  eigendecomposition because a host one at n = 10,000 does not fit a test's time; that
+  eigendecomposition (the library's 64-bit path, selected from 8,192 samples on) is
+  checked here by max |K U - U D| <= 1e-11 max |d| and max |U^T U - I| <= 1e-12.
 """
 
 import numpy as np
@@ -18,6 +19,7 @@ import torch
 from oracle import kinship as ok
 from oracle import lmm as ol
 from tests.test_gpu_parity import ALL, _arrays, _assert_matches_oracle, _snp_info
+from tests.util import eig_residuals
 
 pytestmark = pytest.mark.gpu
 
@@ -90,8 +92,16 @@ def test_precision_form_at_full_sample_size():
     for s in range(0, p_kin, 8192):
         e = min(s + 8192, p_kin)
         acc.add_tile(g[s:e], ld, e - s, mu64[s:e], True)
-    d, U = eigh_device(acc.finalize(p_kin))
+    Kd = acc.finalize(p_kin)
     del acc
+    d, U = eigh_device(Kd)
+    # the 64-bit eigendecomposition at the size it serves: residual and orthogonality
+    # with torch's FP64 GEMMs (independent of the library's kernels)
+    res, orth = eig_residuals(Kd, d, U)
+    print(f"eigendecomposition n = {n}: residual {res:.2e}, orthogonality {orth:.2e}",
+          flush=True)
+    assert res <= 1e-11 and orth <= 1e-12, (res, orth)
+    del Kd
     rng = np.random.default_rng(10)
     gt = g[p_kin:]
     Xh = gt[:, :n].cpu().numpy().astype(np.float64).T

@@ -214,3 +214,35 @@ def test_dgemm2_two_pairs_in_one_pass():
               Ad.data_ptr(), n, n, 1.0, Cd.data_ptr(), n, 0, stream_ptr())
     got = Cd.cpu().numpy().T
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("n,p,leaf", [(8192, 4096, None), (12_289, 24_000, None),
+                                      (12_289, 24_000, 5000)])
+def test_production_sizes_residual_and_orthogonality(n, p, leaf):
+    """eigh_device where it selects the 64-bit path (n >= 8,192), at its production
+    panel width: kinship-like matrices built on the device (rank-deficient at
+    n = 8,192 with p = n / 2; full rank and not a multiple of the 128-column panel at
+    n = 12,289, also with the tridiagonal problem torn into 5,000-row blocks so the
+    divide-and-conquer merges run at this size). max |K U - U D| <= 1e-11 max |d| and
+    max |U^T U - I| <= 1e-12 (torch FP64 GEMMs, independent of the library)."""
+    from jamma_amd.lmm import eigen
+    from tests.util import eig_residuals
+
+    g = torch.Generator(device="cuda").manual_seed(n + p)
+    af = torch.rand(p, device="cuda", dtype=torch.float64, generator=g) * 0.45 + 0.05
+    X = (torch.rand((n, p), device="cuda", dtype=torch.float64, generator=g) < af).double() + \
+        (torch.rand((n, p), device="cuda", dtype=torch.float64, generator=g) < af).double()
+    X -= X.mean(0, keepdim=True)
+    K = (X @ X.t()) / p
+    del X
+    K = 0.5 * (K + K.t())
+    if leaf is None:
+        d, U = eigen.eigh_device(K)
+    else:
+        dd, U = eigen._eigh_large(K, leaf=leaf)
+        d = dd.cpu().numpy()
+    res, orth = eig_residuals(K, d, U)
+    print(f"n = {n}, p = {p}, leaf = {leaf}: residual {res:.2e}, orthogonality {orth:.2e}",
+          flush=True)
+    assert np.all(np.diff(d) >= 0)
+    assert res <= 1e-11 and orth <= 1e-12, (res, orth)

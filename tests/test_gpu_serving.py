@@ -135,3 +135,58 @@ def test_bad_request_fails_alone(cohorts):
         with pytest.raises(ValueError):
             bad.result(timeout=60)
         assert good.result(timeout=120).shape == (100, 12)
+
+
+def test_wide_spectrum_cohort_many_covariates():
+    """A cohort whose spectrum compresses to T in (384, 512] (the kernel with 8 terms
+    per lane and 4 moment rows in registers) with requests of 0-10 covariates, so that
+    one batch holds problems of different c and rows read from memory: every result
+    is the same alone and in the mix, and matches the oracle within 1e-4 in log10 p."""
+    from jamma_amd.device import round_up, to_dev
+    from jamma_amd.io.plink import stats_from_counts
+    from jamma_amd.serve import Request, ResidentCohort, Server
+
+    rng = np.random.default_rng(77)
+    n, p = 1500, 500
+    d = np.sort(rng.uniform(0.01, 5.0, n))
+    U, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    G = balding_nichols(n, p, miss=0.01, seed=77)
+    ld = round_up(n, 64)
+    g8 = np.zeros((p, ld), np.int8)
+    g8[:, :n] = np.where(np.isnan(G), -1, G).astype(np.int8).T
+    counts = np.stack([(G == 0).sum(0), (G == 1).sum(0), (G == 2).sum(0), np.isnan(G).sum(0)], 1)
+    mean32, _, _ = stats_from_counts(counts, np.float32)
+    rc = ResidentCohort("wide", to_dev(g8, torch.int8), ld, to_dev(mean32, torch.float64), d, U)
+    assert 384 < rc.cohort.n_terms <= 512, rc.cohort.n_terms
+    reqs = []
+    for i, ncov in enumerate((0, 10, 3, 7, 1, 5)):
+        W = np.column_stack([np.ones(n), rng.normal(size=(n, ncov))]) if ncov else None
+        y = U @ (rng.normal(size=n) * np.sqrt(0.5 * d + 0.5)) + 0.2 * np.nan_to_num(G[:, i])
+        length = int(rng.integers(40, 160))
+        start = int(rng.integers(0, p - length))
+        reqs.append(Request("wide", y, start, start + length, covariates=W, mode=4))
+    alone = []
+    for r in reqs:
+        srv = Server(slice_snps=48, max_batch_snps=192)
+        srv.add_cohort(rc)
+        with srv:
+            alone.append(srv.submit(_clone(r)).result(timeout=120))
+    srv = Server(slice_snps=48, max_batch_snps=192)
+    srv.add_cohort(rc)
+    with srv:
+        futs = [srv.submit(_clone(r)) for r in reqs]
+        mixed = [f.result(timeout=300) for f in futs]
+        stats = srv.latency_stats()
+    assert stats["max_concurrent"] >= 2, stats
+    UT = U.T
+    for r, a, m in zip(reqs, alone, mixed):
+        assert np.array_equal(a, m, equal_nan=True)
+        X = G[:, r.snp_start:r.snp_end]
+        X = np.where(np.isnan(X), mean32[r.snp_start:r.snp_end][None, :], X)
+        W = r.covariates if r.covariates is not None else np.ones((n, 1))
+        UtW, Uty = UT @ W, UT @ r.phenotype
+        lam0, ll0 = ol.null_model_mle(d, UtW, Uty, W.shape[1])
+        ref = ol.assoc_chunk(4, W.shape[1], d, UtW, Uty, UT @ X, 1.0 / (lam0 * d + 1.0), ll0)
+        for k, col in (("p_wald", 2), ("p_lrt", 7), ("p_score", 8)):
+            dl = np.abs(np.log10(m[:, col]) - np.log10(ref[k]))
+            assert np.nanmax(dl) <= 1e-4, (W.shape[1], k, np.nanmax(dl))

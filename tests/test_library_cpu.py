@@ -58,6 +58,28 @@ def test_host_chi2_matches_scipy():
             assert abs(got - ref) <= 1e-12 * ref
 
 
+def test_host_pvalue_guards_match_the_oracle():
+    """The device's p-value helpers (compiled for the host) follow the reference's
+    guards as the oracle restates them (likelihood_jax.py:231-251, 668-673): F <= 0 ->
+    p = 1, a NaN statistic -> NaN (jnp.maximum / jnp.clip propagate it), a negative LRT
+    statistic -> p = 1, NaN log-likelihood -> NaN."""
+    df = 97.0
+    for f in (-3.0, 0.0, 1e-12, 0.3, 4.0, 250.0, np.inf, np.nan):
+        ref = float(ol._f_pvalue(df, np.array([f]), np.array([True]))[0])
+        got = _lib.lib.jm_host_f_pvalue(df, f)
+        if np.isnan(ref):
+            assert np.isnan(got), f
+        else:
+            assert abs(got - ref) <= 1e-9 * max(ref, 1e-300), (f, got, ref)
+    for h1, h0 in ((-10.0, -12.0), (-12.0, -10.0), (-5.0, -5.0), (np.nan, -3.0), (-3.0, np.nan)):
+        ref = float(ol.lrt_pvalue(np.array([h1]), h0)[0])
+        got = _lib.lib.jm_host_lrt_pvalue(h1, h0)
+        if np.isnan(ref):
+            assert np.isnan(got), (h1, h0)
+        else:
+            assert abs(got - ref) <= 1e-12 * ref, (h1, h0, got, ref)
+
+
 # ------------------------------------------------------------ spectrum plan
 def _kinship_spectrum(n=400, p=4000, seed=0):
     from oracle import kinship as ok
@@ -259,6 +281,46 @@ def test_library_eigen_limit_is_the_32_bit_bound():
 
     assert eg._LIB_MAX_N ** 2 <= 2**31 - 1 < (eg._LIB_MAX_N + 1) ** 2
     assert eg._LARGE_LEAF <= eg._LIB_MAX_N
+
+
+@pytest.mark.parametrize("own_min", [8192, 10**9])
+def test_eigh_device_never_routes_beyond_32_bits_to_rocsolver(monkeypatch, own_min):
+    """eigh_device sends every n > 46,340 to the 64-bit path, even if the switch-over
+    size were raised past it, and never opens rocSOLVER for it. A meta tensor stands in
+    for the matrix (shape only), and the solvers are replaced by stubs, so nothing
+    touches a GPU."""
+    import torch
+
+    from jamma_amd.lmm import eigen as eg
+
+    monkeypatch.setattr(eg, "_OWN_MIN_N", own_min)
+    assert eg._select_solver(eg._LIB_MAX_N + 1) is eg._eigh_large
+    assert eg._select_solver(50_000) is eg._eigh_large
+    assert eg._select_solver(7) is eg._dsyevd
+    assert eg._select_solver(eg._LIB_MAX_N) is (eg._dsyevd if own_min > eg._LIB_MAX_N
+                                               else eg._eigh_large)
+
+    def no_solver(*a, **k):
+        raise AssertionError("rocSOLVER opened for a matrix beyond its 32-bit limit")
+
+    calls = []
+
+    def large_stub(Kd, progress=None):
+        calls.append(tuple(Kd.shape))
+        return torch.arange(Kd.shape[0], dtype=torch.float64) + 1.0, None
+
+    monkeypatch.setattr(eg, "_solver", no_solver)
+    monkeypatch.setattr(eg, "_dsyevd", no_solver)
+    monkeypatch.setattr(eg, "_eigh_large", large_stub)
+    monkeypatch.setattr(eg, "require_gpu", lambda: torch.device("meta"))
+    monkeypatch.setattr(eg, "to_dev", lambda a, dtype=None, device=None: a)
+    K = torch.empty((50_000, 50_000), dtype=torch.float64, device="meta")
+    d, U = eg.eigh_device(K)
+    assert calls == [(50_000, 50_000)] and d.shape == (50_000,) and U is None
+    # the 32-bit routine itself refuses such a size before touching the device
+    monkeypatch.undo()
+    with pytest.raises(ValueError, match="32-bit"):
+        eg._dsyevd(K)
 
 
 def _merge_case(kind, seed):

@@ -60,3 +60,19 @@ def phenotype(G, n_causal=20, h2=0.5, n_cov=3, seed=1):
     if W is not None:
         y = y + W[:, 1:] @ rng.normal(scale=0.3, size=n_cov)
     return y, W
+
+
+def eig_residuals(Kd, d, U):
+    """(max |K U - U D| / max |d|, max |U^T U - I|) with torch's FP64 GEMM."""
+    import torch
+
+    dd = torch.as_tensor(np.asarray(d), dtype=torch.float64, device=Kd.device)
+    Uc = U.contiguous()
+    R = Kd @ Uc
+    R -= Uc * dd[None, :]
+    res = float(R.abs().max()) / float(dd.abs().max())
+    del R
+    O = Uc.t().contiguous() @ Uc
+    O.diagonal().sub_(1.0)
+    orth = float(O.abs().max())
+    return res, orth
