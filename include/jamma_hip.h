@@ -215,6 +215,24 @@ int jm_spectrum_moments(const void* Z, int z_is_f32, int64_t ldz, int64_t n_snps
                         const double* V, int64_t ldv, int nv, const jm_spectrum_plan* plan,
                         double* work, double* mom, int64_t ldm, void* stream);
 
+/*
+ * Allocation-free form of jm_spectrum_moments (no counterpart in the reference:
+ * for callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments and results
+ * (bitwise) without `work`: the per-piece partials go to a static device array of
+ * JM_MOMENTS_SCRATCH_BYTES that each GPU holds for the life of the process (its
+ * address is looked up once per device at the first call). Calls needing more
+ * than that (jm_moments_work_bytes > JM_MOMENTS_SCRATCH_BYTES) are rejected with
+ * JM_ERR_ARG: pass fewer columns per call. The entry point allocates nothing;
+ * calls that overlap on one GPU share the array and race.
+ */
+#define JM_MOMENTS_SCRATCH_BYTES (64ll << 20)
+int64_t jm_moments_scratch_bytes(void);
+int jm_spectrum_moments_noalloc(const void* Z, int z_is_f32, int64_t ldz, int64_t n_snps,
+                                const double* V, int64_t ldv, int nv,
+                                const jm_spectrum_plan* plan, double* mom, int64_t ldm,
+                                void* stream);
+
 /* Per-column descriptor of jm_spectrum_moments_multi: the partner rows
  * (nv x ldv float64) and the output block ((nv+1) x ldm float64) of one column. */
 typedef struct jm_mom_col {

@@ -84,6 +84,9 @@ _SIGS = {
     "jm_moments_work_bytes": ([c_void_p, c_int64, c_int], c_int64),
     "jm_spectrum_moments": ([c_void_p, c_int, c_int64, c_int64, c_void_p, c_int64, c_int, c_void_p,
                              c_void_p, c_void_p, c_int64, c_void_p], c_int),
+    "jm_moments_scratch_bytes": ([], c_int64),
+    "jm_spectrum_moments_noalloc": ([c_void_p, c_int, c_int64, c_int64, c_void_p, c_int64, c_int,
+                                     c_void_p, c_void_p, c_int64, c_void_p], c_int),
     "jm_spectrum_moments_multi": ([c_void_p, c_int, c_int64, c_int64, c_void_p, c_int, c_void_p,
                                    c_void_p, c_void_p], c_int),
     "jm_lmm_assoc_multi": ([c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64,

@@ -17,6 +17,8 @@
 
 #include "jm_common.h"
 
+#include <mutex>
+
 namespace jm {
 
 constexpr int kMomMaxTerms = 16;  // MAX_TERMS_PER_CLUSTER on the host side
@@ -305,4 +307,64 @@ extern "C" int jm_spectrum_moments_multi(const void* Z, int z_is_f32, int64_t ld
                      pl, work, nullptr, 0);
   JM_CHECK_LAUNCH();
   return JM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant: the per-piece partials live in a fixed device array.
+// ---------------------------------------------------------------------------
+// The array is jm::scratch; every GPU the library runs on holds its own copy
+// (JM_MOMENTS_SCRATCH_BYTES) until the process ends. Host code never uses the
+// array's name as an address: its device address is looked up once per device
+// with hipGetSymbolAddress (with that device current) and kept in g_scratch_ptr.
+// k_mom_seg writes every (column, row, slot) partial that k_mom_fold then reads,
+// so nothing has to start at zero; calls that overlap on one GPU share the array
+// and race. The array has external linkage (inside namespace jm): with internal
+// linkage the device code object keeps it as a local symbol under another name
+// than the one the host registers, and the runtime's symbol lookup aborts.
+namespace jm {
+__device__ __align__(256) double scratch[JM_MOMENTS_SCRATCH_BYTES / sizeof(double)];
+}  // namespace jm
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(double** out) {
+  int dev = -1;
+  JM_CHECK_HIP(hipGetDevice(&dev));
+  JM_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    JM_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(jm::scratch)));
+    JM_CHECK_ARG(p != nullptr, "hipGetSymbolAddress returned a null address");
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return JM_OK;
+}
+}  // namespace
+
+extern "C" int64_t jm_moments_scratch_bytes(void) { return JM_MOMENTS_SCRATCH_BYTES; }
+
+extern "C" int jm_spectrum_moments_noalloc(const void* Z, int z_is_f32, int64_t ldz,
+                                           int64_t n_snps, const double* V, int64_t ldv, int nv,
+                                           const jm_spectrum_plan* plan, double* mom, int64_t ldm,
+                                           void* stream) {
+  int rc = check_plan(plan);
+  if (rc) return rc;
+  JM_CHECK_ARG(n_snps >= 0 && nv >= 0, "bad sizes");
+  const int64_t need = jm_moments_work_bytes(plan, n_snps, nv);
+  JM_CHECK_ARG(need <= (int64_t)JM_MOMENTS_SCRATCH_BYTES,
+               "jm_spectrum_moments_noalloc: %lld SNP columns x %d rows x %d slots need %lld "
+               "bytes of partials, more than the fixed %lld-byte scratch; pass fewer columns "
+               "per call",
+               (long long)n_snps, nv + 1, plan->n_slots, (long long)need,
+               (long long)JM_MOMENTS_SCRATCH_BYTES);
+  if (n_snps == 0) return JM_OK;
+  double* work = nullptr;
+  rc = scratch_for_current_device(&work);
+  if (rc) return rc;
+  return jm_spectrum_moments(Z, z_is_f32, ldz, n_snps, V, ldv, nv, plan, work, mom, ldm, stream);
 }

@@ -372,3 +372,26 @@ def test_merge_deflation_keeps_the_spectrum(kind):
         off[r] = 0.0
         assert abs(R[r, r] - v) <= 1e-13 * np.max(np.abs(ref))
         assert np.max(off) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_noalloc_moments_rejects_more_than_its_scratch():
+    """jm_spectrum_moments_noalloc checks the partials' size against its fixed scratch
+    before any device call: a plan with fake (never dereferenced) array pointers and
+    one column too many returns JM_ERR_ARG with a message, on a machine without a GPU."""
+    import ctypes
+
+    pl = _lib.SpectrumPlanC()
+    pl.n, pl.n_terms, pl.n_clusters, pl.n_seg, pl.n_slots = 1000, 200, 50, 60, 250
+    for f in ("cl_terms", "cl_toff", "cl_seg0", "cl_nseg", "seg_start", "seg_len",
+              "seg_cluster", "seg_slot", "delta", "term_cluster", "term_m"):
+        setattr(pl, f, 8)
+    cap = int(_lib.lib.jm_moments_scratch_bytes())
+    per_col = int(_lib.lib.jm_moments_work_bytes(ctypes.byref(pl), 1, 5))
+    assert per_col == 6 * 250 * 8
+    too_many = cap // per_col + 1
+    rc = _lib.lib.jm_spectrum_moments_noalloc(8, 0, too_many, too_many, 8, 1000, 5,
+                                              ctypes.byref(pl), 8, 200, None)
+    assert rc == -1 and "scratch" in _lib.last_error()
+    # zero columns is a no-op that succeeds without touching a device
+    assert _lib.lib.jm_spectrum_moments_noalloc(8, 0, 1, 0, 8, 1000, 5, ctypes.byref(pl), 8,
+                                                200, None) == 0
