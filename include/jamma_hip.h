@@ -428,6 +428,41 @@ int jm_apply_rotations(int64_t r, const int64_t* pq, const double* cs, int64_t n
                        int64_t ldm, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Results files (host code, several threads into one caller buffer)         */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * GEMMA .assoc.txt rows, byte-identical to the reference's writers
+ * (src/jamma/lmm/io.py:12-147 via IncrementalAssocWriter :172-262): row r is
+ *   pre[pre_off[r] .. pre_off[r+1])  (the row's "chr\trs\tps\t")
+ *   n_miss[r] "\t"
+ *   mid[mid_off[r] .. mid_off[r+1])  ("allele1\tallele0\t")
+ *   af[r] as %.3f, then n_stats values of stats row r (row stride ld_stats) as
+ *   "\t%.6e" each, then "\n".
+ * NaN prints "nan" (Python's spelling). Returns the bytes written, or minus the
+ * bytes needed when out_cap is too small (nothing written), INT64_MIN on bad
+ * arguments. n_threads host threads (<= 64) format row ranges in parallel.
+ */
+int64_t jm_format_assoc_rows(int64_t n_rows, const char* pre, const int64_t* pre_off,
+                             const int64_t* n_miss, const char* mid, const int64_t* mid_off,
+                             const double* af, const double* stats, int n_stats,
+                             int64_t ld_stats, char* out, int64_t out_cap, int n_threads);
+
+/* Rows of a matrix as tab-separated %.10g values, one line each: the GEMMA
+ * .cXX.txt / .eigenU.txt (one row per line) and .eigenD.txt (n_cols = 1)
+ * formats (kinship/io.py, lmm/eigen_io.py:98-137). Same return convention. */
+int64_t jm_format_g10_rows(int64_t n_rows, int64_t n_cols, const double* a, int64_t lda, char* out,
+                           int64_t out_cap, int n_threads);
+
+/* Parses whitespace-separated numbers, one matrix row per non-blank line (the
+ * GEMMA text files above), into out (row-major, *n_rows x *n_cols; the column
+ * count is the first line's). Returns the number of values, or -2 when they exceed
+ * out_cap, -3 for a ragged row, -4 for a token that is not a number, -1 on bad
+ * arguments. Conversion by strtod (correctly rounded), rows split over threads. */
+int64_t jm_parse_text_rows(const char* buf, int64_t len, double* out, int64_t out_cap,
+                           int64_t* n_rows, int64_t* n_cols, int n_threads);
+
+/* ------------------------------------------------------------------------ */
 /* Host-callable special functions (the device code compiled for the CPU),   */
 /* exported so the CPU test-suite can check them against SciPy.              */
 /* ------------------------------------------------------------------------ */

@@ -120,6 +120,13 @@ _SIGS = {
                            c_int),
     "jm_apply_rotations": ([c_int64, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p],
                            c_int),
+    "jm_format_assoc_rows": ([c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p, c_void_p, c_int, c_int64, c_void_p, c_int64, c_int],
+                             c_int64),
+    "jm_format_g10_rows": ([c_int64, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_int],
+                           c_int64),
+    "jm_parse_text_rows": ([c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p, c_int],
+                           c_int64),
     "jm_host_betainc": ([c_double, c_double, c_double], c_double),
     "jm_host_chi2_sf1": ([c_double], c_double),
     "jm_host_f_pvalue": ([c_double, c_double], c_double),

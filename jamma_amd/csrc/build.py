@@ -18,7 +18,7 @@ PKG = HERE.parent
 REPO = PKG.parent
 BUILD = HERE / "_build"
 LIB = PKG / "lib" / "libjamma_hip.so"
-SOURCES = ["jm_common.cpp", "decode.hip", "kinship.hip", "rotate.hip", "moments.hip", "lmm.hip",
+SOURCES = ["jm_common.cpp", "format.cpp", "decode.hip", "kinship.hip", "rotate.hip", "moments.hip", "lmm.hip",
            "eigen.hip"]
 ARCH = os.environ.get("JM_OFFLOAD_ARCH", "gfx950")
 

@@ -9,7 +9,9 @@ import numpy as np
 
 
 def read_kinship_matrix(path: Path, n_samples: int | None = None) -> np.ndarray:
-    K = np.loadtxt(path, dtype=np.float64)
+    from jamma_amd.lmm.io import parse_matrix_text
+
+    K = parse_matrix_text(path)
     if K.ndim != 2 or K.shape[0] != K.shape[1]:
         raise ValueError(f"Kinship matrix must be square, got shape {K.shape}")
     if n_samples is not None and K.shape[0] != n_samples:
@@ -21,12 +23,15 @@ def read_kinship_matrix(path: Path, n_samples: int | None = None) -> np.ndarray:
 
 
 def write_kinship_matrix(K: np.ndarray, path: Path) -> None:
-    """10 significant digits (%.10g), tab separated, no header."""
+    """10 significant digits (%.10g), tab separated, no header (the library's
+    multi-threaded formatter; the same bytes as the reference's f-strings)."""
+    from jamma_amd.lmm.io import format_matrix_g10
+
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
-    with open(path, "w") as f:
-        for i in range(K.shape[0]):
-            f.write("\t".join(f"{v:.10g}" for v in K[i]) + "\n")
+    if hasattr(K, "detach"):
+        K = K.detach().cpu().numpy()
+    path.write_bytes(format_matrix_g10(np.asarray(K, dtype=np.float64)))
 
 
 def write_loco_kinship_matrices(loco_kinships: Iterator[tuple[str, np.ndarray]], output_dir: Path,

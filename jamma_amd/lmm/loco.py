@@ -29,8 +29,8 @@ from jamma_amd.lmm.engine import AssocProblem, Cohort
 from jamma_amd.lmm.io import IncrementalAssocWriter
 from jamma_amd.lmm.runner_streaming import (
     TILE_SNPS,
+    TileSink,
     _TEST_TYPE,
-    _make_result,
     _meta_snp_info,
     default_precision,
     snp_filter,
@@ -162,6 +162,9 @@ def run_lmm_loco_multi(
             chr_idx = partitions[chr_name]
             raw, lo, local, means, af, miss_c = _chromosome_tiles(
                 bed, chr_idx, sample_idx, n, maf_threshold, miss_threshold, snps_indices)
+            sinks = [TileSink(lmm_mode, snp_info, af, miss_c,
+                              writers[k] if writers is not None else None, results[k])
+                     for k in range(n_ph)]
             for t0i in range(0, len(local), TILE_SNPS):
                 tl = local[t0i:t0i + TILE_SNPS]
                 gsnp = chr_idx[tl]
@@ -171,15 +174,10 @@ def run_lmm_loco_multi(
                 Z, f32, ldz = cohort.rotate_genotypes(g, ld, mdev, len(tl), precision)
                 for k, pr in enumerate(probs):
                     xm = cohort.moments(Z, f32, ldz, len(tl), pr.V, pr.n_cvt + 1)
-                    o = pr.assoc_moments(xm, len(tl), lmm_mode)[: len(tl)].cpu().numpy()
-                    for j, li in enumerate(tl):
-                        s = int(chr_idx[li])
-                        r = _make_result(lmm_mode, snp_info[s], af[li], miss_c[li], o[j])
-                        if writers is not None:
-                            writers[k].write(r)
-                        else:
-                            results[k].append(r)
+                    sinks[k].push(chr_idx[tl], tl, pr.assoc_moments(xm, len(tl), lmm_mode))
                 del Z
+            for sk in sinks:
+                sk.close()
             if show_progress:
                 logger.info("LOCO chr %s: %d SNPs, T=%d", chr_name, len(local), cohort.n_terms)
             del cohort, probs

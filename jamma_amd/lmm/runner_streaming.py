@@ -22,12 +22,12 @@ from pathlib import Path
 import numpy as np
 import torch
 
-from jamma_amd._lib import OUT
+from jamma_amd._lib import JM_NOUT, OUT
 from jamma_amd.device import require_gpu, round_up, to_dev
 from jamma_amd.io.plink import BedFile, decode, get_plink_metadata, stats_from_counts
 from jamma_amd.lmm.eigen import eigh_device
 from jamma_amd.lmm.engine import AssocProblem, Cohort
-from jamma_amd.lmm.io import IncrementalAssocWriter
+from jamma_amd.lmm.io import IncrementalAssocWriter, snp_text_parts
 from jamma_amd.lmm.stats import AssocResult
 
 logger = logging.getLogger("jamma_amd")
@@ -88,6 +88,50 @@ def _make_result(mode: int, info: dict, af: float, n_miss: int, row: np.ndarray)
     return AssocResult(**meta, beta=g("beta"), se=g("se"), logl_H1=g("logl_H1"),
                        l_remle=g("l_remle"), l_mle=g("l_mle"), p_wald=g("p_wald"),
                        p_lrt=g("p_lrt"), p_score=g("p_score"))
+
+
+class TileSink:
+    """Consumes the device result rows of SNP tiles one tile behind their launch.
+
+    ``push`` starts a non-blocking copy of a tile's rows into pinned host memory and
+    then finishes the previous tile: with a writer its rows go to the writer's
+    background formatter (GEMMA text, no per-SNP Python objects), otherwise they
+    become AssocResults (_yield_chunk_results, results.py:222-286). The device works
+    on the next tile meanwhile. ``close`` finishes the last tile."""
+
+    def __init__(self, mode: int, snp_info, af, miss, writer=None, results=None):
+        self.mode, self.snp_info, self.af, self.miss = mode, snp_info, af, miss
+        self.writer = writer
+        self.results = results if results is not None else []
+        self._pending = None
+
+    def push(self, snps: np.ndarray, stat_idx: np.ndarray, out_dev: torch.Tensor) -> None:
+        """snps: global SNP indices (snp_info rows); stat_idx: their rows of af / miss."""
+        k = len(snps)
+        host = torch.empty((k, JM_NOUT), dtype=torch.float64, pin_memory=True)
+        host.copy_(out_dev[:k], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        prev, self._pending = self._pending, (np.asarray(snps), np.asarray(stat_idx), host, ev)
+        if prev is not None:
+            self._finish(*prev)
+
+    def close(self) -> None:
+        prev, self._pending = self._pending, None
+        if prev is not None:
+            self._finish(*prev)
+
+    def _finish(self, snps, stat_idx, host, ev) -> None:
+        ev.synchronize()
+        rows = host.numpy()
+        if self.writer is not None:
+            parts = snp_text_parts([self.snp_info[int(s)] for s in snps])
+            self.writer.write_rows(parts, self.miss[stat_idx], self.af[stat_idx], rows)
+        else:
+            for j, s in enumerate(snps):
+                i = stat_idx[j]
+                self.results.append(_make_result(self.mode, self.snp_info[int(s)], self.af[i],
+                                                 self.miss[i], rows[j]))
 
 
 def _prepare_problem(kinship, eigenvalues, eigenvectors, phenotypes, covariates, precision, l_min,
@@ -204,6 +248,7 @@ def run_lmm_association_streaming(
         writer = IncrementalAssocWriter(output_path, test_type=_TEST_TYPE[lmm_mode])
         writer.__enter__()
     results: list[AssocResult] = []
+    sink = TileSink(lmm_mode, snp_info, af, miss, writer, results)
     try:
         ld = round_up(n, 64)
         for start in range(0, n_snps, chunk_size):
@@ -221,13 +266,13 @@ def run_lmm_association_streaming(
                                  ld_g=ld)
                 mdev = means_dev_all[torch.as_tensor(tsel, device=means_dev_all.device)]
                 out = prob.associate(g, ld, mdev.contiguous(), len(tsel), lmm_mode, precision)
-                out_h = out[: len(tsel)].cpu().numpy()
-                for j, s in enumerate(tsel):
-                    r = _make_result(lmm_mode, snp_info[s], af[s], miss[s], out_h[j])
-                    if writer is not None:
-                        writer.write(r)
-                    else:
-                        results.append(r)
+                sink.push(tsel, tsel, out)
+        sink.close()
+    except BaseException as e:
+        if writer is not None:
+            writer.__exit__(type(e), e, None)
+            writer = None
+        raise
     finally:
         if writer is not None:
             writer.__exit__(None, None, None)
@@ -304,15 +349,15 @@ def run_lmm_association_jax(
                                     precision, l_min, l_max, n_grid, n_refine)
     ld = round_up(n, 64)
     results = []
+    sink = TileSink(lmm_mode, snp_info, af, miss, None, results)
     for t0i in range(0, len(snp_indices), TILE_SNPS):
         tsel = snp_indices[t0i:t0i + TILE_SNPS]
         gt = np.zeros((len(tsel), ld), dtype=np.int8)
         gt[:, :n] = gi8[tsel]
         g = to_dev(gt, torch.int8)
         mdev = to_dev(means[tsel], torch.float64)
-        out = prob.associate(g, ld, mdev, len(tsel), lmm_mode, precision)[: len(tsel)].cpu().numpy()
-        for j, s in enumerate(tsel):
-            results.append(_make_result(lmm_mode, snp_info[s], af[s], miss[s], out[j]))
+        sink.push(tsel, tsel, prob.associate(g, ld, mdev, len(tsel), lmm_mode, precision))
+    sink.close()
     if show_progress:
         logger.info("LMM association (%s, in-memory, %d SNPs) done", precision, len(snp_indices))
     return results

@@ -97,8 +97,14 @@ class PipelineRunner:
                         (c.eigenvalue_file, "Eigenvalue"), (c.eigenvector_file, "Eigenvector")):
             if p is not None and not Path(p).exists():
                 raise FileNotFoundError(f"{what} file not found: {p}")
-        if not 0.0 <= c.hwe_threshold <= 1.0:
-            raise ValueError(f"hwe_threshold must be in [0, 1], got {c.hwe_threshold}")
+        if c.hwe_threshold < 0:
+            raise ValueError(f"hwe_threshold must be >= 0, got {c.hwe_threshold}")
+        if c.hwe_threshold > 1.0:
+            raise ValueError(f"hwe_threshold must be in [0, 1] (p-value threshold), got "
+                             f"{c.hwe_threshold}")
+        if c.hwe_threshold > 0 and c.loco:  # pipeline.py:234-238
+            raise ValueError("-hwe is not yet supported with -loco mode. Apply HWE filtering as a "
+                             "pre-processing step.")
 
     def parse_phenotypes(self) -> tuple[np.ndarray, int]:
         """.fam column 5 + phenotype_column; "-9"/"NA" -> NaN (pipeline.py:240-293)."""
@@ -140,7 +146,9 @@ class PipelineRunner:
             t1 = time.perf_counter()
             res = run_lmm_loco(c.bfile, phenotypes, covariates=cov, maf_threshold=c.maf,
                                miss_threshold=c.miss, lmm_mode=c.lmm_mode, output_path=assoc_path,
-                               show_progress=c.show_progress, snps_indices=snps_idx,
+                               show_progress=c.show_progress, save_kinship=c.save_kinship,
+                               kinship_output_dir=c.output_dir,
+                               kinship_output_prefix=c.output_prefix, snps_indices=snps_idx,
                                ksnps_indices=ksnps_idx, precision=c.precision)
             t2 = time.perf_counter()
             return PipelineResult(res, n_analyzed, n_snps, assoc_path,
@@ -160,6 +168,21 @@ class PipelineRunner:
             K = kinship_streaming_device(c.bfile, ksnps_indices=ksnps_idx).cpu().numpy()
         if K is not None and c.save_kinship:
             write_kinship_matrix(K, Path(c.output_dir) / f"{c.output_prefix}.cXX.txt")
+        if K is not None and c.write_eigen:
+            # pipeline.py:549-566: eigendecompose the valid-sample kinship, write the GEMMA
+            # eigen files, and hand the decomposition to the association (no second one)
+            from jamma_amd.lmm.eigen import eigh_device
+            from jamma_amd.lmm.eigen_io import write_eigen_files
+
+            valid = ~np.isnan(phenotypes) & (phenotypes != -9.0)
+            Kv = K if np.all(valid) else K[np.ix_(valid, valid)]
+            d, U = eigh_device(Kv)
+            write_eigen_files(d, U, c.output_dir, c.output_prefix)
+            cov0 = self.load_covariates(n_samples)
+            if cov0 is None or not np.any(np.isnan(cov0[valid])):
+                eigenvalues, eigenvectors = d, U
+            # else the runner also drops samples with missing covariates and decomposes
+            # its own subset; the files hold the phenotype-valid samples, as the reference's
         kinship_s = time.perf_counter() - t1
         cov = self.load_covariates(n_samples)
         t2 = time.perf_counter()

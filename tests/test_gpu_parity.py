@@ -394,3 +394,114 @@ def test_loco_runner_matches_oracle(tmp_path):
     p2 = np.concatenate([ref2[c]["p_wald"] for c in sorted(ref2)])
     np.testing.assert_allclose(-np.log10(_arrays(multi[1], ("p_wald",))["p_wald"]), -np.log10(p2),
                                atol=1e-6)
+
+
+def test_streaming_runner_hwe_filter(tmp_path):
+    """-hwe in the streaming runner (runner_streaming.py:265-305): counts over the
+    valid samples, SNPs with chi-square HWE p < threshold removed, the others'
+    results unchanged. Expected set from the closed form with SciPy's chi2."""
+    from scipy import stats as sst
+
+    from jamma_amd.lmm import run_lmm_association_streaming
+
+    rng = np.random.default_rng(21)
+    n, p = 300, 120
+    G = rng.binomial(2, rng.uniform(0.1, 0.5, p), size=(n, p)).astype(float)
+    # HWE violations: heterozygote deficit / excess
+    G[:, 10] = np.where(rng.random(n) < 0.5, 0.0, 2.0)
+    G[:, 20] = np.where(rng.random(n) < 0.85, 1.0, 0.0)
+    G[rng.random((n, p)) < 0.01] = np.nan
+    y = rng.normal(size=n) + 0.2 * np.nan_to_num(G[:, 3])
+    y[:7] = np.nan
+    pre = tmp_path / "hwe"
+    opl.write_plink(pre, G, phenotypes=y)
+    K = ok.centered_kinship(G)
+    base = run_lmm_association_streaming(pre, y, kinship=K, lmm_mode=1, precision="f64")
+    thr = 1e-3
+    filt = run_lmm_association_streaming(pre, y, kinship=K, lmm_mode=1, precision="f64",
+                                         hwe_threshold=thr)
+    Gv = G[~np.isnan(y)]
+    aa, ab, bb = ((Gv == v).sum(0).astype(float) for v in (0, 1, 2))
+    tot = aa + ab + bb
+    with np.errstate(invalid="ignore", divide="ignore"):
+        f = (2 * aa + ab) / (2 * tot)
+        e = [tot * f ** 2, 2 * tot * f * (1 - f), tot * (1 - f) ** 2]
+        chi = sum((o - x) ** 2 / x for o, x in zip((aa, ab, bb), e))
+    hwe_p = sst.chi2.sf(np.nan_to_num(chi, nan=0.0), 1)
+    rs_base = [r.rs for r in base]
+    keep = {f"rs{j:06d}" for j in range(p) if hwe_p[j] >= thr}
+    assert [r.rs for r in filt] == [r for r in rs_base if r in keep]
+    assert "rs000010" not in [r.rs for r in filt] and "rs000020" not in [r.rs for r in filt]
+    assert "rs000010" in rs_base and "rs000020" in rs_base
+    by_rs = {r.rs: r for r in base}
+    for r in filt:
+        assert r.p_wald == by_rs[r.rs].p_wald and r.beta == by_rs[r.rs].beta
+
+
+def test_runner_file_output_equals_per_result_lines(tmp_path):
+    """The streaming runner's file path (native tile formatter on the writer thread)
+    writes exactly the lines the per-result formatter gives for the results the
+    in-memory path returns, for every mode (lmm/io.py:12-262)."""
+    from jamma_amd.lmm import run_lmm_association_streaming
+    from jamma_amd.lmm.io import _HEADERS, format_line
+
+    G, y, K, C = _fixture_data()
+    for mode, kind in ((1, "wald"), (2, "lrt"), (3, "score"), (4, "all")):
+        res = run_lmm_association_streaming(synthetic_prefix(), y, kinship=K, covariates=C,
+                                            lmm_mode=mode, chunk_size=96, precision="f64")
+        out = tmp_path / f"m{mode}.assoc.txt"
+        assert run_lmm_association_streaming(synthetic_prefix(), y, kinship=K, covariates=C,
+                                             lmm_mode=mode, chunk_size=96, output_path=out,
+                                             precision="f64") == []
+        want = _HEADERS[kind] + "\n" + "".join(format_line(r, kind) + "\n" for r in res)
+        assert out.read_text() == want, mode
+
+
+def test_gwas_write_eigen_then_reuse(tmp_path):
+    """gwas(write_eigen=True) writes the valid-sample eigendecomposition in GEMMA format
+    (pipeline.py:549-566); a second gwas() reading those files (-d / -u) gives the same
+    association to the files' 10 significant digits."""
+    import jamma_amd
+    from jamma_amd.lmm.eigen_io import read_eigen_files
+
+    pre = synthetic_prefix()
+    r1 = jamma_amd.gwas(pre, lmm_mode=4, output_dir=tmp_path / "a", output_prefix="t",
+                        write_eigen=True, save_kinship=True, show_progress=False,
+                        precision="f64")
+    dp, up = tmp_path / "a" / "t.eigenD.txt", tmp_path / "a" / "t.eigenU.txt"
+    d, U = read_eigen_files(dp, up, n_samples=r1.n_samples)
+    K = np.loadtxt(tmp_path / "a" / "t.cXX.txt")
+    gem = np.loadtxt(GOLDEN / "gemma_synthetic" / "gemma_kinship.cXX.txt")
+    assert np.max(np.abs(K - gem)) < 1e-9
+    y = fam_phenotypes(pre)
+    valid = ~np.isnan(y)
+    Kv = gem[np.ix_(valid, valid)]
+    assert np.max(np.abs((U * d) @ U.T - Kv)) < 1e-8
+    r2 = jamma_amd.gwas(pre, lmm_mode=4, output_dir=tmp_path / "b", output_prefix="t",
+                        eigenvalue_file=dp, eigenvector_file=up, show_progress=False,
+                        precision="f64")
+    a = load_assoc(tmp_path / "a" / "t.assoc.txt")
+    b = load_assoc(tmp_path / "b" / "t.assoc.txt")
+    assert list(a["rs"]) == list(b["rs"]) and r1.n_samples == r2.n_samples
+    for k in ("p_wald", "p_lrt", "p_score"):
+        np.testing.assert_allclose(-np.log10(b[k]), -np.log10(a[k]), atol=2e-6, err_msg=k)
+
+
+def test_gwas_loco_saves_kinships(tmp_path):
+    """gwas(loco=True, save_kinship=True) forwards the kinship output to the LOCO runner
+    (pipeline.py:491-506): one .loco.cXX.chr<c>.txt per chromosome, equal to the
+    oracle's LOCO kinships at 10 significant digits."""
+    import jamma_amd
+
+    G = balding_nichols(150, 300, miss=0.01, seed=31)
+    chrom = np.array([str(1 + (j * 3) // 300) for j in range(300)])
+    y, _ = phenotype(G, n_cov=0, seed=31)
+    pre = tmp_path / "lc"
+    opl.write_plink(pre, G, phenotypes=y, chromosomes=chrom)
+    jamma_amd.gwas(pre, lmm_mode=1, loco=True, save_kinship=True, output_dir=tmp_path / "o",
+                   output_prefix="r", show_progress=False, precision="f64")
+    ref = ok.loco_kinship(G, chrom, 0.01, 0.05)
+    for c in ("1", "2", "3"):
+        K = np.loadtxt(tmp_path / "o" / f"r.loco.cXX.chr{c}.txt")
+        np.testing.assert_allclose(K, ref[c], rtol=1e-9, atol=1e-12)
+    assert (tmp_path / "o" / "r.assoc.txt").exists()

@@ -395,3 +395,187 @@ def test_noalloc_moments_rejects_more_than_its_scratch():
     # zero columns is a no-op that succeeds without touching a device
     assert _lib.lib.jm_spectrum_moments_noalloc(8, 0, 1, 0, 8, 1000, 5, ctypes.byref(pl), 8,
                                                 200, None) == 0
+
+
+def test_hwe_pvalues_closed_form():
+    """compute_hwe_pvalues (core/snp_filter.py:94-150) on crafted genotype counts:
+    counts in exact HWE give chi2 = 0 -> p = 1; (50, 0, 50) has p = 1/2 and expected
+    (25, 50, 25), so chi2 = 25 + 50 + 25 = 100; (81, 18, 1) is exact HWE at p = 0.9;
+    (60, 20, 20): p = 0.7, expected (49, 42, 9), chi2 = 121/49 + 484/42 + 121/9;
+    monomorphic and empty SNPs give NaN chi2 -> p = 1 by the reference's convention."""
+    from jamma_amd.lmm.runner_streaming import hwe_pvalues
+
+    aa = np.array([25, 50, 81, 60, 100, 0, 0])
+    ab = np.array([50, 0, 18, 20, 0, 0, 0])
+    bb = np.array([25, 50, 1, 20, 0, 0, 40])
+    chi = np.array([0.0, 100.0, 0.0, 121 / 49 + 484 / 42 + 121 / 9, 0.0, 0.0, 0.0])
+    got = hwe_pvalues(aa, ab, bb)
+    want = stats.chi2.sf(chi, 1)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-300)
+    assert got[0] == 1.0 and got[4] == 1.0 and got[5] == 1.0 and got[6] == 1.0
+    assert abs(got[1] - special.erfc(np.sqrt(50.0))) <= 1e-12 * got[1]
+    # the filter keeps p >= threshold (runner_streaming.py:298-305)
+    assert list(got >= 1e-3) == [True, False, True, False, True, True, True]
+
+
+# ------------------------------------------------------------ native writer
+FIXTURE_FILES = {
+    "wald": "gemma_synthetic/gemma_assoc.assoc.txt",
+    "lrt": "gemma_synthetic/gemma_lrt.assoc.txt",
+    "score": "gemma_score/gemma_score.assoc.txt",
+    "all": "gemma_all_tests/gemma_all_covar.assoc.txt",
+}
+
+
+@pytest.mark.parametrize("kind", list(FIXTURE_FILES))
+def test_native_rows_reproduce_gemma_files(kind):
+    """jm_format_assoc_rows re-creates every data line of the GEMMA fixture files byte
+    for byte from their parsed fields (af as .3f, statistics as .6e), in the files'
+    own column sets (GEMMA 0.98's -lmm 1 / -lmm 4 files carry no logl_H1 column)."""
+    from jamma_amd.lmm.io import format_rows, snp_text_parts
+
+    lines = (GOLDEN / FIXTURE_FILES[kind]).read_text().splitlines()
+    head = lines[0].split("\t")
+    rows = [dict(zip(head, ln.split("\t"))) for ln in lines[1:]]
+    infos = [{"chr": r["chr"], "rs": r["rs"], "pos": int(r["ps"]), "a1": r["allele1"],
+              "a0": r["allele0"]} for r in rows]
+    stat_cols = head[7:]
+    stats = np.array([[float(r[c]) for c in stat_cols] for r in rows])
+    data = format_rows(snp_text_parts(infos), [int(r["n_miss"]) for r in rows],
+                       [float(r["af"]) for r in rows], stats, n_threads=3)
+    assert data.decode().splitlines() == lines[1:]
+
+
+def test_native_rows_match_python_formatting_on_hard_values():
+    """Values where a formatter could differ from Python's format(): ties at the 7th
+    significant digit, subnormals, huge and tiny exponents, +-0, +-inf, NaN of either
+    sign, af rounding at .0005; checked against the reference's f-strings."""
+    from jamma_amd.lmm.io import format_rows, snp_text_parts
+
+    rng = np.random.default_rng(4)
+    special_vals = [0.0, -0.0, np.inf, -np.inf, np.nan, -np.nan, 5e-324, -2.5e-308,
+                    1.0000005, 1.0000015, 2.5e-7, 9.9999995e10, 1e300, -1e-300,
+                    0.1, 1 / 3, 123456.5, 0.00012345675]
+    rand = (rng.standard_normal(3000) * 10.0 ** rng.integers(-300, 300, 3000)).tolist()
+    vals = np.array(special_vals + rand)
+    nst = 5
+    k = len(vals) // nst
+    stats = vals[: k * nst].reshape(k, nst)
+    af = np.concatenate([[0.0005, 0.0015, 0.0025, 0.1235, 0.9995, 1.0, np.nan],
+                         rng.uniform(0, 1, k - 7)])
+    n_miss = rng.integers(0, 10**6, k)
+    infos = [{"chr": str(1 + j % 22), "rs": f"rs{j}", "pos": j * 7, "a1": "A", "a0": "T"}
+             for j in range(k)]
+    got = format_rows(snp_text_parts(infos), n_miss, af, stats, n_threads=7).decode()
+    want = "".join("\t".join([i["chr"], i["rs"], str(i["pos"]), str(int(m)), "A", "T",
+                              f"{a:.3f}"] + [f"{v:.6e}" for v in row]) + "\n"
+                   for i, m, a, row in zip(infos, n_miss, af, stats))
+    assert got == want
+
+
+def test_native_g10_matches_reference_writer():
+    """%.10g rows (kinship / eigen files) equal the reference's f"{v:.10g}" joins."""
+    from jamma_amd.lmm.io import format_matrix_g10
+
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((37, 23)) * 10.0 ** rng.integers(-12, 12, (37, 23))
+    A[0, :6] = [0.0, -0.0, np.nan, 1e-5, 123456789012.0, 2.0 / 3.0]
+    want = "".join("\t".join(f"{v:.10g}" for v in row) + "\n" for row in A)
+    assert format_matrix_g10(A, n_threads=4).decode() == want
+    d = rng.standard_normal(11)
+    assert format_matrix_g10(d).decode() == "".join(f"{v:.10g}\n" for v in d)
+
+
+def test_incremental_writer_tiles_keep_order(tmp_path):
+    """write_rows queues tiles on the background thread; mixing them with per-result
+    write() keeps submission order, and the file equals the per-result writer's."""
+    from jamma_amd.lmm.io import IncrementalAssocWriter, snp_text_parts
+    from jamma_amd.lmm.runner_streaming import _make_result
+
+    rng = np.random.default_rng(6)
+    n = 1000
+    out = rng.standard_normal((n, _lib.JM_NOUT))
+    infos = [{"chr": "2", "rs": f"rs{j}", "pos": 10 * j, "a1": "C", "a0": "G"} for j in range(n)]
+    af = rng.uniform(0, 1, n)
+    miss = rng.integers(0, 50, n)
+    for mode, kind in ((1, "wald"), (2, "lrt"), (3, "score"), (4, "all")):
+        a, b = tmp_path / f"a{mode}.txt", tmp_path / f"b{mode}.txt"
+        with IncrementalAssocWriter(a, kind) as w:
+            for t0 in range(0, 600, 150):
+                sl = slice(t0, t0 + 150)
+                w.write_rows(snp_text_parts(infos[sl]), miss[sl], af[sl], out[sl])
+            for j in range(600, n):
+                w.write(_make_result(mode, infos[j], af[j], miss[j], out[j]))
+        with IncrementalAssocWriter(b, kind) as w:
+            for j in range(n):
+                w.write(_make_result(mode, infos[j], af[j], miss[j], out[j]))
+        assert a.read_bytes() == b.read_bytes()
+
+
+# ------------------------------------------------------------ kinship / eigen files
+def test_kinship_file_parse_matches_loadtxt(tmp_path):
+    """The native parser reads the GEMMA kinship fixture bit-identically to
+    np.loadtxt, and write_kinship_matrix reproduces the reference's %.10g text."""
+    from jamma_amd.kinship.io import read_kinship_matrix, write_kinship_matrix
+
+    path = GOLDEN / "gemma_synthetic" / "gemma_kinship.cXX.txt"
+    K = read_kinship_matrix(path, n_samples=100)
+    assert np.array_equal(K, np.loadtxt(path))
+    out = tmp_path / "k.cXX.txt"
+    write_kinship_matrix(K, out)
+    assert out.read_text() == "".join("\t".join(f"{v:.10g}" for v in row) + "\n" for row in K)
+    with pytest.raises(ValueError, match="does not match expected"):
+        read_kinship_matrix(path, n_samples=99)
+
+
+def test_eigen_files_round_trip_and_validation(tmp_path):
+    """write_eigen_files / read_eigen_files (lmm/eigen_io.py:42-164): GEMMA text at 10
+    significant digits round-trips to 5e-10 relative; the reference's shape checks
+    raise with its messages."""
+    from jamma_amd.lmm.eigen_io import read_eigen_files, write_eigen_files
+
+    rng = np.random.default_rng(8)
+    n = 57
+    d = np.sort(rng.uniform(0, 3, n))
+    U, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    dp, up = write_eigen_files(d, U, tmp_path / "e", "cohort")
+    assert dp.name == "cohort.eigenD.txt" and up.name == "cohort.eigenU.txt"
+    assert dp.read_text() == "".join(f"{v:.10g}\n" for v in d)
+    d2, U2 = read_eigen_files(dp, up, n_samples=n)
+    assert np.array_equal(d2, np.loadtxt(dp)) and np.array_equal(U2, np.loadtxt(up))
+    np.testing.assert_allclose(d2, d, rtol=5e-10, atol=0)
+    np.testing.assert_allclose(U2, U, rtol=5e-10, atol=0)
+    with pytest.raises(ValueError, match="expected n_samples=58"):
+        read_eigen_files(dp, up, n_samples=58)
+    bad_u = tmp_path / "rect.txt"
+    bad_u.write_text("1\t2\t3\n4\t5\t6\n")
+    with pytest.raises(ValueError, match="must be square"):
+        read_eigen_files(dp, bad_u)
+    small_u = tmp_path / "small.txt"
+    small_u.write_text("1\t0\n0\t1\n")
+    with pytest.raises(ValueError, match="does not match eigenvector"):
+        read_eigen_files(dp, small_u)
+    ragged = tmp_path / "ragged.txt"
+    ragged.write_text("1\t2\n3\n")
+    with pytest.raises(ValueError, match="different lengths"):
+        read_eigen_files(dp, ragged)
+
+
+def test_pipeline_flag_validation(tmp_path):
+    """PipelineRunner.validate_inputs (pipeline.py:142-238) without a GPU: -hwe with
+    -loco, hwe out of range, -k with -loco, -d without -u."""
+    from jamma_amd.pipeline import PipelineConfig, PipelineRunner
+
+    pre = GOLDEN / "gemma_synthetic" / "test"
+
+    def check(match, **kw):
+        with pytest.raises(ValueError, match=match):
+            PipelineRunner(PipelineConfig(bfile=pre, **kw)).validate_inputs()
+
+    check("not yet supported with -loco", loco=True, hwe_threshold=1e-6)
+    check("must be >= 0", hwe_threshold=-0.1)
+    check(r"must be in \[0, 1\]", hwe_threshold=1.5)
+    check("mutually exclusive", loco=True, kinship_file=GOLDEN / "gemma_synthetic" /
+          "gemma_kinship.cXX.txt")
+    check("must be provided together", eigenvalue_file=tmp_path / "x.eigenD.txt")
+    PipelineRunner(PipelineConfig(bfile=pre, hwe_threshold=1e-6)).validate_inputs()
