@@ -667,10 +667,18 @@ def serving_session(args):
         h2 = rng.uniform(0.2, 0.8)
         ncov = int(rng.integers(0, 11))
         W = np.column_stack([np.ones(n), rng.normal(size=(n, ncov))]) if ncov else None
-        y = rng.normal(size=n)
+        # genetic phenotype: 50 causal SNPs of the cohort (mean-imputed genotypes on the
+        # device), standardised genetic value scaled to h2, plus noise and covariate effects
+        causal = torch.as_tensor(np.sort(rng.choice(rc.p, 50, replace=False)), device=dev)
+        gx = rc.g[causal, :n].to(torch.float64)
+        gx = torch.where(gx < 0, rc.means[causal][:, None], gx)
+        gv = (torch.as_tensor(rng.normal(size=50), device=dev) @ gx).cpu().numpy()
+        gv = (gv - gv.mean()) / (gv.std() + 1e-12)
+        y = np.sqrt(h2) * gv + np.sqrt(1 - h2) * rng.normal(size=n)
+        if ncov:
+            y = y + W[:, 1:] @ rng.normal(0, 0.3, ncov)
         length = int(np.exp(rng.uniform(np.log(1000), np.log(50000))))
         start = int(rng.integers(0, rc.p - length))
-        y = np.sqrt(1 - h2) * y + np.sqrt(h2) * rng.normal(size=n)
         reqs.append(Request(name, y, start, start + length, covariates=W, mode=4))
     gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
     log(f"serving {len(reqs)} requests at {args.rate}/s")
@@ -686,6 +694,24 @@ def serving_session(args):
     stats = srv.latency_stats()
     log(f"serving: p50 {stats['p50_s'] * 1e3:.1f} ms, p99 {stats['p99_s'] * 1e3:.1f} ms, "
         f"{stats['snps_per_s']:,.0f} SNPs/s")
+    # saturation: the same requests all submitted at once (a fresh server on the same
+    # resident cohorts), SNPs/s over the time to drain them
+    sat = Server(slice_snps=4096, max_batch_snps=16384)
+    for rc in srv.cohorts.values():
+        sat.add_cohort(rc)
+    with sat:
+        t0 = time.perf_counter()
+        futs = [sat.submit(Request(r.cohort, r.phenotype, r.snp_start, r.snp_end, r.covariates,
+                                   r.mode)) for r in reqs]
+        for f in futs:
+            f.result(timeout=1200)
+        sat_wall = time.perf_counter() - t0
+    total = sum(r.n_snps for r in reqs)
+    stats["saturation_snps_per_s"] = total / sat_wall
+    stats["saturation_wall_s"] = sat_wall
+    stats["offered_snps_per_s"] = total / wall
+    log(f"serving saturation: {total / sat_wall:,.0f} SNPs/s ({len(reqs)} requests at once, "
+        f"{sat_wall:.2f} s)")
     return stats, wall, setup, sizes, p
 
 

@@ -1,37 +1,51 @@
 """Continuous-batching association server over resident cohorts.
 
 A cohort (kinship eigendecomposition + int8 genotypes + imputation means) stays
-resident in HBM. A request is one phenotype (+ covariates) against a
-contiguous SNP range of a cohort. Requests are admitted against an HBM budget
-as memory frees up; every scheduler step takes the next slice of SNPs of every
-active request of a cohort and runs them as ONE batch: one rotation launch
-(gathered genotype rows), one spectrum-moments launch (per-column request
-descriptors) and one lambda-search launch (per-SNP problem index). Requests
-that arrive while others run join the next step, so their work overlaps on
-the device; each cohort uses its own stream. Per-SNP results do not depend on
-the batch composition (every output is computed by a fixed instruction
-sequence of its own column / SNP).
+resident in HBM. A request is one phenotype (+ covariates) against a contiguous SNP
+range of a cohort. The scheduler thread only launches device work and never waits
+for it:
+
+* admission reserves HBM budget in arrival order (first fit, with aging: a request
+  that later ones have overtaken ``max_skips`` times blocks further overtaking until
+  it fits); the per-request setup (rotated covariates / phenotype, their spectrum
+  moments, the device null model, the grid tables) then runs on a worker thread and
+  a setup stream of the cohort, so the null model's host read-back never stalls the
+  scheduler;
+* every scheduler step takes the next slice of every active request of a cohort and
+  launches, on that cohort's stream: ONE genotype pack + rotation for the whole batch
+  (a gather of the requests' SNP rows, whose index list goes up through pinned
+  memory without a host wait), then per request the spectrum moments of its columns
+  and the lambda search + tests (16 lanes per SNP when the problem allows), writing
+  straight into the request's result rows;
+* at most ``max_inflight`` steps per cohort are queued ahead of the device; steps,
+  result copies (to pinned memory) and completions are tracked with events that the
+  loop polls, so requests that arrive while others run join the next step and their
+  work overlaps on the device (one stream per cohort, all at default priority);
+* an error in a step fails that cohort's active requests alone (their budget is
+  released) and the loop goes on.
+Per-SNP results do not depend on the batch: a SNP's column of the rotation, its
+moments and its search are computed by the same instructions whatever else is in
+the step.
 
 The reference has no serving layer; its per-request work is
-run_lmm_association_streaming (src/jamma/lmm/runner_streaming.py:94-628) with
-a precomputed eigendecomposition (prepare.py:77-111).
+run_lmm_association_streaming (src/jamma/lmm/runner_streaming.py:94-628) with a
+precomputed eigendecomposition (prepare.py:77-111).
 """
 
 from __future__ import annotations
 
-import ctypes
 import itertools
 import threading
 import time
-from concurrent.futures import Future
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
 import torch
 
 from jamma_amd import _lib
-from jamma_amd.device import ptr, require_gpu, round_up, to_dev
-from jamma_amd.lmm.engine import ctypes_ref, AssocProblem, Cohort
+from jamma_amd.device import ptr, require_gpu
+from jamma_amd.lmm.engine import AssocProblem, Cohort, ctypes_ref
 
 
 @dataclass
@@ -67,77 +81,93 @@ class ResidentCohort:
         self.cohort = Cohort(eigenvalues, eigenvectors,
                              precision="both" if precision == "mixed" else "f64")
         self.n = self.cohort.n
-        self.stream = torch.cuda.Stream()  # default priority
+        self.stream = torch.cuda.Stream()        # steps (default priority)
+        self.setup_stream = torch.cuda.Stream()  # per-request setup (default priority)
 
     def resident_bytes(self) -> int:
         return self.cohort.hbm_bytes() + self.g.numel() + self.means.numel() * 8
 
 
-MOMCOL_DTYPE = np.dtype([("V", np.uint64), ("ldv", np.int64), ("nv", np.int32),
-                         ("reserved", np.int32), ("out", np.uint64), ("ldm", np.int64)])
-assert MOMCOL_DTYPE.itemsize == ctypes.sizeof(_lib.MomCol)
-
-
 class _Active:
     """Per-request device state while admitted."""
 
-    def __init__(self, req: Request, rc: ResidentCohort, slice_snps: int):
+    def __init__(self, req: Request, rc: ResidentCohort, prob: AssocProblem, slice_snps: int):
         self.req = req
         self.rc = rc
-        self.prob = AssocProblem(rc.cohort, req.phenotype, req.covariates)
+        self.prob = prob
         self.next = req.snp_start
-        self.out = torch.empty((req.n_snps, _lib.JM_NOUT), dtype=torch.float64, device=rc.cohort.device)
+        dev = rc.cohort.device
         T = rc.cohort.n_terms
-        self.rows = self.prob.n_cvt + 2
-        self.xmom = torch.empty((slice_snps, self.rows, T), dtype=torch.float64,
-                                device=rc.cohort.device)
+        self.rows = prob.n_cvt + 2
+        self.out = torch.empty((req.n_snps, _lib.JM_NOUT), dtype=torch.float64, device=dev)
+        self.xmom = torch.empty((slice_snps, self.rows, T), dtype=torch.float64, device=dev)
+        self.host = None       # pinned copy of out once the last slice is launched
+        self.done_event = None
+        self.cost = 0
 
     @staticmethod
     def cost_bytes(rc: ResidentCohort, req: Request, slice_snps: int) -> int:
+        """HBM held by the request while admitted: problem tables, rotated covariates,
+        its moments slice and result rows, and its share of a step's buffers (rotated
+        columns, packed genotypes, moment partials) at full slice."""
         c = 1 if req.covariates is None else np.asarray(req.covariates).shape[1]
         T = rc.cohort.n_terms
         ni = (c + 3) * (c + 2) // 2
-        tables = (T * 51 + 51 * ni + ni * T + 4 * T) * 8 + (c + 1) * rc.n * 8 * 2
-        return tables + slice_snps * (c + 2) * T * 8 + req.n_snps * _lib.JM_NOUT * 8
+        n = rc.n
+        tables = (T * 51 + 51 * ni + ni * T + 4 * T) * 8 + (c + 1) * n * 8 * 3
+        own = slice_snps * (c + 2) * T * 8 + req.n_snps * _lib.JM_NOUT * 8
+        z_col = n * (8 if rc.precision == "f64" else 4) + (n + 64)  # Z + packed genotypes
+        work_col = (c + 2) * rc.cohort.plan.n_slots * 8
+        return tables + own + slice_snps * (z_col + work_col)
 
-    def done(self) -> bool:
-        return self.next >= self.req.snp_end
+    def remaining(self) -> int:
+        return self.req.snp_end - self.next
 
 
 class AdmissionQueue:
-    """First-fit admission in arrival order against a byte budget (host logic).
+    """First-fit admission in arrival order against a byte budget, with aging (host
+    logic, no device work).
 
-    ``try_admit(cost, create)`` is called for queued requests in order; a request
-    is admitted when ``used + cost <= budget`` and ``create`` succeeds. A request
-    whose ``create`` raises fails alone (its error is returned) without holding
-    budget; requests that do not fit stay queued in order.
-    """
+    ``admit(cost_of)`` walks the queue in order and reserves budget for every item
+    with ``used + cost <= budget``; an item that does not fit stays queued and counts
+    one skip each time a later item is admitted past it. Once the oldest waiting item
+    has ``max_skips`` skips, nothing behind it is admitted until it fits, so a large
+    request cannot be starved by a stream of small ones."""
 
-    def __init__(self, budget: int):
+    def __init__(self, budget: int, max_skips: int = 8):
         self.budget = int(budget)
         self.used = 0
+        self.max_skips = int(max_skips)
         self.queue: list = []
+        self._skips: dict = {}
 
     def fits_ever(self, cost: int) -> bool:
         return cost <= self.budget
 
-    def admit(self, cost_of, create):
-        """-> (admitted [(item, obj, cost)], failed [(item, exc)])."""
-        admitted, failed, keep = [], [], []
+    def admit(self, cost_of):
+        """-> [(item, cost)] admitted, budget reserved, in queue order."""
+        admitted, keep = [], []
+        blocked = False
+        waiting = []  # items passed over so far in this walk
         for item in self.queue:
             cost = cost_of(item)
-            if self.used + cost <= self.budget:
-                try:
-                    obj = create(item)
-                except Exception as e:  # noqa: BLE001 - reported to the request
-                    failed.append((item, e))
-                    continue
+            if not blocked and self.used + cost <= self.budget:
                 self.used += cost
-                admitted.append((item, obj, cost))
+                admitted.append((item, cost))
+                for w in waiting:
+                    self._skips[id(w)] = self._skips.get(id(w), 0) + 1
             else:
                 keep.append(item)
+                waiting.append(item)
+                if self._skips.get(id(item), 0) >= self.max_skips:
+                    blocked = True  # aged: no more overtaking
+        for item, _ in admitted:
+            self._skips.pop(id(item), None)
         self.queue = keep
-        return admitted, failed
+        return admitted
+
+    def skips(self, item) -> int:
+        return self._skips.get(id(item), 0)
 
     def release(self, cost: int) -> None:
         self.used -= cost
@@ -147,21 +177,28 @@ class Server:
     """Scheduler thread owning the device work of every resident cohort."""
 
     def __init__(self, hbm_budget_bytes: int | None = None, slice_snps: int = 2048,
-                 max_batch_snps: int = 16384):
+                 max_batch_snps: int = 16384, max_inflight: int = 2, setup_workers: int = 2,
+                 max_skips: int = 8):
         self.device = require_gpu()
         self.cohorts: dict[str, ResidentCohort] = {}
         self.slice_snps = slice_snps
         self.max_batch = max_batch_snps
+        self.max_inflight = max_inflight
         if hbm_budget_bytes is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget_bytes = int(free * 0.5)
-        self.adm = AdmissionQueue(hbm_budget_bytes)
+        self.adm = AdmissionQueue(hbm_budget_bytes, max_skips)
         self._active: dict[str, list[_Active]] = {}
+        self._inflight: dict[str, list[torch.cuda.Event]] = {}
+        self._setup: list = []  # (req, cost, future of AssocProblem)
+        self._pool = ThreadPoolExecutor(max_workers=max(1, setup_workers),
+                                        thread_name_prefix="jm-setup")
         self._cv = threading.Condition()
         self._ids = itertools.count()
         self._thread = None
         self._stop = False
         self.completed: list[Request] = []
+        self.failed: list[Request] = []
         self.snps_done = 0
         self.batches = 0
         self.max_concurrent = 0
@@ -170,6 +207,7 @@ class Server:
     def add_cohort(self, rc: ResidentCohort) -> None:
         self.cohorts[rc.name] = rc
         self._active[rc.name] = []
+        self._inflight[rc.name] = []
 
     # -- requests -------------------------------------------------------------
     def submit(self, req: Request) -> Future:
@@ -201,6 +239,7 @@ class Server:
             self._cv.notify()
         if self._thread:
             self._thread.join()
+        self._pool.shutdown(wait=True)
 
     def __enter__(self):
         return self.start()
@@ -209,109 +248,159 @@ class Server:
         self.stop()
 
     # -- scheduler --------------------------------------------------------------
-    def _admit(self) -> None:
-        """First-fit admission in arrival order against the HBM budget."""
+    def _create(self, req: Request) -> AssocProblem:
+        """Worker thread: the request's device tables on the cohort's setup stream."""
+        rc = self.cohorts[req.cohort]
+        with torch.cuda.device(rc.cohort.device), torch.cuda.stream(rc.setup_stream):
+            prob = AssocProblem(rc.cohort, req.phenotype, req.covariates)
+            rc.setup_stream.synchronize()  # tables complete before the steps read them
+        return prob
 
-        def create(req):
-            rc = self.cohorts[req.cohort]
-            with torch.cuda.stream(rc.stream):
-                return _Active(req, rc, self.slice_snps)
-
-        admitted, failed = self.adm.admit(
-            lambda r: _Active.cost_bytes(self.cohorts[r.cohort], r, self.slice_snps), create)
-        for req, exc in failed:
+    def _fail(self, req: Request, exc: BaseException, cost: int) -> None:
+        with self._cv:
+            self.adm.release(cost)
+        req.t_done = time.perf_counter()
+        self.failed.append(req)
+        if not req.future.done():
             req.future.set_exception(exc)
-        for req, act, cost in admitted:
-            act.cost = cost
-            req.t_admit = time.perf_counter()
-            self._active[req.cohort].append(act)
 
-    def _step_cohort(self, name: str) -> list[_Active]:
-        acts = self._active[name]
+    def _admit(self) -> None:
+        with self._cv:
+            admitted = self.adm.admit(
+                lambda r: _Active.cost_bytes(self.cohorts[r.cohort], r, self.slice_snps))
+        for req, cost in admitted:
+            req.t_admit = time.perf_counter()
+            self._setup.append((req, cost, self._pool.submit(self._create, req)))
+        keep = []
+        for req, cost, fut in self._setup:
+            if not fut.done():
+                keep.append((req, cost, fut))
+                continue
+            exc = fut.exception()
+            if exc is not None:
+                self._fail(req, exc, cost)
+                continue
+            rc = self.cohorts[req.cohort]
+            try:
+                with torch.cuda.stream(rc.stream):
+                    act = _Active(req, rc, fut.result(), self.slice_snps)
+            except Exception as e:  # noqa: BLE001 - reported to the request
+                self._fail(req, e, cost)
+                continue
+            act.cost = cost
+            self._active[req.cohort].append(act)
+        self._setup = keep
+
+    def _launch_step(self, name: str) -> bool:
+        """Launch the next step of a cohort if it has work and room in flight."""
+        infl = self._inflight[name]
+        while infl and infl[0].query():
+            infl.pop(0)
+        if len(infl) >= self.max_inflight:
+            return False
+        acts = [a for a in self._active[name] if a.remaining() > 0]
         if not acts:
-            return []
+            return False
         rc = self.cohorts[name]
         co = rc.cohort
         T = co.n_terms
         with torch.cuda.stream(rc.stream):
-            cols, vptr, ldv, nv, xptr, snp_prob, dests = [], [], [], [], [], [], []
+            dests, cols = [], []
             budget = self.max_batch
-            for k, a in enumerate(acts):
+            for a in acts:
                 if budget <= 0:
                     break
-                take = min(self.slice_snps, a.req.snp_end - a.next, budget)
-                if take <= 0:
-                    continue
+                take = min(self.slice_snps, a.remaining(), budget)
                 budget -= take
                 cols.append(np.arange(a.next, a.next + take, dtype=np.int32))
-                stride = a.rows * T * 8
-                xptr.append(a.xmom.data_ptr() + stride * np.arange(take, dtype=np.uint64))
-                vptr.append(np.full(take, a.prob.V.data_ptr(), dtype=np.uint64))
-                ldv.append(np.full(take, a.prob.V.shape[1], dtype=np.int64))
-                nv.append(np.full(take, a.prob.n_cvt + 1, dtype=np.int32))
-                snp_prob.append(np.full(take, k, dtype=np.int32))
                 dests.append((a, a.next - a.req.snp_start, take))
                 a.next += take
-            if not cols:
-                return []
             B = int(sum(len(c) for c in cols))
-            col_idx = to_dev(np.concatenate(cols), torch.int32)
+            host_idx = torch.empty((B,), dtype=torch.int32, pin_memory=True)
+            host_idx.numpy()[:] = np.concatenate(cols)
+            col_idx = torch.empty((B,), dtype=torch.int32, device=co.device)
+            col_idx.copy_(host_idx, non_blocking=True)
             Z, f32, ldz = co.rotate_genotypes(rc.g, rc.ld, rc.means, B, rc.precision, col_idx)
-            desc = np.zeros(B, dtype=MOMCOL_DTYPE)
-            desc["V"] = np.concatenate(vptr)
-            desc["ldv"] = np.concatenate(ldv)
-            desc["nv"] = np.concatenate(nv)
-            xp = np.concatenate(xptr)
-            desc["out"] = xp
-            desc["ldm"] = T
-            desc_arr = desc
-            desc_dev = to_dev(np.frombuffer(desc_arr, dtype=np.uint8), torch.uint8)
-            nv_max = max(a.prob.n_cvt for a in acts) + 1
-            work = co.moments_work(B, nv_max)
-            _lib.call("jm_spectrum_moments_multi", ptr(Z), int(f32), ldz, B, ptr(desc_dev),
-                      nv_max, ctypes_ref(co.plan_c), ptr(work), rc.stream.cuda_stream)
-            probs = (_lib.LmmProblem * len(acts))(*[a.prob.prob for a in acts])
-            probs_dev = to_dev(np.frombuffer(probs, dtype=np.uint8), torch.uint8)
-            modes = to_dev(np.array([a.req.mode for a in acts], dtype=np.int32), torch.int32)
-            sp = to_dev(np.concatenate(snp_prob), torch.int32)
-            sx = to_dev(xp.view(np.int64), torch.int64)
-            out = torch.empty((B, _lib.JM_NOUT), dtype=torch.float64, device=co.device)
-            max_cvt = max(a.prob.n_cvt for a in acts)
-            _lib.call("jm_lmm_assoc_multi", ptr(probs_dev), ptr(modes), len(acts), max_cvt, T,
-                      ptr(sp), ptr(sx), T, B, ptr(out), rc.stream.cuda_stream)
+            nv_max = max(a.prob.n_cvt for a, _, _ in dests) + 1
+            take_max = max(t for _, _, t in dests)
+            work = co.moments_work(take_max, nv_max)
+            esz = 4 if f32 else 8
             o = 0
             for a, off, take in dests:
-                a.out[off:off + take].copy_(out[o:o + take])
+                _lib.call("jm_spectrum_moments", Z.data_ptr() + o * esz, int(f32), ldz, take,
+                          ptr(a.prob.V), a.prob.V.shape[1], a.prob.n_cvt + 1,
+                          ctypes_ref(co.plan_c), ptr(work), ptr(a.xmom), T, rc.stream.cuda_stream)
+                _lib.call("jm_lmm_assoc", ctypes_ref(a.prob.prob), int(a.req.mode), ptr(a.xmom),
+                          T, take, a.out.data_ptr() + off * _lib.JM_NOUT * 8,
+                          rc.stream.cuda_stream)
                 o += take
-            self.batches += 1
-            self.snps_done += B
-            finished = [a for a in acts if a.done()]
-            if finished:
-                rc.stream.synchronize()
-            # keep the host arrays referenced by the launches alive until here
-            del desc_arr, probs
-        return finished
+                if a.remaining() == 0:
+                    a.host = torch.empty(a.out.shape, dtype=torch.float64, pin_memory=True)
+                    a.host.copy_(a.out, non_blocking=True)
+                    a.done_event = torch.cuda.Event()
+                    a.done_event.record(rc.stream)
+            ev = torch.cuda.Event()
+            ev.record(rc.stream)
+            infl.append(ev)
+            del Z, work, col_idx  # stream-ordered frees: reused only by later work
+        self.batches += 1
+        self.snps_done += B
+        return True
+
+    def _fail_cohort(self, name: str, exc: BaseException) -> None:
+        """A step failed: its cohort's active requests fail alone; the device work
+        already queued on the cohort's stream is drained before their buffers go."""
+        rc = self.cohorts[name]
+        try:
+            rc.stream.synchronize()
+        except Exception:  # noqa: BLE001 - the stream's own error is the one reported
+            pass
+        for a in self._active[name]:
+            self._fail(a.req, exc, a.cost)
+        self._active[name] = []
+        self._inflight[name] = []
+
+    def _poll_done(self) -> int:
+        n_done = 0
+        for name, acts in self._active.items():
+            for a in list(acts):
+                if a.done_event is None or not a.done_event.query():
+                    continue
+                res = a.host.numpy().copy()
+                a.req.t_done = time.perf_counter()
+                acts.remove(a)
+                with self._cv:
+                    self.adm.release(a.cost)
+                self.completed.append(a.req)
+                a.req.future.set_result(res)
+                n_done += 1
+        return n_done
+
+    def _busy(self) -> bool:
+        return bool(self.adm.queue or self._setup or any(self._active.values()))
 
     def _loop(self) -> None:
         while True:
             with self._cv:
-                while not self._stop and not self.adm.queue and not any(self._active.values()):
+                while not self._stop and not self._busy():
                     self._cv.wait()
-                if self._stop and not self.adm.queue and not any(self._active.values()):
+                if self._stop and not self._busy():
                     return
-                self._admit()
+            self._admit()
             self.max_concurrent = max(self.max_concurrent,
                                       sum(len(v) for v in self._active.values()))
+            launched = False
             for name in list(self._active):
-                finished = self._step_cohort(name)
-                for a in finished:
-                    res = a.out.cpu().numpy()
-                    a.req.t_done = time.perf_counter()
-                    self._active[name].remove(a)
-                    with self._cv:
-                        self.adm.release(a.cost)
-                    self.completed.append(a.req)
-                    a.req.future.set_result(res)
+                try:
+                    launched |= self._launch_step(name)
+                except Exception as e:  # noqa: BLE001 - isolate the cohort's requests
+                    self._fail_cohort(name, e)
+            done = self._poll_done()
+            if not launched and not done:
+                # everything is queued on the device (or still being set up): wait a
+                # little for completions, setups or new requests
+                with self._cv:
+                    self._cv.wait(timeout=0.0005)
 
     # -- reporting -------------------------------------------------------------
     def latency_stats(self) -> dict:
@@ -323,4 +412,5 @@ class Server:
         return {"requests": int(lat.size), "p50_s": float(np.percentile(lat, 50)),
                 "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
                 "snps": int(snps), "snps_per_s": snps / span if span > 0 else float("nan"),
-                "batches": self.batches, "max_concurrent": self.max_concurrent}
+                "batches": self.batches, "max_concurrent": self.max_concurrent,
+                "failed": len(self.failed)}

@@ -123,6 +123,49 @@ def test_admission_budget_queues_and_completes(cohorts):
         tiny.submit(Request("a", np.zeros(rc.n), 0, rc.p, covariates=None))
 
 
+def test_step_error_fails_its_cohort_alone(cohorts, monkeypatch):
+    """A launch that fails inside a scheduler step (here jm_lmm_assoc, once) fails the
+    requests active on that cohort with the error, releases their budget, and the
+    loop keeps serving: a later request on the same cohort and one on another cohort
+    complete and match their results alone."""
+    from jamma_amd import _lib
+    from jamma_amd.serve import Request, Server
+
+    real = _lib.call
+    state = {"armed": True}
+
+    def flaky(name, *args):
+        if name == "jm_lmm_assoc" and state["armed"]:
+            state["armed"] = False
+            raise RuntimeError("jm_lmm_assoc failed (rc=-2): injected step error")
+        return real(name, *args)
+
+    rca, rcb = cohorts["a"][0], cohorts["b"][0]
+    rng = np.random.default_rng(9)
+    ya, yb, yc = (rng.normal(size=rc.n) for rc in (rca, rca, rcb))
+    srv = Server(slice_snps=64)
+    for c, *_ in cohorts.values():
+        srv.add_cohort(c)
+    ref = Server(slice_snps=64)
+    for c, *_ in cohorts.values():
+        ref.add_cohort(c)
+    with ref:
+        want_a2 = ref.submit(Request("a", yb, 100, 300)).result(timeout=120)
+        want_b = ref.submit(Request("b", yc, 0, 150)).result(timeout=120)
+    monkeypatch.setattr(_lib, "call", flaky)
+    with srv:
+        bad = srv.submit(Request("a", ya, 0, 200))
+        with pytest.raises(RuntimeError, match="injected step error"):
+            bad.result(timeout=120)
+        assert not state["armed"]
+        good_a = srv.submit(Request("a", yb, 100, 300))
+        good_b = srv.submit(Request("b", yc, 0, 150))
+        assert np.array_equal(good_a.result(timeout=120), want_a2, equal_nan=True)
+        assert np.array_equal(good_b.result(timeout=120), want_b, equal_nan=True)
+        stats = srv.latency_stats()
+    assert stats["failed"] == 1 and srv.adm.used == 0
+
+
 def test_bad_request_fails_alone(cohorts):
     from jamma_amd.serve import Request, Server
 
