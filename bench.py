@@ -83,6 +83,8 @@ def main() -> int:
                          "default: the cohort's choice by sample size)")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 path (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-guard", action="store_true",
+                    help="skip the FP64 recomputation of mixed p-values near the thresholds")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving session that adds request p50/p99 to the line")
     ap.add_argument("--workload", default="precision",
@@ -128,7 +130,7 @@ def main() -> int:
     from jamma_amd.io.plink import stats_from_counts
     from jamma_amd.kinship.compute import KinshipAccumulator
     from jamma_amd.lmm.eigen import eigh_device
-    from jamma_amd.lmm.engine import AssocProblem, Cohort
+    from jamma_amd.lmm.engine import GUARD_BAND, AssocProblem, Cohort, guard_rows
 
     dev = require_gpu()
     n, p = args.n, args.p
@@ -202,6 +204,7 @@ def main() -> int:
 
     tiles = [(s, min(s + args.tile, pf)) for s in range(0, pf, args.tile)]
     out_all = {}
+    retested = []
 
     def run_step(precision, timers=None):
         outs = []
@@ -224,6 +227,12 @@ def main() -> int:
                 evs[3].record()
                 timers.append(evs)
             outs.append(o[: e - s])
+        if precision == "mixed" and not args.no_guard:
+            # p-values within the guard band of a significance threshold: FP64 path
+            full = torch.cat(outs)
+            rows = guard_rows(full, args.mode)
+            retested.append(prob.retest_f64(gf, ld, means, rows, args.mode, full))
+            outs = [full]
         return outs
 
     results = {}
@@ -263,6 +272,8 @@ def main() -> int:
                             "p_zero_in_both": int(np.sum((a[:, col] == 0) & (b[:, col] == 0)))}
         parity["beta_sign_agreement"] = bool(np.array_equal(np.sign(a[:, 0]), np.sign(b[:, 0])))
         parity["n_sig_5e-8_wald"] = int(np.sum(b[:, 2] < 5e-8))
+        parity["guard_band_log10p"] = None if args.no_guard else GUARD_BAND
+        parity["retested_f64_per_step"] = retested[-1] if retested else 0
 
     # ---- CPU baseline: the oracle on a bounded sample of the same workload ----
     cpu = None
@@ -422,31 +433,37 @@ def loco(args) -> int:
         (W[:, 1:] @ rng.normal(0, 0.3, (3, n_ph)))
     means = to_dev(mean32.astype(np.float64), torch.float64)
 
-    torch.cuda.synchronize()
-    t_all = time.perf_counter()
-    # per-chromosome int8 partial Grams (one staging budget shared by the 19)
-    accs = [KinshipAccumulator(n, stage_bytes=(1 << 31) // n_chr) for _ in range(n_chr)]
-    for c in range(n_chr):
-        sel = idx_all[chrom[idx_all] == c]
+    def accumulate(sel):
+        acc = KinshipAccumulator(n, max_snps=len(sel))
         for s in range(0, len(sel), 16384):
             ss = torch.as_tensor(sel[s:s + 16384], device=dev)
-            accs[c].add_tile(g[ss].contiguous(), ld, len(ss), mu64[sel[s:s + 16384]], True)
-    full = KinshipAccumulator(n)
-    for a in accs:
-        full.S += a.S
-        full.u += a.u
-        full.C = a.C.clone() if full.C is None else full.C + a.C
-        full.s0 += a.s0
-        full.p += a.p
+            acc.add_tile(g[ss].contiguous(), ld, len(ss), mu64[sel[s:s + 16384]], True)
+        acc.release()
+        return acc
+
+    torch.cuda.synchronize()
+    base_mem = torch.cuda.memory_allocated()
+    torch.cuda.reset_peak_memory_stats()
+    t_all = time.perf_counter()
+    # the full int8 Gram + FP64 corrections once; each chromosome's partial Gram is
+    # recomputed from its own SNPs when its LOCO kinship is formed, then freed
+    full = accumulate(idx_all)
     torch.cuda.synchronize()
     t_kin = time.perf_counter() - t_all
-    log(f"loco: partial Grams of {n_chr} chromosomes + total in {t_kin:.2f}s")
+    kin_peak = 0
+    log(f"loco: full Gram in {t_kin:.2f}s")
     t_eig = t_rot = t_assoc = 0.0
     out = {"mixed": [], "f64": []}
     for c in range(n_chr):
         t0 = time.perf_counter()
-        K = full.finalize(full.p - accs[c].p, sub=accs[c])
-        accs[c] = None  # this chromosome's partial Gram is no longer needed
+        sel_c = idx_all[chrom[idx_all] == c]
+        acc_c = accumulate(sel_c)
+        K = full.finalize(full.p - acc_c.p, sub=acc_c)
+        torch.cuda.synchronize()
+        kin_peak = max(kin_peak, torch.cuda.max_memory_allocated() - base_mem)
+        del acc_c
+        t_kin += time.perf_counter() - t0
+        t0 = time.perf_counter()
         d, U = eigh_device(K)
         del K
         torch.cuda.synchronize()
@@ -499,6 +516,8 @@ def loco(args) -> int:
                    "n": n, "p": p, "chromosomes": n_chr, "phenotypes": n_ph, "tile_snps": tile},
         "seconds": {"kinship": t_kin, "eigendecompositions": t_eig, "association_mixed": t_assoc,
                     "timed_total": timed, "wall_including_f64_parity": total},
+        "kinship_peak_hbm_gb": kin_peak / 1e9,
+        "peak_hbm_gb": (torch.cuda.max_memory_allocated() - base_mem) / 1e9,
         "parity_mixed_vs_f64": parity,
     }
     print(json.dumps(line), flush=True)

@@ -475,6 +475,90 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
 
 using namespace jm;
 
+namespace jm {
+
+// ---------------------------------------------------------------------------
+// FP64 rotation of a few SNP columns (n_snps <= 16): HBM-bound GEMV form
+// ---------------------------------------------------------------------------
+// A 64 x 128 MFMA tile spends almost all of its work on padding when only a few
+// columns are rotated (the FP64 recomputation of a handful of SNPs), and then reads
+// U^T at a fraction of HBM speed. Here each block owns 16 consecutive rows of U^T
+// (4 per wave); the <= 16 genotype columns are staged through LDS as int8, k-major
+// (16 bytes per eigen-index: one ds_read_b128 per lane gives a lane all columns of
+// its k), 4096 eigen-indices per stage. Lanes stride over k, so every U^T row is
+// read once, coalesced; per lane and row the partial sums of the 16 columns are
+// then summed over the wavefront in a fixed order (deterministic).
+constexpr int kNarrowCols = 16;
+constexpr int kNarrowRowsPerWave = 4;
+constexpr int kNarrowChunk = 4096;
+
+__global__ void __launch_bounds__(256) k_rotate_f64_narrow(const double* __restrict__ UT,
+                                                           int64_t ld_u, const int8_t* __restrict__ g,
+                                                           int64_t ld_g,
+                                                           const double* __restrict__ mean,
+                                                           const int32_t* __restrict__ col,
+                                                           int64_t n, int ncols,
+                                                           double* __restrict__ Z, int64_t ldz) {
+  __shared__ __align__(16) int8_t xs[kNarrowChunk * kNarrowCols];  // 64 KB
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t row0 = (int64_t)blockIdx.x * (4 * kNarrowRowsPerWave);
+  double mu[kNarrowCols];
+#pragma unroll
+  for (int c = 0; c < kNarrowCols; ++c) {
+    const int64_t r = c < ncols ? (col ? col[c] : c) : 0;
+    mu[c] = c < ncols ? mean[r] : 0.0;
+  }
+  double acc[kNarrowRowsPerWave][kNarrowCols];
+#pragma unroll
+  for (int q = 0; q < kNarrowRowsPerWave; ++q)
+#pragma unroll
+    for (int c = 0; c < kNarrowCols; ++c) acc[q][c] = 0.0;
+  for (int64_t k0 = 0; k0 < n; k0 += kNarrowChunk) {
+    const int len = (int)min((int64_t)kNarrowChunk, n - k0);
+    const int len4 = (len + 3) & ~3;  // rows are zero-padded to ld_g (a multiple of 16)
+    __syncthreads();  // the previous stage has been read
+    // stage: column c, eigen-indices k..k+3 of this chunk -> xs[(k + j) * 16 + c]
+    for (int e = threadIdx.x; e < kNarrowCols * (len4 / 4); e += blockDim.x) {
+      const int c = e / (len4 / 4), k = (e % (len4 / 4)) * 4;
+      uint32_t v = 0;
+      if (c < ncols) {
+        const int64_t r = col ? col[c] : c;
+        v = *(const uint32_t*)(g + r * ld_g + k0 + k);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xs[(k + j) * kNarrowCols + c] = (int8_t)(v >> (8 * j));
+    }
+    __syncthreads();
+    for (int k = lane; k < len; k += 64) {
+      const u32x4_t raw = *(const u32x4_t*)(xs + k * kNarrowCols);
+      double x[kNarrowCols];
+#pragma unroll
+      for (int c = 0; c < kNarrowCols; ++c) {
+        const int8_t b = (int8_t)(raw[c >> 2] >> (8 * (c & 3)));
+        x[c] = b < 0 ? mu[c] : (double)b;
+      }
+#pragma unroll
+      for (int q = 0; q < kNarrowRowsPerWave; ++q) {
+        const int64_t i = row0 + wave + 4 * q;
+        const double u = i < n ? UT[i * ld_u + k0 + k] : 0.0;
+#pragma unroll
+        for (int c = 0; c < kNarrowCols; ++c) acc[q][c] = fma(u, x[c], acc[q][c]);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kNarrowRowsPerWave; ++q) {
+    const int64_t i = row0 + wave + 4 * q;
+#pragma unroll
+    for (int c = 0; c < kNarrowCols; ++c) {
+      const double v = wave_sum(acc[q][c]);
+      if (lane == 0 && i < n && c < ncols) Z[i * ldz + c] = v;
+    }
+  }
+}
+
+}  // namespace jm
+
 extern "C" int jm_rotate_f64(const double* UT, int64_t ld_u, const int8_t* g, int64_t ld_g,
                              const double* mean, const int32_t* col_idx, int64_t n, int64_t n_snps,
                              double* Z, int64_t ldz, void* stream) {
@@ -483,6 +567,14 @@ extern "C" int jm_rotate_f64(const double* UT, int64_t ld_u, const int8_t* g, in
   JM_CHECK_ARG(UT && g && mean && Z, "null pointer");
   JM_CHECK_ARG(ld_u >= n && ld_g >= n && ldz >= n_snps, "bad leading dimension");
   JM_CHECK_ARG(ld_g % 16 == 0 && ((uintptr_t)g % 16) == 0, "g must be 16-byte aligned rows");
+  if (n_snps <= kNarrowCols) {  // a few columns: the HBM-bound GEMV form
+    const int64_t blocks = (n + 4 * kNarrowRowsPerWave - 1) / (4 * kNarrowRowsPerWave);
+    hipLaunchKernelGGL(k_rotate_f64_narrow, dim3((unsigned)blocks), dim3(256), 0,
+                       (hipStream_t)stream, UT, ld_u, g, ld_g, mean, col_idx, n, (int)n_snps, Z,
+                       ldz);
+    JM_CHECK_LAUNCH();
+    return JM_OK;
+  }
   const int nrb = ceil_div(n, F_BM), ncb = ceil_div(n_snps, F_BN);
   BSrcGeno src{g, ld_g, mean, col_idx};
   hipLaunchKernelGGL(k_gemm_f64<BSrcGeno>, dim3(nrb * ncb), dim3(256), 0, (hipStream_t)stream,

@@ -35,7 +35,10 @@ class KinshipAccumulator:
     reference's 10,000-SNP batches cost one large launch instead of many small ones.
     The imputation terms u and C are applied per tile."""
 
-    def __init__(self, n: int, device=None, stage_bytes: int = 1 << 31):
+    def __init__(self, n: int, device=None, stage_bytes: int = 1 << 31,
+                 max_snps: int | None = None):
+        """max_snps (optional): the SNPs this accumulator will see at most; the staging
+        buffer is not made wider than that."""
         self.device = device or require_gpu()
         self.n = n
         self.n_pad = round_up(n, 256)
@@ -45,6 +48,8 @@ class KinshipAccumulator:
         self.s0 = 0.0
         self.p = 0
         self.stage_cols = int(min(1 << 17, max(128, (stage_bytes // self.n_pad) // 128 * 128)))
+        if max_snps is not None:
+            self.stage_cols = int(min(self.stage_cols, max(128, round_up(max_snps, 128))))
         self._stage = None
         self._fill = 0
         # optional list: each SYRK launch appends its (start, end) torch.cuda.Event pair
@@ -264,40 +269,34 @@ def compute_kinship_streaming(bed_path: Path, chunk_size: int = 10_000, maf_thre
 def loco_kinship_streaming_device(bed_path: Path, chunk_size: int = 10_000,
                                   maf_threshold: float = 0.0, miss_threshold: float = 1.0,
                                   ksnps_indices=None) -> Iterator[tuple[str, torch.Tensor]]:
-    """Per-chromosome int32 partial Grams; K_c = (S_full - S_c) / (p - p_c)."""
+    """LOCO kinships K_c = (S_full - S_c) / (p - p_c) (kinship/compute.py:693-916),
+    yielded one chromosome at a time in sorted order.
+
+    The full accumulators (int32 S, FP64 u and C) are built in one pass; each
+    chromosome's partial Gram is then recomputed from its own SNPs, subtracted and
+    freed before the next, so at most two sets of n x n accumulators are live (at
+    n = 20,000: 2 x (1.6 GB int32 + 3.2 GB FP64) instead of one set per chromosome).
+    The extra work is one more int8 GEMM over the SNPs, the cheapest stage."""
     bed = BedFile(bed_path)
     chrom = get_plink_metadata(bed_path)["chromosome"]
     counts, idx = _streaming_selection(bed, chunk_size, maf_threshold, miss_threshold,
                                        ksnps_indices)
-    chr_f = chrom[idx]
-    names = sorted(set(chr_f.tolist()))
-    # one staging budget shared by the per-chromosome accumulators
-    accs = {c: KinshipAccumulator(bed.n_samples, stage_bytes=(1 << 31) // len(names))
-            for c in names}
-
-    def split(sel):
-        cs = chrom[sel]
-        return [(accs[c], sel[cs == c]) for c in sorted(set(cs.tolist()))]
-
-    _accumulate_streaming(bed, idx, counts, chunk_size, split)
-    full = KinshipAccumulator(bed.n_samples)
+    names = sorted(set(chrom[idx].tolist()))
+    full = KinshipAccumulator(bed.n_samples, max_snps=len(idx))
+    _accumulate_streaming(bed, idx, counts, chunk_size, lambda sel: [(full, sel)])
+    full.release()
     for c in names:
-        a = accs[c]
-        full.S += a.S
-        full.u += a.u
-        if a.C is not None:
-            if full.C is None:
-                full.C = torch.zeros_like(a.C)
-            full.C += a.C
-        full.s0 += a.s0
-        full.p += a.p
-    for c in names:
-        p_loco = full.p - accs[c].p
+        sel_c = idx[chrom[idx] == c]
+        p_loco = full.p - len(sel_c)
         if p_loco == 0:
             raise ValueError(f"Cannot compute LOCO kinship: all {full.p} filtered SNPs are on "
                              f"chromosome '{c}'.")
-        yield c, full.finalize(p_loco, sub=accs[c])
-        del accs[c]
+        acc = KinshipAccumulator(bed.n_samples, max_snps=len(sel_c))
+        _accumulate_streaming(bed, sel_c, counts, chunk_size, lambda sel: [(acc, sel)])
+        K = full.finalize(p_loco, sub=acc)
+        del acc
+        yield c, K
+        del K
 
 
 def compute_loco_kinship_streaming(bed_path: Path, chunk_size: int = 10_000,
@@ -331,28 +330,27 @@ def compute_loco_kinship(genotypes: np.ndarray, chromosome_for_each_snp: np.ndar
     mu = _mu64(counts)
     ld = round_up(n, 64)
     names = sorted(set(chrom[idx].tolist()))
-    accs = {c: KinshipAccumulator(n) for c in names}
-    for c in names:
-        sel_c = idx[chrom[idx] == c]
-        if c != names[0]:
-            accs[names[names.index(c) - 1]].release()  # chromosomes run one after another
-        for s in range(0, len(sel_c), batch_size):
-            sel = sel_c[s:s + batch_size]
+
+    def accumulate(sel_all):
+        acc = KinshipAccumulator(n, max_snps=len(sel_all))
+        for s in range(0, len(sel_all), batch_size):
+            sel = sel_all[s:s + batch_size]
             tile = np.zeros((len(sel), ld), dtype=np.int8)
             tile[:, :n] = gi8[sel]
-            accs[c].add_tile(to_dev(tile, torch.int8), ld, len(sel), mu[sel],
-                             bool(counts[sel, 3].any()))
-    full = KinshipAccumulator(n)
+            acc.add_tile(to_dev(tile, torch.int8), ld, len(sel), mu[sel],
+                         bool(counts[sel, 3].any()))
+        acc.release()
+        return acc
+
+    # the full accumulators once, then each chromosome's partial Gram on its own
+    # (at most two sets of n x n accumulators live, as in the streaming form)
+    full = accumulate(idx)
     for c in names:
-        full.S += accs[c].S
-        full.u += accs[c].u
-        if accs[c].C is not None:
-            full.C = accs[c].C.clone() if full.C is None else full.C + accs[c].C
-        full.s0 += accs[c].s0
-        full.p += accs[c].p
-    for c in names:
-        p_loco = full.p - accs[c].p
-        if p_loco == 0:
+        sel_c = idx[chrom[idx] == c]
+        if full.p - len(sel_c) == 0:
             raise ValueError(f"Cannot compute LOCO kinship: all {full.p} filtered SNPs are on "
                              f"chromosome '{c}'. LOCO requires SNPs on multiple chromosomes.")
-        yield c, full.finalize(p_loco, sub=accs[c]).cpu().numpy()
+        acc = accumulate(sel_c)
+        K = full.finalize(full.p - len(sel_c), sub=acc).cpu().numpy()
+        del acc
+        yield c, K

@@ -62,6 +62,33 @@ def pair_tables(n_cvt: int) -> tuple[np.ndarray, np.ndarray]:
 
 TWO_PLANE_MAX_N = 16384  # largest cohort rotated with the two-plane split by default
 
+# Significance thresholds whose agreement the mixed path guarantees, and the band (in
+# log10 p, twice the 1e-4 accuracy bar of the mixed path) around each of them inside
+# which a mixed-precision p-value is recomputed on the FP64 path: a p-value can only
+# land on the other side of a threshold than its FP64 value if it lies within its
+# error of the threshold, so after the recomputation every reported p-value agrees
+# with the FP64 path on every threshold.
+GUARD_THRESHOLDS = (0.05, 0.01, 1e-3, 5e-8)
+GUARD_BAND = 2e-4
+_P_COLS = {1: (2,), 2: (7,), 3: (8,), 4: (2, 7, 8)}
+
+
+def guard_rows(out, mode: int, band: float | None = None):
+    """Rows of a result block (device tensor or host array, n x JM_NOUT) with a p-value
+    of the mode within `band` (log10 units; default GUARD_BAND) of a guarded threshold."""
+    band = GUARD_BAND if band is None else band
+    cols = list(_P_COLS[mode])
+    logt = np.log10(np.array(GUARD_THRESHOLDS))
+    if isinstance(out, torch.Tensor):
+        lp = torch.log10(out[:, cols])
+        t = torch.as_tensor(logt, dtype=lp.dtype, device=lp.device)
+        near = (lp[:, :, None] - t).abs() <= band
+        return torch.nonzero(near.any(dim=2).any(dim=1)).flatten()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lp = np.log10(np.asarray(out)[:, cols])
+    near = np.abs(lp[:, :, None] - logt) <= band
+    return np.nonzero(near.any(axis=2).any(axis=1))[0]
+
 
 class Cohort:
     """Eigendecomposition of one kinship, resident on the device."""
@@ -281,6 +308,22 @@ class AssocProblem:
             _lib.call("jm_lmm_assoc", ctypes_ref(self.prob), int(mode), ptr(xmom),
                       self.cohort.n_terms, n_snps, ptr(out), stream_ptr())
         return out
+
+    def retest_f64(self, g: torch.Tensor, ld_g: int, means: torch.Tensor, rows, mode: int,
+                   out: torch.Tensor) -> int:
+        """Recompute SNP rows `rows` of g (with their means, indexed alike) on the FP64
+        path and write them into the same rows of `out`. Returns the count."""
+        if len(rows) == 0:
+            return 0
+        idx = torch.as_tensor(rows, device=self.cohort.device)
+        col_idx = idx.to(torch.int32).contiguous()
+        co = self.cohort
+        k = int(idx.numel())
+        Z, f32, ldz = co.rotate_genotypes(g, ld_g, means, k, "f64", col_idx)
+        xm = co.moments(Z, f32, ldz, k, self.V, self.n_cvt + 1)
+        o = self.assoc_moments(xm, k, mode)
+        out[idx.to(torch.int64)] = o[:k]
+        return k
 
     def associate(self, g: torch.Tensor, ld_g: int, means: torch.Tensor, n_snps: int, mode: int,
                   precision: str = "mixed") -> torch.Tensor:

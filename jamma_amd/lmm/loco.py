@@ -174,7 +174,8 @@ def run_lmm_loco_multi(
                 Z, f32, ldz = cohort.rotate_genotypes(g, ld, mdev, len(tl), precision)
                 for k, pr in enumerate(probs):
                     xm = cohort.moments(Z, f32, ldz, len(tl), pr.V, pr.n_cvt + 1)
-                    sinks[k].push(chr_idx[tl], tl, pr.assoc_moments(xm, len(tl), lmm_mode))
+                    sinks[k].push(chr_idx[tl], tl, pr.assoc_moments(xm, len(tl), lmm_mode),
+                                  (pr, g, ld, mdev) if precision == "mixed" else None)
                 del Z
             for sk in sinks:
                 sk.close()

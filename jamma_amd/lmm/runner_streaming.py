@@ -26,7 +26,7 @@ from jamma_amd._lib import JM_NOUT, OUT
 from jamma_amd.device import require_gpu, round_up, to_dev
 from jamma_amd.io.plink import BedFile, decode, get_plink_metadata, stats_from_counts
 from jamma_amd.lmm.eigen import eigh_device
-from jamma_amd.lmm.engine import AssocProblem, Cohort
+from jamma_amd.lmm.engine import AssocProblem, Cohort, guard_rows
 from jamma_amd.lmm.io import IncrementalAssocWriter, snp_text_parts
 from jamma_amd.lmm.stats import AssocResult
 
@@ -104,15 +104,21 @@ class TileSink:
         self.writer = writer
         self.results = results if results is not None else []
         self._pending = None
+        self.retested = 0
 
-    def push(self, snps: np.ndarray, stat_idx: np.ndarray, out_dev: torch.Tensor) -> None:
-        """snps: global SNP indices (snp_info rows); stat_idx: their rows of af / miss."""
+    def push(self, snps: np.ndarray, stat_idx: np.ndarray, out_dev: torch.Tensor,
+             guard=None) -> None:
+        """snps: global SNP indices (snp_info rows); stat_idx: their rows of af / miss.
+        guard (mixed precision): (problem, g, ld_g, means) of the tile, so that rows
+        with a p-value within the guard band of a significance threshold are
+        recomputed on the FP64 path before they are consumed."""
         k = len(snps)
         host = torch.empty((k, JM_NOUT), dtype=torch.float64, pin_memory=True)
         host.copy_(out_dev[:k], non_blocking=True)
         ev = torch.cuda.Event()
         ev.record()
-        prev, self._pending = self._pending, (np.asarray(snps), np.asarray(stat_idx), host, ev)
+        prev, self._pending = self._pending, (np.asarray(snps), np.asarray(stat_idx), host, ev,
+                                              out_dev, guard)
         if prev is not None:
             self._finish(*prev)
 
@@ -121,9 +127,16 @@ class TileSink:
         if prev is not None:
             self._finish(*prev)
 
-    def _finish(self, snps, stat_idx, host, ev) -> None:
+    def _finish(self, snps, stat_idx, host, ev, out_dev, guard) -> None:
         ev.synchronize()
         rows = host.numpy()
+        if guard is not None:
+            near = guard_rows(rows, self.mode)
+            if len(near):
+                prob, g, ld_g, means = guard
+                prob.retest_f64(g, ld_g, means, near, self.mode, out_dev)
+                rows[near] = out_dev[torch.as_tensor(near, device=out_dev.device)].cpu().numpy()
+                self.retested += len(near)
         if self.writer is not None:
             parts = snp_text_parts([self.snp_info[int(s)] for s in snps])
             self.writer.write_rows(parts, self.miss[stat_idx], self.af[stat_idx], rows)
@@ -265,8 +278,10 @@ def run_lmm_association_streaming(
                                  snp_idx=(tsel - start).astype(np.int32), want_counts=False,
                                  ld_g=ld)
                 mdev = means_dev_all[torch.as_tensor(tsel, device=means_dev_all.device)]
-                out = prob.associate(g, ld, mdev.contiguous(), len(tsel), lmm_mode, precision)
-                sink.push(tsel, tsel, out)
+                mdev = mdev.contiguous()
+                out = prob.associate(g, ld, mdev, len(tsel), lmm_mode, precision)
+                sink.push(tsel, tsel, out,
+                          (prob, g, ld, mdev) if precision == "mixed" else None)
         sink.close()
     except BaseException as e:
         if writer is not None:
@@ -356,7 +371,8 @@ def run_lmm_association_jax(
         gt[:, :n] = gi8[tsel]
         g = to_dev(gt, torch.int8)
         mdev = to_dev(means[tsel], torch.float64)
-        sink.push(tsel, tsel, prob.associate(g, ld, mdev, len(tsel), lmm_mode, precision))
+        sink.push(tsel, tsel, prob.associate(g, ld, mdev, len(tsel), lmm_mode, precision),
+                  (prob, g, ld, mdev) if precision == "mixed" else None)
     sink.close()
     if show_progress:
         logger.info("LMM association (%s, in-memory, %d SNPs) done", precision, len(snp_indices))

@@ -505,3 +505,70 @@ def test_gwas_loco_saves_kinships(tmp_path):
         K = np.loadtxt(tmp_path / "o" / f"r.loco.cXX.chr{c}.txt")
         np.testing.assert_allclose(K, ref[c], rtol=1e-9, atol=1e-12)
     assert (tmp_path / "o" / "r.assoc.txt").exists()
+
+
+def test_mixed_guard_band_recomputes_in_fp64(monkeypatch):
+    """Mixed-precision p-values within the guard band of a significance threshold are
+    recomputed on the FP64 path: with the band widened to a decade, every SNP with a
+    p-value in [5e-9, 0.5] (any test) gets the FP64 path's row (to rounding), the
+    others keep their mixed results (within the 1e-4 bar of the FP64 ones)."""
+    from jamma_amd.lmm import engine, run_lmm_association_jax
+
+    G = balding_nichols(400, 500, miss=0.01, seed=41)
+    y, W = phenotype(G[:, :300], n_cov=2, seed=41, n_causal=30, h2=0.6)
+    K = ok.centered_kinship(G[:, 300:])
+    info = _snp_info(300)
+    f64 = _arrays(run_lmm_association_jax(G[:, :300], y, K, info, covariates=W, lmm_mode=4,
+                                          precision="f64"), ALL)
+    monkeypatch.setattr(engine, "GUARD_BAND", 1.0)
+    mixed = _arrays(run_lmm_association_jax(G[:, :300], y, K, info, covariates=W, lmm_mode=4,
+                                            precision="mixed"), ALL)
+    P = np.stack([f64[k] for k in ("p_wald", "p_lrt", "p_score")], 1)
+    with np.errstate(divide="ignore"):
+        lp = np.log10(P)
+    dist = np.nanmin(np.abs(lp[:, :, None] - np.log10([0.05, 0.01, 1e-3, 5e-8])), axis=(1, 2))
+    # SNPs within 1e-3 of the band's edge could be selected or not (the mixed p-value
+    # decides): they are held to neither statement
+    near, far = dist <= 1.0 - 1e-3, dist > 1.0 + 1e-3
+    assert near.sum() > 50 and far.sum() > 0
+    # the recomputed rows take the FP64 rotation of a few columns (another summation
+    # order than the tile kernel's): the FP64 path's rows to rounding
+    _assert_matches_oracle({k: mixed[k][near] for k in ALL}, {k: f64[k][near] for k in ALL}, ALL)
+    for k in ("p_wald", "p_lrt", "p_score"):
+        d = np.abs(np.log10(mixed[k][far]) - np.log10(f64[k][far]))
+        assert np.nanmax(d) <= 1e-4, k
+
+
+@pytest.mark.parametrize("n,ncols,gather", [(333, 1, False), (1031, 7, True), (5000, 16, True),
+                                            (4099, 16, False)])
+def test_rotation_f64_few_columns(n, ncols, gather):
+    """jm_rotate_f64 with at most 16 SNP columns (the GEMV form: U^T streamed once,
+    columns staged in LDS) against NumPy FP64, with missing calls mean-imputed, a
+    row gather, and sizes that are not multiples of the 4-byte staging or the
+    4096-index stages."""
+    from jamma_amd import _lib
+    from jamma_amd.device import ptr, round_up, stream_ptr
+
+    rng = np.random.default_rng(n + ncols)
+    UT = rng.normal(size=(n, n))
+    p = 40
+    ld = round_up(n, 64)
+    g = np.zeros((p, ld), dtype=np.int8)
+    g[:, :n] = rng.integers(0, 3, size=(p, n))
+    g[:, :n][rng.random((p, n)) < 0.03] = -1
+    mean = rng.uniform(0.1, 1.9, p)
+    rows = rng.choice(p, ncols, replace=False).astype(np.int32) if gather else \
+        np.arange(ncols, dtype=np.int32)
+    X = np.where(g[rows, :n] < 0, mean[rows][:, None], g[rows, :n]).T  # (n, ncols)
+    ref = UT @ X
+    dev = _dev()
+    Z = torch.full((n, ncols), np.nan, dtype=torch.float64, device=dev)
+    UTd = torch.as_tensor(UT, device=dev)
+    gd = torch.as_tensor(g, device=dev)
+    md = torch.as_tensor(mean, device=dev)
+    cd = torch.as_tensor(rows, device=dev) if gather else None
+    _lib.call("jm_rotate_f64", ptr(UTd), n, ptr(gd), ld, ptr(md), ptr(cd), n, ncols, ptr(Z),
+              ncols, stream_ptr())
+    got = Z.cpu().numpy()
+    scale = np.abs(UT) @ np.abs(X)
+    assert np.all(np.abs(got - ref) <= 1e-13 * scale)

@@ -579,3 +579,24 @@ def test_pipeline_flag_validation(tmp_path):
           "gemma_kinship.cXX.txt")
     check("must be provided together", eigenvalue_file=tmp_path / "x.eigenD.txt")
     PipelineRunner(PipelineConfig(bfile=pre, hwe_threshold=1e-6)).validate_inputs()
+
+
+def test_guard_rows_selects_p_values_near_thresholds():
+    """Rows whose p-value (of the mode's tests) lies within the band, in log10 units,
+    of 0.05, 0.01, 1e-3 or 5e-8 are the ones the mixed path recomputes in FP64; NaN and
+    p = 0 never qualify."""
+    from jamma_amd.lmm.engine import guard_rows
+
+    out = np.full((8, _lib.JM_NOUT), 0.5)
+    out[1, 2] = 0.05 * 10 ** 1.5e-4     # Wald just above 0.05, inside the band
+    out[2, 7] = 0.01 * 10 ** -3e-4      # LRT outside the band
+    out[3, 8] = 5e-8                    # Score on the threshold
+    out[4, 2] = np.nan
+    out[5, 2] = 0.0
+    out[6, 7] = 1e-3 * 10 ** -1e-4      # LRT inside
+    out[7, 2] = 1e-3 * 10 ** 5e-5       # Wald inside, but mode 2 does not test Wald
+    assert list(guard_rows(out, 4)) == [1, 3, 6, 7]
+    assert list(guard_rows(out, 1)) == [1, 7]
+    assert list(guard_rows(out, 2)) == [6]
+    assert list(guard_rows(out, 3)) == [3]
+    assert list(guard_rows(out, 4, band=1e-6)) == [3]
