@@ -73,6 +73,22 @@ int jm_transpose_i8(const int8_t* in, int64_t rows, int64_t cols, int64_t ld_in,
                     int64_t ld_out, int8_t* out0, int64_t ld_out0, void* stream);
 
 /*
+ * Dosage (non-integer) genotypes, float64 (compute_centered_kinship and the
+ * in-memory runner accept any float matrix with NaN for missing:
+ * kinship/compute.py:93-226, runner_jax.py:52-481):
+ *   jm_center_columns_f64: impute_and_center (kinship/missing.py:22-61) of a
+ *     sample-major tile X (n x cols, row stride ld) in place: mu_j = nanmean of
+ *     column j (0 if all NaN, written to mean_out[j] when not NULL), X <- X - mu_j,
+ *     NaN -> 0. The kinship then accumulates Xc Xc^T with jm_dgemm.
+ *   jm_impute_rows_f64: SNP-major rows X (rows x n, stride ld), NaN -> mean[r],
+ *     before the FP64 rotation (jm_gemm_nt_f64) of the association.
+ */
+int jm_center_columns_f64(double* X, int64_t n, int64_t cols, int64_t ld, double* mean_out,
+                          void* stream);
+int jm_impute_rows_f64(double* X, int64_t rows, int64_t n, int64_t ld, const double* mean,
+                       void* stream);
+
+/*
  * Replaces _accumulate_kinship (src/jamma/kinship/compute.py:46-57) inside the
  * batch loops of compute_centered_kinship (:205-214) and
  * compute_kinship_streaming (:663-681), together with impute_and_center

@@ -65,6 +65,8 @@ _SIGS = {
                        c_void_p, c_void_p], c_int),
     "jm_transpose_i8": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_void_p,
                          c_int64, c_void_p], c_int),
+    "jm_center_columns_f64": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p], c_int),
+    "jm_impute_rows_f64": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p], c_int),
     "jm_kinship_syrk_i8": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p], c_int),
     "jm_kinship_impute_terms": ([c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_void_p,
                                  c_void_p, c_void_p, c_int, c_void_p], c_int),

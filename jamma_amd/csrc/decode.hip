@@ -14,6 +14,8 @@
 
 #include "jm_common.h"
 
+#include <algorithm>
+
 namespace jm {
 
 __global__ void __launch_bounds__(256) k_decode_bed(const uint8_t* __restrict__ bed,
@@ -122,6 +124,78 @@ extern "C" int jm_transpose_i8(const int8_t* in, int64_t rows, int64_t cols, int
   const dim3 grid((unsigned)ceil_div(cols, 64), (unsigned)ceil_div(rows, 64));
   hipLaunchKernelGGL(k_transpose_i8, grid, dim3(256), 0, (hipStream_t)stream, in, rows, cols,
                      ld_in, out, ld_out, out0, ld_out0);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Dosage (non-integer) genotypes in float64
+// ---------------------------------------------------------------------------
+namespace jm {
+
+// impute_and_center (kinship/missing.py:22-61) of a sample-major tile X (n x cols,
+// row stride ld): per column j, mu_j = mean of the non-NaN entries (0 when all are
+// NaN); X[i][j] <- X[i][j] - mu_j, NaN -> 0. One thread per column, rows walked in
+// order (coalesced across the threads of a wave; a fixed summation order).
+__global__ void __launch_bounds__(256) k_center_columns_f64(double* __restrict__ X, int64_t n,
+                                                            int64_t cols, int64_t ld,
+                                                            double* __restrict__ mean_out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  double s = 0.0;
+  int64_t cnt = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double v = X[i * ld + j];
+    if (v == v) {
+      s += v;
+      ++cnt;
+    }
+  }
+  const double mu = cnt > 0 ? s / (double)cnt : 0.0;
+  if (mean_out) mean_out[j] = mu;
+  for (int64_t i = 0; i < n; ++i) {
+    const double v = X[i * ld + j];
+    X[i * ld + j] = (v == v) ? v - mu : 0.0;
+  }
+}
+
+// SNP-major rows X (rows x n, row stride ld): NaN -> mean[r] (mean imputation of the
+// association runners, runner_jax.py / runner_streaming.py:429-434).
+__global__ void __launch_bounds__(256) k_impute_rows_f64(double* __restrict__ X, int64_t rows,
+                                                         int64_t n, int64_t ld,
+                                                         const double* __restrict__ mean) {
+  const int64_t r = blockIdx.y;
+  if (r >= rows) return;
+  const double m = mean[r];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = X[r * ld + i];
+    if (!(v == v)) X[r * ld + i] = m;
+  }
+}
+
+}  // namespace jm
+
+extern "C" int jm_center_columns_f64(double* X, int64_t n, int64_t cols, int64_t ld,
+                                     double* mean_out, void* stream) {
+  JM_CHECK_ARG(n >= 1 && cols >= 0 && ld >= cols, "bad sizes");
+  if (cols == 0) return JM_OK;
+  JM_CHECK_ARG(X != nullptr, "null X");
+  hipLaunchKernelGGL(jm::k_center_columns_f64, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, X, n, cols, ld, mean_out);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int jm_impute_rows_f64(double* X, int64_t rows, int64_t n, int64_t ld,
+                                  const double* mean, void* stream) {
+  JM_CHECK_ARG(rows >= 0 && n >= 1 && ld >= n, "bad sizes");
+  JM_CHECK_ARG(rows <= 65535, "at most 65535 rows per call");
+  if (rows == 0) return JM_OK;
+  JM_CHECK_ARG(X && mean, "null pointer");
+  const unsigned bx = (unsigned)std::min<int64_t>((n + 255) / 256, 64);
+  hipLaunchKernelGGL(jm::k_impute_rows_f64, dim3(bx, (unsigned)rows), dim3(256), 0,
+                     (hipStream_t)stream, X, rows, n, ld, mean);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

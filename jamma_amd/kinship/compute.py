@@ -13,6 +13,7 @@ jm_kinship_finalize). Per-SNP statistics come from jm_decode_bed counts.
 from __future__ import annotations
 
 import logging
+import warnings
 from collections.abc import Iterator
 from pathlib import Path
 
@@ -168,10 +169,63 @@ def _counts(gi8: np.ndarray) -> np.ndarray:
                     axis=1).astype(np.int64)
 
 
+def is_hard_calls(genotypes: np.ndarray) -> bool:
+    """True when every non-NaN value is 0, 1 or 2 (the int8 path applies)."""
+    g = np.asarray(genotypes)
+    v = g[~np.isnan(g)]
+    return bool(np.all((v == 0) | (v == 1) | (v == 2)))
+
+
+def snp_stats_dense(genotypes: np.ndarray):
+    """compute_snp_stats (core/snp_filter.py:36-56) on a dense matrix with NaN: per-SNP
+    nanmean, missing count and nanvar in the input's dtype, NaN -> 0."""
+    g = np.asarray(genotypes)
+    miss = np.sum(np.isnan(g), axis=0)
+    with np.errstate(invalid="ignore"), warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        mean = np.nanmean(g, axis=0)
+        var = np.nanvar(g, axis=0)
+    return (np.nan_to_num(mean, nan=0.0).astype(np.float64), miss.astype(np.int64),
+            np.nan_to_num(var, nan=0.0).astype(np.float64))
+
+
+def dosage_kinship_device(genotypes: np.ndarray, batch_size: int = 10000,
+                          maf_threshold: float = 0.0, miss_threshold: float = 1.0) -> torch.Tensor:
+    """compute_centered_kinship for non-integer genotypes (kinship/compute.py:93-226):
+    the reference's filter on nanmean / missing count / nanvar, then per batch of SNPs
+    impute_and_center on the device (jm_center_columns_f64) and K += Xc Xc^T on FP64
+    MFMA (jm_dgemm), K /= p."""
+    g = np.asarray(genotypes)
+    n, p0 = g.shape
+    mean, miss, var = snp_stats_dense(g)
+    af = mean / 2.0
+    maf = np.minimum(af, 1.0 - af)
+    idx = np.where((maf >= maf_threshold) & (miss / n <= miss_threshold) & (var > 0))[0]
+    if len(idx) == 0:
+        raise ValueError(f"No SNPs passed filtering (maf>={maf_threshold}, miss<={miss_threshold},"
+                         f" polymorphic). Original SNP count: {p0}")
+    dev = require_gpu()
+    K = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    st = stream_ptr()
+    for s in range(0, len(idx), batch_size):
+        sel = idx[s:s + batch_size]
+        b = len(sel)
+        X = to_dev(np.ascontiguousarray(g[:, sel], dtype=np.float64), torch.float64, dev)
+        _lib.call("jm_center_columns_f64", ptr(X), n, b, b, None, st)
+        # column-major view: X is (b x n) with ld b; K (n x n) += X^T X
+        _lib.call("jm_dgemm", 1, 0, n, n, b, 1.0, ptr(X), b, ptr(X), b, 1.0, ptr(K), n, 0, st)
+        del X
+    K /= float(len(idx))
+    return K
+
+
 def centered_kinship_device(genotypes: np.ndarray, batch_size: int = 10000,
                             maf_threshold: float = 0.0, miss_threshold: float = 1.0) -> torch.Tensor:
-    """compute_centered_kinship on the device -> device float64 K."""
+    """compute_centered_kinship on the device -> device float64 K. Hard calls (0/1/2)
+    take the int8 path; any other values the FP64 dosage path."""
     g = np.asarray(genotypes)
+    if not is_hard_calls(g):
+        return dosage_kinship_device(g, batch_size, maf_threshold, miss_threshold)
     n = g.shape[0]
     gi8 = _integral_i8(g)
     counts = _counts(gi8)

@@ -325,6 +325,24 @@ class AssocProblem:
         out[idx.to(torch.int64)] = o[:k]
         return k
 
+    def associate_dense(self, X: torch.Tensor, means: torch.Tensor, n_snps: int,
+                        mode: int) -> torch.Tensor:
+        """One tile of dosage genotypes: X SNP-major float64 rows (n_snps x n, NaN =
+        missing, imputed in place with `means`), FP64 rotation (jm_gemm_nt_f64), then
+        moments and the lambda search + tests as for hard calls."""
+        co = self.cohort
+        n = co.n
+        if tuple(X.shape) != (n_snps, n) or X.dtype != torch.float64 or not X.is_contiguous():
+            raise ValueError("X must be contiguous float64 (n_snps x n)")
+        st = stream_ptr()
+        _lib.call("jm_impute_rows_f64", ptr(X), n_snps, n, n, ptr(means), st)
+        ldz = max(1, n_snps)
+        Z = torch.empty((n, ldz), dtype=torch.float64, device=co.device)
+        _lib.call("jm_gemm_nt_f64", ptr(co.UT), co.ld_u, ptr(X), n, n, n_snps, n, ptr(Z), ldz, st)
+        xmom = co.moments(Z, False, ldz, n_snps, self.V, self.n_cvt + 1)
+        del Z
+        return self.assoc_moments(xmom, n_snps, mode)
+
     def associate(self, g: torch.Tensor, ld_g: int, means: torch.Tensor, n_snps: int, mode: int,
                   precision: str = "mixed") -> torch.Tensor:
         """One SNP tile: rotation -> moments -> lambda search + tests (device out)."""

@@ -337,8 +337,9 @@ def run_lmm_association_jax(
 ) -> list[AssocResult]:
     """In-memory runner (src/jamma/lmm/runner_jax.py:52-481) on the device engine.
 
-    Genotypes must be hard calls 0/1/2 with NaN for missing (the PLINK case);
-    SNP statistics follow the input dtype (float32 from a PLINK load, as in
+    Hard calls 0/1/2 (NaN = missing; the PLINK case) take the int8 decode and either
+    rotation; any other values (dosages) are mean-imputed on the device and rotated
+    in FP64. SNP statistics follow the input dtype (float32 from a PLINK load, as in
     the reference's compute_snp_stats)."""
     del use_gpu, check_memory
     precision = precision or default_precision()
@@ -353,10 +354,21 @@ def run_lmm_association_jax(
         genotypes = genotypes[valid, :]
     n, p = genotypes.shape
     gi8, integral = encode_genotypes_i8(genotypes)
-    if not integral:
-        raise ValueError("genotypes must be hard calls 0/1/2 (NaN = missing)")
-    counts = counts_from_i8(gi8)
-    mask, means, af, miss = snp_filter(counts, n, maf_threshold, miss_threshold, stat_dtype)
+    if integral:
+        counts = counts_from_i8(gi8)
+        mask, means, af, miss = snp_filter(counts, n, maf_threshold, miss_threshold, stat_dtype)
+    else:
+        # dosages: the reference's filter on nanmean / missing / nanvar (input dtype),
+        # FP64 rotation of the mean-imputed values (no int8 packing, no bf16 split)
+        from jamma_amd.kinship.compute import snp_stats_dense
+
+        means, miss, var = snp_stats_dense(genotypes)
+        af = means / 2.0
+        maf = np.minimum(af, 1.0 - af)
+        mask = (maf >= maf_threshold) & (miss / n <= miss_threshold) & (var > 0)
+        if precision != "f64":
+            logger.info("dosage genotypes: FP64 rotation (the mixed path needs hard calls)")
+        precision = "f64"
     snp_indices = np.where(mask)[0]
     if len(snp_indices) == 0:
         return []
@@ -365,6 +377,17 @@ def run_lmm_association_jax(
     ld = round_up(n, 64)
     results = []
     sink = TileSink(lmm_mode, snp_info, af, miss, None, results)
+    if not integral:
+        for t0i in range(0, len(snp_indices), TILE_SNPS):
+            tsel = snp_indices[t0i:t0i + TILE_SNPS]
+            X = to_dev(np.ascontiguousarray(np.asarray(genotypes[:, tsel], dtype=np.float64).T),
+                       torch.float64)
+            mdev = to_dev(means[tsel], torch.float64)
+            sink.push(tsel, tsel, prob.associate_dense(X, mdev, len(tsel), lmm_mode))
+        sink.close()
+        if show_progress:
+            logger.info("LMM association (f64, dosages, in-memory, %d SNPs) done", len(snp_indices))
+        return results
     for t0i in range(0, len(snp_indices), TILE_SNPS):
         tsel = snp_indices[t0i:t0i + TILE_SNPS]
         gt = np.zeros((len(tsel), ld), dtype=np.int8)

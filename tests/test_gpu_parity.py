@@ -572,3 +572,49 @@ def test_rotation_f64_few_columns(n, ncols, gather):
     got = Z.cpu().numpy()
     scale = np.abs(UT) @ np.abs(X)
     assert np.all(np.abs(got - ref) <= 1e-13 * scale)
+
+
+def _dosages(n, p, seed, miss=0.02):
+    rng = np.random.default_rng(seed)
+    af = rng.uniform(0.05, 0.5, p)
+    G = rng.binomial(2, af, size=(n, p)).astype(np.float64)
+    G += rng.normal(0, 0.15, size=(n, p))  # imputation-like dosages in [0, 2]
+    G = np.clip(G, 0.0, 2.0)
+    G[rng.random((n, p)) < miss] = np.nan
+    G[:, 3] = 1.25  # monomorphic: filtered
+    return G
+
+
+@pytest.mark.parametrize("n,p,batch", [(211, 900, 256), (640, 1500, 10000)])
+def test_dosage_kinship_matches_oracle(n, p, batch):
+    """compute_centered_kinship on non-integer genotypes (kinship/compute.py:93-226):
+    device centring (jm_center_columns_f64) + FP64 MFMA accumulation (jm_dgemm) vs the
+    oracle's impute_and_center + Xc Xc^T / p, and the reference's filters."""
+    from jamma_amd.kinship import compute_centered_kinship
+
+    G = _dosages(n, p, n)
+    K = compute_centered_kinship(G, batch_size=batch)
+    ref = ok.centered_kinship(G)
+    assert np.max(np.abs(K - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+    K2 = compute_centered_kinship(G, batch_size=batch, maf_threshold=0.1, miss_threshold=0.03)
+    ref2 = ok.centered_kinship(G, maf_threshold=0.1, miss_threshold=0.03)
+    assert np.max(np.abs(K2 - ref2)) < 1e-12 * max(1.0, np.max(np.abs(ref2)))
+
+
+def test_dosage_association_matches_oracle():
+    """run_lmm_association_jax on dosages: device mean imputation + FP64 rotation, then
+    the same moments and search as hard calls, against the oracle run on the same
+    float64 matrix (statistics in float64, the input dtype)."""
+    from jamma_amd.lmm import run_lmm_association_jax
+
+    G = _dosages(300, 400, 5)
+    y, W = phenotype(np.nan_to_num(G[:, :150]), n_cov=2, seed=5)
+    K = ok.centered_kinship(G[:, 150:])
+    for precision in ("f64", "mixed"):  # dosages always take the FP64 rotation
+        res = run_lmm_association_jax(G[:, :150], y, K, _snp_info(150), covariates=W,
+                                      lmm_mode=4, precision=precision)
+        ref = ol.run_lmm(G[:, :150], y, kinship=K, covariates=W, lmm_mode=4,
+                         stat_dtype=np.float64)
+        assert [r.rs for r in res] == [f"rs{j:04d}" for j in ref["snp_indices"]]
+        assert "rs0003" not in [r.rs for r in res]
+        _assert_matches_oracle(_arrays(res, ALL), ref, ALL)

@@ -600,3 +600,25 @@ def test_guard_rows_selects_p_values_near_thresholds():
     assert list(guard_rows(out, 2)) == [6]
     assert list(guard_rows(out, 3)) == [3]
     assert list(guard_rows(out, 4, band=1e-6)) == [3]
+
+
+def test_jamma_import_alias():
+    """`jamma.*` imports of a reference caller resolve to this package's modules (the
+    same module objects), with runner_jax mapped to the module holding the in-memory
+    runner; modules outside the hot path raise ImportError."""
+    import importlib
+
+    import jamma
+    import jamma_amd.kinship.compute
+    import jamma_amd.lmm.runner_streaming
+
+    assert importlib.import_module("jamma.kinship.compute") is jamma_amd.kinship.compute
+    assert importlib.import_module("jamma.lmm.runner_jax") is jamma_amd.lmm.runner_streaming
+    from jamma.kinship import compute_centered_kinship
+    from jamma.lmm import AssocResult as A1
+    from jamma.lmm.stats import AssocResult as A2
+
+    assert A1 is A2 and compute_centered_kinship is jamma_amd.kinship.compute.compute_centered_kinship
+    assert jamma.gwas is jamma_amd.gwas
+    with pytest.raises(ImportError):
+        importlib.import_module("jamma.cli")
