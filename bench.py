@@ -151,6 +151,7 @@ def main() -> int:
     pf = gf.shape[0]
     means = to_dev(mean32[idx], torch.float64)
     mu64, _, _ = stats_from_counts(counts[idx], np.float64)
+    miss_f = counts[idx, 3]
     log(f"{pf} SNPs pass the filters")
 
     # ---- kinship on int8 MFMA (kinship-filter = var > 0 over all samples) ----
@@ -163,7 +164,7 @@ def main() -> int:
     ktile = args.kin_tile
     for s in range(0, pf, ktile):
         e = min(s + ktile, pf)
-        acc.add_tile(gf[s:e], ld, e - s, mu64[s:e], True)
+        acc.add_tile(gf[s:e], ld, e - s, mu64[s:e], True, int(miss_f[s:e].sum()))
     acc.flush()
     torch.cuda.synchronize()
     syrk_ms = sum(a.elapsed_time(b) for a, b in acc.syrk_timer)
@@ -437,7 +438,8 @@ def loco(args) -> int:
         acc = KinshipAccumulator(n, max_snps=len(sel))
         for s in range(0, len(sel), 16384):
             ss = torch.as_tensor(sel[s:s + 16384], device=dev)
-            acc.add_tile(g[ss].contiguous(), ld, len(ss), mu64[sel[s:s + 16384]], True)
+            acc.add_tile(g[ss].contiguous(), ld, len(ss), mu64[sel[s:s + 16384]], True,
+                         int(counts[sel[s:s + 16384], 3].sum()))
         acc.release()
         return acc
 
@@ -664,7 +666,7 @@ def serving_session(args):
         acc = KinshipAccumulator(n)
         for s in range(0, p, 8192):
             e = min(s + 8192, p)
-            acc.add_tile(g[s:e], ld, e - s, mu64[s:e], True)
+            acc.add_tile(g[s:e], ld, e - s, mu64[s:e], True, int(counts[s:e, 3].sum()))
         K = acc.finalize(p)
         del acc
         t1 = time.perf_counter()

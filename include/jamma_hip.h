@@ -112,13 +112,18 @@ int jm_kinship_syrk_i8(const int8_t* gT0, int64_t ld_gT, int64_t n, int64_t n_sn
  *   mu[j]    = float64 nanmean of SNP j (host computes from jm_decode_bed counts)
  *   u[i]    += sum_j (g_ij >= 0 ? g_ij mu_j : mu_j^2)
  *   C[k][i] += sum_{j: g_kj missing} mu_j * (g_ij >= 0 ? g_ij : mu_j / 2)
- * g is the SNP-major tile (n_snps x ld_g), gT the sample-major one
- * (n x ld_gT). C (n x n float64) is only touched for samples with missing
- * calls in the tile.
+ * g is the SNP-major tile (n_snps x ld_g, rows 16-byte aligned), gT the
+ * sample-major one (n x ld_gT, rows 16-byte aligned and zero-padded to a multiple
+ * of 16). With has_missing, n_missing is the tile's number of missing calls (from
+ * the decode counts) and miss_ws holds jm_kinship_miss_ws_len(n, n_snps,
+ * n_missing) int32 of 8-byte aligned device scratch: the per-sample lists of
+ * missing SNPs and the tile's genotypes as 2-bit codes are built there first.
  */
+int64_t jm_kinship_miss_ws_len(int64_t n, int64_t n_snps, int64_t n_missing);
 int jm_kinship_impute_terms(const int8_t* g, int64_t ld_g, const int8_t* gT, int64_t ld_gT,
                             int64_t n, int64_t n_snps, const double* mu, double* u, double* C,
-                            int has_missing, void* stream);
+                            int has_missing, int32_t* miss_ws, int64_t miss_ws_len,
+                            int64_t n_missing, void* stream);
 
 /*
  * K[i][k] = (S_ik - u_i - u_k + s0 + C[i][k] + C[k][i]) / p_snps

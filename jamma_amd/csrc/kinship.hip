@@ -202,97 +202,270 @@ __global__ void __launch_bounds__(256) k_impute_u(const int8_t* __restrict__ gT,
   if (threadIdx.x == 0) u[i] += ((red[0] + red[1]) + (red[2] + red[3]));
 }
 
-// C[k][i] += sum_{j: sample k missing at j} mu_j * Y_ij over one SNP tile.
-// Block (k, i-chunk of 2048): the missing SNPs of row k are compacted into LDS
-// (in SNP order), then each thread accumulates 8 consecutive samples i. Per
-// SNP the contribution mu_j * Y_ij is one of four values indexed by the 2-bit
-// code of g_ij (0, mu, 2 mu, missing -> mu^2 / 2), so the inner loop is a
-// select and an add; four SNPs are in flight at a time.
-constexpr int C_CHUNK = 4096;
-__device__ __forceinline__ double code_val(uint32_t code, double v1, double v2, double vm) {
-  const double lo = (code & 1u) ? v1 : 0.0;
-  const double hi = (code & 1u) ? vm : v2;
-  return (code & 2u) ? hi : lo;
+// C[k][i] += sum_{j: sample k missing at j} mu_j * Y_ij over one SNP tile, in four
+// steps. The missing calls of every sample are first listed (k_miss_count,
+// k_miss_scan, k_miss_fill: a CSR of SNP indices per sample, in SNP order, built once
+// per tile), and the tile's genotypes are packed to 2-bit codes (k_pack_codes: the
+// int8 value's low two bits, 0 / 1 / 2 / 3 = missing; 16 samples per 32-bit word).
+// k_impute_C then gives each wavefront one sample k and 512 samples i (8 per lane);
+// per listed SNP j the contribution is mu_j * x with x the code of g_ij converted to
+// FP64 (0 / 1 / 2, and mu_j / 2 for a missing call), one fused multiply-add per
+// element (the kernel is VALU-bound: a conversion, a compare, a 64-bit select and the
+// FMA per element), one 4-byte load per two lanes, sixteen SNPs in flight, with the
+// list and the means on scalar loads. The blocks are ordered so that each XCD sweeps
+// the 512-sample chunks one after another: the codes of one chunk (128 B per SNP, 2 MB
+// for 16k SNPs) stay in that XCD's L2 while all samples k read them. Each C element
+// is summed by one lane in list order (deterministic).
+constexpr int C_ICHUNK = 512;  // samples i per wavefront (8 per lane)
+constexpr int C_WAVES = 4;     // samples k per block
+
+// The factor multiplying mu_j in mu_j * Y_ij: the genotype 0 / 1 / 2 (code 0-2), or
+// mu_j / 2 at a missing call (code 3, h = mu_j / 2): a conversion and one select.
+__device__ __forceinline__ double code_x(uint32_t code, double h) {
+  return code == 3u ? h : (double)code;
 }
 
-__global__ void __launch_bounds__(256) k_impute_C(const int8_t* __restrict__ g, int64_t ld_g,
-                                                  const int8_t* __restrict__ gT, int64_t ld_gT,
-                                                  int64_t n, int64_t K,
-                                                  const double* __restrict__ mu,
+// cnt[k] = missing calls (-1) of sample k among the tile's K SNPs; gT rows are
+// zero-padded to a multiple of 128 (16-byte loads stay inside the row).
+__global__ void __launch_bounds__(256) k_miss_count(const int8_t* __restrict__ gT, int64_t ld,
+                                                    int64_t n, int64_t K, int32_t* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= n) return;
+  int c = 0;
+  for (int64_t j = (int64_t)lane * 16; j < K; j += 64 * 16) {
+    const uint4 v = *reinterpret_cast<const uint4*>(gT + k * ld + j);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      c += (j + e < K && ((w[e >> 2] >> (8 * (e & 3))) & 0xFFu) == 0xFFu) ? 1 : 0;
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m, 64);
+  if (lane == 0) cnt[k] = c;
+}
+
+// In place: cnt[0..n) -> exclusive offsets off[0..n], off[n] = total (one block).
+__global__ void __launch_bounds__(1024) k_miss_scan(int32_t* __restrict__ off, int64_t n) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (n + 1023) / 1024;
+  const int64_t a = t * per, b = min(n, a + per);
+  int64_t s = 0;
+  for (int64_t i = a; i < b; ++i) s += off[i];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int64_t run = 0;
+    for (int q = 0; q < 1024; ++q) {
+      const int64_t v = part[q];
+      part[q] = run;
+      run += v;
+    }
+    off[n] = (int32_t)run;
+  }
+  __syncthreads();
+  int64_t run = part[t];
+  for (int64_t i = a; i < b; ++i) {
+    const int32_t v = off[i];
+    off[i] = (int32_t)run;
+    run += v;
+  }
+}
+
+// list[off[k] + q] = SNP index of the q-th missing call of sample k (SNP order);
+// entries at or past cap are not written.
+__global__ void __launch_bounds__(256) k_miss_fill(const int8_t* __restrict__ gT, int64_t ld,
+                                                   int64_t n, int64_t K,
+                                                   const int32_t* __restrict__ off,
+                                                   int32_t* __restrict__ list, int64_t cap) {
+  const int lane = threadIdx.x & 63;
+  const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= n) return;
+  int64_t pos = off[k];
+  for (int64_t j0 = 0; j0 < K; j0 += 64 * 16) {
+    const int64_t j = j0 + (int64_t)lane * 16;
+    uint32_t mask = 0;
+    if (j < K) {
+      const uint4 v = *reinterpret_cast<const uint4*>(gT + k * ld + j);
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
+        if (j + e < K && ((w[e >> 2] >> (8 * (e & 3))) & 0xFFu) == 0xFFu) mask |= 1u << e;
+    }
+    const int c = __popc(mask);
+    int incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += v;
+    }
+    int64_t p = pos + incl - c;
+    for (int e = 0; e < 16; ++e)
+      if (mask & (1u << e)) {
+        if (p < cap) list[p] = (int32_t)(j + e);
+        ++p;
+      }
+    pos += __shfl(incl, 63, 64);
+  }
+}
+
+// gc[j][w] (ldc words per SNP row): the 2-bit codes of samples 16w .. 16w + 15 of
+// SNP j (low bits first); samples past n are code 0.
+__global__ void __launch_bounds__(256) k_pack_codes(const int8_t* __restrict__ g, int64_t ld_g,
+                                                    int64_t n, int64_t K,
+                                                    uint32_t* __restrict__ gc, int64_t ldc) {
+  const int64_t words = (n + 15) / 16;
+  const int64_t total = K * words;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e / words, wd = e % words;
+    const int64_t i0 = wd * 16;
+    uint32_t code = 0;
+    if (i0 + 16 <= n) {
+      const uint4 v = *reinterpret_cast<const uint4*>(g + j * ld_g + i0);
+      const uint32_t b[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int q = 0; q < 16; ++q) code |= ((b[q >> 2] >> (8 * (q & 3))) & 3u) << (2 * q);
+    } else {
+      for (int q = 0; i0 + q < n; ++q) code |= ((uint32_t)(uint8_t)g[j * ld_g + i0 + q] & 3u) << (2 * q);
+    }
+    gc[j * ldc + wd] = code;
+  }
+}
+
+typedef const int32_t __attribute__((address_space(4))) c_i32;
+typedef const double __attribute__((address_space(4))) c_f64;
+
+__global__ void __launch_bounds__(256) k_impute_C(const uint32_t* __restrict__ gc, int64_t ldc,
+                                                  int64_t n, const double* __restrict__ mu,
+                                                  const int32_t* __restrict__ off,
+                                                  const int32_t* __restrict__ list, int64_t cap,
                                                   double* __restrict__ C) {
-  __shared__ int32_t list[C_CHUNK];
-  __shared__ int32_t wsum[5];
-  const int64_t k = blockIdx.x;
-  const int64_t ibase = (int64_t)blockIdx.y * 2048 + threadIdx.x * 8;
-  const bool full = ibase + 8 <= n;  // 8 samples in range: one 8-byte load per SNP
+  const int lane = threadIdx.x & 63;
+  // wave-uniform values in scalar registers: the list, its offsets and the means then
+  // come through scalar loads, and only the genotype codes are vector loads
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ni = (int)((n + C_ICHUNK - 1) / C_ICHUNK);
+  const int nk = (int)((n + C_WAVES - 1) / C_WAVES);
+  // XCD-contiguous ranges of the chunk-major order (blocks b, b + 8, ... share an XCD)
+  const int total = ni * nk;
+  const int x = blockIdx.x % 8, jb = blockIdx.x / 8;
+  const int base = total / 8, rem = total % 8;
+  const int t = x * base + (x < rem ? x : rem) + jb;
+  const int ic = t / nk, kg = t % nk;
+  const int64_t k = (int64_t)kg * C_WAVES + w;
+  if (k >= n) return;
+  const c_i32* soff = (const c_i32*)off;
+  const c_i32* slist = (const c_i32*)list;
+  const c_f64* smu = (const c_f64*)mu;
+  const int64_t i0 = (int64_t)ic * C_ICHUNK + lane * 8;
+  const bool live = i0 < n;
+  const uint32_t* gl = gc + i0 / 16;  // word of this lane's 8 samples
+  const int sh = (lane & 1) * 16;
+  int64_t q0 = soff[k], q1 = soff[k + 1];
+  if (q1 > cap) q1 = cap;
   double acc[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) acc[e] = 0.0;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int64_t j0 = 0; j0 < K; j0 += C_CHUNK) {
-    // each thread inspects 16 consecutive SNPs of row k
-    const int64_t jt = j0 + threadIdx.x * 16;
-    uint32_t mask = 0;
+  constexpr int U = 16;  // SNPs in flight
+  int64_t q = q0;
+  for (; q + U <= q1; q += U) {
+    int32_t j[U];
+    uint32_t v[U];
+    double m[U];
 #pragma unroll
-    for (int e = 0; e < 16; ++e)
-      if (jt + e < K && gT[k * ld_gT + jt + e] < 0) mask |= (1u << e);
-    int cnt = __popc(mask);
-    int incl = cnt;
+    for (int u = 0; u < U; ++u) j[u] = slist[q + u];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int v = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += v;
+    for (int u = 0; u < U; ++u) {
+      m[u] = smu[j[u]];
+      v[u] = live ? gl[(int64_t)j[u] * ldc] >> sh : 0u;
     }
-    if (lane == 63) wsum[w] = incl;
-    __syncthreads();
-    int wofs = 0;
-    for (int q = 0; q < w; ++q) wofs += wsum[q];
-    const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    int pos = wofs + incl - cnt;
-    for (int e = 0; e < 16; ++e)
-      if (mask & (1u << e)) list[pos++] = (int32_t)(jt + e);
-    __syncthreads();
-    if (ibase < n) {
-      int q = 0;
-      for (; q + 4 <= total; q += 4) {
-        uint64_t v[4];
-        double m[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int64_t j = list[q + u];
-          m[u] = mu[j];
-          const int8_t* gr = g + j * ld_g + ibase;
-          if (full) {
-            v[u] = *reinterpret_cast<const uint64_t*>(gr);
-          } else {
-            v[u] = 0;
-            for (int e = 0; e < 8; ++e)
-              if (ibase + e < n) v[u] |= (uint64_t)(uint8_t)gr[e] << (8 * e);
-          }
-        }
+    for (int u = 0; u < U; ++u) {
+      const double h = 0.5 * m[u];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const double v1 = m[u], v2 = 2.0 * m[u], vm = m[u] * (0.5 * m[u]);
-#pragma unroll
-          for (int e = 0; e < 8; ++e)
-            acc[e] += code_val((uint32_t)(v[u] >> (8 * e)) & 3u, v1, v2, vm);
-        }
-      }
-      for (; q < total; ++q) {
-        const int64_t j = list[q];
-        const double mj = mu[j];
-        const int8_t* gr = g + j * ld_g + ibase;
-        const double v1 = mj, v2 = 2.0 * mj, vm = mj * (0.5 * mj);
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-          if (ibase + e < n) acc[e] += code_val((uint32_t)(uint8_t)gr[e] & 3u, v1, v2, vm);
-      }
+      for (int e = 0; e < 8; ++e) acc[e] = fma(m[u], code_x((v[u] >> (2 * e)) & 3u, h), acc[e]);
     }
-    __syncthreads();
+  }
+  for (; q < q1; ++q) {
+    const int64_t jj = slist[q];
+    const double mj = smu[jj];
+    const uint32_t vv = live ? gl[jj * ldc] >> sh : 0u;
+    const double h = 0.5 * mj;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) acc[e] = fma(mj, code_x((vv >> (2 * e)) & 3u, h), acc[e]);
   }
 #pragma unroll
   for (int e = 0; e < 8; ++e)
-    if (ibase + e < n) C[k * n + ibase + e] += acc[e];
+    if (i0 + e < n) C[k * n + i0 + e] += acc[e];
+}
+
+// K for the tile pair (ti, tk), ti >= tk, of 32 x 32 blocks: both K[i][k] (rows of
+// block ti) and its mirror K[k][i] are written, and C[i][k] + C[k][i] is formed from
+// two tiles staged in LDS, so every global access is a coalesced row segment (the
+// element-wise form reads C and the upper triangle of S column-wise). S is read
+// from its lower triangle only.
+constexpr int KF_T = 32;
+__global__ void __launch_bounds__(256) k_kin_finalize_tiled(
+    const int32_t* __restrict__ S, const double* __restrict__ u, const double* __restrict__ C,
+    double s0, const int32_t* __restrict__ Ss, const double* __restrict__ us,
+    const double* __restrict__ Cs, double s0s, int64_t n, double p, double* __restrict__ K) {
+  __shared__ double ct[KF_T][KF_T + 1];  // C[k][i] - Cs[k][i] of the mirror tile, as [k][i]
+  __shared__ double kt[KF_T][KF_T + 1];  // K tile [i][k], read back transposed
+  __shared__ int32_t st[KF_T][KF_T + 1]; // diagonal tile: S[i][k] for the mirror entries
+  // linear block -> lower-triangle tile pair (ti, tk)
+  const int64_t b = blockIdx.x;
+  int64_t ti = (int64_t)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= b) ++ti;
+  while (ti * (ti + 1) / 2 > b) --ti;
+  const int64_t tk = b - ti * (ti + 1) / 2;
+  const int64_t i0 = ti * KF_T, k0 = tk * KF_T;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows x 32 columns per pass
+  // mirror tile: C[k][i] for k in block tk (rows), i in block ti (columns)
+  for (int r = ty; r < KF_T; r += 8) {
+    const int64_t k = k0 + r, i = i0 + tx;
+    double c = 0.0;
+    if (k < n && i < n) {
+      if (C) c = C[k * n + i];
+      if (Cs) c -= Cs[k * n + i];
+    }
+    ct[r][tx] = c;
+    if (ti == tk) st[r][tx] = (k < n && i < n) ? S[k * n + i] - (Ss ? Ss[k * n + i] : 0) : 0;
+  }
+  __syncthreads();
+  const double sv0 = s0 - (us ? s0s : 0.0);
+  for (int r = ty; r < KF_T; r += 8) {
+    const int64_t i = i0 + r, k = k0 + tx;
+    double kv = 0.0;
+    if (i < n && k < n) {
+      int64_t sv;
+      if (ti != tk || r >= tx) {  // lower triangle
+        sv = S[i * n + k];
+        if (Ss) sv -= Ss[i * n + k];
+      } else {
+        sv = st[tx][r];  // S[k][i], k > i: its lower-triangle entry
+      }
+      double ui = u[i], uk = u[k];
+      if (us) {
+        ui -= us[i];
+        uk -= us[k];
+      }
+      double c = 0.0;
+      if (C) c = C[i * n + k];
+      if (Cs) c -= Cs[i * n + k];
+      c += ct[tx][r];
+      kv = ((double)sv - ui - uk + sv0 + c) / p;
+      K[i * n + k] = kv;
+    }
+    kt[r][tx] = kv;
+  }
+  if (ti == tk) return;
+  __syncthreads();
+  for (int r = ty; r < KF_T; r += 8) {
+    const int64_t k = k0 + r, i = i0 + tx;
+    if (k < n && i < n) K[k * n + i] = kt[tx][r];
+  }
 }
 
 __global__ void k_kin_finalize(const int32_t* __restrict__ S, const double* __restrict__ u,
@@ -356,9 +529,19 @@ extern "C" int jm_kinship_syrk_i8(const int8_t* gT0, int64_t ld_gT, int64_t n, i
   return JM_OK;
 }
 
+// workspace (int32 units): offsets (n + 1) | lists (n_missing) | 2-bit codes (K rows of
+// ceil(n / 16) words, rounded up to an even count so that rows stay 8-byte aligned)
+static int64_t codes_ld(int64_t n) { return (((n + 15) / 16) + 1) / 2 * 2; }
+
+extern "C" int64_t jm_kinship_miss_ws_len(int64_t n, int64_t n_snps, int64_t n_missing) {
+  if (n < 1 || n_snps < 0 || n_missing < 0) return 0;
+  return n + 1 + (n_missing + 1) / 2 * 2 + n_snps * codes_ld(n);
+}
+
 extern "C" int jm_kinship_impute_terms(const int8_t* g, int64_t ld_g, const int8_t* gT,
                                        int64_t ld_gT, int64_t n, int64_t n_snps, const double* mu,
-                                       double* u, double* C, int has_missing, void* stream) {
+                                       double* u, double* C, int has_missing, int32_t* miss_ws,
+                                       int64_t miss_ws_len, int64_t n_missing, void* stream) {
   JM_CHECK_ARG(n >= 1 && n_snps >= 0, "bad sizes");
   if (n_snps == 0) return JM_OK;
   JM_CHECK_ARG(gT && mu && u, "null pointer");
@@ -368,9 +551,36 @@ extern "C" int jm_kinship_impute_terms(const int8_t* g, int64_t ld_g, const int8
   JM_CHECK_LAUNCH();
   if (has_missing) {
     JM_CHECK_ARG(g && C && ld_g >= n, "missing-call correction needs g and C");
-    JM_CHECK_ARG(ld_g % 8 == 0 && ((uintptr_t)g % 8) == 0, "g rows must be 8-byte aligned");
-    const dim3 grid((unsigned)n, (unsigned)ceil_div(n, 2048));
-    hipLaunchKernelGGL(k_impute_C, grid, dim3(256), 0, st, g, ld_g, gT, ld_gT, n, n_snps, mu, C);
+    JM_CHECK_ARG(ld_g % 16 == 0 && ((uintptr_t)g % 16) == 0, "g rows must be 16-byte aligned");
+    JM_CHECK_ARG(ld_gT % 16 == 0 && ((uintptr_t)gT % 16) == 0 && ld_gT >= ((n_snps + 15) / 16) * 16,
+                 "gT rows must be 16-byte aligned and padded to a multiple of 16");
+    JM_CHECK_ARG(n_missing >= 0 && n_missing < (1LL << 31),
+                 "n_missing out of range (at most 2^31 - 1 missing calls per tile)");
+    JM_CHECK_ARG(miss_ws && ((uintptr_t)miss_ws % 8) == 0 &&
+                     miss_ws_len >= jm_kinship_miss_ws_len(n, n_snps, n_missing),
+                 "missing-call correction needs an 8-byte aligned workspace of "
+                 "jm_kinship_miss_ws_len(n, n_snps, n_missing) int32");
+    JM_CHECK_ARG(n < (1LL << 31) && n_snps < (1LL << 31), "sizes exceed int32 indices");
+    const int64_t cap = n_missing;
+    int32_t* off = miss_ws;
+    int32_t* list = miss_ws + n + 1;
+    const int64_t ldc = codes_ld(n);
+    // n + 1 + even(n_missing): the code rows start 4-byte aligned past the lists
+    uint32_t* gc = (uint32_t*)(miss_ws + n + 1 + (n_missing + 1) / 2 * 2);
+    const unsigned rb = (unsigned)((n + 3) / 4);
+    hipLaunchKernelGGL(k_miss_count, dim3(rb), dim3(256), 0, st, gT, ld_gT, n, n_snps, off);
+    JM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_miss_scan, dim3(1), dim3(1024), 0, st, off, n);
+    JM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_miss_fill, dim3(rb), dim3(256), 0, st, gT, ld_gT, n, n_snps, off, list,
+                       cap);
+    JM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_pack_codes, dim3(4096), dim3(256), 0, st, g, ld_g, n, n_snps, gc, ldc);
+    JM_CHECK_LAUNCH();
+    const int64_t blocks = ((n + C_ICHUNK - 1) / C_ICHUNK) * ((n + C_WAVES - 1) / C_WAVES);
+    JM_CHECK_ARG(blocks < (1LL << 31), "n too large");
+    hipLaunchKernelGGL(k_impute_C, dim3((unsigned)blocks), dim3(256), 0, st, gc, ldc, n, mu, off,
+                       list, cap, C);
     JM_CHECK_LAUNCH();
   }
   return JM_OK;
@@ -383,8 +593,16 @@ extern "C" int jm_kinship_finalize(const int32_t* S, const double* u, const doub
   JM_CHECK_ARG(n >= 1 && p_snps > 0, "bad sizes");
   JM_CHECK_ARG(S && u && K, "null pointer");
   JM_CHECK_ARG((S_sub == nullptr) == (u_sub == nullptr), "S_sub and u_sub go together");
-  hipLaunchKernelGGL(k_kin_finalize, dim3(4096), dim3(256), 0, (hipStream_t)stream, S, u, C, s0,
-                     S_sub, u_sub, C_sub, s0_sub, n, p_snps, K);
+  const int64_t nt = (n + KF_T - 1) / KF_T;
+  const int64_t pairs = nt * (nt + 1) / 2;
+  if (pairs < (1LL << 31)) {
+    hipLaunchKernelGGL(k_kin_finalize_tiled, dim3((unsigned)pairs), dim3(256), 0,
+                       (hipStream_t)stream, S, u, C, s0, S_sub, u_sub, C_sub, s0_sub, n, p_snps,
+                       K);
+  } else {
+    hipLaunchKernelGGL(k_kin_finalize, dim3(4096), dim3(256), 0, (hipStream_t)stream, S, u, C,
+                       s0, S_sub, u_sub, C_sub, s0_sub, n, p_snps, K);
+  }
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

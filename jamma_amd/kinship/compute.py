@@ -93,8 +93,10 @@ class KinshipAccumulator:
         self._stage = None
 
     def add_tile(self, g: torch.Tensor, ld_g: int, n_snps: int, mu: np.ndarray,
-                 has_missing: bool) -> None:
-        """g: SNP-major int8 (n_snps x ld_g, -1 missing); mu: float64 nanmean per SNP."""
+                 has_missing: bool, n_missing: int | None = None) -> None:
+        """g: SNP-major int8 (n_snps x ld_g, -1 missing); mu: float64 nanmean per SNP;
+        n_missing: the tile's number of missing calls (from the decode counts; counted
+        on the device when not given)."""
         if n_snps == 0:
             return
         n = self.n
@@ -122,8 +124,15 @@ class KinshipAccumulator:
         mu_d = to_dev(np.asarray(mu, dtype=np.float64), torch.float64, self.device)
         if has_missing and self.C is None:
             self.C = torch.zeros((n, n), dtype=torch.float64, device=self.device)
+        ws, ws_len = None, 0
+        if has_missing:
+            if n_missing is None:
+                n_missing = int((g[:n_snps, :n] < 0).sum())
+            ws_len = int(_lib.lib.jm_kinship_miss_ws_len(n, n_snps, int(n_missing)))
+            ws = torch.empty((ws_len,), dtype=torch.int32, device=self.device)
         _lib.call("jm_kinship_impute_terms", ptr(g), ld_g, ptr(gT), ldt, n, n_snps, ptr(mu_d),
-                  ptr(self.u), ptr(self.C) if has_missing else None, int(has_missing), st)
+                  ptr(self.u), ptr(self.C) if has_missing else None, int(has_missing), ptr(ws),
+                  ws_len, int(n_missing or 0), st)
         for m in np.asarray(mu, dtype=np.float64):
             self.s0 += m * m
         self.p += n_snps
@@ -242,7 +251,8 @@ def centered_kinship_device(genotypes: np.ndarray, batch_size: int = 10000,
         sel = idx[s:s + batch_size]
         tile = np.zeros((len(sel), ld), dtype=np.int8)
         tile[:, :n] = gi8[sel]
-        acc.add_tile(to_dev(tile, torch.int8), ld, len(sel), mu[sel], bool(counts[sel, 3].any()))
+        acc.add_tile(to_dev(tile, torch.int8), ld, len(sel), mu[sel], bool(counts[sel, 3].any()),
+                     int(counts[sel, 3].sum()))
     return acc.finalize(len(idx))
 
 
@@ -297,7 +307,8 @@ def _accumulate_streaming(bed, idx, counts, chunk_size, acc_for):
         for acc, part in acc_for(sel):
             g, _, _ = decode(raw, bed.bytes_per_snp, len(part), n_out=n,
                              snp_idx=(part - start).astype(np.int32), want_counts=False, ld_g=ld)
-            acc.add_tile(g, ld, len(part), mu[part], bool(counts[part, 3].any()))
+            acc.add_tile(g, ld, len(part), mu[part], bool(counts[part, 3].any()),
+                         int(counts[part, 3].sum()))
 
 
 def kinship_streaming_device(bed_path: Path, chunk_size: int = 10_000, maf_threshold: float = 0.0,
@@ -392,7 +403,7 @@ def compute_loco_kinship(genotypes: np.ndarray, chromosome_for_each_snp: np.ndar
             tile = np.zeros((len(sel), ld), dtype=np.int8)
             tile[:, :n] = gi8[sel]
             acc.add_tile(to_dev(tile, torch.int8), ld, len(sel), mu[sel],
-                         bool(counts[sel, 3].any()))
+                         bool(counts[sel, 3].any()), int(counts[sel, 3].sum()))
         acc.release()
         return acc
 
