@@ -313,7 +313,11 @@ def main() -> int:
     rot_ms = mixed["kernel_ms_per_tile"][3]  # the rotation kernel alone
     tile_snps = pf / len(tiles)
     achieved = 2.0 * n * n * tile_snps / (rot_ms * 1e-3)
-    traffic = hbm_traffic("k_rotate_bf16x3_packed", n, args.tile)
+    traffic, traffic_status = hbm_traffic("k_rotate_bf16x3_packed", n, args.tile)
+    vendor = vendor_bf16_gemm_tflops(n, int(tile_snps), n)
+    issue_tflops = (args.planes + 1) * achieved / 1e12
+    log(f"rotation: {achieved / 1e12:.0f} TFLOP/s algorithmic, {issue_tflops:.0f} issued; "
+        f"hipBLASLt bf16 GEMM at ({n} x {int(tile_snps)} x {n}): {vendor:.0f} TFLOP/s")
     line = {
         "metric": METRIC,
         "value": mixed["snps_per_s"],
@@ -340,6 +344,9 @@ def main() -> int:
                                     f"planes of U^T issues {args.planes + 1} bf16 MFMA products "
                                     "per algorithmic FLOP",
                      "mfma_issue_frac": (args.planes + 1) * achieved / PEAK["bf16"],
+                     "traffic_source": traffic_status,
+                     "vendor_bf16_gemm_tflops": vendor,
+                     "issued_over_vendor_gemm": issue_tflops / vendor,
                      "launch_ms": rot_ms, "snps_per_launch": tile_snps},
         "cpu_baseline": cpu,
         "fp64_path": results.get("f64"),
@@ -368,19 +375,59 @@ def main() -> int:
     return 0
 
 
+KERNEL_SOURCES = {"k_rotate_bf16x3_packed": "rotate.hip", "k_gemm_f64_geno": "rotate.hip",
+                  "k_syrk_i8_main": "kinship.hip", "k_syrk_i8_tail": "kinship.hip",
+                  "k_mom_seg_f32": "moments.hip", "k_mom_fold": "moments.hip",
+                  "k_lmm_grid": "lmm.hip", "k_lmm_assoc16": "lmm.hip"}
+
+
+def source_hash(kernel: str) -> str | None:
+    """First 12 hex digits of the SHA-256 of the source file holding `kernel`."""
+    import hashlib
+
+    name = KERNEL_SOURCES.get(kernel)
+    if name is None:
+        return None
+    path = Path(__file__).resolve().parent / "jamma_amd" / "csrc" / name
+    return hashlib.sha256(path.read_bytes()).hexdigest()[:12]
+
+
 def hbm_traffic(kernel: str, n: int, tile: int):
     """HBM bytes per launch of `kernel` at (n, tile) from the committed rocprofv3 counter
     summary (profiles/hbm_traffic.txt: FETCH_SIZE x fetch scale + WRITE_SIZE x write scale,
-    the scales measured on a copy kernel of known size). None when not measured."""
+    the scales measured on a copy kernel of known size), valid only while the kernel's
+    source file has the hash recorded beside the line. -> (bytes or None, status)."""
     path = Path(__file__).resolve().parent / "profiles" / "hbm_traffic.txt"
     if not path.exists():
-        return None
+        return None, "not measured"
+    want = source_hash(kernel)
     for line in path.read_text().splitlines():
         f = line.split()
         if len(f) >= 5 and not line.startswith("#") and f[0] == kernel and \
                 int(f[1]) == n and int(f[2]) == tile:
-            return float(f[3]) + float(f[4])
-    return None
+            if len(f) < 6 or f[5] != want:
+                return None, f"stale (measured on source {f[5] if len(f) > 5 else '?'}, now {want})"
+            return float(f[3]) + float(f[4]), f"measured on source {want}"
+    return None, "not measured at this shape"
+
+
+def vendor_bf16_gemm_tflops(m: int, n: int, k: int) -> float:
+    """hipBLASLt bf16 GEMM (torch.matmul) TFLOP/s at the rotation's shape on this device:
+    the practical MFMA ceiling the rotation kernel is compared with."""
+    import torch
+
+    A = torch.randn(m, k, device="cuda", dtype=torch.bfloat16)
+    B = torch.randn(k, n, device="cuda", dtype=torch.bfloat16)
+    for _ in range(3):
+        C = A @ B
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        C = A @ B
+    e1.record()
+    torch.cuda.synchronize()
+    del A, B, C
+    return 2.0 * m * n * k / (e0.elapsed_time(e1) / 10 * 1e-3) / 1e12
 
 
 def loco(args) -> int:

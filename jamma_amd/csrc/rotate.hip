@@ -411,34 +411,45 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
       sel[2] = v[2];
       sel[3] = v[3];
     };
-    bf16x8_t x1[8];
-#pragma unroll 1
-    for (int pl = 0; pl < NPL; ++pl) {
-      bf16x8_t af[4];
+    // every LDS read of the step is issued up front (hi plane, the genotype
+    // selectors, then the other planes), so the later planes' fragments arrive
+    // while the hi plane's MFMAs run instead of stalling each plane's first MFMA
+    bf16x8_t af[NPL][4];
+    uint32_t sel[8][4];
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) {
+      const int r = wr + 16 * ti + lr;
+      const int c = lk ^ ((r >> 2) & 3);
+      memcpy(&af[0][ti], base + r * 64 + 16 * c, 16);
+    }
+#pragma unroll
+    for (int tj = 0; tj < 8; ++tj) gsel(tj, sel[tj]);
+#pragma unroll
+    for (int pl = 1; pl < NPL; ++pl)
 #pragma unroll
       for (int ti = 0; ti < 4; ++ti) {
         const int r = wr + 16 * ti + lr;
         const int c = lk ^ ((r >> 2) & 3);
-        memcpy(&af[ti], base + pl * V3_APLANE + r * 64 + 16 * c, 16);
+        memcpy(&af[pl][ti], base + pl * V3_APLANE + r * 64 + 16 * c, 16);
       }
-      if (pl == 0) {  // hi plane first: X1 and X2 of a column block from one widening
+    // hi plane first: X1 and X2 of a column block from one widening
+    bf16x8_t x1[8];
 #pragma unroll
-        for (int tj = 0; tj < 8; ++tj) {
-          uint32_t sel[4];
-          gsel(tj, sel);
-          bf16x8_t x2;
-          widen8s(sel, mm[tj], x1[tj], x2);
+    for (int tj = 0; tj < 8; ++tj) {
+      bf16x8_t x2;
+      widen8s(sel[tj], mm[tj], x1[tj], x2);
 #pragma unroll
-          for (int ti = 0; ti < 4; ++ti)
-            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ti], x2, acc[ti][tj], 0, 0, 0);
-        }
-      }
+      for (int ti = 0; ti < 4; ++ti)
+        acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][ti], x2, acc[ti][tj], 0, 0, 0);
+    }
+#pragma unroll
+    for (int pl = 0; pl < NPL; ++pl)
 #pragma unroll
       for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
         for (int tj = 0; tj < 8; ++tj)
-          acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ti], x1[tj], acc[ti][tj], 0, 0, 0);
-    }
+          acc[ti][tj] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[pl][ti], x1[tj], acc[ti][tj], 0, 0, 0);
   };
 #pragma unroll
   for (int st = 0; st < NBUF - 1; ++st)

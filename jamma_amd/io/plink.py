@@ -82,8 +82,61 @@ def validate_plink_dimensions(bfile) -> None:
                          f"({expected} bytes)")
 
 
+# .bed payloads held in HBM for the duration of a resident_bed() block: path ->
+# {"key": (size, mtime), "payload": device uint8 tensor, "counts": {sample key: counts}}
+_RESIDENT: dict[str, dict] = {}
+
+
+def _file_key(path: Path):
+    st = path.stat()
+    return (st.st_size, st.st_mtime_ns)
+
+
+class resident_bed:
+    """Context manager: the .bed payload of `bfile` is copied to HBM once and every
+    BedFile opened on it inside the block reads windows from that copy (no PCIe
+    transfer per pass), and per-SNP genotype counts computed by one pass are reused by
+    the next pass over the same samples. The reference decodes the file four times per
+    gwas() (two kinship passes, two association passes); here it crosses PCIe once.
+    Falls back to windowed copies when the payload exceeds `max_fraction` of free HBM."""
+
+    def __init__(self, bfile, max_fraction: float = 0.25):
+        self.path = Path(f"{bfile}.bed")
+        self.max_fraction = max_fraction
+        self._owner = False
+
+    def __enter__(self):
+        key = str(self.path.resolve())
+        if key in _RESIDENT:  # nested use: the outer block owns the copy
+            return self
+        size = self.path.stat().st_size - 3
+        free, _ = torch.cuda.mem_get_info()
+        if size <= 0 or size > self.max_fraction * free:
+            return self
+        with open(self.path, "rb") as f:
+            if f.read(3) != BED_MAGIC:
+                raise ValueError(f"{self.path} is not a SNP-major PLINK .bed file")
+        mm = np.memmap(self.path, dtype=np.uint8, mode="r", offset=3, shape=(size,))
+        dev = torch.empty((size,), dtype=torch.uint8, device=require_gpu())
+        step = 1 << 28  # 256 MB host chunks
+        with warnings.catch_warnings():  # the read-only memory map is only read (copied)
+            warnings.simplefilter("ignore", UserWarning)
+            for lo in range(0, size, step):
+                hi = min(size, lo + step)
+                dev[lo:hi].copy_(torch.from_numpy(np.ascontiguousarray(mm[lo:hi])))
+        _RESIDENT[key] = {"key": _file_key(self.path), "payload": dev, "counts": {}}
+        self._owner = True
+        return self
+
+    def __exit__(self, *exc):
+        if self._owner:
+            _RESIDENT.pop(str(self.path.resolve()), None)
+        return False
+
+
 class BedFile:
-    """Memory-mapped SNP-major .bed payload with device-side decoding."""
+    """SNP-major .bed payload with device-side decoding: memory-mapped and copied to
+    the device window by window, or read from the HBM copy of a resident_bed block."""
 
     def __init__(self, bfile):
         self.bfile = bfile
@@ -98,14 +151,30 @@ class BedFile:
             raise ValueError(f"{path} is not a SNP-major PLINK .bed file")
         self._mm = np.memmap(path, dtype=np.uint8, mode="r", offset=3,
                              shape=(self.n_snps * self.bytes_per_snp,))
+        res = _RESIDENT.get(str(path.resolve()))
+        self._res = res if res is not None and res["key"] == _file_key(path) else None
+
+    @property
+    def resident(self) -> bool:
+        return self._res is not None
 
     def window_to_device(self, start: int, end: int) -> torch.Tensor:
         """Raw bytes of SNPs [start, end) as a device uint8 tensor."""
         lo, hi = start * self.bytes_per_snp, end * self.bytes_per_snp
+        if self._res is not None:
+            return self._res["payload"][lo:hi]
         with warnings.catch_warnings():  # the read-only memory map is only read (copied)
             warnings.simplefilter("ignore", UserWarning)
             host = torch.from_numpy(np.ascontiguousarray(self._mm[lo:hi]))
         return host.to(require_gpu(), non_blocking=False)
+
+    def cached_counts(self, sample_key):
+        """Counts of an earlier pass over the same samples inside a resident_bed block."""
+        return None if self._res is None else self._res["counts"].get(sample_key)
+
+    def cache_counts(self, sample_key, counts: np.ndarray) -> None:
+        if self._res is not None:
+            self._res["counts"][sample_key] = counts
 
 
 def decode(raw_dev: torch.Tensor, bytes_per_snp: int, n_snps: int, sample_idx=None,
@@ -142,3 +211,13 @@ def stats_from_counts(counts: np.ndarray, dtype=np.float32):
     mean = mean.astype(dtype).astype(np.float64)
     poly = ((c[:, 0] > 0).astype(int) + (c[:, 1] > 0) + (c[:, 2] > 0)) >= 2
     return mean, c[:, 3].copy(), poly
+
+
+def sample_key(sample_idx, n_total: int):
+    """Cache key of a sample selection: None for all samples in file order."""
+    if sample_idx is None:
+        return None
+    idx = np.asarray(sample_idx, dtype=np.int64)
+    if len(idx) == n_total and np.array_equal(idx, np.arange(n_total)):
+        return None
+    return idx.tobytes()

@@ -22,7 +22,13 @@ import torch
 
 from jamma_amd import _lib
 from jamma_amd.device import ptr, require_gpu, round_up, stream_ptr, to_dev
-from jamma_amd.io.plink import BedFile, decode, get_plink_metadata, stats_from_counts
+from jamma_amd.io.plink import (
+    BedFile,
+    decode,
+    get_plink_metadata,
+    sample_key,
+    stats_from_counts,
+)
 
 logger = logging.getLogger("jamma_amd")
 
@@ -266,6 +272,10 @@ def compute_centered_kinship(genotypes: np.ndarray, batch_size: int = 10000,
 
 def _bed_counts(bed: BedFile, chunk_size: int, sample_idx=None, n_out=None) -> np.ndarray:
     n_out = bed.n_samples if n_out is None else n_out
+    key = sample_key(sample_idx, bed.n_samples)
+    hit = bed.cached_counts(key)
+    if hit is not None:
+        return hit.copy()
     counts = np.zeros((bed.n_snps, 4), dtype=np.int64)
     for start in range(0, bed.n_snps, chunk_size):
         end = min(start + chunk_size, bed.n_snps)
@@ -273,6 +283,7 @@ def _bed_counts(bed: BedFile, chunk_size: int, sample_idx=None, n_out=None) -> n
         _, c, _ = decode(raw, bed.bytes_per_snp, end - start, sample_idx=sample_idx, n_out=n_out,
                          want_g=False)
         counts[start:end] = c[: end - start].cpu().numpy()
+    bed.cache_counts(key, counts.copy())
     return counts
 
 

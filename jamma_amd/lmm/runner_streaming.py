@@ -24,7 +24,7 @@ import torch
 
 from jamma_amd._lib import JM_NOUT, OUT
 from jamma_amd.device import require_gpu, round_up, to_dev
-from jamma_amd.io.plink import BedFile, decode, get_plink_metadata, stats_from_counts
+from jamma_amd.io.plink import BedFile, decode, get_plink_metadata, sample_key, stats_from_counts
 from jamma_amd.lmm.eigen import eigh_device
 from jamma_amd.lmm.engine import AssocProblem, Cohort, guard_rows
 from jamma_amd.lmm.io import IncrementalAssocWriter, snp_text_parts
@@ -231,14 +231,21 @@ def run_lmm_association_streaming(
     sample_idx = None if np.all(valid) else np.where(valid)[0].astype(np.int32)
     bed = BedFile(bed_path)
 
-    # ---- pass 1: per-SNP counts on the device (float32 statistics) ----
-    counts = np.zeros((n_snps, 4), dtype=np.int64)
-    for start in range(0, n_snps, chunk_size):
-        end = min(start + chunk_size, n_snps)
-        raw = bed.window_to_device(start, end)
-        _, c, _ = decode(raw, bed.bytes_per_snp, end - start, sample_idx=sample_idx, n_out=n,
-                         want_g=False)
-        counts[start:end] = c[: end - start].cpu().numpy()
+    # ---- pass 1: per-SNP counts on the device (float32 statistics); inside a
+    # resident_bed block the kinship's pass over the same samples is reused ----
+    skey = sample_key(sample_idx, bed.n_samples)
+    counts = bed.cached_counts(skey)
+    if counts is None:
+        counts = np.zeros((n_snps, 4), dtype=np.int64)
+        for start in range(0, n_snps, chunk_size):
+            end = min(start + chunk_size, n_snps)
+            raw = bed.window_to_device(start, end)
+            _, c, _ = decode(raw, bed.bytes_per_snp, end - start, sample_idx=sample_idx, n_out=n,
+                             want_g=False)
+            counts[start:end] = c[: end - start].cpu().numpy()
+        bed.cache_counts(skey, counts.copy())
+    else:
+        counts = counts.copy()
     mask, means, af, miss = snp_filter(counts, n, maf_threshold, miss_threshold, np.float32)
     if snps_indices is not None:
         idx = np.asarray(snps_indices, dtype=np.int64)

@@ -15,7 +15,7 @@ from pathlib import Path
 import numpy as np
 
 from jamma_amd.io.covariate import read_covariate_file
-from jamma_amd.io.plink import get_plink_metadata, validate_plink_dimensions
+from jamma_amd.io.plink import get_plink_metadata, resident_bed, validate_plink_dimensions
 from jamma_amd.kinship.compute import kinship_streaming_device
 from jamma_amd.kinship.io import read_kinship_matrix, write_kinship_matrix
 from jamma_amd.lmm.runner_streaming import run_lmm_association_streaming
@@ -129,9 +129,16 @@ class PipelineRunner:
         return cov
 
     def run(self) -> PipelineResult:
+        """The reference's run (pipeline.py:420-609) with the .bed payload resident in HBM
+        for the whole run: every pass (kinship statistics and Gram, association
+        statistics and tiles) decodes from that one copy."""
+        self.validate_inputs()
+        with resident_bed(self.config.bfile):
+            return self._run()
+
+    def _run(self) -> PipelineResult:
         c = self.config
         t0 = time.perf_counter()
-        self.validate_inputs()
         meta = get_plink_metadata(c.bfile)
         n_samples, n_snps = meta["n_samples"], meta["n_snps"]
         phenotypes, n_analyzed = self.parse_phenotypes()

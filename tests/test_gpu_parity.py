@@ -618,3 +618,31 @@ def test_dosage_association_matches_oracle():
         assert [r.rs for r in res] == [f"rs{j:04d}" for j in ref["snp_indices"]]
         assert "rs0003" not in [r.rs for r in res]
         _assert_matches_oracle(_arrays(res, ALL), ref, ALL)
+
+
+def test_resident_bed_decodes_from_one_copy(tmp_path):
+    """Inside resident_bed, every pass decodes from one HBM copy of the .bed payload and
+    the association's pass-1 counts are the kinship pass's (all samples valid): kinship
+    and association results are identical to the windowed-copy path's."""
+    from jamma_amd.io.plink import BedFile, resident_bed
+    from jamma_amd.kinship.compute import kinship_streaming_device
+    from jamma_amd.lmm import run_lmm_association_streaming
+
+    G = balding_nichols(160, 700, miss=0.02, seed=61)
+    y, W = phenotype(G, n_cov=1, seed=61)
+    pre = tmp_path / "res"
+    opl.write_plink(pre, G, phenotypes=y)
+    K0 = kinship_streaming_device(pre, chunk_size=256).cpu().numpy()
+    r0 = run_lmm_association_streaming(pre, y, kinship=K0, covariates=W, lmm_mode=4,
+                                       chunk_size=256, precision="f64")
+    with resident_bed(pre):
+        assert BedFile(pre).resident
+        K1 = kinship_streaming_device(pre, chunk_size=256).cpu().numpy()
+        assert BedFile(pre).cached_counts(None) is not None
+        r1 = run_lmm_association_streaming(pre, y, kinship=K1, covariates=W, lmm_mode=4,
+                                           chunk_size=256, precision="f64")
+    assert not BedFile(pre).resident
+    assert np.array_equal(K0, K1)
+    assert [r.rs for r in r0] == [r.rs for r in r1]
+    for k in ALL:
+        assert np.array_equal(_arrays(r0, (k,))[k], _arrays(r1, (k,))[k], equal_nan=True), k
