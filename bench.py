@@ -57,6 +57,17 @@ def gen_cohort(n: int, p: int, n_pop: int, fst: float, miss: float, seed: int, l
     return g
 
 
+def cpu_model() -> str:
+    """Host CPU model name (first 'model name' line of /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def counts_of(g, n):
     import torch
 
@@ -88,14 +99,16 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving session that adds request p50/p99 to the line")
     ap.add_argument("--workload", default="precision",
-                    choices=["precision", "scale", "loco", "pipeline", "serving", "copy"],
+                    choices=["precision", "small", "scale", "loco", "pipeline", "serving", "copy"],
                     help="precision: the headline config; scale: n=50,000 x 200,000 SNPs, 5 "
                          "subpopulations (Fst 0.02), 0.5%% missing, Wald test in 10,000-SNP "
                          "tiles; loco: n=20,000 x 300,000 SNPs on 19 chromosomes, 8 "
                          "correlated phenotypes, leave-one-chromosome-out kinships, -lmm 4; "
                          "pipeline: the precision form written as a PLINK fileset and run "
                          "through gwas() (.bed on disk -> .assoc.txt); "
-                         "serving: resident cohorts + Poisson request stream (p50/p99)")
+                         "serving: resident cohorts + Poisson request stream (p50/p99); "
+                         "small: the plain FP64 method at n=1,940 x 12,226 SNPs (-lmm 1) with "
+                         "the oracle timed on the whole job")
     ap.add_argument("--pops", type=int, default=3, help="Balding-Nichols subpopulations")
     ap.add_argument("--fst", type=float, default=0.05)
     ap.add_argument("--miss", type=float, default=0.01, help="missing-call rate")
@@ -118,6 +131,8 @@ def main() -> int:
         return loco(args)
     if args.workload == "pipeline":
         return pipeline(args)
+    if args.workload == "small":
+        return small(args)
     if args.workload == "scale":
         args.n, args.p, args.pops, args.fst, args.miss = 50_000, 200_000, 5, 0.02, 0.005
         args.tile, args.kin_tile, args.mode = 10_000, 10_000, 1
@@ -298,7 +313,8 @@ def main() -> int:
         ref = {k: np.concatenate([r[k] for r in parts]) for k in parts[0]}
         cpu_s = time.perf_counter() - t0
         cores = len(os.sched_getaffinity(0))
-        cpu = {"value": args.cpu_sample / cpu_s, "unit": "SNPs/s", "cores": cores, "kind": "port",
+        cpu = {"value": args.cpu_sample / cpu_s, "unit": "SNPs/s", "cores": cores,
+               "cpu_model": cpu_model(), "kind": "port",
                "sample": f"{args.cpu_sample} SNPs of the same workload (rotation + REML/MLE grid+"
                          f"golden + Wald/LRT/Score, numpy FP64, {cores} host threads), "
                          f"{cpu_s:.1f}s"}
@@ -645,6 +661,113 @@ def pipeline(args) -> int:
         "config": {"workload": f"pipeline: precision form n={n} x {p} SNPs through gwas(), "
                                f"3 covariates + intercept, -lmm 4", "n": n, "p": p},
         "mixed": runs["mixed"], "f64": runs["f64"],
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def small(args) -> int:
+    """The plain method at 1,940 samples x 12,226 SNPs, FP64: genotypes ~ Binomial(2, p_j)
+    with p_j ~ U(0.05, 0.5), 0.5% of calls missing (mean-imputed); phenotype = 50 causal
+    SNP effects beta ~ N(0, 0.05) + N(0, 1); intercept only; centred kinship, its
+    eigendecomposition, then the per-SNP REML Wald test (-lmm 1).
+
+    Device: int8 kinship + 64-bit eigendecomposition + FP64 rotation and lambda search
+    over every SNP. CPU: the oracle on the whole job (NumPy centred kinship, LAPACK
+    eigh, the same association), on the host cores, with its p-values as the check."""
+    import numpy as np
+    import torch
+
+    from jamma_amd.device import require_gpu, round_up, to_dev
+    from jamma_amd.io.plink import stats_from_counts
+    from jamma_amd.kinship.compute import KinshipAccumulator
+    from jamma_amd.lmm.eigen import eigh_device
+    from jamma_amd.lmm.engine import AssocProblem, Cohort
+    from oracle import kinship as ok
+    from oracle import lmm as ol
+
+    dev = require_gpu()
+    n, p = 1940, 12226
+    ld = round_up(n, 64)
+    g = gen_cohort(n, p, 1, 1e-6, 0.005, args.seed, ld)  # one population: Binomial(2, p_j)
+    counts = counts_of(g, n)
+    mean32, miss, poly = stats_from_counts(counts, np.float32)
+    af = mean32 / 2
+    keep = (np.minimum(af, 1 - af) >= 0.01) & (miss / n <= 0.05) & poly
+    idx = np.where(keep)[0]
+    gf = g[torch.as_tensor(idx, device=dev)].contiguous()
+    pf = gf.shape[0]
+    means = to_dev(mean32[idx], torch.float64)
+    mu64, _, _ = stats_from_counts(counts[idx], np.float64)
+    rng = np.random.default_rng(args.seed)
+    causal = np.sort(rng.choice(pf, 50, replace=False))
+    Gh = gf[:, :n].cpu().numpy().astype(np.float64).T  # (n, pf), -1 = missing
+    Gh[Gh < 0] = np.nan
+    Gc = np.where(np.isnan(Gh[:, causal]), mu64[causal][None, :], Gh[:, causal])
+    y = Gc @ rng.normal(0.0, 0.05, 50) + rng.normal(size=n)
+    W = np.ones((n, 1))
+
+    def device_job():
+        acc = KinshipAccumulator(n)
+        for s in range(0, pf, 16384):
+            e = min(s + 16384, pf)
+            acc.add_tile(gf[s:e], ld, e - s, mu64[s:e], True, int(counts[idx[s:e], 3].sum()))
+        K = acc.finalize(pf)
+        d, U = eigh_device(K)
+        co = Cohort(d, U, precision="f64")
+        prob = AssocProblem(co, y, W)
+        torch.cuda.synchronize()
+        t_a = time.perf_counter()
+        outs = [prob.associate(gf[s:min(s + 8192, pf)], ld, means[s:min(s + 8192, pf)],
+                               min(s + 8192, pf) - s, 1, "f64")[: min(s + 8192, pf) - s]
+                for s in range(0, pf, 8192)]
+        torch.cuda.synchronize()
+        return torch.cat(outs).cpu().numpy(), time.perf_counter() - t_a
+
+    for _ in range(max(1, args.warmup)):
+        device_job()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    walls, assoc = [], []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        out, ta = device_job()
+        walls.append(time.perf_counter() - t1)
+        assoc.append(ta)
+    wall = float(np.mean(walls))
+    log(f"small: device job {wall * 1e3:.1f} ms ({pf} SNPs), association {np.mean(assoc) * 1e3:.1f} ms")
+
+    # the oracle on the whole job (kinship -> eigh -> association), host threads
+    cores = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    K_ref = ok.centered_kinship(Gh)
+    t_kin = time.perf_counter() - t0
+    d_ref, U_ref = ol.eigendecompose(K_ref)
+    t_eig = time.perf_counter() - t0 - t_kin
+    ref = ol.run_lmm(Gh, y, eigenvalues=d_ref, eigenvectors=U_ref, maf_threshold=0.0,
+                     miss_threshold=1.0, lmm_mode=1, stat_dtype=np.float32)
+    cpu_s = time.perf_counter() - t0
+    t_assoc = cpu_s - t_kin - t_eig
+    with np.errstate(divide="ignore"):
+        dl = float(np.nanmax(np.abs(np.log10(out[:, 2]) - np.log10(ref["p_wald"]))))
+    log(f"small: oracle {cpu_s:.1f}s on {cores} threads (kinship {t_kin:.1f}s, eigh {t_eig:.1f}s, "
+        f"association {t_assoc:.1f}s); max |dlog10 p_wald| {dl:.2e}")
+    line = {
+        "metric": "whole-job SNPs/s, plain FP64 method (-gk 1, eigendecomposition, -lmm 1)",
+        "value": pf / wall, "unit": "SNPs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic (Binomial(2, p_j) genotypes, 0.5% missing)",
+        "config": {"workload": f"small: n={n} x {p} SNPs ({pf} pass the filters), intercept "
+                               f"only, centred kinship + eigendecomposition + REML Wald",
+                   "n": n, "p": p},
+        "association_snps_per_s": pf / float(np.mean(assoc)),
+        "parity": {"wald_vs_oracle_max_abs_dlog10p": dl, "n_snps": pf},
+        "cpu_baseline": {"value": pf / cpu_s, "unit": "SNPs/s", "cores": cores,
+                         "cpu_model": cpu_model(), "kind": "port",
+                         "sample": f"the whole job ({pf} SNPs): NumPy centred kinship "
+                                   f"{t_kin:.1f}s, LAPACK eigh {t_eig:.1f}s, association "
+                                   f"{t_assoc:.1f}s",
+                         "association_snps_per_s": pf / t_assoc},
     }
     print(json.dumps(line), flush=True)
     return 0

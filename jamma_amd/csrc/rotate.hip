@@ -243,18 +243,26 @@ constexpr int v3_stage() { return NPL * V3_APLANE + V3_BBYTES; }
 // holds the selectors of two genotypes. Per-column tables:
 //   X1: lo bytes {00, 80, 00, lo(m1)}, hi bytes {00, 3F, 40, hi(m1)}  (0, 1.0, 2.0, m1)
 //   X2: lo bytes {00, 00, 00, lo(m2)}, hi bytes {00, 00, 00, hi(m2)}
-// mm = m1 | m2 << 16. 4 table + 8 permute VALU per 8 genotypes for X1 and X2.
-__device__ __forceinline__ void widen8s(const uint32_t (&sel)[4], uint32_t mm, bf16x8_t& x1,
+// mm = m1 | m2 << 16. The 4 table words of a column are made once per launch;
+// 8 permute VALU per 8 genotypes for X1 and X2.
+struct widen_tab {  // byte tables of one column: X1 {lo, hi} and X2 {lo, hi}
+  uint32_t t1lo, t1hi, t2lo, t2hi;
+};
+__device__ __forceinline__ widen_tab widen_tables(uint32_t mm) {
+  widen_tab t;
+  t.t1lo = __builtin_amdgcn_perm(mm, 0x00008000u, 0x04020100u);  // lo(m1) in byte 3
+  t.t1hi = __builtin_amdgcn_perm(mm, 0x00403F00u, 0x05020100u);  // hi(m1) in byte 3
+  t.t2lo = __builtin_amdgcn_perm(mm, 0u, 0x060C0C0Cu);           // lo(m2) in byte 3
+  t.t2hi = mm & 0xFF000000u;                                     // hi(m2) in byte 3
+  return t;
+}
+__device__ __forceinline__ void widen8s(const uint32_t (&sel)[4], const widen_tab& t, bf16x8_t& x1,
                                         bf16x8_t& x2) {
-  const uint32_t t1lo = __builtin_amdgcn_perm(mm, 0x00008000u, 0x04020100u);  // lo(m1) in byte 3
-  const uint32_t t1hi = __builtin_amdgcn_perm(mm, 0x00403F00u, 0x05020100u);  // hi(m1) in byte 3
-  const uint32_t t2lo = __builtin_amdgcn_perm(mm, 0u, 0x060C0C0Cu);           // lo(m2) in byte 3
-  const uint32_t t2hi = mm & 0xFF000000u;                                     // hi(m2) in byte 3
   uint32_t w1[4], w2[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    w1[q] = __builtin_amdgcn_perm(t1hi, t1lo, sel[q]);
-    w2[q] = __builtin_amdgcn_perm(t2hi, t2lo, sel[q]);
+    w1[q] = __builtin_amdgcn_perm(t.t1hi, t.t1lo, sel[q]);
+    w2[q] = __builtin_amdgcn_perm(t.t2hi, t.t2lo, sel[q]);
   }
   memcpy(&x1, w1, 16);
   memcpy(&x2, w2, 16);
@@ -346,20 +354,26 @@ __global__ void k_pack_geno(const int8_t* __restrict__ g, int64_t ld_g,
   }
 }
 
-// BF16x3 rotation on packed operands: 256x256 block tile, 8 waves in a 4 x 2
-// grid of 64x128 wave tiles (128 accumulators per lane, two waves per SIMD),
-// both operands copied block by block into NBUF LDS stages with
-// global_load_lds_dwordx4 (the copies of stages kt+1 .. kt+NBUF-1 in flight
-// while stage kt is multiplied), one barrier per 32-deep K step. Genotype
-// fragments are widened to X1 and X2 in one pass per column block (the two share
-// the code-to-byte selectors and the LDS read). Per K step:
+// BF16x3 rotation on packed operands: 256x256 block tile, 8 waves (two per
+// SIMD) in a grid of (16 WR) x (16 WC) wave tiles, WR x WC = 32 MFMA blocks of
+// 16x16 (128 accumulators per lane), both operands copied block by block into
+// NBUF LDS stages with global_load_lds_dwordx4 (the copies of stages kt+1 ..
+// kt+NBUF-1 in flight while stage kt is multiplied), one barrier per 32-deep K
+// step. Genotype fragments are widened to X1 and X2 in one pass per column
+// block (the two share the code-to-byte selectors and the LDS read); each wave
+// widens the columns of its own tile, so WR = 8 (2 x 4 waves of 128 x 64)
+// widens each column twice per block where WR = 4 (4 x 2 waves of 64 x 128)
+// widens it four times. Per K step:
 //   NPL = 2 (UT = hi + lo):        hi x X2, hi x X1, lo x X1   (3 products)
 //   NPL = 3 (UT = hi + mid + lo):  hi x X2, hi x X1, mid x X1, lo x X1
-template <int NPL, int NBUF>
+template <int NPL, int NBUF, int WR>
 __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
     const uint16_t* __restrict__ planes, int64_t per_plane, const uint8_t* __restrict__ gp,
     const uint32_t* __restrict__ mmv, int64_t M, int64_t n_cols, int64_t nks,
     float* __restrict__ Z, int64_t ldz) {
+  constexpr int WC = 32 / WR;   // 16-column blocks per wave
+  constexpr int NWC = 16 / WC;  // waves across the 256 columns
+  static_assert(WR * WC == 32 && (8 / NWC) * WR * 16 == V3_BM, "wave grid");
   constexpr int STAGE = v3_stage<NPL>();
   constexpr int ADMA = NPL * 2;    // 1 KB UT copies per wave per stage (NPL x 16 KB / 8 waves)
   constexpr int DPS = ADMA + 2;    // copies per wave per stage (+ 2 KB of genotype selectors)
@@ -371,7 +385,7 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
   int rb, cb;
   xcd_tile_2d(blockIdx.x, nrb, ncb, rb, cb);
   const int64_t i0 = (int64_t)rb * V3_BM, s0 = (int64_t)cb * V3_BN;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 128;
+  const int wr = (wave / NWC) * 16 * WR, wc = (wave % NWC) * 16 * WC;
   const int lr = lane & 15, lk = lane >> 4;
   // copy sources: plane pl's block (rb, s) and genotype block (cb, s)
   const uint8_t* ap = (const uint8_t*)planes + (int64_t)rb * nks * UTP_BLOCK + lane * 16;
@@ -389,17 +403,17 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
       lds_copy16(bp + sk * GP_BLOCK + (2 * wave + u) * 1024,
                  base + NPL * V3_APLANE + (2 * wave + u) * 1024);
   };
-  uint32_t mm[8];
+  widen_tab tab[WC];
 #pragma unroll
-  for (int tj = 0; tj < 8; ++tj) {
+  for (int tj = 0; tj < WC; ++tj) {
     const int64_t sc = s0 + wc + 16 * tj + lr;
-    mm[tj] = sc < n_cols ? mmv[sc] : 0u;
+    tab[tj] = widen_tables(sc < n_cols ? mmv[sc] : 0u);
   }
-  f4_t acc[4][8];
+  f4_t acc[WR][WC];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < WR; ++a)
 #pragma unroll
-    for (int b = 0; b < 8; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < WC; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
   auto compute = [&](const uint8_t* base) {
     auto gsel = [&](int tj, uint32_t (&sel)[4]) {  // selectors of 8 samples of column j
       const int j = wc + 16 * tj + lr;
@@ -414,40 +428,40 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
     // every LDS read of the step is issued up front (hi plane, the genotype
     // selectors, then the other planes), so the later planes' fragments arrive
     // while the hi plane's MFMAs run instead of stalling each plane's first MFMA
-    bf16x8_t af[NPL][4];
-    uint32_t sel[8][4];
+    bf16x8_t af[NPL][WR];
+    uint32_t sel[WC][4];
 #pragma unroll
-    for (int ti = 0; ti < 4; ++ti) {
+    for (int ti = 0; ti < WR; ++ti) {
       const int r = wr + 16 * ti + lr;
       const int c = lk ^ ((r >> 2) & 3);
       memcpy(&af[0][ti], base + r * 64 + 16 * c, 16);
     }
 #pragma unroll
-    for (int tj = 0; tj < 8; ++tj) gsel(tj, sel[tj]);
+    for (int tj = 0; tj < WC; ++tj) gsel(tj, sel[tj]);
 #pragma unroll
     for (int pl = 1; pl < NPL; ++pl)
 #pragma unroll
-      for (int ti = 0; ti < 4; ++ti) {
+      for (int ti = 0; ti < WR; ++ti) {
         const int r = wr + 16 * ti + lr;
         const int c = lk ^ ((r >> 2) & 3);
         memcpy(&af[pl][ti], base + pl * V3_APLANE + r * 64 + 16 * c, 16);
       }
     // hi plane first: X1 and X2 of a column block from one widening
-    bf16x8_t x1[8];
+    bf16x8_t x1[WC];
 #pragma unroll
-    for (int tj = 0; tj < 8; ++tj) {
+    for (int tj = 0; tj < WC; ++tj) {
       bf16x8_t x2;
-      widen8s(sel[tj], mm[tj], x1[tj], x2);
+      widen8s(sel[tj], tab[tj], x1[tj], x2);
 #pragma unroll
-      for (int ti = 0; ti < 4; ++ti)
+      for (int ti = 0; ti < WR; ++ti)
         acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][ti], x2, acc[ti][tj], 0, 0, 0);
     }
 #pragma unroll
     for (int pl = 0; pl < NPL; ++pl)
 #pragma unroll
-      for (int ti = 0; ti < 4; ++ti)
+      for (int ti = 0; ti < WR; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < 8; ++tj)
+        for (int tj = 0; tj < WC; ++tj)
           acc[ti][tj] =
               __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[pl][ti], x1[tj], acc[ti][tj], 0, 0, 0);
   };
@@ -471,9 +485,9 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
     cur = cur + 1 == NBUF ? 0 : cur + 1;
   }
 #pragma unroll
-  for (int ti = 0; ti < 4; ++ti)
+  for (int ti = 0; ti < WR; ++ti)
 #pragma unroll
-    for (int tj = 0; tj < 8; ++tj)
+    for (int tj = 0; tj < WC; ++tj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
@@ -662,11 +676,11 @@ extern "C" int jm_rotate_bf16x3(const uint16_t* planes, int nplanes, const uint8
   const int64_t nks = (n + 31) / 32;
   const int64_t per_plane = jm_packed_ut_bytes(n, nplanes) / (2 * nplanes);  // bf16 per plane
   const int nrb = ceil_div(n, V3_BM), ncb = ceil_div(n_snps, V3_BN);
-  if (nplanes == 2)  // 2 x 48 KB stages (measured faster than 3 at n = 10k)
-    hipLaunchKernelGGL((k_rotate_bf16x3_packed<2, 2>), dim3(nrb * ncb), dim3(512), 0,
+  if (nplanes == 2)  // 2 x 48 KB stages, 128 x 64 wave tiles (measured fastest at n = 10k)
+    hipLaunchKernelGGL((k_rotate_bf16x3_packed<2, 2, 8>), dim3(nrb * ncb), dim3(512), 0,
                        (hipStream_t)stream, planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
-  else  // 2 x 64 KB stages
-    hipLaunchKernelGGL((k_rotate_bf16x3_packed<3, 2>), dim3(nrb * ncb), dim3(512), 0,
+  else  // 2 x 64 KB stages, 64 x 128 wave tiles (fewer registers live with three planes)
+    hipLaunchKernelGGL((k_rotate_bf16x3_packed<3, 2, 4>), dim3(nrb * ncb), dim3(512), 0,
                        (hipStream_t)stream, planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
   JM_CHECK_LAUNCH();
   return JM_OK;
