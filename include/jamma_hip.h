@@ -351,12 +351,16 @@ int jm_lmm_assoc(const jm_lmm_problem* prob, int mode, const double* xmom, int64
  * problem probs[snp_prob[s]] (device array of n_probs problems, all on one
  * cohort so they share n_terms) with mode modes[snp_prob[s]], from the moments
  * block starting at snp_xmom[s] (row stride ldm); results go to row s of out.
- * max_cvt = max n_cvt over the problems. One launch serves every request
- * admitted on the cohort; a SNP's result does not depend on the batch.
+ * max_cvt = max n_cvt over the problems, n_clusters = the cohort plan's cluster
+ * count (both size the kernel's LDS). One launch serves every request admitted
+ * on the cohort; a SNP's result does not depend on the batch. With n_terms <= 256
+ * the 16-lanes-per-SNP kernel runs, each block taking 16 consecutive SNPs and the
+ * rows of their problems (fastest when a problem's SNPs are contiguous).
  */
 int jm_lmm_assoc_multi(const jm_lmm_problem* probs, const int32_t* modes, int n_probs, int max_cvt,
-                       int n_terms, const int32_t* snp_prob, const double* const* snp_xmom,
-                       int64_t ldm, int64_t n_snps, double* out, void* stream);
+                       int n_terms, int n_clusters, const int32_t* snp_prob,
+                       const double* const* snp_xmom, int64_t ldm, int64_t n_snps, double* out,
+                       void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Large-cohort eigendecomposition (n*n > 2^31 - 1). Together with rocBLAS'   */

@@ -93,8 +93,8 @@ _SIGS = {
                                      c_void_p, c_void_p, c_int64, c_void_p], c_int),
     "jm_spectrum_moments_multi": ([c_void_p, c_int, c_int64, c_int64, c_void_p, c_int, c_void_p,
                                    c_void_p, c_void_p], c_int),
-    "jm_lmm_assoc_multi": ([c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64,
-                            c_int64, c_void_p, c_void_p], c_int),
+    "jm_lmm_assoc_multi": ([c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                            c_int64, c_int64, c_void_p, c_void_p], c_int),
     "jm_lmm_null": ([POINTER(LmmProblem), c_void_p, c_void_p], c_int),
     "jm_lmm_prepare": ([POINTER(LmmProblem), c_void_p], c_int),
     "jm_lmm_assoc": ([POINTER(LmmProblem), c_int, c_void_p, c_int64, c_int64, c_void_p, c_void_p],

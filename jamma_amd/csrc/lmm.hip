@@ -1360,10 +1360,18 @@ static size_t assoc16_lds(int KT, int nc, int C) {
          (size_t)(TB + tet(nc + 1) + NR) * sizeof(int);
 }
 
-template <int KT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_lmm_assoc16(jm_lmm_problem pr, int mode,
-                                                     const double* __restrict__ xmom, int64_t ldm,
-                                                     int64_t n_snps, double* __restrict__ out) {
+// One block = 16 consecutive SNPs of the launch (4 per wavefront). MULTI: SNP s
+// is tested under probs[snp_prob[s]] (problems of one cohort: the same spectrum
+// plan, n and T) from the moments block at snp_xmom[s]; a block splits its 16 SNPs
+// into runs of one problem and reloads the problem's LDS rows (shared pair rows,
+// recursion table) when the problem changes, so a batch of requests runs as one
+// launch. The LDS layout is that of the widest problem (max_cvt).
+template <int KT, bool MULTI>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_lmm_assoc16(
+    jm_lmm_problem pr_one, int mode_one, const double* __restrict__ xmom, int64_t ldm,
+    int64_t n_snps, double* __restrict__ out, const jm_lmm_problem* __restrict__ probs,
+    const int32_t* __restrict__ modes, const int32_t* __restrict__ snp_prob,
+    const double* const* __restrict__ snp_xmom, int max_cvt) {
   extern __shared__ double smem[];
   constexpr int TB = kGL * KT;
   const int lane = threadIdx.x & 63;
@@ -1372,37 +1380,41 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   const int j = lane & 15;
   const int h = j >> 3;  // 0: row A (REML, lambda A), 1: row B (MLE, lambda B)
   const int q8 = j & 7;
-  const int nc = pr.n_cvt, T = pr.n_terms, n = pr.n, G = pr.n_grid, C = pr.n_clusters;
-  const int ni = (nc + 3) * (nc + 2) / 2;
-  const int nrows = nc + 2;  // per-SNP moment rows: W_1..W_c, y, xx
-  const int n_sh = (nc + 1) * (nc + 2) / 2;
-  const int NR = nrows + 1 + n_sh;  // dot-product rows: SNP rows, logdet, shared rows
-  const int per_grp = assoc16_group_doubles(ni, C);
+  const jm_lmm_problem& p0 = MULTI ? probs[0] : pr_one;  // the cohort's plan, n and T
+  const int T = p0.n_terms, n = p0.n, C = p0.n_clusters;
+  const int ncm = MULTI ? max_cvt : pr_one.n_cvt;  // LDS layout of the widest problem
+  const int ni_m = (ncm + 3) * (ncm + 2) / 2;
+  const int n_sh_m = (ncm + 1) * (ncm + 2) / 2;
+  const int per_grp = assoc16_group_doubles(ni_m, C);
   double* c_twl = smem;                       // TB: logdet weight / power (0: m == 0, t >= T)
   double* c_cdc = c_twl + TB;                 // C cluster centres
   double* c_cw = c_cdc + C;                   // C cluster weights
   double* srows = c_cw + C;                   // n_sh x TB shared pair rows, zero-padded
-  double* groups = srows + (size_t)n_sh * TB;  // 16 x per_grp
+  double* groups = srows + (size_t)n_sh_m * TB;  // 16 x per_grp
   int* c_tinf = (int*)(groups + 16 * per_grp);  // TB: m | cluster << 8
   int* rtab = c_tinf + TB;                    // tet(nc + 1) recursion updates
-  int* vslot = rtab + tet(nc + 1);            // NR: packed-row slot of each dot-product row
+  int* vslot = rtab + tet(ncm + 1);           // NR: packed-row slot of each dot-product row
   for (int t = threadIdx.x; t < TB; t += blockDim.x) {
     const bool act = t < T;
-    const int m = act ? pr.term_m[t] : 0;
-    const int cl = act ? pr.term_cluster[t] : 0;
+    const int m = act ? p0.term_m[t] : 0;
+    const int cl = act ? p0.term_cluster[t] : 0;
     c_tinf[t] = m | (cl << 8);
-    c_twl[t] = (act && m > 0) ? pr.logdet_mom[t] / (double)m : 0.0;
+    c_twl[t] = (act && m > 0) ? p0.logdet_mom[t] / (double)m : 0.0;
   }
   for (int c = threadIdx.x; c < C; c += blockDim.x) {
-    c_cdc[c] = pr.cl_center[c];
-    c_cw[c] = pr.cl_weight[c];
+    c_cdc[c] = p0.cl_center[c];
+    c_cw[c] = p0.cl_weight[c];
   }
-  for (int k = 0; k < n_sh; ++k) {
-    const int jj = shared_pair_index(k, nc);
-    for (int t = threadIdx.x; t < TB; t += blockDim.x)
-      srows[(size_t)k * TB + t] = t < T ? pr.shared_mom[(int64_t)jj * T + t] : 0.0;
-  }
-  {
+  // the problem's shared pair rows, recursion table and row slots
+  auto load_problem = [&](const jm_lmm_problem& pr) {
+    const int nc = pr.n_cvt;
+    const int n_sh = (nc + 1) * (nc + 2) / 2;
+    const int NR = nc + 2 + 1 + n_sh;
+    for (int k = 0; k < n_sh; ++k) {
+      const int jj = shared_pair_index(k, nc);
+      for (int t = threadIdx.x; t < TB; t += blockDim.x)
+        srows[(size_t)k * TB + t] = t < T ? pr.shared_mom[(int64_t)jj * T + t] : 0.0;
+    }
     const int m = nc + 2;
     for (int e = threadIdx.x; e < tet(nc + 1); e += blockDim.x) {
       int p = 1;
@@ -1416,39 +1428,67 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       const int a = p + 1 + ap, b = a + rem;
       rtab[e] = ab_index(a, b, nc) | (ab_index(a, p, nc) << 8) | (ab_index(b, p, nc) << 16);
     }
-  }
-  for (int r = threadIdx.x; r < NR; r += blockDim.x)
-    vslot[r] = r < nrows ? xpair_index(r, nc) : (r == nrows ? ni : shared_pair_index(r - nrows - 1, nc));
+    for (int r = threadIdx.x; r < NR; r += blockDim.x)
+      vslot[r] = r < nc + 2 ? xpair_index(r, nc)
+                            : (r == nc + 2 ? (nc + 3) * (nc + 2) / 2
+                                           : shared_pair_index(r - nc - 3, nc));
+  };
+  if (!MULTI) load_problem(pr_one);
   __syncthreads();
 
-  double* PA = groups + (size_t)(wib * 4 + grp) * per_grp;  // ni entries + logdet slot
-  double* PB = PA + ni + 1;
-  double* iab_ld = PB + ni + 1;  // kMaxLevels
-  double* H = iab_ld + kMaxLevels;  // 2 C: (h_A, h_B) per cluster
-  double* Ph = h ? PB : PA;
   int tinf[KT];
 #pragma unroll
   for (int k = 0; k < KT; ++k) tinf[k] = c_tinf[j * KT + k];
   const double phi = 0.6180339887498949;
-  const bool do_reml = (mode == 1 || mode == 4);
-  const bool do_mle = (mode == 2 || mode == 4);
-  const bool do_score = (mode == 3 || mode == 4);
-  const bool do_golden = do_reml || do_mle;
-  const double df = (double)(n - nc - 1);
   const double nn = (double)n;
-  const double cr = 0.5 * df * (log(df) - log(2.0 * M_PI) - 1.0);
   const double cm = 0.5 * nn * (log(nn) - log(2.0 * M_PI) - 1.0);
-  const int st_final = 3 + pr.n_iter, st_score = 4 + pr.n_iter;
-  const int64_t n_quads = (n_snps + 3) / 4;
-  for (int64_t wq = (int64_t)blockIdx.x * 4 + wib; wq < n_quads; wq += (int64_t)gridDim.x * 4) {
-    const int64_t s_raw = wq * 4 + grp;
-    const bool active = s_raw < n_snps;
-    const int64_t s = active ? s_raw : n_snps - 1;
-    const double* xm = xmom + s * (int64_t)nrows * ldm;
+  int loaded = MULTI ? -1 : 0;
+  const int64_t n_units = (n_snps + 15) / 16;
+  for (int64_t un = blockIdx.x; un < n_units; un += gridDim.x) {
+   const int64_t u_lo = un * 16, u_hi = min(u_lo + 16, n_snps);
+   for (int64_t sub = u_lo; sub < u_hi;) {
+    int pidx = 0;
+    int64_t s_end = u_hi;
+    if constexpr (MULTI) {  // the run of SNPs of one problem starting at sub (block-uniform)
+      pidx = snp_prob[sub];
+      s_end = sub + 1;
+      while (s_end < u_hi && snp_prob[s_end] == pidx) ++s_end;
+      if (pidx != loaded) {
+        __syncthreads();  // every wavefront is done with the previous problem's rows
+        load_problem(probs[pidx]);
+        __syncthreads();
+        loaded = pidx;
+      }
+    }
+    const jm_lmm_problem& pr = MULTI ? probs[pidx] : pr_one;
+    const int mode = MULTI ? modes[pidx] : mode_one;
+    const int nc = pr.n_cvt, G = pr.n_grid;
+    const int ni = (nc + 3) * (nc + 2) / 2;
+    const int nrows = nc + 2;  // per-SNP moment rows: W_1..W_c, y, xx
+    const int n_sh = (nc + 1) * (nc + 2) / 2;
+    const int NR = nrows + 1 + n_sh;  // dot-product rows: SNP rows, logdet, shared rows
+    double* PA = groups + (size_t)(wib * 4 + grp) * per_grp;  // ni entries + logdet slot
+    double* PB = PA + ni + 1;
+    double* iab_ld = PB + ni + 1;  // kMaxLevels
+    double* H = iab_ld + kMaxLevels;  // 2 C: (h_A, h_B) per cluster
+    double* Ph = h ? PB : PA;
+    const bool do_reml = (mode == 1 || mode == 4);
+    const bool do_mle = (mode == 2 || mode == 4);
+    const bool do_score = (mode == 3 || mode == 4);
+    const bool do_golden = do_reml || do_mle;
+    const double df = (double)(n - nc - 1);
+    const double cr = 0.5 * df * (log(df) - log(2.0 * M_PI) - 1.0);
+    const int st_final = 3 + pr.n_iter, st_score = 4 + pr.n_iter;
+    const int64_t s_raw = u_lo + 4 * wib + grp;
+    const bool active = s_raw >= sub && s_raw < s_end;
+    const int64_t s = active ? s_raw : sub;  // an idle group repeats a SNP of the run, unwritten
+    const double* xm = MULTI ? snp_xmom[s] : xmom + s * (int64_t)nrows * ldm;
     double* os = out + s * JM_NOUT;
-    // grid argmax indices left by k_lmm_grid, then columns a mode does not compute are NaN
-    const int gi_r = do_golden ? (int)os[JM_OUT_L_REMLE] : 0;
-    const int gi_m = do_golden ? (int)os[JM_OUT_L_MLE] : 0;
+    // grid argmax indices left by k_lmm_grid, then columns a mode does not compute are
+    // NaN. An idle group stands in for a SNP that another wavefront may be writing: it
+    // reads no index (0), and indices are kept inside the grid.
+    const int gi_r = (do_golden && active) ? min(max((int)os[JM_OUT_L_REMLE], 0), G - 1) : 0;
+    const int gi_m = (do_golden && active) ? min(max((int)os[JM_OUT_L_MLE], 0), G - 1) : 0;
     wave_sync();
     if (active && j < JM_NOUT) os[j] = NAN;
     int evals = 0;
@@ -1684,6 +1724,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     }
     if (active && j == 0) os[JM_OUT_EVALS] = (double)evals;
     wave_sync();
+    sub = s_end;
+   }
   }
 }
 
@@ -1752,7 +1794,8 @@ static void launch_kt(const jm_lmm_problem* one, int mode_one, const double* xmo
 static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* xmom, int64_t ldm,
                         int64_t n_snps, double* out, const jm_lmm_problem* probs,
                         const int32_t* modes, const int32_t* snp_prob,
-                        const double* const* snp_xmom, int T, int max_cvt, void* stream) {
+                        const double* const* snp_xmom, int T, int max_cvt, int n_clusters,
+                        void* stream) {
   JM_CHECK_ARG(T >= 1, "n_terms=%d < 1", T);
   const int ni_max = (max_cvt + 3) * (max_cvt + 2) / 2;
   hipStream_t st = (hipStream_t)stream;
@@ -1781,20 +1824,30 @@ static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* x
     }
     JM_CHECK_LAUNCH();
   }
-  // single problem with at most 256 terms: 16 lanes per SNP when the LDS fits
-  // (JM_LMM_WAVE_PER_SNP=1 keeps the wavefront-per-SNP kernel, for comparisons)
-  if (one) {
+  // at most 256 terms: 16 lanes per SNP when the LDS fits (one problem, or a batch of
+  // problems of one cohort; JM_LMM_WAVE_PER_SNP=1 keeps the wavefront-per-SNP kernel,
+  // for comparisons)
+  {
     const char* env = getenv("JM_LMM_WAVE_PER_SNP");
     const bool wave = env && env[0] == '1';
     const int kt16 = T <= 64 ? 4 : (T <= 128 ? 8 : (T <= 192 ? 12 : (T <= 256 ? 16 : 0)));
-    const size_t lds16 = kt16 ? assoc16_lds(kt16, one->n_cvt, one->n_clusters) : 0;
-    if (!wave && kt16 && one->n_clusters >= 1 && lds16 <= 64 * 1024) {
-      const int64_t quads = (n_snps + 3) / 4;
-      int64_t b16 = (quads + 3) / 4;
+    const size_t lds16 = kt16 ? assoc16_lds(kt16, max_cvt, n_clusters) : 0;
+    if (!wave && kt16 && n_clusters >= 1 && lds16 <= 64 * 1024) {
+      int64_t b16 = (n_snps + 15) / 16;
       if (b16 > 256 * 8) b16 = 256 * 8;
-#define JM_LAUNCH16(KT_)                                                                         \
-  hipLaunchKernelGGL(k_lmm_assoc16<KT_>, dim3((unsigned)b16), dim3(256), lds16, st, *one, mode_one, \
-                     xmom, ldm, n_snps, out)
+      jm_lmm_problem dummy;
+      memset(&dummy, 0, sizeof(dummy));
+#define JM_LAUNCH16(KT_)                                                                        \
+  do {                                                                                          \
+    if (one)                                                                                    \
+      hipLaunchKernelGGL((k_lmm_assoc16<KT_, false>), dim3((unsigned)b16), dim3(256), lds16, st, \
+                         *one, mode_one, xmom, ldm, n_snps, out, nullptr, nullptr, nullptr,     \
+                         nullptr, max_cvt);                                                     \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_lmm_assoc16<KT_, true>), dim3((unsigned)b16), dim3(256), lds16, st,  \
+                         dummy, 0, nullptr, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom, \
+                         max_cvt);                                                              \
+  } while (0)
       if (kt16 == 4)
         JM_LAUNCH16(4);
       else if (kt16 == 8)
@@ -1875,20 +1928,21 @@ extern "C" int jm_lmm_assoc(const jm_lmm_problem* prob, int mode, const double* 
                    prob->null_shared,
                "tables not prepared");
   return launch_assoc(prob, mode, xmom, ldm, n_snps, out, nullptr, nullptr, nullptr, nullptr,
-                      prob->n_terms, prob->n_cvt, stream);
+                      prob->n_terms, prob->n_cvt, prob->n_clusters, stream);
 }
 
 extern "C" int jm_lmm_assoc_multi(const jm_lmm_problem* probs, const int32_t* modes, int n_probs,
-                                  int max_cvt, int n_terms, const int32_t* snp_prob,
-                                  const double* const* snp_xmom, int64_t ldm, int64_t n_snps,
-                                  double* out, void* stream) {
+                                  int max_cvt, int n_terms, int n_clusters,
+                                  const int32_t* snp_prob, const double* const* snp_xmom,
+                                  int64_t ldm, int64_t n_snps, double* out, void* stream) {
   JM_CHECK_ARG(n_snps >= 0 && n_probs >= 1, "bad sizes");
   if (n_snps == 0) return JM_OK;
   JM_CHECK_ARG(probs && modes && snp_prob && snp_xmom && out, "null pointer");
   JM_CHECK_ARG(max_cvt >= 1 && max_cvt <= 20, "max_cvt out of range [1,20]");
   JM_CHECK_ARG(n_terms >= 1 && ldm >= n_terms, "bad n_terms / ldm");
+  JM_CHECK_ARG(n_clusters >= 1, "n_clusters < 1");
   return launch_assoc(nullptr, 0, nullptr, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                      n_terms, max_cvt, stream);
+                      n_terms, max_cvt, n_clusters, stream);
 }
 
 extern "C" double jm_host_betainc(double a, double b, double x) { return betainc_reg(a, b, x); }

@@ -380,7 +380,8 @@ def assoc_multi(problems, modes, snp_prob, snp_xmom, out: torch.Tensor, stream=N
     sx = to_dev(snp_xmom.view(np.int64), torch.int64, dev)
     max_cvt = max(p.n_cvt for p in problems)
     _lib.call("jm_lmm_assoc_multi", ptr(probs_dev), ptr(modes_dev), len(problems), max_cvt, T,
-              ptr(sp), ptr(sx), T, B, ptr(out), stream_ptr(stream))
+              int(problems[0].prob.n_clusters), ptr(sp), ptr(sx), T, B, ptr(out),
+              stream_ptr(stream))
     if stream is not None:
         # the launch reads these device copies on `stream`: the caching allocator must not
         # hand their memory out again before that stream has passed the launch

@@ -12,11 +12,14 @@ for it:
   a setup stream of the cohort, so the null model's host read-back never stalls the
   scheduler;
 * every scheduler step takes the next slice of every active request of a cohort and
-  launches, on that cohort's stream: ONE genotype pack + rotation for the whole batch
-  (a gather of the requests' SNP rows, whose index list goes up through pinned
-  memory without a host wait), then per request the spectrum moments of its columns
-  and the lambda search + tests (16 lanes per SNP when the problem allows), writing
-  straight into the request's result rows;
+  launches, on that cohort's stream, one pass of the hot path over the whole batch:
+  ONE genotype pack + rotation (a gather of the requests' SNP rows), ONE spectrum-
+  moments launch (a descriptor per column: its request's partner rows and output
+  block) and ONE lambda search + tests launch (per-SNP problem index; 16 lanes per SNP
+  for spectra of up to 256 terms, each block reloading a problem's rows where the
+  batch moves to the next request); the step's descriptors go up in one pinned,
+  non-blocking copy, and the results are copied into each request's rows on the
+  stream;
 * at most ``max_inflight`` steps per cohort are queued ahead of the device; steps,
   result copies (to pinned memory) and completions are tracked with events that the
   loop polls, so requests that arrive while others run join the next step and their
@@ -34,6 +37,7 @@ precomputed eigendecomposition (prepare.py:77-111).
 
 from __future__ import annotations
 
+import ctypes
 import itertools
 import threading
 import time
@@ -173,6 +177,57 @@ class AdmissionQueue:
         self.used -= cost
 
 
+_MOMCOL = np.dtype([("V", "<u8"), ("ldv", "<i8"), ("nv", "<i4"), ("res", "<i4"), ("out", "<u8"),
+                    ("ldm", "<i8")])
+
+
+def _step_descriptors(dests, co, T):
+    """The device descriptors of one step, laid out in one pinned buffer: the gather
+    index of every batch column (rotation), its moments descriptor (jm_mom_col: the
+    request's partner rows, the column's block of the request's moments), its problem
+    index and moments address (lambda search), then the problems and their modes.
+    -> (pinned uint8 tensor, {name: (byte offset, bytes)})."""
+    B = int(sum(t for _, _, t in dests))
+    sizes = {"col_idx": 4 * B, "cols": _MOMCOL.itemsize * B, "snp_prob": 4 * B,
+             "snp_xmom": 8 * B, "probs": ctypes.sizeof(_lib.LmmProblem) * len(dests),
+             "modes": 4 * len(dests)}
+    views, off = {}, 0
+    for k, nb in sizes.items():
+        views[k] = (off, nb)
+        off += (nb + 15) // 16 * 16
+    stage = torch.empty((max(off, 16),), dtype=torch.uint8, pin_memory=True)
+    raw = stage.numpy()
+
+    def arr(k, dt):
+        o, nb = views[k]
+        return raw[o:o + nb].view(dt)
+
+    col_idx, cols = arr("col_idx", np.int32), arr("cols", _MOMCOL)
+    snp_prob, snp_xmom = arr("snp_prob", np.int32), arr("snp_xmom", np.uint64)
+    o = 0
+    for k, (a, _, take) in enumerate(dests):
+        rows = a.prob.n_cvt + 2
+        blk = rows * T * 8
+        r = np.arange(take)
+        col_idx[o:o + take] = a.next - take + r
+        addr = np.uint64(a.xmom.data_ptr()) + (r * blk).astype(np.uint64)
+        c = cols[o:o + take]
+        c["V"] = a.prob.V.data_ptr()
+        c["ldv"] = a.prob.V.shape[1]
+        c["nv"] = a.prob.n_cvt + 1
+        c["res"] = 0
+        c["out"] = addr
+        c["ldm"] = T
+        snp_prob[o:o + take] = k
+        snp_xmom[o:o + take] = addr
+        o += take
+    probs = (_lib.LmmProblem * len(dests))(*[a.prob.prob for a, _, _ in dests])
+    po, pn = views["probs"]
+    raw[po:po + pn] = np.frombuffer(bytes(probs), dtype=np.uint8)
+    arr("modes", np.int32)[:] = [int(a.req.mode) for a, _, _ in dests]
+    return stage, views
+
+
 class Server:
     """Scheduler thread owning the device work of every resident cohort."""
 
@@ -305,34 +360,34 @@ class Server:
         co = rc.cohort
         T = co.n_terms
         with torch.cuda.stream(rc.stream):
-            dests, cols = [], []
+            dests = []
             budget = self.max_batch
             for a in acts:
                 if budget <= 0:
                     break
                 take = min(self.slice_snps, a.remaining(), budget)
                 budget -= take
-                cols.append(np.arange(a.next, a.next + take, dtype=np.int32))
                 dests.append((a, a.next - a.req.snp_start, take))
                 a.next += take
-            B = int(sum(len(c) for c in cols))
-            host_idx = torch.empty((B,), dtype=torch.int32, pin_memory=True)
-            host_idx.numpy()[:] = np.concatenate(cols)
-            col_idx = torch.empty((B,), dtype=torch.int32, device=co.device)
-            col_idx.copy_(host_idx, non_blocking=True)
-            Z, f32, ldz = co.rotate_genotypes(rc.g, rc.ld, rc.means, B, rc.precision, col_idx)
+            B = int(sum(t for _, _, t in dests))
+            stage, views = _step_descriptors(dests, co, T)
+            dev_buf = torch.empty((stage.numel(),), dtype=torch.uint8, device=co.device)
+            dev_buf.copy_(stage, non_blocking=True)  # one upload, no host wait
+            dv = {k: dev_buf[o:o + nb] for k, (o, nb) in views.items()}
+            Z, f32, ldz = co.rotate_genotypes(rc.g, rc.ld, rc.means, B, rc.precision,
+                                              dv["col_idx"].view(torch.int32))
             nv_max = max(a.prob.n_cvt for a, _, _ in dests) + 1
-            take_max = max(t for _, _, t in dests)
-            work = co.moments_work(take_max, nv_max)
-            esz = 4 if f32 else 8
+            work = co.moments_work(B, nv_max)
+            st = rc.stream.cuda_stream
+            _lib.call("jm_spectrum_moments_multi", Z.data_ptr(), int(f32), ldz, B,
+                      ptr(dv["cols"]), nv_max, ctypes_ref(co.plan_c), ptr(work), st)
+            sout = torch.empty((B, _lib.JM_NOUT), dtype=torch.float64, device=co.device)
+            _lib.call("jm_lmm_assoc_multi", ptr(dv["probs"]), ptr(dv["modes"]), len(dests),
+                      nv_max - 1, T, int(dests[0][0].prob.prob.n_clusters),
+                      ptr(dv["snp_prob"]), ptr(dv["snp_xmom"]), T, B, ptr(sout), st)
             o = 0
             for a, off, take in dests:
-                _lib.call("jm_spectrum_moments", Z.data_ptr() + o * esz, int(f32), ldz, take,
-                          ptr(a.prob.V), a.prob.V.shape[1], a.prob.n_cvt + 1,
-                          ctypes_ref(co.plan_c), ptr(work), ptr(a.xmom), T, rc.stream.cuda_stream)
-                _lib.call("jm_lmm_assoc", ctypes_ref(a.prob.prob), int(a.req.mode), ptr(a.xmom),
-                          T, take, a.out.data_ptr() + off * _lib.JM_NOUT * 8,
-                          rc.stream.cuda_stream)
+                a.out[off:off + take].copy_(sout[o:o + take])
                 o += take
                 if a.remaining() == 0:
                     a.host = torch.empty(a.out.shape, dtype=torch.float64, pin_memory=True)
@@ -342,7 +397,7 @@ class Server:
             ev = torch.cuda.Event()
             ev.record(rc.stream)
             infl.append(ev)
-            del Z, work, col_idx  # stream-ordered frees: reused only by later work
+            del Z, work, dev_buf, sout  # stream-ordered frees: reused only by later work
         self.batches += 1
         self.snps_done += B
         return True

@@ -9,8 +9,10 @@ jm_lmm_assoc picks, by the number of spectrum terms T and covariate columns c:
   T > 512                         KT = 8 in blocks of 512 terms (multi-block)
 and rows beyond those held in registers (c + 2 > 4 or 8) are read from memory; the
 shared covariate/phenotype rows sit in LDS only when they fit 48 KB. The serving
-form jm_lmm_assoc_multi always takes the wavefront-per-SNP kernel with problems of
-different c (and modes) in one launch.
+form jm_lmm_assoc_multi takes the same kernels with problems of different c (and
+modes) in one launch: for T <= 256 the 16-lane kernel, which splits each block's 16
+SNPs into runs of one problem (the test interleaves problems SNP by SNP, so every
+run is one SNP long and every block reloads its rows 16 times).
 
 Cohorts are built from chosen spectra (random orthogonal basis, eigenvalues drawn
 uniformly or log-uniformly) whose compressed plans land in each range, with c = 1,

@@ -124,7 +124,7 @@ def test_admission_budget_queues_and_completes(cohorts):
 
 
 def test_step_error_fails_its_cohort_alone(cohorts, monkeypatch):
-    """A launch that fails inside a scheduler step (here jm_lmm_assoc, once) fails the
+    """A launch that fails inside a scheduler step (here jm_lmm_assoc_multi, once) fails the
     requests active on that cohort with the error, releases their budget, and the
     loop keeps serving: a later request on the same cohort and one on another cohort
     complete and match their results alone."""
@@ -135,9 +135,9 @@ def test_step_error_fails_its_cohort_alone(cohorts, monkeypatch):
     state = {"armed": True}
 
     def flaky(name, *args):
-        if name == "jm_lmm_assoc" and state["armed"]:
+        if name == "jm_lmm_assoc_multi" and state["armed"]:
             state["armed"] = False
-            raise RuntimeError("jm_lmm_assoc failed (rc=-2): injected step error")
+            raise RuntimeError("jm_lmm_assoc_multi failed (rc=-2): injected step error")
         return real(name, *args)
 
     rca, rcb = cohorts["a"][0], cohorts["b"][0]
