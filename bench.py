@@ -464,7 +464,7 @@ def loco(args) -> int:
     from jamma_amd.io.plink import stats_from_counts
     from jamma_amd.kinship.compute import KinshipAccumulator
     from jamma_amd.lmm.eigen import eigh_device
-    from jamma_amd.lmm.engine import AssocProblem, Cohort
+    from jamma_amd.lmm.engine import GUARD_BAND, AssocProblem, Cohort, guard_rows
 
     dev = require_gpu()
     n, p, n_chr, n_ph, tile = 20_000, 300_000, 19, 8, args.tile
@@ -518,6 +518,7 @@ def loco(args) -> int:
     kin_peak = 0
     log(f"loco: full Gram in {t_kin:.2f}s")
     t_eig = t_rot = t_assoc = 0.0
+    retested = 0
     out = {"mixed": [], "f64": []}
     for c in range(n_chr):
         t0 = time.perf_counter()
@@ -547,9 +548,14 @@ def loco(args) -> int:
                 ss = torch.as_tensor(sel[s:s + tile], device=dev)
                 gt = g[ss].contiguous()
                 Z, f32, ldz = cohort.rotate_genotypes(gt, ld, means[ss], len(ss), prec)
-                res[prec].append(torch.stack(
-                    [pr.assoc_moments(cohort.moments(Z, f32, ldz, len(ss), pr.V, pr.n_cvt + 1),
-                                      len(ss), 4)[: len(ss)] for pr in probs], 1).cpu())
+                outs = []
+                for pr in probs:
+                    o = pr.assoc_moments(cohort.moments(Z, f32, ldz, len(ss), pr.V,
+                                                        pr.n_cvt + 1), len(ss), 4)[: len(ss)]
+                    if prec == "mixed":  # guard band: p near a threshold -> FP64 path
+                        retested += pr.retest_f64(gt, ld, means[ss], guard_rows(o, 4), 4, o)
+                    outs.append(o)
+                res[prec].append(torch.stack(outs, 1).cpu())
                 del Z, gt
         for prec in res:
             out[prec].append(torch.cat(res[prec]))
@@ -569,6 +575,8 @@ def loco(args) -> int:
         parity[name] = {"max_abs_dlog10p": float(np.nanmax(dl)), "significance_flips": flips}
     parity["beta_sign_agreement"] = bool(np.array_equal(np.sign(a[:, :, 0]), np.sign(b[:, :, 0])))
     parity["n_sig_5e-8_wald"] = int(np.sum(b[:, :, 2] < 5e-8))
+    parity["guard_band_log10p"] = GUARD_BAND
+    parity["retested_f64"] = int(retested)
     timed = t_kin + t_eig + t_assoc
     line = {
         "metric": "LOCO SNP x phenotype tests/s (kinships + eigendecompositions + association)",
