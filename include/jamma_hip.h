@@ -363,6 +363,97 @@ int jm_lmm_assoc_multi(const jm_lmm_problem* probs, const int32_t* modes, int n_
                        void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Native serving runtime. No reference counterpart: the reference runs one    */
+/* association per call (run_lmm_association_streaming,                        */
+/* lmm/runner_streaming.py:94-628); this is the continuous-batching loop that  */
+/* keeps cohorts resident and serves requests as they arrive.                  */
+/* ------------------------------------------------------------------------ */
+#define JM_SERVE_F64 0   /* cohort precision: FP64 rotation (UT)            */
+#define JM_SERVE_MIXED 1 /* BF16x3 rotation (packed planes)                 */
+#define JM_SERVE_STUB 1  /* jm_serve_config.flags: no HIP call at all       */
+#define JM_SERVE_MANUAL 2 /* with STUB: no thread, jm_serve_run_once drives it */
+
+typedef struct jm_serve_config {
+  int64_t slice_snps;   /* SNPs a request contributes to one step */
+  int64_t max_batch;    /* SNPs per step (>= slice_snps) */
+  int32_t max_reqs;     /* requests per step */
+  int32_t max_inflight; /* steps queued on a cohort's stream ahead of the device */
+  int32_t max_skips;    /* aging: overtakes a waiting request tolerates */
+  int32_t max_cvt;      /* widest problem (covariate columns incl. intercept) accepted */
+  int32_t flags;        /* JM_SERVE_STUB */
+  int32_t reserved;
+} jm_serve_config;
+
+typedef struct jm_serve_cohort_desc {
+  int32_t n;            /* samples */
+  int32_t precision;    /* JM_SERVE_F64 or JM_SERVE_MIXED */
+  int64_t n_snps;       /* genotype rows */
+  const int8_t* g;      /* device, n_snps x ld_g int8 SNP-major, -1 missing, 8-byte rows */
+  int64_t ld_g;
+  const double* means;  /* device, n_snps imputation means */
+  const double* UT;     /* device U^T (n x ld_u), FP64 cohorts */
+  int64_t ld_u;
+  const uint16_t* planes; /* device jm_pack_ut_bf16x3 output, mixed cohorts */
+  int32_t nplanes;
+  int32_t n_terms;      /* of the cohort's spectrum plan */
+  int32_t n_clusters;
+  int32_t reserved;
+} jm_serve_cohort_desc;
+
+typedef struct jm_serve_done {
+  int64_t id;
+  int32_t status;       /* JM_OK, or the error code of the step that failed it */
+  int32_t reserved;
+  char message[240];    /* the error message, empty on success */
+} jm_serve_done;
+
+typedef struct jm_serve_stats_t {
+  int64_t queued, active, steps, snps, arena_used, arena_bytes;
+  int64_t max_active;   /* most requests admitted at once so far */
+  int64_t reserved;     /* arena bytes of the cohorts' step workspaces */
+} jm_serve_stats_t;
+
+typedef struct jm_serve jm_serve;
+
+/* Arena bytes of a cohort's step workspace (carved from the arena by
+ * jm_serve_add_cohort) and of one admitted request (its moments slice). */
+int64_t jm_serve_cohort_ws_bytes(const jm_serve_config* cfg, const jm_serve_cohort_desc* c,
+                                 const jm_spectrum_plan* plan);
+int64_t jm_serve_request_ws_bytes(const jm_serve_config* cfg, int n_terms, int nv);
+
+/* Starts the scheduler thread over a caller-owned device arena (256-byte aligned;
+ * every device buffer the runtime uses comes from it, so serving never calls
+ * hipMalloc). With JM_SERVE_STUB the arena may be NULL: no HIP call is made and
+ * steps complete as they are formed (admission and batching logic without a GPU). */
+int jm_serve_create(const jm_serve_config* cfg, void* arena, int64_t arena_bytes, jm_serve** out);
+/* Registers a resident cohort (its arrays stay caller-owned and must outlive the
+ * runtime): carves its step workspace, creates its stream and pinned slots. */
+int jm_serve_add_cohort(jm_serve* s, const jm_serve_cohort_desc* c, const jm_spectrum_plan* plan,
+                        int* cohort_id);
+/* Queues a request: a prepared problem (jm_lmm_prepare and jm_lmm_null done; its
+ * arrays must live until the request completes) with its partner rows V
+ * ((n_cvt + 1) x ldv), a test mode 1..4 and SNP rows [snp_start, snp_end) of the
+ * cohort; results go to out ((snp_end - snp_start) x JM_NOUT float64, device),
+ * complete once the request's completion record has been polled. */
+int jm_serve_submit(jm_serve* s, int cohort, const jm_lmm_problem* prob, const double* V,
+                    int64_t ldv, int mode, int64_t snp_start, int64_t snp_end, double* out,
+                    int64_t* request_id);
+/* Completion records (non-blocking): up to max_recs into recs, count in *n_recs. */
+int jm_serve_poll(jm_serve* s, jm_serve_done* recs, int max_recs, int* n_recs);
+/* Blocks up to timeout_ms for a completion record; returns how many are waiting. */
+int jm_serve_wait(jm_serve* s, int timeout_ms);
+int jm_serve_stats(jm_serve* s, jm_serve_stats_t* st);
+/* JM_SERVE_MANUAL only: one pass of the loop (admission, a step per cohort,
+ * completions); returns launched steps * 65536 + completed requests. */
+int64_t jm_serve_run_once(jm_serve* s);
+/* Failure-handling hook: the cohort's next step fails on the host before any
+ * launch (its active requests fail with JM_ERR_UNSUPPORTED, the loop goes on). */
+int jm_serve_inject_step_error(jm_serve* s, int cohort);
+/* Drains every queued and admitted request, then stops the thread and frees the
+ * streams, events and pinned slots (the arena stays the caller's). */
+int jm_serve_destroy(jm_serve* s);
+
+/* ------------------------------------------------------------------------ */
 /* Large-cohort eigendecomposition (n*n > 2^31 - 1). Together with rocBLAS'   */
 /* 64-bit level-2 routines and rocSOLVER dstedc on the halves, these replace  */
 /* np.linalg.eigh (lmm/eigen.py:94) where the library's 32-bit tridiagonal    */

@@ -58,6 +58,40 @@ class SpectrumPlanC(ctypes.Structure):
     ]
 
 
+class ServeConfig(ctypes.Structure):
+    """Mirror of jm_serve_config."""
+
+    _fields_ = [("slice_snps", c_int64), ("max_batch", c_int64), ("max_reqs", c_int32),
+                ("max_inflight", c_int32), ("max_skips", c_int32), ("max_cvt", c_int32),
+                ("flags", c_int32), ("reserved", c_int32)]
+
+
+class ServeCohortDesc(ctypes.Structure):
+    """Mirror of jm_serve_cohort_desc."""
+
+    _fields_ = [("n", c_int32), ("precision", c_int32), ("n_snps", c_int64), ("g", c_void_p),
+                ("ld_g", c_int64), ("means", c_void_p), ("UT", c_void_p), ("ld_u", c_int64),
+                ("planes", c_void_p), ("nplanes", c_int32), ("n_terms", c_int32),
+                ("n_clusters", c_int32), ("reserved", c_int32)]
+
+
+class ServeDone(ctypes.Structure):
+    """Mirror of jm_serve_done."""
+
+    _fields_ = [("id", c_int64), ("status", c_int32), ("reserved", c_int32),
+                ("message", ctypes.c_char * 240)]
+
+
+class ServeStats(ctypes.Structure):
+    """Mirror of jm_serve_stats_t."""
+
+    _fields_ = [("queued", c_int64), ("active", c_int64), ("steps", c_int64), ("snps", c_int64),
+                ("arena_used", c_int64), ("arena_bytes", c_int64), ("max_active", c_int64),
+                ("reserved", c_int64)]
+
+
+JM_SERVE_F64, JM_SERVE_MIXED, JM_SERVE_STUB, JM_SERVE_MANUAL = 0, 1, 1, 2
+
 _SIGS = {
     "jm_version": ([], c_int),
     "jm_last_error": ([ctypes.c_char_p, c_size_t], c_int),
@@ -135,6 +169,20 @@ _SIGS = {
     "jm_host_chi2_sf1": ([c_double], c_double),
     "jm_host_f_pvalue": ([c_double, c_double], c_double),
     "jm_host_lrt_pvalue": ([c_double, c_double], c_double),
+    "jm_serve_cohort_ws_bytes": ([POINTER(ServeConfig), POINTER(ServeCohortDesc), c_void_p],
+                                 c_int64),
+    "jm_serve_request_ws_bytes": ([POINTER(ServeConfig), c_int, c_int], c_int64),
+    "jm_serve_create": ([POINTER(ServeConfig), c_void_p, c_int64, POINTER(c_void_p)], c_int),
+    "jm_serve_add_cohort": ([c_void_p, POINTER(ServeCohortDesc), c_void_p, POINTER(c_int)],
+                            c_int),
+    "jm_serve_submit": ([c_void_p, c_int, POINTER(LmmProblem), c_void_p, c_int64, c_int, c_int64,
+                         c_int64, c_void_p, POINTER(c_int64)], c_int),
+    "jm_serve_poll": ([c_void_p, POINTER(ServeDone), c_int, POINTER(c_int)], c_int),
+    "jm_serve_wait": ([c_void_p, c_int], c_int),
+    "jm_serve_stats": ([c_void_p, POINTER(ServeStats)], c_int),
+    "jm_serve_run_once": ([c_void_p], c_int64),
+    "jm_serve_inject_step_error": ([c_void_p, c_int], c_int),
+    "jm_serve_destroy": ([c_void_p], c_int),
 }
 
 EXPORTED = tuple(_SIGS)

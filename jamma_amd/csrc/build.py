@@ -19,7 +19,7 @@ REPO = PKG.parent
 BUILD = HERE / "_build"
 LIB = PKG / "lib" / "libjamma_hip.so"
 SOURCES = ["jm_common.cpp", "format.cpp", "decode.hip", "kinship.hip", "rotate.hip", "moments.hip", "lmm.hip",
-           "eigen.hip"]
+           "eigen.hip", "serve.cpp"]
 ARCH = os.environ.get("JM_OFFLOAD_ARCH", "gfx950")
 
 

@@ -101,14 +101,21 @@ def test_alone_equals_mix_and_oracle(cohorts):
 
 
 def test_admission_budget_queues_and_completes(cohorts):
+    """An arena that holds the cohorts' step workspaces and 1.5 requests admits one
+    request at a time; every request still completes. A request that can never fit
+    is refused at submission, and a cohort whose workspace does not fit at add time."""
     from jamma_amd.serve import Request, Server
-    from jamma_amd.serve.server import _Active
+    from jamma_amd.serve.server import cohort_workspace_bytes, request_workspace_bytes, serve_config
 
     rng = np.random.default_rng(1)
     reqs = _requests(rng, cohorts, 5)
     rc = cohorts["a"][0]
-    one = max(_Active.cost_bytes(cohorts[r.cohort][0], r, 64) for r in reqs)
-    srv = Server(hbm_budget_bytes=int(one * 1.5), slice_snps=64)
+    cfg = serve_config(slice_snps=64)
+    ws = sum((cohort_workspace_bytes(c, cfg) + 255) // 256 * 256 for c, *_ in cohorts.values())
+    one = max(request_workspace_bytes(cohorts[r.cohort][0], cfg,
+                                      1 if r.covariates is None else r.covariates.shape[1])
+              for r in reqs)
+    srv = Server(hbm_budget_bytes=ws + int(one * 1.5), slice_snps=64)
     for c, *_ in cohorts.values():
         srv.add_cohort(c)
     with srv:
@@ -117,28 +124,25 @@ def test_admission_budget_queues_and_completes(cohorts):
         stats = srv.latency_stats()
     assert stats["max_concurrent"] < len(reqs)  # the budget forces queueing
     assert all(np.isfinite(x[:, 2]).all() for x in res)
-    tiny = Server(hbm_budget_bytes=1024)
+    ws_a = (cohort_workspace_bytes(rc, cfg) + 255) // 256 * 256
+    tiny = Server(hbm_budget_bytes=ws_a + 1024, slice_snps=64)
     tiny.add_cohort(rc)
     with pytest.raises(MemoryError):
         tiny.submit(Request("a", np.zeros(rc.n), 0, rc.p, covariates=None))
+    tiny.stop()
+    tinier = Server(hbm_budget_bytes=1024, slice_snps=64)
+    with pytest.raises(MemoryError):
+        tinier.add_cohort(rc)
+    tinier.stop()
 
 
-def test_step_error_fails_its_cohort_alone(cohorts, monkeypatch):
-    """A launch that fails inside a scheduler step (here jm_lmm_assoc_multi, once) fails the
-    requests active on that cohort with the error, releases their budget, and the
-    loop keeps serving: a later request on the same cohort and one on another cohort
-    complete and match their results alone."""
-    from jamma_amd import _lib
+def test_step_error_fails_its_cohort_alone(cohorts):
+    """A step that fails inside the runtime (the failure hook: the cohort's next step
+    fails on the host before any launch) fails the requests active on that cohort with
+    the error, releases their arena space, and the loop keeps serving: a later request
+    on the same cohort and one on another cohort complete and match their results
+    alone."""
     from jamma_amd.serve import Request, Server
-
-    real = _lib.call
-    state = {"armed": True}
-
-    def flaky(name, *args):
-        if name == "jm_lmm_assoc_multi" and state["armed"]:
-            state["armed"] = False
-            raise RuntimeError("jm_lmm_assoc_multi failed (rc=-2): injected step error")
-        return real(name, *args)
 
     rca, rcb = cohorts["a"][0], cohorts["b"][0]
     rng = np.random.default_rng(9)
@@ -152,18 +156,18 @@ def test_step_error_fails_its_cohort_alone(cohorts, monkeypatch):
     with ref:
         want_a2 = ref.submit(Request("a", yb, 100, 300)).result(timeout=120)
         want_b = ref.submit(Request("b", yc, 0, 150)).result(timeout=120)
-    monkeypatch.setattr(_lib, "call", flaky)
     with srv:
+        srv.inject_step_error("a")
         bad = srv.submit(Request("a", ya, 0, 200))
         with pytest.raises(RuntimeError, match="injected step error"):
             bad.result(timeout=120)
-        assert not state["armed"]
         good_a = srv.submit(Request("a", yb, 100, 300))
         good_b = srv.submit(Request("b", yc, 0, 150))
         assert np.array_equal(good_a.result(timeout=120), want_a2, equal_nan=True)
         assert np.array_equal(good_b.result(timeout=120), want_b, equal_nan=True)
         stats = srv.latency_stats()
-    assert stats["failed"] == 1 and srv.adm.used == 0
+        st = srv.stats()
+    assert stats["failed"] == 1 and st["arena_used"] == st["reserved"]
 
 
 def test_bad_request_fails_alone(cohorts):
