@@ -351,14 +351,18 @@ int jm_lmm_assoc(const jm_lmm_problem* prob, int mode, const double* xmom, int64
  * problem probs[snp_prob[s]] (device array of n_probs problems, all on one
  * cohort so they share n_terms) with mode modes[snp_prob[s]], from the moments
  * block starting at snp_xmom[s] (row stride ldm); results go to row s of out.
- * max_cvt = max n_cvt over the problems, n_clusters = the cohort plan's cluster
- * count (both size the kernel's LDS). One launch serves every request admitted
- * on the cohort; a SNP's result does not depend on the batch. With n_terms <= 256
- * the 16-lanes-per-SNP kernel runs, each block taking 16 consecutive SNPs and the
- * rows of their problems (fastest when a problem's SNPs are contiguous).
+ * max_cvt = the widest problem's n_cvt (sizes the LDS layouts); n_clusters = the
+ * cohort plan's cluster count. width_cvt (0: max_cvt; else in [max_cvt, 20]) picks
+ * the kernel variant, and with it the summation order: launches with one
+ * width_cvt give a SNP the same result bit for bit whatever else is in the batch
+ * (the serving runtime passes its configured max_cvt). With n_terms <= 256 the
+ * 16-lanes-per-SNP kernel runs when its LDS fits at width_cvt (the shared pair
+ * rows from global memory when they do not fit in LDS: same arithmetic), each
+ * block taking 16 consecutive SNPs and the rows of their problems (fastest when a
+ * problem's SNPs are contiguous).
  */
 int jm_lmm_assoc_multi(const jm_lmm_problem* probs, const int32_t* modes, int n_probs, int max_cvt,
-                       int n_terms, int n_clusters, const int32_t* snp_prob,
+                       int width_cvt, int n_terms, int n_clusters, const int32_t* snp_prob,
                        const double* const* snp_xmom, int64_t ldm, int64_t n_snps, double* out,
                        void* stream);
 
@@ -397,7 +401,10 @@ typedef struct jm_serve_cohort_desc {
   int32_t nplanes;
   int32_t n_terms;      /* of the cohort's spectrum plan */
   int32_t n_clusters;
-  int32_t reserved;
+  int32_t max_batch;    /* SNPs per step of this cohort, slice_snps .. the server's
+                           max_batch (0: the server's max_batch); a smaller cap keeps a
+                           large cohort's steps short, so other work queued behind them
+                           on the device waits less */
 } jm_serve_cohort_desc;
 
 typedef struct jm_serve_done {
@@ -433,8 +440,9 @@ int jm_serve_add_cohort(jm_serve* s, const jm_serve_cohort_desc* c, const jm_spe
 /* Queues a request: a prepared problem (jm_lmm_prepare and jm_lmm_null done; its
  * arrays must live until the request completes) with its partner rows V
  * ((n_cvt + 1) x ldv), a test mode 1..4 and SNP rows [snp_start, snp_end) of the
- * cohort; results go to out ((snp_end - snp_start) x JM_NOUT float64, device),
- * complete once the request's completion record has been polled. */
+ * cohort; results go to out ((snp_end - snp_start) x JM_NOUT float64, device memory
+ * or pinned host memory from hipHostMalloc: each step copies its rows there on the
+ * cohort's stream), complete once the request's completion record has been polled. */
 int jm_serve_submit(jm_serve* s, int cohort, const jm_lmm_problem* prob, const double* V,
                     int64_t ldv, int mode, int64_t snp_start, int64_t snp_end, double* out,
                     int64_t* request_id);
@@ -443,6 +451,14 @@ int jm_serve_poll(jm_serve* s, jm_serve_done* recs, int max_recs, int* n_recs);
 /* Blocks up to timeout_ms for a completion record; returns how many are waiting. */
 int jm_serve_wait(jm_serve* s, int timeout_ms);
 int jm_serve_stats(jm_serve* s, jm_serve_stats_t* st);
+/* paused != 0: no admission and no new step until resumed (steps already launched
+ * complete and are reported); requests submitted meanwhile wait in the queue, so a
+ * caller can hand over a set of requests that then start together. Destroy resumes. */
+int jm_serve_pause(jm_serve* s, int paused);
+/* The cohort's stream (NULL with JM_SERVE_STUB): a caller that prepares requests on
+ * it adds no stream of its own, so its work never waits behind another cohort's
+ * step in a shared hardware queue; it runs in order with the cohort's steps. */
+int jm_serve_cohort_stream(jm_serve* s, int cohort, void** stream);
 /* JM_SERVE_MANUAL only: one pass of the loop (admission, a step per cohort,
  * completions); returns launched steps * 65536 + completed requests. */
 int64_t jm_serve_run_once(jm_serve* s);

@@ -72,7 +72,7 @@ class ServeCohortDesc(ctypes.Structure):
     _fields_ = [("n", c_int32), ("precision", c_int32), ("n_snps", c_int64), ("g", c_void_p),
                 ("ld_g", c_int64), ("means", c_void_p), ("UT", c_void_p), ("ld_u", c_int64),
                 ("planes", c_void_p), ("nplanes", c_int32), ("n_terms", c_int32),
-                ("n_clusters", c_int32), ("reserved", c_int32)]
+                ("n_clusters", c_int32), ("max_batch", c_int32)]
 
 
 class ServeDone(ctypes.Structure):
@@ -127,8 +127,8 @@ _SIGS = {
                                      c_void_p, c_void_p, c_int64, c_void_p], c_int),
     "jm_spectrum_moments_multi": ([c_void_p, c_int, c_int64, c_int64, c_void_p, c_int, c_void_p,
                                    c_void_p, c_void_p], c_int),
-    "jm_lmm_assoc_multi": ([c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
-                            c_int64, c_int64, c_void_p, c_void_p], c_int),
+    "jm_lmm_assoc_multi": ([c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p,
+                            c_void_p, c_int64, c_int64, c_void_p, c_void_p], c_int),
     "jm_lmm_null": ([POINTER(LmmProblem), c_void_p, c_void_p], c_int),
     "jm_lmm_prepare": ([POINTER(LmmProblem), c_void_p], c_int),
     "jm_lmm_assoc": ([POINTER(LmmProblem), c_int, c_void_p, c_int64, c_int64, c_void_p, c_void_p],
@@ -181,6 +181,8 @@ _SIGS = {
     "jm_serve_wait": ([c_void_p, c_int], c_int),
     "jm_serve_stats": ([c_void_p, POINTER(ServeStats)], c_int),
     "jm_serve_run_once": ([c_void_p], c_int64),
+    "jm_serve_pause": ([c_void_p, c_int], c_int),
+    "jm_serve_cohort_stream": ([c_void_p, c_int, POINTER(c_void_p)], c_int),
     "jm_serve_inject_step_error": ([c_void_p, c_int], c_int),
     "jm_serve_destroy": ([c_void_p], c_int),
 }

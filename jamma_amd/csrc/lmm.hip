@@ -1343,19 +1343,21 @@ __device__ __forceinline__ double oct_sum(double v, int lane) {
   return v;
 }
 
-// LDS of k_lmm_assoc16 (bytes): block constants, the shared pair rows and 16
-// per-group areas (two packed rows with their logdet slot, Iab level logs and
-// the h table of both lambdas), then the int tables.
+// LDS of k_lmm_assoc16 (bytes): block constants, the shared pair rows (none
+// when SG: they are read from global memory) and 16 per-group areas (two packed
+// rows with their logdet slot, Iab level logs and the h table of both lambdas),
+// then the int tables.
 __host__ __device__ constexpr int assoc16_group_doubles(int ni, int C) {
   return 2 * (ni + 1) + kMaxLevels + 2 * C;
 }
-static size_t assoc16_lds(int KT, int nc, int C) {
+static size_t assoc16_lds(int KT, int nc, int C, bool sg) {
   const int TB = kGL * KT;
   const int ni = (nc + 3) * (nc + 2) / 2;
   const int n_sh = (nc + 1) * (nc + 2) / 2;
   const int nrows = nc + 2;
   const int NR = nrows + 1 + n_sh;
-  return (size_t)(TB + 2 * C + (size_t)n_sh * TB + 16 * (size_t)assoc16_group_doubles(ni, C)) *
+  return (size_t)(TB + 2 * C + (sg ? 0 : (size_t)n_sh * TB) +
+                  16 * (size_t)assoc16_group_doubles(ni, C)) *
              sizeof(double) +
          (size_t)(TB + tet(nc + 1) + NR) * sizeof(int);
 }
@@ -1365,8 +1367,11 @@ static size_t assoc16_lds(int KT, int nc, int C) {
 // plan, n and T) from the moments block at snp_xmom[s]; a block splits its 16 SNPs
 // into runs of one problem and reloads the problem's LDS rows (shared pair rows,
 // recursion table) when the problem changes, so a batch of requests runs as one
-// launch. The LDS layout is that of the widest problem (max_cvt).
-template <int KT, bool MULTI>
+// launch. The LDS layout is that of the widest problem (max_cvt). SG: the shared
+// pair rows are read from global memory (L2-resident: every SNP of a problem
+// reads the same rows) instead of LDS, for layouts whose rows do not fit; the
+// values and the order of the multiply-adds are the same, so are the results.
+template <int KT, bool MULTI, bool SG>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_lmm_assoc16(
     jm_lmm_problem pr_one, int mode_one, const double* __restrict__ xmom, int64_t ldm,
     int64_t n_snps, double* __restrict__ out, const jm_lmm_problem* __restrict__ probs,
@@ -1390,7 +1395,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   double* c_cdc = c_twl + TB;                 // C cluster centres
   double* c_cw = c_cdc + C;                   // C cluster weights
   double* srows = c_cw + C;                   // n_sh x TB shared pair rows, zero-padded
-  double* groups = srows + (size_t)n_sh_m * TB;  // 16 x per_grp
+  double* groups = srows + (SG ? 0 : (size_t)n_sh_m * TB);  // 16 x per_grp
   int* c_tinf = (int*)(groups + 16 * per_grp);  // TB: m | cluster << 8
   int* rtab = c_tinf + TB;                    // tet(nc + 1) recursion updates
   int* vslot = rtab + tet(ncm + 1);           // NR: packed-row slot of each dot-product row
@@ -1410,10 +1415,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     const int nc = pr.n_cvt;
     const int n_sh = (nc + 1) * (nc + 2) / 2;
     const int NR = nc + 2 + 1 + n_sh;
-    for (int k = 0; k < n_sh; ++k) {
-      const int jj = shared_pair_index(k, nc);
-      for (int t = threadIdx.x; t < TB; t += blockDim.x)
-        srows[(size_t)k * TB + t] = t < T ? pr.shared_mom[(int64_t)jj * T + t] : 0.0;
+    if (!SG) {
+      for (int k = 0; k < n_sh; ++k) {
+        const int jj = shared_pair_index(k, nc);
+        for (int t = threadIdx.x; t < TB; t += blockDim.x)
+          srows[(size_t)k * TB + t] = t < T ? pr.shared_mom[(int64_t)jj * T + t] : 0.0;
+      }
     }
     const int m = nc + 2;
     for (int e = threadIdx.x; e < tet(nc + 1); e += blockDim.x) {
@@ -1463,6 +1470,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     const jm_lmm_problem& pr = MULTI ? probs[pidx] : pr_one;
     const int mode = MULTI ? modes[pidx] : mode_one;
     const int nc = pr.n_cvt, G = pr.n_grid;
+    const double* shm = pr.shared_mom;  // SG: the shared pair rows, (ni x T)
     const int ni = (nc + 3) * (nc + 2) / 2;
     const int nrows = nc + 2;  // per-SNP moment rows: W_1..W_c, y, xx
     const int n_sh = (nc + 1) * (nc + 2) / 2;
@@ -1588,12 +1596,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             sa = ldA;
             sb = ldB;
           } else if (r < NR) {
-            const double* row = srows + (size_t)(r - nrows - 1) * TB + j * KT;
+            if constexpr (SG) {  // packed pair index vslot[r] of shared pair r - nrows - 1
+              const double* row = shm + (int64_t)vslot[r] * T + j * KT;
 #pragma unroll
-            for (int k = 0; k < KT; ++k) {
-              const double x = row[k];
-              sa = fma(kA[k], x, sa);
-              sb = fma(kB[k], x, sb);
+              for (int k = 0; k < KT; ++k) {
+                const double x = (j * KT + k < T) ? row[k] : 0.0;
+                sa = fma(kA[k], x, sa);
+                sb = fma(kB[k], x, sb);
+              }
+            } else {
+              const double* row = srows + (size_t)(r - nrows - 1) * TB + j * KT;
+#pragma unroll
+              for (int k = 0; k < KT; ++k) {
+                const double x = row[k];
+                sa = fma(kA[k], x, sa);
+                sb = fma(kB[k], x, sb);
+              }
             }
           }
           v[2 * e] = sa;
@@ -1794,9 +1812,13 @@ static void launch_kt(const jm_lmm_problem* one, int mode_one, const double* xmo
 static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* xmom, int64_t ldm,
                         int64_t n_snps, double* out, const jm_lmm_problem* probs,
                         const int32_t* modes, const int32_t* snp_prob,
-                        const double* const* snp_xmom, int T, int max_cvt, int n_clusters,
-                        void* stream) {
+                        const double* const* snp_xmom, int T, int max_cvt, int sel_cvt,
+                        int n_clusters, void* stream) {
+  // max_cvt sizes the LDS layouts (the widest problem of the launch); sel_cvt >=
+  // max_cvt picks the lambda-search kernel (16 lanes or a wavefront per SNP, which
+  // sum in different orders), so launches with one sel_cvt agree bit for bit
   JM_CHECK_ARG(T >= 1, "n_terms=%d < 1", T);
+  if (sel_cvt < max_cvt) sel_cvt = max_cvt;
   const int ni_max = (max_cvt + 3) * (max_cvt + 2) / 2;
   hipStream_t st = (hipStream_t)stream;
   // grid stage (skips Score-only SNPs): one wave per SNP, a Pab column per lane in LDS
@@ -1831,22 +1853,34 @@ static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* x
     const char* env = getenv("JM_LMM_WAVE_PER_SNP");
     const bool wave = env && env[0] == '1';
     const int kt16 = T <= 64 ? 4 : (T <= 128 ? 8 : (T <= 192 ? 12 : (T <= 256 ? 16 : 0)));
-    const size_t lds16 = kt16 ? assoc16_lds(kt16, max_cvt, n_clusters) : 0;
-    if (!wave && kt16 && n_clusters >= 1 && lds16 <= 64 * 1024) {
+    // the shared pair rows go to LDS when they fit, else they are read from global
+    // memory (same arithmetic): the kernel choice depends on T, the cluster count and
+    // max_cvt only
+    const bool sg = kt16 && assoc16_lds(kt16, max_cvt, n_clusters, false) > 64 * 1024;
+    const size_t lds16 = kt16 ? assoc16_lds(kt16, max_cvt, n_clusters, sg) : 0;
+    const bool fits = kt16 && assoc16_lds(kt16, sel_cvt, n_clusters, true) <= 64 * 1024;
+    if (!wave && kt16 && n_clusters >= 1 && fits) {
       int64_t b16 = (n_snps + 15) / 16;
       if (b16 > 256 * 8) b16 = 256 * 8;
       jm_lmm_problem dummy;
       memset(&dummy, 0, sizeof(dummy));
-#define JM_LAUNCH16(KT_)                                                                        \
+#define JM_LAUNCH16_SG(KT_, SG_)                                                                \
   do {                                                                                          \
     if (one)                                                                                    \
-      hipLaunchKernelGGL((k_lmm_assoc16<KT_, false>), dim3((unsigned)b16), dim3(256), lds16, st, \
-                         *one, mode_one, xmom, ldm, n_snps, out, nullptr, nullptr, nullptr,     \
+      hipLaunchKernelGGL((k_lmm_assoc16<KT_, false, SG_>), dim3((unsigned)b16), dim3(256), lds16, \
+                         st, *one, mode_one, xmom, ldm, n_snps, out, nullptr, nullptr, nullptr, \
                          nullptr, max_cvt);                                                     \
     else                                                                                        \
-      hipLaunchKernelGGL((k_lmm_assoc16<KT_, true>), dim3((unsigned)b16), dim3(256), lds16, st,  \
-                         dummy, 0, nullptr, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom, \
-                         max_cvt);                                                              \
+      hipLaunchKernelGGL((k_lmm_assoc16<KT_, true, SG_>), dim3((unsigned)b16), dim3(256), lds16,  \
+                         st, dummy, 0, nullptr, ldm, n_snps, out, probs, modes, snp_prob,       \
+                         snp_xmom, max_cvt);                                                    \
+  } while (0)
+#define JM_LAUNCH16(KT_)           \
+  do {                             \
+    if (sg)                        \
+      JM_LAUNCH16_SG(KT_, true);   \
+    else                           \
+      JM_LAUNCH16_SG(KT_, false);  \
   } while (0)
       if (kt16 == 4)
         JM_LAUNCH16(4);
@@ -1857,6 +1891,7 @@ static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* x
       else
         JM_LAUNCH16(16);
 #undef JM_LAUNCH16
+#undef JM_LAUNCH16_SG
       JM_CHECK_LAUNCH();
       return JM_OK;
     }
@@ -1928,21 +1963,23 @@ extern "C" int jm_lmm_assoc(const jm_lmm_problem* prob, int mode, const double* 
                    prob->null_shared,
                "tables not prepared");
   return launch_assoc(prob, mode, xmom, ldm, n_snps, out, nullptr, nullptr, nullptr, nullptr,
-                      prob->n_terms, prob->n_cvt, prob->n_clusters, stream);
+                      prob->n_terms, prob->n_cvt, prob->n_cvt, prob->n_clusters, stream);
 }
 
 extern "C" int jm_lmm_assoc_multi(const jm_lmm_problem* probs, const int32_t* modes, int n_probs,
-                                  int max_cvt, int n_terms, int n_clusters,
+                                  int max_cvt, int width_cvt, int n_terms, int n_clusters,
                                   const int32_t* snp_prob, const double* const* snp_xmom,
                                   int64_t ldm, int64_t n_snps, double* out, void* stream) {
   JM_CHECK_ARG(n_snps >= 0 && n_probs >= 1, "bad sizes");
   if (n_snps == 0) return JM_OK;
   JM_CHECK_ARG(probs && modes && snp_prob && snp_xmom && out, "null pointer");
   JM_CHECK_ARG(max_cvt >= 1 && max_cvt <= 20, "max_cvt out of range [1,20]");
+  JM_CHECK_ARG(width_cvt == 0 || (width_cvt >= max_cvt && width_cvt <= 20),
+               "width_cvt must be 0 or in [max_cvt, 20]");
   JM_CHECK_ARG(n_terms >= 1 && ldm >= n_terms, "bad n_terms / ldm");
   JM_CHECK_ARG(n_clusters >= 1, "n_clusters < 1");
   return launch_assoc(nullptr, 0, nullptr, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                      n_terms, max_cvt, n_clusters, stream);
+                      n_terms, max_cvt, width_cvt ? width_cvt : max_cvt, n_clusters, stream);
 }
 
 extern "C" double jm_host_betainc(double a, double b, double x) { return betainc_reg(a, b, x); }

@@ -11,12 +11,13 @@
 //   * one HIP stream (default priority) and `max_inflight` pinned descriptor slots,
 //     each with an event; a slot is reused only once its event has completed;
 //   * a step takes the next slice (<= slice_snps) of every active request, up to
-//     max_batch SNPs and max_reqs requests, writes the step's descriptors into the
-//     slot (gather index, moments column descriptors, per-SNP problem index and
-//     moments address, the problems and their modes), copies them up with one
-//     hipMemcpyAsync, and launches pack + rotation (or the FP64 rotation), ONE
-//     jm_spectrum_moments_multi and ONE jm_lmm_assoc_multi over the whole batch,
-//     then copies each request's rows into its result block on the stream;
+//     the cohort's max_batch SNPs and max_reqs requests, writes the step's
+//     descriptors into the slot (gather index, moments column descriptors, per-SNP
+//     problem index and moments address, the problems and their modes), copies them
+//     up with one hipMemcpyAsync, and launches pack + rotation (or the FP64
+//     rotation), ONE jm_spectrum_moments_multi and ONE jm_lmm_assoc_multi over the
+//     whole batch, then copies each request's rows into its result block (device or
+//     pinned host memory) on the stream;
 //   * a request completes when the event of the step that took its last slice has
 //     completed; its arena space is released then.
 // Admission walks the queue in submission order (first fit, with aging: a request
@@ -148,6 +149,7 @@ struct jm_serve {
   int64_t next_id = 1;
   int64_t steps = 0, snps = 0, max_active = 0;
   bool manual = false;  // no thread: jm_serve_run_once drives the loop
+  bool paused = false;  // no admission and no new step (completions still polled)
   int inject_fail = -1;  // cohort whose next step fails (failure-handling tests)
   int64_t reserved = 0;  // arena bytes held by the cohorts' step workspaces
 };
@@ -165,6 +167,11 @@ void finish(jm_serve* s, Request* r, int status, const char* msg) {
   delete r;
 }
 
+// SNPs per step of a cohort: its own cap if it has one, else the server's.
+int64_t cohort_batch(const jm_serve_config& cfg, const jm_serve_cohort_desc& d) {
+  return d.max_batch > 0 ? std::min<int64_t>(d.max_batch, cfg.max_batch) : cfg.max_batch;
+}
+
 int64_t request_bytes(const jm_serve* s, const Cohort& c, int nv) {
   return align_up(s->cfg.slice_snps * (int64_t)(nv + 1) * c.d.n_terms * (int64_t)sizeof(double));
 }
@@ -172,7 +179,7 @@ int64_t request_bytes(const jm_serve* s, const Cohort& c, int nv) {
 // Step workspace layout of a cohort (device, in the arena) and its descriptor block.
 int64_t layout(const jm_serve_config& cfg, const jm_serve_cohort_desc& d,
                const jm_spectrum_plan* plan, Cohort* c) {
-  const int64_t MB = cfg.max_batch, MR = cfg.max_reqs;
+  const int64_t MB = cohort_batch(cfg, d), MR = cfg.max_reqs;
   int64_t dd = 0;
   auto put = [&](int64_t bytes) {
     const int64_t o = dd;
@@ -293,10 +300,11 @@ int launch_step(jm_serve* s, int ci) {
   std::vector<Take> takes;
   int64_t B = 0;
   int max_cvt = 1;
+  const int64_t MB = cohort_batch(s->cfg, c.d);
   for (Request* r : c.active) {
     if (r->remaining() <= 0) continue;
-    if (B >= s->cfg.max_batch || (int64_t)takes.size() >= s->cfg.max_reqs) break;
-    const int64_t t = std::min({s->cfg.slice_snps, r->remaining(), s->cfg.max_batch - B});
+    if (B >= MB || (int64_t)takes.size() >= s->cfg.max_reqs) break;
+    const int64_t t = std::min({s->cfg.slice_snps, r->remaining(), MB - B});
     takes.push_back({r, r->next - r->start, t});
     B += t;
     max_cvt = std::max(max_cvt, r->prob.n_cvt);
@@ -378,16 +386,20 @@ int launch_step(jm_serve* s, int ci) {
                                  max_cvt + 1, &c.plan, (double*)(dev + c.o_work), st);
   if (rc) return rc;
   double* sout = (double*)(dev + c.o_sout);
+  // the search kernel is chosen for the configured max_cvt, not for this step's
+  // widest problem: its variant, and with it every result bit, is then the same
+  // whatever else shares the step
   rc = jm_lmm_assoc_multi((const jm_lmm_problem*)(ddesc + c.d_probs),
-                          (const int32_t*)(ddesc + c.d_modes), (int)takes.size(), max_cvt, T,
+                          (const int32_t*)(ddesc + c.d_modes), (int)takes.size(), max_cvt,
+                          (int)s->cfg.max_cvt, T,
                           c.d.n_clusters, (const int32_t*)(ddesc + c.d_prob),
                           (const double* const*)(ddesc + c.d_xmom), T, B, sout, st);
   if (rc) return rc;
   o = 0;
   for (const Take& t : takes) {
     JM_CHECK_HIP(hipMemcpyAsync(t.r->out + t.off * JM_NOUT, sout + o * JM_NOUT,
-                                (size_t)t.take * JM_NOUT * sizeof(double),
-                                hipMemcpyDeviceToDevice, st));
+                                (size_t)t.take * JM_NOUT * sizeof(double), hipMemcpyDefault,
+                                st));
     o += t.take;
   }
   JM_CHECK_HIP(hipEventRecord(slot->ev, st));
@@ -403,10 +415,16 @@ bool busy(const jm_serve* s) {
   return false;
 }
 
+bool in_flight(const jm_serve* s) {
+  for (const auto& c : s->cohorts)
+    if (c->seq_done < c->seq_launched) return true;
+  return false;
+}
+
 // One pass: admit, then per cohort poll / launch a step / poll. Caller holds s->mu.
 // -> launched steps * 65536 + completed requests.
 int64_t iterate(jm_serve* s) {
-  admit(s);
+  if (!s->paused) admit(s);
   int64_t active = 0;
   for (const auto& c : s->cohorts) active += (int64_t)c->active.size();
   s->max_active = std::max(s->max_active, active);
@@ -414,7 +432,7 @@ int64_t iterate(jm_serve* s) {
   for (size_t ci = 0; ci < s->cohorts.size(); ++ci) {
     Cohort& c = *s->cohorts[ci];
     completed += poll_cohort(s, c);
-    const int rc = launch_step(s, (int)ci);
+    const int rc = s->paused ? 0 : launch_step(s, (int)ci);
     if (rc > 0) {
       ++launched;
     } else if (rc < 0) {
@@ -438,9 +456,9 @@ void loop(jm_serve* s) {
     const int64_t r = iterate(s);
     const bool launched = r >= 65536, completed = (r % 65536) != 0;
     if (!launched && !completed) {
-      if (!busy(s)) {
-        if (s->stop) break;
-        s->cv.wait(lk);
+      if (!busy(s) || (s->paused && !in_flight(s))) {
+        if (s->stop && !busy(s)) break;
+        s->cv.wait(lk);  // submit, resume and destroy notify
       } else {
         // device work in flight (or requests waiting for budget): wait a little
         s->cv.wait_for(lk, std::chrono::microseconds(200));
@@ -506,6 +524,9 @@ extern "C" int jm_serve_add_cohort(jm_serve* s, const jm_serve_cohort_desc* d,
                                                            : (d->UT && d->ld_u >= d->n))),
                "missing cohort arrays");
   JM_CHECK_ARG(d->n_snps < (1LL << 31), "n_snps exceeds int32 SNP indices");
+  JM_CHECK_ARG(d->max_batch == 0 || d->max_batch >= s->cfg.slice_snps,
+               "a cohort's max_batch (%d) must be 0 or at least slice_snps (%lld)", d->max_batch,
+               (long long)s->cfg.slice_snps);
   auto c = std::make_unique<Cohort>();
   c->d = *d;
   c->plan = *plan;
@@ -638,6 +659,22 @@ extern "C" int jm_serve_stats(jm_serve* s, jm_serve_stats_t* st) {
   return JM_OK;
 }
 
+extern "C" int jm_serve_pause(jm_serve* s, int paused) {
+  JM_CHECK_ARG(s, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->paused = paused != 0;
+  s->cv.notify_all();
+  return JM_OK;
+}
+
+extern "C" int jm_serve_cohort_stream(jm_serve* s, int cohort, void** stream) {
+  JM_CHECK_ARG(s && stream, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  JM_CHECK_ARG(cohort >= 0 && cohort < (int)s->cohorts.size(), "unknown cohort %d", cohort);
+  *stream = (void*)s->cohorts[(size_t)cohort]->stream;
+  return JM_OK;
+}
+
 extern "C" int jm_serve_inject_step_error(jm_serve* s, int cohort) {
   JM_CHECK_ARG(s, "null argument");
   std::lock_guard<std::mutex> lk(s->mu);
@@ -651,6 +688,7 @@ extern "C" int jm_serve_destroy(jm_serve* s) {
   {
     std::lock_guard<std::mutex> lk(s->mu);
     s->stop = true;
+    s->paused = false;  // destroy drains everything
     s->cv.notify_all();
   }
   if (s->thread.joinable()) s->thread.join();  // drains every admitted and queued request

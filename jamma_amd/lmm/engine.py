@@ -353,10 +353,13 @@ class AssocProblem:
         return self.assoc_moments(xmom, n_snps, mode)
 
 
-def assoc_multi(problems, modes, snp_prob, snp_xmom, out: torch.Tensor, stream=None) -> None:
+def assoc_multi(problems, modes, snp_prob, snp_xmom, out: torch.Tensor, stream=None,
+                width_cvt: int = 0) -> None:
     """jm_lmm_assoc_multi: SNP s is tested under problems[snp_prob[s]] (AssocProblems of
     one cohort) in mode modes[snp_prob[s]], from the moments block at device address
-    snp_xmom[s] ((n_cvt+2) x T float64 rows); its results go to row s of out."""
+    snp_xmom[s] ((n_cvt+2) x T float64 rows); its results go to row s of out.
+    width_cvt (0: the widest problem's n_cvt) picks the kernel variant: launches with
+    one width_cvt give a SNP the same bits whatever else is in the batch."""
     if not problems:
         raise ValueError("no problems")
     co = problems[0].cohort
@@ -379,9 +382,9 @@ def assoc_multi(problems, modes, snp_prob, snp_xmom, out: torch.Tensor, stream=N
     sp = to_dev(snp_prob, torch.int32, dev)
     sx = to_dev(snp_xmom.view(np.int64), torch.int64, dev)
     max_cvt = max(p.n_cvt for p in problems)
-    _lib.call("jm_lmm_assoc_multi", ptr(probs_dev), ptr(modes_dev), len(problems), max_cvt, T,
-              int(problems[0].prob.n_clusters), ptr(sp), ptr(sx), T, B, ptr(out),
-              stream_ptr(stream))
+    _lib.call("jm_lmm_assoc_multi", ptr(probs_dev), ptr(modes_dev), len(problems), max_cvt,
+              int(width_cvt), T, int(problems[0].prob.n_clusters), ptr(sp), ptr(sx), T, B,
+              ptr(out), stream_ptr(stream))
     if stream is not None:
         # the launch reads these device copies on `stream`: the caching allocator must not
         # hand their memory out again before that stream has passed the launch

@@ -7,8 +7,8 @@ csrc/serve.cpp), which owns a scheduler thread and one HBM arena given to it her
 
 * this module prepares each request on worker threads (rotated covariates and
   phenotype, their spectrum moments, the device null model and grid tables, on the
-  cohort's setup stream) and hands the prepared problem to the runtime with a result
-  block of its own;
+  cohort's stream, in order with its steps) and hands the prepared problem to the
+  runtime with a pinned host result block of its own;
 * the runtime admits requests in submission order against the arena (first fit, with
   aging: a request that later ones overtook `max_skips` times stops further
   overtaking), and every step of a cohort takes the next slice of each active
@@ -78,13 +78,13 @@ class ResidentCohort:
         self.cohort = Cohort(eigenvalues, eigenvectors,
                              precision="both" if precision == "mixed" else "f64")
         self.n = self.cohort.n
-        self.setup_stream = torch.cuda.Stream()  # per-request setup (default priority)
 
     def resident_bytes(self) -> int:
         return self.cohort.hbm_bytes() + self.g.numel() + self.means.numel() * 8
 
-    def desc(self) -> _lib.ServeCohortDesc:
-        """The runtime's description of this cohort (device arrays stay ours)."""
+    def desc(self, max_batch: int = 0) -> _lib.ServeCohortDesc:
+        """The runtime's description of this cohort (device arrays stay ours); max_batch
+        caps the SNPs of one step (0: the server's cap)."""
         co = self.cohort
         mixed = self.precision == "mixed"
         return _lib.ServeCohortDesc(
@@ -92,7 +92,7 @@ class ResidentCohort:
             n_snps=self.p, g=ptr(self.g), ld_g=self.ld, means=ptr(self.means), UT=ptr(co.UT),
             ld_u=co.ld_u, planes=ptr(co.split) if mixed else None,
             nplanes=co.bf16_planes if mixed else 0, n_terms=co.n_terms,
-            n_clusters=int(co.plan_c.n_clusters))
+            n_clusters=int(co.plan_c.n_clusters), max_batch=int(max_batch))
 
 
 def serve_config(slice_snps: int = 2048, max_batch_snps: int = 16384, max_inflight: int = 2,
@@ -102,9 +102,21 @@ def serve_config(slice_snps: int = 2048, max_batch_snps: int = 16384, max_inflig
                             max_cvt=max_cvt, flags=0)
 
 
-def cohort_workspace_bytes(rc: ResidentCohort, cfg: _lib.ServeConfig) -> int:
-    """Arena bytes of a cohort's step workspace under `cfg`."""
-    return int(_lib.lib.jm_serve_cohort_ws_bytes(ctypes.byref(cfg), ctypes.byref(rc.desc()),
+def cohort_step_snps(n: int, cfg: _lib.ServeConfig, step_work: float) -> int:
+    """SNPs per step of a cohort of n samples: at most step_work / n^2 (the rotation
+    of a step costs 2 n^2 FLOP per SNP), whole slices, between one slice and the
+    server's max_batch. Short steps of a large cohort bound how long other work queued
+    behind them on the device (another cohort's step, a request's setup) waits."""
+    sl = int(cfg.slice_snps)
+    cap = int(step_work // (float(n) * n)) // sl * sl
+    return max(sl, min(int(cfg.max_batch), cap))
+
+
+def cohort_workspace_bytes(rc: ResidentCohort, cfg: _lib.ServeConfig, max_batch: int = 0) -> int:
+    """Arena bytes of a cohort's step workspace under `cfg` (steps of at most
+    max_batch SNPs; 0: cfg.max_batch)."""
+    return int(_lib.lib.jm_serve_cohort_ws_bytes(ctypes.byref(cfg),
+                                                 ctypes.byref(rc.desc(max_batch)),
                                                  ctypes.byref(rc.cohort.plan_c)))
 
 
@@ -119,10 +131,15 @@ class Server:
 
     def __init__(self, hbm_budget_bytes: int | None = None, slice_snps: int = 2048,
                  max_batch_snps: int = 16384, max_inflight: int = 2, setup_workers: int = 2,
-                 max_skips: int = 8, max_reqs: int = 64, max_cvt: int = 12):
+                 max_skips: int = 8, max_reqs: int = 64, max_cvt: int = 12,
+                 step_work: float | None = None):
+        """step_work: n^2 x SNPs of one step of a cohort (default: max_batch_snps SNPs
+        at n = 10,000); see cohort_step_snps."""
         self.device = require_gpu()
         self.cfg = serve_config(slice_snps, max_batch_snps, max_inflight, max_skips, max_reqs,
                                 max_cvt)
+        self.step_work = float(step_work) if step_work is not None else \
+            float(self.cfg.max_batch) * 1e8
         if hbm_budget_bytes is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget_bytes = int(free * 0.5)
@@ -134,6 +151,8 @@ class Server:
                   ctypes.byref(self._h))
         self.cohorts: dict[str, ResidentCohort] = {}
         self._cid: dict[str, int] = {}
+        self.step_snps: dict[str, int] = {}  # SNPs per step of each cohort
+        self._streams: dict[str, torch.cuda.ExternalStream] = {}
         self._reserved = 0
         self._pending: dict[int, tuple] = {}  # runtime id -> (request, problem, out)
         self._setups = 0
@@ -149,13 +168,20 @@ class Server:
 
     # -- cohorts --------------------------------------------------------------
     def add_cohort(self, rc: ResidentCohort) -> None:
-        need = cohort_workspace_bytes(rc, self.cfg)
+        mb = cohort_step_snps(rc.n, self.cfg, self.step_work)
+        need = cohort_workspace_bytes(rc, self.cfg, mb)
         if need > self.budget - self._reserved:
             raise MemoryError(f"the HBM budget cannot hold cohort {rc.name!r}'s step workspace "
                               f"({need} bytes)")
         cid = ctypes.c_int()
-        _lib.call("jm_serve_add_cohort", self._h, ctypes.byref(rc.desc()),
+        _lib.call("jm_serve_add_cohort", self._h, ctypes.byref(rc.desc(mb)),
                   ctypes.byref(rc.cohort.plan_c), ctypes.byref(cid))
+        self.step_snps[rc.name] = mb
+        h = ctypes.c_void_p()
+        _lib.call("jm_serve_cohort_stream", self._h, cid.value, ctypes.byref(h))
+        # request setup runs on the cohort's own stream: no extra stream shares a
+        # hardware queue with (and waits behind) another cohort's steps
+        self._streams[rc.name] = torch.cuda.ExternalStream(h.value, device=rc.cohort.device)
         self._reserved += (need + 255) // 256 * 256
         self.cohorts[rc.name] = rc
         self._cid[rc.name] = cid.value
@@ -187,16 +213,18 @@ class Server:
         the hand-over to the runtime."""
         rc = self.cohorts[req.cohort]
         try:
-            with torch.cuda.device(rc.cohort.device), torch.cuda.stream(rc.setup_stream):
+            st = self._streams[req.cohort]
+            with torch.cuda.device(rc.cohort.device), torch.cuda.stream(st):
                 prob = AssocProblem(rc.cohort, req.phenotype, req.covariates)
-                out = torch.empty((req.n_snps, _lib.JM_NOUT), dtype=torch.float64,
-                                  device=rc.cohort.device)
-                rc.setup_stream.synchronize()  # tables complete before the steps read them
+                st.synchronize()  # tables complete before the steps read them
+            # pinned host result block: each step copies its rows there on the cohort's
+            # stream, so a completed request needs no further device-to-host copy
+            out = torch.empty((req.n_snps, _lib.JM_NOUT), dtype=torch.float64, pin_memory=True)
             rid = ctypes.c_int64()
             with self._lock:  # registered before the runtime can complete it
                 _lib.call("jm_serve_submit", self._h, self._cid[req.cohort],
                           ctypes.byref(prob.prob), ptr(prob.V), prob.V.shape[1], int(req.mode),
-                          req.snp_start, req.snp_end, ptr(out), ctypes.byref(rid))
+                          req.snp_start, req.snp_end, out.data_ptr(), ctypes.byref(rid))
                 self._pending[rid.value] = (req, prob, out)
                 req.t_admit = time.perf_counter()
         except Exception as e:  # noqa: BLE001 - reported to the request
@@ -211,7 +239,7 @@ class Server:
         with self._lock:
             req, prob, out = self._pending.pop(rec.id)
         if rec.status == 0:
-            res = out.cpu().numpy()  # the step's event has completed: the rows are final
+            res = out.numpy().copy()  # the step's event has completed: the rows are final
             req.t_done = time.perf_counter()
             self.completed.append(req)
             req.future.set_result(res)
@@ -251,6 +279,7 @@ class Server:
             self._final_stats = self.stats()
             _lib.call("jm_serve_destroy", self._h)
             self._h = ctypes.c_void_p()
+            self._streams.clear()  # the runtime's streams are gone
             self.arena = None  # back to the caching allocator
 
     def __del__(self):
@@ -267,6 +296,25 @@ class Server:
 
     def __exit__(self, *exc):
         self.stop()
+
+    def pause(self) -> None:
+        """No admission and no new step until resume(); steps already launched finish."""
+        _lib.call("jm_serve_pause", self._h, 1)
+
+    def resume(self) -> None:
+        _lib.call("jm_serve_pause", self._h, 0)
+
+    def wait_prepared(self, timeout: float = 300.0) -> None:
+        """Blocks until every submitted request has been prepared and handed to the
+        runtime (or failed in preparation)."""
+        t_end = time.perf_counter() + timeout
+        while True:
+            with self._lock:
+                if self._setups == 0:
+                    return
+            if time.perf_counter() > t_end:
+                raise TimeoutError("request preparation did not finish")
+            time.sleep(0.002)
 
     def inject_step_error(self, cohort: str) -> None:
         """Failure-handling hook: the cohort's next step fails before any launch."""

@@ -161,3 +161,34 @@ def test_multi_problem_batch(name):
             rtol = 5e-5 if key.startswith("l_") else 1e-9
             np.testing.assert_allclose(out[sel, col], single[:, col], rtol=rtol,
                                        err_msg=f"{name} multi vs single c={c} {key}")
+
+
+@pytest.mark.parametrize("name", list(SPECTRA))
+def test_multi_result_bits_do_not_depend_on_the_batch(name):
+    """With one width_cvt, a problem's SNPs give the same bits launched alone and
+    interleaved with wider problems (the LDS layout follows the batch, the kernel
+    variant only the width)."""
+    from jamma_amd import _lib
+    from jamma_amd.lmm.engine import assoc_multi
+
+    d, UT, UtX, co, Z, _ = _cohort(name)
+    specs = [(1, 4), (4, 4), (11, 4), (3, 1)]
+    probs, modes, xms = [], [], []
+    for c, mode in specs:
+        pb, xm, _ = _problem(name, c)
+        probs.append(pb)
+        modes.append(mode)
+        xms.append(xm)
+    T = co.n_terms
+    snp_prob = np.arange(P, dtype=np.int32) % len(specs)
+    addr = np.array([xms[k].data_ptr() + s * (probs[k].n_cvt + 2) * T * 8
+                     for s, k in enumerate(snp_prob)], dtype=np.uint64)
+    mix = torch.full((P, _lib.JM_NOUT), -7.0, dtype=torch.float64, device="cuda")
+    assoc_multi(probs, modes, snp_prob, addr, mix, width_cvt=11)
+    mix = mix.cpu().numpy()
+    for k, (c, mode) in enumerate(specs):
+        sel = np.where(snp_prob == k)[0]
+        alone = torch.full((len(sel), _lib.JM_NOUT), -7.0, dtype=torch.float64, device="cuda")
+        assoc_multi([probs[k]], [mode], np.zeros(len(sel), np.int32), addr[sel], alone,
+                    width_cvt=11)
+        assert np.array_equal(alone.cpu().numpy(), mix[sel], equal_nan=True), (name, c)

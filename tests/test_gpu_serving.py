@@ -75,11 +75,17 @@ def test_alone_equals_mix_and_oracle(cohorts):
             srv.add_cohort(rc)
         with srv:
             alone.append(srv.submit(_clone(r)).result(timeout=120))
-    srv = Server(slice_snps=64, max_batch_snps=256)
+    # the mix also caps each cohort's steps by n^2 x SNPs: 128 SNPs at n = 300, one
+    # 64-SNP slice at n = 517
+    srv = Server(slice_snps=64, max_batch_snps=256, step_work=64 * 517 ** 2)
     for rc, *_ in cohorts.values():
         srv.add_cohort(rc)
+    assert srv.step_snps == {"a": 128, "b": 64}
     with srv:
+        srv.pause()  # every request prepared and queued first: the steps mix them
         futs = [srv.submit(_clone(r)) for r in reqs]
+        srv.wait_prepared()
+        srv.resume()
         mixed = [f.result(timeout=300) for f in futs]
         stats = srv.latency_stats()
     assert stats["max_concurrent"] >= 2, stats
@@ -221,7 +227,10 @@ def test_wide_spectrum_cohort_many_covariates():
     srv = Server(slice_snps=48, max_batch_snps=192)
     srv.add_cohort(rc)
     with srv:
+        srv.pause()
         futs = [srv.submit(_clone(r)) for r in reqs]
+        srv.wait_prepared()
+        srv.resume()
         mixed = [f.result(timeout=300) for f in futs]
         stats = srv.latency_stats()
     assert stats["max_concurrent"] >= 2, stats

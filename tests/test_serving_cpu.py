@@ -23,9 +23,9 @@ def _plan():
     return p
 
 
-def _desc():
+def _desc(max_batch=0):
     return _lib.ServeCohortDesc(n=N, precision=_lib.JM_SERVE_MIXED, n_snps=100_000, ld_g=128,
-                                nplanes=2, n_terms=T, n_clusters=C)
+                                nplanes=2, n_terms=T, n_clusters=C, max_batch=max_batch)
 
 
 def _cfg(max_batch=SLICE, max_skips=8, manual=True):
@@ -39,9 +39,9 @@ def _req_bytes(cfg, n_cvt):
 
 
 class Runtime:
-    def __init__(self, cfg, extra_bytes, n_cohorts=1):
+    def __init__(self, cfg, extra_bytes, n_cohorts=1, max_batch=0):
         self.cfg = cfg
-        self.plan, self.desc = _plan(), _desc()
+        self.plan, self.desc = _plan(), _desc(max_batch)
         self.ws = int(_lib.lib.jm_serve_cohort_ws_bytes(ctypes.byref(cfg), ctypes.byref(self.desc),
                                                         ctypes.byref(self.plan)))
         assert self.ws > 0
@@ -136,6 +136,30 @@ def test_step_limits_and_batches():
     rt.close()
 
 
+def test_cohort_step_cap():
+    """A cohort's own max_batch caps its steps below the server's and shrinks its
+    step workspace; a cap below one slice is refused."""
+    cfg = _cfg(max_batch=4 * SLICE)
+    full = Runtime(cfg, 0)
+    rt = Runtime(cfg, 8 * _req_bytes(cfg, 2), max_batch=SLICE)
+    assert rt.ws < full.ws
+    full.close()
+    ids = [rt.submit(2, n_snps=SLICE, start=SLICE * k)[1] for k in range(3)]
+    rt.run()  # one slice per step although the server allows four
+    assert [d[0] for d in rt.done()] == ids[:1]
+    rt.run(2)
+    assert [d[0] for d in rt.done()] == ids[1:]
+    assert rt.stats().steps == 3
+    rt.close()
+    h = ctypes.c_void_p()
+    _lib.call("jm_serve_create", ctypes.byref(cfg), None, 1 << 30, ctypes.byref(h))
+    cid = ctypes.c_int()
+    with pytest.raises(RuntimeError, match="at least slice_snps"):
+        _lib.call("jm_serve_add_cohort", h, ctypes.byref(_desc(SLICE - 1)), ctypes.byref(_plan()),
+                  ctypes.byref(cid))
+    _lib.call("jm_serve_destroy", h)
+
+
 def test_rejections():
     cfg = _cfg()
     rt = Runtime(cfg, _req_bytes(cfg, 2))
@@ -191,6 +215,28 @@ def test_threaded_destroy_drains_queued_requests():
     for k in range(10):
         assert rt.submit(4, n_snps=2 * SLICE, start=64 * k)[0] == 0
     rt.close()  # returns once every request has been served
+
+
+def test_pause_holds_admission_and_steps():
+    cfg = _cfg(max_batch=4 * SLICE, manual=False)
+    rt = Runtime(cfg, 8 * _req_bytes(cfg, 2))
+    _lib.call("jm_serve_pause", rt.h, 1)
+    ids = [rt.submit(2, start=SLICE * k)[1] for k in range(3)]
+    assert _lib.lib.jm_serve_wait(rt.h, 50) == 0  # nothing runs while paused
+    st = rt.stats()
+    assert st.queued == 3 and st.steps == 0
+    _lib.call("jm_serve_pause", rt.h, 0)
+    seen = []
+    for _ in range(100):
+        if len(seen) == 3:
+            break
+        _lib.lib.jm_serve_wait(rt.h, 100)
+        seen += rt.done()
+    assert sorted(d[0] for d in seen) == ids
+    assert rt.stats().steps == 1  # admitted together: one step holds all three
+    _lib.call("jm_serve_pause", rt.h, 1)
+    rt.submit(2)
+    rt.close()  # destroy resumes and drains
 
 
 def test_manual_mode_requires_stub():
