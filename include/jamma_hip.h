@@ -534,6 +534,22 @@ int jm_tridiag_panel_step(int64_t len, int i, const double* ap, int64_t lda, con
  * alpha = -tau/2 (w . v), w += alpha v (tau read from device memory). */
 int jm_tridiag_finish(int64_t len, const double* tau, double* w, const double* v, void* stream);
 
+/* One fused column step i of the blocked reduction of a panel (LAPACK dlatrd,
+ * lower, with dlarfg), replacing the seven launches of jm_tridiag_panel_update,
+ * jm_tridiag_larfg, jm_symv_lower, jm_tridiag_panel_step and jm_tridiag_finish by
+ * three: the finish of column i-1 and the update of column i, then the symv with
+ * the panel dot products (the reflector formed on the fly), then w and the stored
+ * reflector. Ap = A_loc(0, 0) and Wp = W_loc(0, 0) are the panel's top-left
+ * corners (column-major, lda / ldw >= Np; W's rows are panel-relative), tau and e
+ * the panel's entries. Columns 0 .. nbp-1 are stepped in order; after the last
+ * one, if it has a reflector (nbp - 1 < Np - 1), jm_tridiag_finish completes its
+ * w. work: jm_tridiag_column_work_doubles(Np) doubles, kept between the steps of
+ * a panel (block partials in a fixed order: deterministic). Replaces the host
+ * LAPACK dsyevd's reduction under the reference's eigh (lmm/eigen.py:28-128). */
+int64_t jm_tridiag_column_work_doubles(int64_t N);
+int jm_tridiag_column(int64_t Np, int i, double* Ap, int64_t lda, double* Wp, int64_t ldw,
+                      double* tau, double* e, double* work, int64_t work_doubles, void* stream);
+
 /* Rank-one update D + rho w w^T, D = diag(dl) strictly ascending, rho > 0, no w_i
  * = 0 (the deflated problem of a divide-and-conquer merge; K eigen-pairs). Root j
  * = dorg[j] + tau[j] (lam[j] holds the sum) with dorg[j] the pole dl[j] or

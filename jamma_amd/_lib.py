@@ -145,6 +145,9 @@ _SIGS = {
                          c_void_p, c_int64, c_double, c_void_p, c_int64, c_int, c_void_p, c_int64,
                          c_void_p], c_int),
     "jm_tridiag_larfg": ([c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "jm_tridiag_column_work_doubles": ([c_int64], c_int64),
+    "jm_tridiag_column": ([c_int64, c_int, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p,
+                           c_void_p, c_int64, c_void_p], c_int),
     "jm_tridiag_panel_update": ([c_int64, c_int, c_void_p, c_int64, c_void_p, c_int64, c_void_p,
                                  c_void_p], c_int),
     "jm_tridiag_panel_work_doubles": ([c_int64], c_int64),

@@ -368,15 +368,16 @@ __device__ __forceinline__ double reduce16_lanes(double (&v)[16], int lane) {
   return t;
 }
 
-__global__ void __launch_bounds__(256) k_symv_lower(int64_t N, const double* __restrict__ A,
-                                                    int64_t lda, const double* __restrict__ x,
-                                                    int chunk, int nchmax,
-                                                    double* __restrict__ P,
-                                                    double* __restrict__ Y1) {
-  __shared__ double red[4][4][64];  // [2 parities x 2 tiles][wave][lane]
+// The work of symv block (J, c) (see above); vx.val(i) gives x_i and vx.first()
+// x_0, so that a caller can form x on the fly (the fused column step scales the
+// reflector this way; its x_0 is the unit entry). Rows of off-diagonal tiles are
+// never 0.
+template <class XF>
+__device__ __forceinline__ void symv_tile(int64_t N, const double* __restrict__ A, int64_t lda,
+                                          const XF& vx, int chunk, int nchmax, int64_t J, int c,
+                                          double* __restrict__ P, double* __restrict__ Y1,
+                                          double (*red)[4][64]) {
   const int64_t nt = (N + SV_T - 1) / SV_T;
-  const int64_t J = blockIdx.x;
-  const int c = blockIdx.y;
   const int64_t I0 = J + (int64_t)c * chunk;
   if (I0 >= nt) return;
   const int64_t I1 = (I0 + chunk < nt) ? I0 + chunk : nt;
@@ -386,9 +387,10 @@ __global__ void __launch_bounds__(256) k_symv_lower(int64_t N, const double* __r
   double xj[16], t[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    xj[k] = (col0 + k < N) ? x[col0 + k] : 0.0;
+    xj[k] = (col0 + k < N) ? vx.val(col0 + k) : 0.0;
     t[k] = 0.0;
   }
+  if (col0 == 0) xj[0] = vx.first();
   // the diagonal tile (first tile of chunk 0): symmetric from its lower half
   if (I0 == J) {
     const int64_t row = J * SV_T + lane;
@@ -420,7 +422,7 @@ __global__ void __launch_bounds__(256) k_symv_lower(int64_t N, const double* __r
       a0[k] = rok0 ? A[row0 + (col0 + k) * lda] : 0.0;
       a1[k] = rok1 ? A[row1 + (col0 + k) * lda] : 0.0;
     }
-    const double xi0 = rok0 ? x[row0] : 0.0, xi1 = rok1 ? x[row1] : 0.0;
+    const double xi0 = rok0 ? vx.val(row0) : 0.0, xi1 = rok1 ? vx.val(row1) : 0.0;
     double yr0 = 0.0, yr1 = 0.0;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
@@ -442,6 +444,22 @@ __global__ void __launch_bounds__(256) k_symv_lower(int64_t N, const double* __r
   }
   const double tot = reduce16_lanes(t, lane);
   if ((lane & 3) == 0) Y1[(J * nchmax + c) * SV_T + 16 * g + ((lane >> 2) & 15)] = tot;
+}
+
+struct PlainVec {
+  const double* x;
+  __device__ double val(int64_t i) const { return x[i]; }
+  __device__ double first() const { return x[0]; }
+};
+
+__global__ void __launch_bounds__(256) k_symv_lower(int64_t N, const double* __restrict__ A,
+                                                    int64_t lda, const double* __restrict__ x,
+                                                    int chunk, int nchmax,
+                                                    double* __restrict__ P,
+                                                    double* __restrict__ Y1) {
+  __shared__ double red[4][4][64];  // [2 parities x 2 tiles][wave][lane]
+  symv_tile(N, A, lda, PlainVec{x}, chunk, nchmax, (int64_t)blockIdx.x, (int)blockIdx.y, P, Y1,
+            red);
 }
 
 // One block per row tile R (64 rows, one per lane); wave w sums P[R][J] for
@@ -474,6 +492,287 @@ __global__ void __launch_bounds__(1024) k_symv_reduce(int64_t N, int64_t nt, int
 #pragma unroll
     for (int q = 0; q < 16; ++q) tt += part[q][lane];
     y[i] = tt;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused column step of the blocked reduction: three kernels per column instead of
+// seven (k_panel_update, k_larfg_unit, k_symv_lower, k_symv_reduce, k_panel_dots,
+// k_panel_apply, k_latrd_finish), the same LAPACK dlatrd / dlarfg arithmetic with
+// the column-wide reductions split into fixed-order block partials:
+//   k_col_a (column i, rows i..): dlatrd's finish of column i-1 (w <- tau w +
+//     alpha v, alpha = -tau/2 (tau w).v from k_col_c's partials), then the update of
+//     column i with the panel's earlier reflectors (the c = i-1 terms use the just
+//     finished w), then per block max|x| and sum (x / max)^2 of the reflector's x;
+//   k_col_b: every block combines those pairs into ||x|| and forms dlarfg's beta,
+//     tau and f = 1/(alpha - beta) itself; the symv tiles and the blocks of the panel
+//     dot products work on u = (alpha - beta, x) = v / f (the column as stored, its
+//     first entry replaced: a scale on the fly would hold 34 more registers in the
+//     symv), giving A22 u, W^T u, V^T u; block 0 stores tau, e = beta and f;
+//   k_col_c (one 64-row tile per block): y = f A22 u from the symv partials, the dots
+//     times f, w = y - V (W^T v) - W (V^T v) (stored before dlatrd's finish),
+//     v = (1, f x) written into A's column, and the block's partial of (tau w).v for
+//     the next k_col_a.
+// Panel-relative coordinates: A_loc(r, c) = Ap[r + c lda], W_loc(r, c) = Wp[r + c ldw].
+// ---------------------------------------------------------------------------
+constexpr int CA_T = 256;  // rows per k_col_a block
+
+// workspace of the fused step (doubles): symv partials, dot partials, norm pairs,
+// finish partials, scalars (f, the unscaled w at the pivot row)
+struct ColLayout {
+  int64_t sym, dot, nrm, fin, sc, total;
+};
+__host__ __device__ inline ColLayout col_layout(int64_t N) {
+  const int64_t nt = (N + SV_T - 1) / SV_T;
+  ColLayout L;
+  L.sym = 0;
+  L.dot = nt * nt * SV_T + nt * (nt / 4 + 1) * SV_T;
+  L.nrm = L.dot + ((N + PD_ROWS - 1) / PD_ROWS + 1) * 2 * PANEL_MAX;
+  L.fin = L.nrm + 2 * ((N + CA_T - 1) / CA_T + 1);
+  L.sc = L.fin + (N + SV_T - 1) / SV_T + 1;
+  L.total = L.sc + 8;
+  return L;
+}
+
+// block-wide sum / max of one value per thread, every thread gets the result
+// (fixed order: a wave butterfly, then the waves in order)
+template <int NT>
+__device__ __forceinline__ double block_sum_all(double v, double* red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int q = 0; q < NT / 64; ++q) t += red[q];
+  return t;
+}
+template <int NT>
+__device__ __forceinline__ double block_max_all(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, shfl_xor_d(v, o));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int q = 0; q < NT / 64; ++q) t = fmax(t, red[q]);
+  return t;
+}
+
+// dlatrd's finish of one element: w <- tau w, then w + alpha v
+__device__ __forceinline__ double finish_elem(double tau, double w, double alpha, double v) {
+  const double x = tau * w;
+  return x + alpha * v;
+}
+
+__global__ void __launch_bounds__(CA_T) k_col_a(int64_t len, int i, int do_norm,
+                                                double* __restrict__ Acol,
+                                                const double* __restrict__ a0, int64_t lda,
+                                                double* __restrict__ w0, int64_t ldw,
+                                                const double* __restrict__ tau_prev,
+                                                const double* __restrict__ fin_part, int n_fin,
+                                                const double* __restrict__ sc,
+                                                double* __restrict__ nrm_part) {
+  __shared__ double xs[2][PANEL_MAX];
+  __shared__ double red[CA_T / 64];
+  const int64_t r = (int64_t)blockIdx.x * CA_T + threadIdx.x;
+  double tp = 0.0, alpha = 0.0;
+  if (i > 0) {
+    tp = *tau_prev;
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n_fin; q += CA_T) s += fin_part[q];
+    alpha = -0.5 * tp * block_sum_all<CA_T>(s, red);
+    if (threadIdx.x < (unsigned)i) {
+      const int c = threadIdx.x;
+      // W_loc(i, c), A_loc(i, c); column i-1 of W is finished here (its pivot row's
+      // unscaled value comes from sc[1]: row 0 is rewritten by this kernel)
+      xs[0][c] = (c == i - 1) ? finish_elem(tp, sc[1], alpha, 1.0) : w0[(int64_t)c * ldw];
+      xs[1][c] = a0[(int64_t)c * lda];
+    }
+  }
+  __syncthreads();
+  double y = 0.0;
+  if (r < len) {
+    y = Acol[r];
+    if (i > 0) {
+      const double wf = finish_elem(tp, w0[r + (int64_t)(i - 1) * ldw], alpha,
+                                    a0[r + (int64_t)(i - 1) * lda]);
+      w0[r + (int64_t)(i - 1) * ldw] = wf;
+      double s0 = 0.0, s1 = 0.0;
+#pragma unroll 8
+      for (int c = 0; c < i; ++c) {
+        const double wc = (c == i - 1) ? wf : w0[r + (int64_t)c * ldw];
+        s0 += a0[r + (int64_t)c * lda] * xs[0][c];
+        s1 += wc * xs[1][c];
+      }
+      y -= s0 + s1;
+      Acol[r] = y;
+    }
+  }
+  if (do_norm) {  // the reflector's x: rows 2 .. len-1 of the updated column
+    const double ax = (r >= 2 && r < len) ? fabs(y) : 0.0;
+    const double mb = block_max_all<CA_T>(ax, red);
+    double q = 0.0;
+    if (mb > 0.0) {
+      const double t = ax / mb;
+      q = t * t;
+    }
+    q = block_sum_all<CA_T>(q, red);
+    if (threadIdx.x == 0) {
+      nrm_part[2 * blockIdx.x] = mb;
+      nrm_part[2 * blockIdx.x + 1] = q;
+    }
+  }
+}
+
+struct ReflVec {  // u = (u0, x): x = the column below the pivot, as stored
+  const double* x;
+  double u0;
+  __device__ double val(int64_t i) const { return x[i]; }  // i >= 1
+  __device__ double first() const { return u0; }
+};
+
+// dlarfg from the block partials (every block): beta, tau, f = 1/(alpha - beta) and
+// u0 = alpha - beta (f = 1, u0 = 1 when x = 0: H = I)
+__device__ __forceinline__ void larfg_from_parts(const double* __restrict__ nrm_part, int n_nrm,
+                                                 double alpha, double* red, double& f,
+                                                 double& u0, double& tau, double& e) {
+  double m = 0.0;
+  for (int q = threadIdx.x; q < n_nrm; q += blockDim.x) m = fmax(m, nrm_part[2 * q]);
+  const double scale = block_max_all<256>(m, red);
+  double ss = 0.0;
+  if (scale > 0.0)
+    for (int q = threadIdx.x; q < n_nrm; q += blockDim.x) {
+      const double rb = nrm_part[2 * q] / scale;
+      ss += nrm_part[2 * q + 1] * (rb * rb);
+    }
+  ss = block_sum_all<256>(ss, red);
+  const double xnorm = scale * sqrt(ss);
+  if (xnorm == 0.0) {  // H = I
+    f = 1.0;
+    u0 = 1.0;
+    tau = 0.0;
+    e = alpha;
+    return;
+  }
+  const double beta = -copysign(hypot(alpha, xnorm), alpha);
+  u0 = alpha - beta;
+  f = 1.0 / u0;
+  tau = (beta - alpha) / beta;
+  e = beta;
+}
+
+// Blocks [0, nsym): symv tile (J = b % nt, chunk c = b / nt) of y = A22 v; blocks
+// [nsym, nsym + nblk 2i): partial dot products of the panels' columns with v (as
+// k_panel_dots).
+__global__ void __launch_bounds__(256) k_col_b(
+    int64_t L, int i, const double* __restrict__ A22,
+                                               int64_t lda, const double* __restrict__ x,
+                                               const double* __restrict__ nrm_part, int n_nrm,
+                                               int64_t nt, int chunk, int nchmax, int64_t nsym,
+                                               int nblk, const double* __restrict__ ap,
+                                               const double* __restrict__ wp, int64_t ldw,
+                                               double* __restrict__ P, double* __restrict__ Y1,
+                                               double* __restrict__ dpart,
+                                               double* __restrict__ tau, double* __restrict__ e,
+                                               double* __restrict__ sc) {
+  __shared__ double red[4][4][64];
+  double f, u0, tv, ev;
+  larfg_from_parts(nrm_part, n_nrm, x[0], &red[0][0][0], f, u0, tv, ev);
+  __syncthreads();  // red is reused below
+  const int64_t b = blockIdx.x;
+  if (b == 0 && threadIdx.x == 0) {
+    *tau = tv;
+    *e = ev;
+    sc[0] = f;
+  }
+  const ReflVec vx{x, u0};
+  if (b < nsym) {
+    symv_tile(L, A22, lda, vx, chunk, nchmax, b % nt, (int)(b / nt), P, Y1, red);
+    return;
+  }
+  const int64_t d = b - nsym;
+  const int q = (int)(d / nblk);
+  const int64_t bx = d % nblk;
+  const double* col = q < i ? wp + (int64_t)q * ldw : ap + (int64_t)(q - i) * lda;
+  const int64_t r0 = bx * PD_ROWS;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < PD_ROWS / 256; ++k) {
+    const int64_t r = r0 + threadIdx.x + 256 * k;
+    if (r < L) s += col[r] * (r == 0 ? vx.first() : vx.val(r));
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[0][0][threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    dpart[bx * 2 * PANEL_MAX + q] = (red[0][0][0] + red[0][0][1]) + (red[0][0][2] + red[0][0][3]);
+}
+
+// One block (16 waves) per 64-row tile R of the column step (L rows).
+__global__ void __launch_bounds__(1024) k_col_c(int64_t L, int i, int64_t nt, int chunk,
+                                                int nchmax, int nblk,
+                                                const double* __restrict__ P,
+                                                const double* __restrict__ Y1,
+                                                const double* __restrict__ dpart,
+                                                const double* __restrict__ ap, int64_t lda,
+                                                const double* __restrict__ wp, int64_t ldw,
+                                                double* __restrict__ x,
+                                                double* __restrict__ w,
+                                                double* __restrict__ gram_col,
+                                                const double* __restrict__ tau,
+                                                double* __restrict__ sc,
+                                                double* __restrict__ fin_part) {
+  __shared__ double part[16][64];
+  __shared__ double ts[2 * PANEL_MAX];
+  const int64_t R = blockIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // t1 = W^T v (c < i), t2 = V^T v (i <= c < 2i): the dot partials in block order
+  if (threadIdx.x < (unsigned)(2 * i)) {
+    double s = 0.0;
+    for (int bb = 0; bb < nblk; ++bb) s += dpart[(int64_t)bb * 2 * PANEL_MAX + threadIdx.x];
+    s *= sc[0];  // the dots were taken with u = v / f
+    ts[threadIdx.x] = s;
+    if (R == 0 && threadIdx.x >= (unsigned)i) gram_col[threadIdx.x - i] = s;
+  }
+  // y: the symv partials of row tile R (as k_symv_reduce)
+  const double* pr = P + R * nt * SV_T + lane;
+  double s = 0.0;
+  int64_t J = wv;
+  for (; J + 48 <= R; J += 64) {
+    const double a0 = pr[J * SV_T], a1 = pr[(J + 16) * SV_T];
+    const double a2 = pr[(J + 32) * SV_T], a3 = pr[(J + 48) * SV_T];
+    s += (a0 + a1) + (a2 + a3);
+  }
+  for (; J <= R; J += 16) s += pr[J * SV_T];
+  __syncthreads();  // ts complete
+  // the panel corrections of row 64 R + lane, columns c = wv, wv + 16, ...
+  const double f = sc[0];
+  const int64_t r = R * SV_T + lane;
+  double cs = 0.0;
+  if (r < L)
+    for (int c = wv; c < i; c += 16)
+      cs += ap[r + (int64_t)c * lda] * ts[c] + wp[r + (int64_t)c * ldw] * ts[i + c];
+  part[wv][lane] = f * s - cs;
+  __syncthreads();
+  if (wv == 0) {
+    double yu = 0.0;
+    const int nch = (int)((nt - R + chunk - 1) / chunk);
+    for (int c = 0; c < nch; ++c) yu += Y1[(R * nchmax + c) * SV_T + lane];
+    double y = f * yu;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) y += part[q][lane];
+    double pf = 0.0;
+    if (r < L) {
+      const double v = (r == 0) ? 1.0 : x[r] * f;
+      w[r] = y;
+      x[r] = v;
+      pf = ((*tau) * y) * v;
+      if (r == 0) sc[1] = y;
+    }
+    pf = wave_sum(pf);
+    if (lane == 0) fin_part[R] = pf;
   }
 }
 
@@ -804,6 +1103,54 @@ extern "C" int jm_tridiag_finish(int64_t len, const double* tau, double* w, cons
   if (len == 0) return JM_OK;
   JM_CHECK_ARG(tau && w && v, "null pointer");
   hipLaunchKernelGGL(k_latrd_finish, dim3(1), dim3(1024), 0, (hipStream_t)stream, len, tau, w, v);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int64_t jm_tridiag_column_work_doubles(int64_t N) {
+  return N < 1 ? 0 : col_layout(N).total;
+}
+
+extern "C" int jm_tridiag_column(int64_t Np, int i, double* Ap, int64_t lda, double* Wp,
+                                 int64_t ldw, double* tau, double* e, double* work,
+                                 int64_t work_doubles, void* stream) {
+  JM_CHECK_ARG(Np >= 1 && i >= 0 && i < PANEL_MAX && i < Np, "bad sizes");
+  JM_CHECK_ARG(Ap && Wp && tau && e && work, "null pointer");
+  JM_CHECK_ARG(lda >= Np && ldw >= Np, "lda / ldw < Np");
+  const ColLayout Ly = col_layout(Np);
+  JM_CHECK_ARG(work_doubles >= Ly.total, "workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  double* sc = work + Ly.sc;
+  const int64_t len = Np - i;      // column i from row i
+  const bool refl = i < Np - 1;    // column i has a reflector
+  const int na = (int)((len + CA_T - 1) / CA_T);
+  const int n_fin = i > 0 ? (int)((len + SV_T - 1) / SV_T) : 0;  // column i-1's row tiles
+  hipLaunchKernelGGL(k_col_a, dim3((unsigned)na), dim3(CA_T), 0, st, len, i, refl ? 1 : 0,
+                     Ap + i + (int64_t)i * lda, Ap + i, lda, Wp + i, ldw,
+                     i > 0 ? tau + i - 1 : nullptr, work + Ly.fin, n_fin, sc, work + Ly.nrm);
+  JM_CHECK_LAUNCH();
+  if (!refl) return JM_OK;
+  const int64_t L = Np - i - 1;
+  int64_t nt;
+  int chunk, nchmax;
+  symv_shape(L, nt, chunk, nchmax);
+  const int64_t ntN = (Np + SV_T - 1) / SV_T;
+  double* P = work + Ly.sym;
+  double* Y1 = P + ntN * ntN * SV_T;
+  const int nblk = (int)((L + PD_ROWS - 1) / PD_ROWS);
+  const int64_t nsym = nt * nchmax;
+  const int64_t nb_total = nsym + (int64_t)nblk * 2 * i;
+  JM_CHECK_ARG(nb_total < (1LL << 31), "grid too large");
+  double* x = Ap + (i + 1) + (int64_t)i * lda;  // A_loc(i + 1, i): alpha, then x
+  hipLaunchKernelGGL(k_col_b, dim3((unsigned)nb_total), dim3(256), 0, st, L, i,
+                     Ap + (i + 1) + (int64_t)(i + 1) * lda, lda, x, work + Ly.nrm, na, nt, chunk,
+                     nchmax, nsym, nblk, Ap + (i + 1), Wp + (i + 1), ldw, P, Y1, work + Ly.dot,
+                     tau + i, e + i, sc);
+  JM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_col_c, dim3((unsigned)nt), dim3(1024), 0, st, L, i, nt, chunk, nchmax,
+                     nblk, P, Y1, work + Ly.dot, Ap + (i + 1), lda, Wp + (i + 1), ldw, x,
+                     Wp + (i + 1) + (int64_t)i * ldw, Wp + (int64_t)i * ldw, tau + i, sc,
+                     work + Ly.fin);
   JM_CHECK_LAUNCH();
   return JM_OK;
 }

@@ -162,12 +162,27 @@ def _tridiagonalize(A: torch.Tensor, nb: int, progress=None):
     def pw(r, c):
         return wb + 8 * (c * ld + r)
 
+    # three launches per column (jm_tridiag_column); JM_TRIDIAG_UNFUSED=1 takes the
+    # seven-launch sequence of the separate entry points (for comparisons)
+    fused = os.environ.get("JM_TRIDIAG_UNFUSED", "0") != "1"
+    if fused:
+        ncol = int(_lib.lib.jm_tridiag_column_work_doubles(n))
+        col_work = torch.empty(max(ncol, 1), dtype=f64, device=dev)
     gram = []
     next_report = 8192
     for p0 in range(0, n, nb):
         Np = n - p0
         nbp = min(nb, Np)
-        for i in range(nbp):
+        if fused:
+            tp, ep = tau.data_ptr() + 8 * p0, e.data_ptr() + 8 * p0
+            for i in range(nbp):
+                _lib.call("jm_tridiag_column", Np, i, pa(p0, p0), ld, pw(0, 0), ld, tp, ep,
+                          col_work.data_ptr(), ncol, st)
+            il = nbp - 1
+            if il < Np - 1:  # the panel's last reflector: its w still needs dlatrd's finish
+                _lib.call("jm_tridiag_finish", Np - il - 1, tau.data_ptr() + 8 * (p0 + il),
+                          pw(il + 1, il), pa(p0 + il + 1, p0 + il), st)
+        for i in range(0 if fused else nbp):
             gc = p0 + i
             if i > 0:  # apply the panel's earlier reflectors to column i
                 _lib.call("jm_tridiag_panel_update", Np - i, i, pa(gc, p0), ld, pw(i, 0), ld,
