@@ -515,7 +515,7 @@ __global__ void __launch_bounds__(1024) k_symv_reduce(int64_t N, int64_t nt, int
 //     the next k_col_a.
 // Panel-relative coordinates: A_loc(r, c) = Ap[r + c lda], W_loc(r, c) = Wp[r + c ldw].
 // ---------------------------------------------------------------------------
-constexpr int CA_T = 256;  // rows per k_col_a block
+constexpr int CA_T = 64;  // rows per k_col_a block
 
 // workspace of the fused step (doubles): symv partials, dot partials, norm pairs,
 // finish partials, scalars (f, the unscaled w at the pivot row)
@@ -566,60 +566,74 @@ __device__ __forceinline__ double finish_elem(double tau, double w, double alpha
   return x + alpha * v;
 }
 
-__global__ void __launch_bounds__(CA_T) k_col_a(int64_t len, int i, int do_norm,
-                                                double* __restrict__ Acol,
-                                                const double* __restrict__ a0, int64_t lda,
-                                                double* __restrict__ w0, int64_t ldw,
-                                                const double* __restrict__ tau_prev,
-                                                const double* __restrict__ fin_part, int n_fin,
-                                                const double* __restrict__ sc,
-                                                double* __restrict__ nrm_part) {
+// 64 rows per block (lane = row), the panel columns c split over the 4 waves
+// (c = wave, wave + 4, ...), their sums added in wave order.
+__global__ void __launch_bounds__(256) k_col_a(int64_t len, int i, int do_norm,
+                                               double* __restrict__ Acol,
+                                               const double* __restrict__ a0, int64_t lda,
+                                               double* __restrict__ w0, int64_t ldw,
+                                               const double* __restrict__ tau_prev,
+                                               const double* __restrict__ fin_part, int n_fin,
+                                               const double* __restrict__ sc,
+                                               double* __restrict__ nrm_part) {
   __shared__ double xs[2][PANEL_MAX];
-  __shared__ double red[CA_T / 64];
-  const int64_t r = (int64_t)blockIdx.x * CA_T + threadIdx.x;
+  __shared__ double red[4];
+  __shared__ double part[4][64];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t r = (int64_t)blockIdx.x * CA_T + lane;
   double tp = 0.0, alpha = 0.0;
   if (i > 0) {
     tp = *tau_prev;
     double s = 0.0;
-    for (int q = threadIdx.x; q < n_fin; q += CA_T) s += fin_part[q];
-    alpha = -0.5 * tp * block_sum_all<CA_T>(s, red);
+    for (int q = threadIdx.x; q < n_fin; q += 256) s += fin_part[q];
+    alpha = -0.5 * tp * block_sum_all<256>(s, red);
     if (threadIdx.x < (unsigned)i) {
       const int c = threadIdx.x;
       // W_loc(i, c), A_loc(i, c); column i-1 of W is finished here (its pivot row's
-      // unscaled value comes from sc[1]: row 0 is rewritten by this kernel)
+      // value before the finish comes from sc[1]: row 0 is rewritten by this kernel)
       xs[0][c] = (c == i - 1) ? finish_elem(tp, sc[1], alpha, 1.0) : w0[(int64_t)c * ldw];
       xs[1][c] = a0[(int64_t)c * lda];
     }
   }
   __syncthreads();
+  double s = 0.0;
+  if (i > 0 && r < len) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int c = wv; c < i; c += 4) {
+      double wc = w0[r + (int64_t)c * ldw];
+      if (c == i - 1) {
+        wc = finish_elem(tp, wc, alpha, a0[r + (int64_t)c * lda]);
+        w0[r + (int64_t)c * ldw] = wc;
+      }
+      s0 += a0[r + (int64_t)c * lda] * xs[0][c];
+      s1 += wc * xs[1][c];
+    }
+    s = s0 + s1;
+  }
+  part[wv][lane] = s;
+  __syncthreads();
+  if (wv != 0) return;
   double y = 0.0;
   if (r < len) {
     y = Acol[r];
     if (i > 0) {
-      const double wf = finish_elem(tp, w0[r + (int64_t)(i - 1) * ldw], alpha,
-                                    a0[r + (int64_t)(i - 1) * lda]);
-      w0[r + (int64_t)(i - 1) * ldw] = wf;
-      double s0 = 0.0, s1 = 0.0;
-#pragma unroll 8
-      for (int c = 0; c < i; ++c) {
-        const double wc = (c == i - 1) ? wf : w0[r + (int64_t)c * ldw];
-        s0 += a0[r + (int64_t)c * lda] * xs[0][c];
-        s1 += wc * xs[1][c];
-      }
-      y -= s0 + s1;
+      y -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
       Acol[r] = y;
     }
   }
   if (do_norm) {  // the reflector's x: rows 2 .. len-1 of the updated column
     const double ax = (r >= 2 && r < len) ? fabs(y) : 0.0;
-    const double mb = block_max_all<CA_T>(ax, red);
+    double mb = ax;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) mb = fmax(mb, shfl_xor_d(mb, o));
     double q = 0.0;
     if (mb > 0.0) {
       const double t = ax / mb;
       q = t * t;
     }
-    q = block_sum_all<CA_T>(q, red);
-    if (threadIdx.x == 0) {
+    q = wave_sum(q);
+    if (lane == 0) {
       nrm_part[2 * blockIdx.x] = mb;
       nrm_part[2 * blockIdx.x + 1] = q;
     }
@@ -740,10 +754,11 @@ __global__ void __launch_bounds__(1024) k_col_c(int64_t L, int i, int64_t nt, in
   const double* pr = P + R * nt * SV_T + lane;
   double s = 0.0;
   int64_t J = wv;
-  for (; J + 48 <= R; J += 64) {
-    const double a0 = pr[J * SV_T], a1 = pr[(J + 16) * SV_T];
-    const double a2 = pr[(J + 32) * SV_T], a3 = pr[(J + 48) * SV_T];
-    s += (a0 + a1) + (a2 + a3);
+  for (; J + 112 <= R; J += 128) {  // eight independent loads in flight
+    double a[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = pr[(J + 16 * k) * SV_T];
+    s += ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   }
   for (; J <= R; J += 16) s += pr[J * SV_T];
   __syncthreads();  // ts complete
@@ -1125,7 +1140,7 @@ extern "C" int jm_tridiag_column(int64_t Np, int i, double* Ap, int64_t lda, dou
   const bool refl = i < Np - 1;    // column i has a reflector
   const int na = (int)((len + CA_T - 1) / CA_T);
   const int n_fin = i > 0 ? (int)((len + SV_T - 1) / SV_T) : 0;  // column i-1's row tiles
-  hipLaunchKernelGGL(k_col_a, dim3((unsigned)na), dim3(CA_T), 0, st, len, i, refl ? 1 : 0,
+  hipLaunchKernelGGL(k_col_a, dim3((unsigned)na), dim3(256), 0, st, len, i, refl ? 1 : 0,
                      Ap + i + (int64_t)i * lda, Ap + i, lda, Wp + i, ldw,
                      i > 0 ? tau + i - 1 : nullptr, work + Ly.fin, n_fin, sc, work + Ly.nrm);
   JM_CHECK_LAUNCH();
