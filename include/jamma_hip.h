@@ -107,6 +107,26 @@ int jm_kinship_syrk_i8(const int8_t* gT0, int64_t ld_gT, int64_t n, int64_t n_sn
                        void* stream);
 
 /*
+ * The same S += X0 X0^T (replacing the same reference code) from a 2-bit packed
+ * operand, a quarter of the int8 operand's bytes; widened to int8 in registers,
+ * exact int32 accumulation on int8 MFMA, S bit-identical to jm_kinship_syrk_i8.
+ *   jm_pack_gram_2bit: SNP-major int8 genotypes g (n_snps rows of ld_g, -1 =
+ *     missing, as jm_decode_bed writes them) -> SNP columns col0 .. col0 + n_snps
+ *     of gP (col0 a multiple of 16): sample-major rows of ldp bytes (ldp a multiple
+ *     of 64, 4 SNPs per byte), round_up(n, 256) rows of which rows >= n must be zero
+ *     (never written). The codes from col0 + n_snps up to the next multiple of 256
+ *     are written as 0; require round_up(col0 + n_snps, 256) <= 4 * ldp. Within a
+ *     256-SNP group the 16-SNP words are interleaved for the GEMM's fragment reads.
+ *   jm_kinship_syrk_2bit: S (n x n, lower triangle) += X0 X0^T over SNP columns
+ *     [0, n_snps) of gP; every 256-SNP group touched must be fully written
+ *     (columns of the gaps between packed tiles read as zero genotypes).
+ */
+int jm_pack_gram_2bit(const int8_t* g, int64_t ld_g, int64_t n_snps, int64_t n, uint8_t* gP,
+                      int64_t ldp, int64_t col0, void* stream);
+int jm_kinship_syrk_2bit(const uint8_t* gP, int64_t ldp, int64_t n, int64_t n_snps, int32_t* S,
+                         void* stream);
+
+/*
  * Per-tile mean-imputation / centring terms, accumulated over SNP tiles
  * (the algebra of impute_and_center, kinship/missing.py:48-61):
  *   mu[j]    = float64 nanmean of SNP j (host computes from jm_decode_bed counts)

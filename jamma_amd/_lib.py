@@ -102,6 +102,9 @@ _SIGS = {
     "jm_center_columns_f64": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p], c_int),
     "jm_impute_rows_f64": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p], c_int),
     "jm_kinship_syrk_i8": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p], c_int),
+    "jm_pack_gram_2bit": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int64,
+                           c_void_p], c_int),
+    "jm_kinship_syrk_2bit": ([c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p], c_int),
     "jm_kinship_miss_ws_len": ([c_int64, c_int64, c_int64], c_int64),
     "jm_kinship_impute_terms": ([c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_void_p,
                                  c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int64, c_void_p],

@@ -16,6 +16,7 @@
 namespace jm {
 
 typedef int i4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // S += X0 X0^T on int8 MFMA
@@ -187,6 +188,209 @@ __global__ void __launch_bounds__(64 * NW) k_syrk_i8(const int8_t* __restrict__ 
             S[i * n + j] += acc[ti][tj][r];
         }
       }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// S += X0 X0^T from 2-bit packed genotypes
+// ---------------------------------------------------------------------------
+// The int8 GEMM above moves 32 KB per 64 SNPs through L2 and LDS for every
+// 256x256 tile and is bound by that stream (without its MFMAs it takes 90 % of
+// its time). Hard-call genotypes with missing calls as 0 need two bits, so this
+// form keeps the operand 2-bit packed in HBM, L2 and LDS (a quarter of the bytes)
+// and widens it to int8 in registers right before the MFMAs.
+//
+// Packed operand gP (jm_pack_gram_2bit): sample-major rows of ldp bytes; SNP
+// group b (SNPs 256b .. 256b + 255) of a row is 64 bytes, and its 16-SNP group
+// Q (SNPs 256b + 16Q ..) is the 32-bit word at byte 64b + 16 (Q & 3) + 4 (Q >> 2)
+// holding the code of SNP 256b + 16Q + e in bits 2e, 2e + 1 (0, 1, 2; missing 0).
+// So the 16 bytes of a row at 16 lk hold the four words of groups lk, lk + 4,
+// lk + 8, lk + 12: the SNPs that lane group lk contributes to each of the four
+// 64-deep MFMA steps of the group (sub-step s = Q >> 2). A word widens to the
+// four dwords (w >> 2k) & 0x03030303, k = 0..3: byte j of dword k is the code of
+// SNP 16Q + 4j + k, a fixed permutation of the word's 16 K positions that the A
+// and B fragments share, so the int32 products are those of the int8 GEMM.
+//
+// Kernel: the same 256x256 lower tiles, XCD patches, 8 waves of 128 x 64 and
+// K-split tail as k_syrk_i8, 4 LDS stages of 2 x 16 KB, each stage 256 SNPs deep
+// (one barrier per 256 SNPs); per stage a wave reads its 12 fragments once
+// (ds_read_b128) and runs 4 sub-steps of 32 MFMAs, widening a fragment's word
+// (7 VALU) before the MFMAs that use it.
+//
+// Measured at n = 10,000 over 99,952 SNPs (one launch): 3.43 ms against 4.45 ms for
+// k_syrk_i8, bit-identical S. Tried: widening once per block into an int8 LDS
+// stage from register-staged packed loads (4.25 ms: each lane's 16-byte loads
+// touch 64 rows per instruction, and the int8 stage adds 32 KB of LDS writes per
+// 64 SNPs), and a fixed MFMA/VALU interleave order (3.56 ms).
+constexpr int SP_G = 256;  // SNPs per stage (64 bytes of a packed row)
+
+__device__ __forceinline__ i4_t widen_codes(uint32_t w) {
+  const uint32_t m = 0x03030303u;
+  return (i4_t){(int)(w & m), (int)((w >> 2) & m), (int)((w >> 4) & m), (int)((w >> 6) & m)};
+}
+
+template <bool ATOMIC>
+__global__ void __launch_bounds__(512) k_syrk_p2(const uint8_t* __restrict__ gP, int64_t ldp,
+                                                 int64_t n, int64_t K, int32_t* __restrict__ S,
+                                                 int n_full, int split) {
+  constexpr int WR = 8, WC = 4, NWC = 4;
+  constexpr int CPO = 2;        // 1 KB copies per operand per wave per stage
+  constexpr int CPW = 2 * CPO;  // copies per wave per stage
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[SY_NBUF * SY_STAGE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nb = (int)((n + SY_B - 1) / SY_B);
+  int t, part = 0;
+  if (!ATOMIC) {
+    const int x = blockIdx.x % 8, j = blockIdx.x / 8;
+    const int base = n_full / 8, rem = n_full % 8;
+    t = x * base + (x < rem ? x : rem) + j;
+  } else {
+    t = n_full + blockIdx.x / split;
+    part = blockIdx.x % split;
+  }
+  int rb, cb;
+  group_tile(t, nb, rb, cb);
+  const int nk_all = (int)((K + SP_G - 1) / SP_G);
+  const int kb0 = ATOMIC ? (int)((int64_t)nk_all * part / split) : 0;
+  const int nk = (ATOMIC ? (int)((int64_t)nk_all * (part + 1) / split) : nk_all) - kb0;
+  const int64_t i0 = (int64_t)rb * SY_B, j0 = (int64_t)cb * SY_B;
+  const int wr = (wave / NWC) * 16 * WR, wc = (wave % NWC) * 16 * WC;
+  const int lr = lane & 15, lk = lane >> 4;
+  // a copy moves 16 rows x 64 B: lane -> row d_r, LDS slot d_s holding the row's
+  // 16-byte chunk d_s ^ ((row >> 1) & 3)
+  const int d_r = lane >> 2, d_s = lane & 3;
+  const uint32_t voff = (uint32_t)(d_r * ldp + 16 * (d_s ^ ((d_r >> 1) & 3)));
+  auto issue = [&](uint8_t* stage, int64_t kb) {
+#pragma unroll
+    for (int op = 0; op < 2; ++op) {
+      const int64_t r0 = op ? j0 : i0;
+#pragma unroll
+      for (int u = 0; u < CPO; ++u) {
+        const int q = wave * CPO + u;  // rows 16q .. 16q+15 of the panel
+        lds_copy16_s(gP + (r0 + 16 * q) * ldp + kb * (SP_G / 4), voff,
+                     stage + op * SY_PANEL + q * 1024);
+      }
+    }
+  };
+  i4_t acc[WR][WC];
+#pragma unroll
+  for (int a = 0; a < WR; ++a)
+#pragma unroll
+    for (int b = 0; b < WC; ++b) acc[a][b] = (i4_t){0, 0, 0, 0};
+#pragma unroll
+  for (int st = 0; st < SY_NBUF - 1; ++st)
+    if (st < nk) issue(lds + st * SY_STAGE, kb0 + st);
+  int cur = 0;  // LDS stage of K step kt
+  for (int kt = 0; kt < nk; ++kt) {
+    if (kt + SY_NBUF - 2 < nk)
+      wait_vmcnt<CPW * (SY_NBUF - 2)>();
+    else
+      wait_vmcnt<0>();
+    __syncthreads();  // every wave's copies of step kt landed; step kt-1's stage is free
+    if (kt + SY_NBUF - 1 < nk) {
+      const int nxt = cur == 0 ? SY_NBUF - 1 : cur - 1;
+      issue(lds + nxt * SY_STAGE, kb0 + kt + SY_NBUF - 1);
+    }
+    const uint8_t* base = lds + cur * SY_STAGE;
+    u4_t ar[WR], br[WC];  // the four sub-steps' words of each fragment
+#pragma unroll
+    for (int ti = 0; ti < WR; ++ti) {
+      const int r = wr + 16 * ti + lr;
+      ar[ti] = *reinterpret_cast<const u4_t*>(base + r * 64 + 16 * (lk ^ ((r >> 1) & 3)));
+    }
+#pragma unroll
+    for (int tj = 0; tj < WC; ++tj) {
+      const int r = wc + 16 * tj + lr;
+      br[tj] = *reinterpret_cast<const u4_t*>(base + SY_PANEL + r * 64 + 16 * (lk ^ ((r >> 1) & 3)));
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      i4_t bq[WC];
+#pragma unroll
+      for (int tj = 0; tj < WC; ++tj) bq[tj] = widen_codes(br[tj][s]);
+#pragma unroll
+      for (int ti = 0; ti < WR; ++ti) {
+        const i4_t aq = widen_codes(ar[ti][s]);
+#pragma unroll
+        for (int tj = 0; tj < WC; ++tj)
+          acc[ti][tj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aq, bq[tj], acc[ti][tj], 0, 0, 0);
+      }
+    }
+    cur = cur + 1 == SY_NBUF ? 0 : cur + 1;
+  }
+#pragma unroll
+  for (int ti = 0; ti < WR; ++ti) {
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int tj = 0; tj < WC; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
+        const int64_t j = j0 + wc + tj * 16 + lr;
+        if (i < n && j < n) {
+          if (ATOMIC)
+            atomicAdd(&S[i * n + j], acc[ti][tj][r]);
+          else
+            S[i * n + j] += acc[ti][tj][r];
+        }
+      }
+  }
+}
+
+// Pack a SNP-major int8 tile (n_snps rows of ld_g, -1 = missing) into the 2-bit
+// operand at SNP column `col0` (a multiple of 16) of gP. One block = 64 samples x
+// 256 SNPs of the tile, staged through LDS (16-byte loads); each thread then
+// builds 4 words (16 SNPs of one sample each). Words from col0 up to the end of
+// the 256-SNP group holding the tile's last SNP are all written (codes past the
+// tile are 0): the GEMM reads whole groups, which then hold zeros past the last
+// SNP until a later tile fills them.
+__global__ void __launch_bounds__(256) k_pack_gram_2bit(const int8_t* __restrict__ g, int64_t ld_g,
+                                                        int64_t n_snps, int64_t n,
+                                                        uint8_t* __restrict__ gP, int64_t ldp,
+                                                        int64_t col0, int64_t n_words) {
+  __shared__ uint32_t tile[256][17];  // SNP row x 64 samples (16 words of 4 code bytes)
+  const int64_t s0 = (int64_t)blockIdx.x * 256, i0 = (int64_t)blockIdx.y * 64;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int idx = t + 256 * q;
+    const int row = idx >> 2, ch = (idx & 3) * 16;  // SNP row, first sample of 16
+    const int64_t s = s0 + row, i = i0 + ch;
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+    if (s < n_snps && i < n) {
+      const int8_t* src = g + s * ld_g + i;
+      if (i + 16 <= n && ((uintptr_t)src % 16) == 0) {
+        const uint4 v = *reinterpret_cast<const uint4*>(src);
+        w[0] = v.x;
+        w[1] = v.y;
+        w[2] = v.z;
+        w[3] = v.w;
+      } else {
+        for (int e = 0; e < 16 && i + e < n; ++e)
+          w[e >> 2] |= (uint32_t)(uint8_t)src[e] << (8 * (e & 3));
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) tile[row][(ch >> 2) + e] = w[e];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int idx = t + 256 * q;
+    const int smp = idx & 63, grp = idx >> 6;  // sample, 16-SNP group of the block tile
+    const int64_t i = i0 + smp;
+    const int64_t wq = (s0 >> 4) + grp;  // word index relative to col0
+    if (i >= n || wq >= n_words) continue;
+    uint32_t word = 0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const uint32_t b = (tile[16 * grp + e][smp >> 2] >> (8 * (smp & 3))) & 0xFFu;
+      const uint32_t code = (b & 0x80u) ? 0u : (b & 3u);  // missing -> 0
+      word |= code << (2 * e);
+    }
+    const int64_t Q = (col0 >> 4) + wq;  // 16-SNP group of the row
+    const int64_t off = (Q >> 4) * 64 + 16 * (Q & 3) + 4 * ((Q >> 2) & 3);
+    *reinterpret_cast<uint32_t*>(gP + i * ldp + off) = word;
   }
 }
 
@@ -533,6 +737,56 @@ extern "C" int jm_kinship_syrk_i8(const int8_t* gT0, int64_t ld_gT, int64_t n, i
   if (total > n_full) {
     hipLaunchKernelGGL((k_syrk_i8<true, 8>), dim3((total - n_full) * split), dim3(512), 0, st,
                        gT0, ld_gT, n, n_snps, S, n_full, split);
+    JM_CHECK_LAUNCH();
+  }
+  return JM_OK;
+}
+
+extern "C" int jm_pack_gram_2bit(const int8_t* g, int64_t ld_g, int64_t n_snps, int64_t n,
+                                 uint8_t* gP, int64_t ldp, int64_t col0, void* stream) {
+  JM_CHECK_ARG(n >= 1 && n_snps >= 0 && ld_g >= n && col0 >= 0, "bad sizes");
+  if (n_snps == 0) return JM_OK;
+  JM_CHECK_ARG(g && gP && ((uintptr_t)gP % 4) == 0, "null or misaligned pointer");
+  JM_CHECK_ARG(col0 % 16 == 0, "col0 must be a multiple of 16");
+  JM_CHECK_ARG(ldp % 64 == 0, "ldp must be a multiple of 64 bytes");
+  const int64_t end = (col0 + n_snps + SP_G - 1) / SP_G * SP_G;  // SNP columns written
+  JM_CHECK_ARG(end <= 4 * ldp, "ldp too small: %lld bytes for %lld SNP columns", (long long)ldp,
+               (long long)end);
+  const int64_t n_words = (end - col0) / 16;
+  const dim3 grid((unsigned)ceil_div(n_words, 16), (unsigned)ceil_div(n, 64));
+  hipLaunchKernelGGL(k_pack_gram_2bit, grid, dim3(256), 0, (hipStream_t)stream, g, ld_g, n_snps, n,
+                     gP, ldp, col0, n_words);
+  JM_CHECK_LAUNCH();
+  return JM_OK;
+}
+
+extern "C" int jm_kinship_syrk_2bit(const uint8_t* gP, int64_t ldp, int64_t n, int64_t n_snps,
+                                    int32_t* S, void* stream) {
+  JM_CHECK_ARG(n >= 1 && n_snps >= 0, "bad sizes");
+  if (n_snps == 0) return JM_OK;
+  JM_CHECK_ARG(gP && S, "null pointer");
+  JM_CHECK_ARG(ldp % 64 == 0 && ((uintptr_t)gP % 16) == 0 &&
+                   (n_snps + SP_G - 1) / SP_G * (SP_G / 4) <= ldp,
+               "gP rows must be 16-byte aligned, ldp a multiple of 64 covering the SNP groups");
+  JM_CHECK_ARG(16 * ldp < (1LL << 32), "ldp too large");
+  const int64_t nb = (n + SY_B - 1) / SY_B;
+  JM_CHECK_ARG(nb * (nb + 1) / 2 < (1LL << 28), "n too large");
+  const int total = (int)(nb * (nb + 1) / 2);
+  const int nk = (int)((n_snps + SP_G - 1) / SP_G);  // 256-SNP stages
+  const int tail = total % 256;
+  int split = tail ? 256 / tail : 1;
+  if (split > nk / 4) split = nk / 4;
+  if (split < 1) split = 1;
+  const int n_full = split > 1 ? total - tail : total;
+  hipStream_t st = (hipStream_t)stream;
+  if (n_full > 0) {
+    hipLaunchKernelGGL(k_syrk_p2<false>, dim3(n_full), dim3(512), 0, st, gP, ldp, n, n_snps, S,
+                       n_full, 1);
+    JM_CHECK_LAUNCH();
+  }
+  if (total > n_full) {
+    hipLaunchKernelGGL(k_syrk_p2<true>, dim3((total - n_full) * split), dim3(512), 0, st, gP, ldp,
+                       n, n_snps, S, n_full, split);
     JM_CHECK_LAUNCH();
   }
   return JM_OK;

@@ -222,29 +222,30 @@ __device__ __forceinline__ double bf16_to_f64(uint16_t h) {
 // ---------------------------------------------------------------------------
 // BF16x3 rotation
 // ---------------------------------------------------------------------------
-// Per 32-deep K step a 256x256 block stages the three UT planes (3 x 256 x 32
-// bf16 = 48 KB) and the genotype selectors (256 x 64 B = 16 KB) from global
-// memory into LDS with global_load_lds_dwordx4 (no register staging), double
-// buffered. LDS rows are 64 bytes, XOR-swizzled in 16-byte chunks so that the
-// ds_read_b128 fragment reads of 16 lanes hit 64 distinct banks: chunk c of
-// row r (A) or column j (genotypes) sits at c ^ ((r >> 2) & 3) (the 4-bank
-// group of a chunk is 4 (r & 3) + c).
+// Per 32-deep K step a 256x256 block stages the UT planes (2 or 3 x 256 x 32
+// bf16 = 16 KB each) and the genotype codes (256 x 32 B = 8 KB) from global
+// memory into LDS with global_load_lds_dwordx4 (no register staging). UT rows
+// in LDS are 64 bytes, XOR-swizzled in 16-byte chunks so that the ds_read_b128
+// fragment reads of 16 lanes hit distinct banks: chunk c of row r sits at
+// c ^ ((r >> 2) & 3) (the 4-bank group of a chunk is 4 (r & 3) + c). Genotype
+// rows are 32 bytes, 8-byte chunk c of SNP j at c ^ ((j >> 2) & 3), so the
+// ds_read_b64 reads of 16 lanes cover all 32 bank pairs.
 constexpr int V3_BM = 256, V3_BN = 256, V3_BK = 32;
 constexpr int V3_APLANE = V3_BM * V3_BK * 2;      // 16 KB per plane
-constexpr int V3_BBYTES = V3_BN * V3_BK * 2;      // 16 KB: two selector bytes per genotype
-// LDS stage of NPL planes + genotypes: 48 KB (NPL = 2) or 64 KB (NPL = 3)
+constexpr int V3_BBYTES = V3_BN * V3_BK;          // 8 KB: one code byte per genotype
+// LDS stage of NPL planes + genotypes: 40 KB (NPL = 2) or 56 KB (NPL = 3)
 template <int NPL>
 constexpr int v3_stage() { return NPL * V3_APLANE + V3_BBYTES; }
 
 // Byte-permute widening of 8 genotypes to bf16. The 2-bit genotype code c (0, 1,
-// 2, missing = 3) indexes 4-entry byte tables; v_perm_b32 picks the low byte of
-// each bf16 from table byte c and the high byte from table byte c + 4, so the
-// packed tile stores, per genotype, the selector pair (c, c + 4) and one dword
-// holds the selectors of two genotypes. Per-column tables:
+// 2, missing = 3, one byte per genotype in the packed tile) indexes 4-entry byte
+// tables: one v_perm_b32 looks up the low bytes of four genotypes' bf16 values,
+// one the high bytes, and two more interleave them into the four bf16 (two
+// dwords). Per-column tables:
 //   X1: lo bytes {00, 80, 00, lo(m1)}, hi bytes {00, 3F, 40, hi(m1)}  (0, 1.0, 2.0, m1)
 //   X2: lo bytes {00, 00, 00, lo(m2)}, hi bytes {00, 00, 00, hi(m2)}
 // mm = m1 | m2 << 16. The 4 table words of a column are made once per launch;
-// 8 permute VALU per 8 genotypes for X1 and X2.
+// 16 permute VALU per 8 genotypes for X1 and X2.
 struct widen_tab {  // byte tables of one column: X1 {lo, hi} and X2 {lo, hi}
   uint32_t t1lo, t1hi, t2lo, t2hi;
 };
@@ -256,14 +257,21 @@ __device__ __forceinline__ widen_tab widen_tables(uint32_t mm) {
   t.t2hi = mm & 0xFF000000u;                                     // hi(m2) in byte 3
   return t;
 }
-__device__ __forceinline__ void widen8s(const uint32_t (&sel)[4], const widen_tab& t, bf16x8_t& x1,
+// four codes (bytes of w) -> four bf16 from the tables (lo, hi): two dwords
+__device__ __forceinline__ void widen4(uint32_t w, uint32_t tlo, uint32_t thi, uint32_t& d0,
+                                       uint32_t& d1) {
+  const uint32_t plo = __builtin_amdgcn_perm(0u, tlo, w);  // byte i = tlo byte code_i
+  const uint32_t phi = __builtin_amdgcn_perm(0u, thi, w);
+  d0 = __builtin_amdgcn_perm(phi, plo, 0x05010400u);  // lo0 hi0 lo1 hi1
+  d1 = __builtin_amdgcn_perm(phi, plo, 0x07030602u);  // lo2 hi2 lo3 hi3
+}
+__device__ __forceinline__ void widen8c(const uint32_t (&w)[2], const widen_tab& t, bf16x8_t& x1,
                                         bf16x8_t& x2) {
   uint32_t w1[4], w2[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    w1[q] = __builtin_amdgcn_perm(t.t1hi, t.t1lo, sel[q]);
-    w2[q] = __builtin_amdgcn_perm(t.t2hi, t.t2lo, sel[q]);
-  }
+  widen4(w[0], t.t1lo, t.t1hi, w1[0], w1[1]);
+  widen4(w[1], t.t1lo, t.t1hi, w1[2], w1[3]);
+  widen4(w[0], t.t2lo, t.t2hi, w2[0], w2[1]);
+  widen4(w[1], t.t2lo, t.t2hi, w2[2], w2[3]);
   memcpy(&x1, w1, 16);
   memcpy(&x2, w2, 16);
 }
@@ -277,13 +285,13 @@ __device__ __forceinline__ void widen8s(const uint32_t (&sel)[4], const widen_ta
 //   UT planes: plane-major, then panel P (256 rows), then K step s (32
 //     columns): a 16 KB block of 256 rows x 64 B, row r's 16-byte chunk c at
 //     r * 64 + 16 * (c ^ ((r >> 2) & 3)). Rows >= n and columns >= n are 0.
-//   genotypes: column block B (256 SNPs), then K step s (32 samples): a 16 KB
-//     block of 256 SNPs x 64 B, SNP j's chunk c (samples 8c .. 8c + 7, two
-//     selector bytes (code, code + 4) each; code = decoded value & 3, missing
-//     0xFF -> 3) at j * 64 + 16 * (c ^ ((j >> 2) & 3)). SNPs >= n_snps and
-//     samples >= n hold code 0.
+//   genotypes: column block B (256 SNPs), then K step s (32 samples): an 8 KB
+//     block of 256 SNPs x 32 B, SNP j's 8-byte chunk c (samples 8c .. 8c + 7,
+//     one code byte each; code = decoded value & 3, missing 0xFF -> 3) at
+//     j * 32 + 8 * (c ^ ((j >> 2) & 3)). SNPs >= n_snps and samples >= n hold
+//     code 0.
 // The swizzles are those of the LDS stage, so a block copies verbatim.
-constexpr int64_t UTP_BLOCK = 16384, GP_BLOCK = 16384;
+constexpr int64_t UTP_BLOCK = 16384, GP_BLOCK = 8192;
 
 __global__ void k_pack_ut(const double* __restrict__ UT, int64_t ld_u, int64_t n, int64_t npan,
                           int64_t nks, int nplanes, uint16_t* __restrict__ planes) {
@@ -317,20 +325,19 @@ __global__ void k_pack_geno(const int8_t* __restrict__ g, int64_t ld_g,
                             int64_t n, int64_t n_snps, int64_t nks, uint8_t* __restrict__ gp,
                             uint32_t* __restrict__ mm) {
   const int64_t nblk = (n_snps + 255) / 256;
-  const int64_t chunks = nblk * nks * 1024;  // 16-byte chunks
+  const int64_t chunks = nblk * nks * 1024;  // 8-byte chunks (8 genotypes each)
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < chunks;
        q += (int64_t)gridDim.x * blockDim.x) {
     const int64_t blk = q >> 10;
-    const int pos = (int)(q & 1023);
+    const int pos = (int)(q & 1023);  // chunk slot within the 8 KB block
     const int j = pos >> 2, cs = pos & 3;
     const int c = cs ^ ((j >> 2) & 3);
     const int64_t B = blk / nks, sk = blk % nks;
     const int64_t snp = B * 256 + j, k0 = sk * 32 + 8 * c;
-    // selector pairs (code, code + 4) of 8 genotypes; padding reads as code 0
-    uint64_t codes = 0;  // byte e = code of sample k0 + e
+    uint64_t codes = 0;  // byte e = code of sample k0 + e; padding reads as code 0
     if (snp < n_snps) {
       const int8_t* row = g + (col ? (int64_t)col[snp] : snp) * ld_g;
-      if (k0 + 8 <= n) {  // rows are 16-byte aligned and k0 % 8 == 0: one 8-byte load
+      if (k0 + 8 <= n) {  // rows are 8-byte aligned and k0 % 8 == 0: one 8-byte load
         codes = *reinterpret_cast<const uint64_t*>(row + k0) & 0x0303030303030303ull;
       } else {
 #pragma unroll
@@ -343,16 +350,48 @@ __global__ void k_pack_geno(const int8_t* __restrict__ g, int64_t ld_g,
         mm[snp] = (uint32_t)h1 | ((uint32_t)f32_to_bf16_rne((float)(mu - bf16_to_f64(h1))) << 16);
       }
     }
-    uint32_t w[4];  // dword h: bytes (c_2h, c_2h + 4, c_2h+1, c_2h+1 + 4)
-#pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      const uint32_t two = (uint32_t)(codes >> (16 * h)) & 0x0303u;  // codes of samples 2h, 2h+1
-      const uint32_t spread = (two & 0x3u) | ((two & 0x300u) << 8);  // bytes 0 and 2
-      w[h] = spread | ((spread + 0x00040004u) << 8);
-    }
-    memcpy(gp + q * 16, w, 16);
+    *reinterpret_cast<uint64_t*>(gp + q * 8) = codes;
   }
 }
+
+// Operands of one K step as seen from a wave. ap / bp: this lane's copy sources
+// (plane 0 of row panel rb, genotype block cb); a stage holds NPL UT planes then
+// the genotype codes. Each wave copies NPL x 2 KB of UT and 1 KB of codes.
+template <int NPL>
+struct v3_ops {
+  const uint8_t* ap;
+  const uint8_t* bp;
+  int64_t pstride;  // bytes per UT plane
+  int wave, wr, wc, lr, lk;
+  static constexpr int ADMA = NPL * 2;  // 1 KB UT copies per wave per stage
+  static constexpr int DPS = ADMA + 1;  // copies per wave per stage
+  __device__ __forceinline__ void issue(uint8_t* base, int64_t sk) const {
+#pragma unroll
+    for (int u = 0; u < ADMA; ++u) {
+      const int q = wave * ADMA + u;  // plane q >> 4, kilobyte q & 15
+      lds_copy16(ap + (q >> 4) * pstride + sk * UTP_BLOCK + (q & 15) * 1024,
+                 base + (q >> 4) * V3_APLANE + (q & 15) * 1024);
+    }
+    lds_copy16(bp + sk * GP_BLOCK + wave * 1024, base + NPL * V3_APLANE + wave * 1024);
+  }
+  // UT fragment of plane pl, 16-row block ti of the wave: A[row][8 lk .. 8 lk + 7]
+  __device__ __forceinline__ bf16x8_t a_frag(const uint8_t* base, int pl, int ti) const {
+    const int r = wr + 16 * ti + lr;
+    const int c = lk ^ ((r >> 2) & 3);
+    bf16x8_t v;
+    memcpy(&v, base + pl * V3_APLANE + r * 64 + 16 * c, 16);
+    return v;
+  }
+  // codes of samples 8 lk .. 8 lk + 7 of the wave's column 16 tj + lr (one 8-byte read)
+  __device__ __forceinline__ void g_codes(const uint8_t* base, int tj, uint32_t (&w)[2]) const {
+    const int j = wc + 16 * tj + lr;
+    const int c = lk ^ ((j >> 2) & 3);
+    uint64_t v;
+    memcpy(&v, base + NPL * V3_APLANE + j * 32 + 8 * c, 8);
+    w[0] = (uint32_t)v;
+    w[1] = (uint32_t)(v >> 32);
+  }
+};
 
 // BF16x3 rotation on packed operands: 256x256 block tile, 8 waves (two per
 // SIMD) in a grid of (16 WR) x (16 WC) wave tiles, WR x WC = 32 MFMA blocks of
@@ -360,10 +399,7 @@ __global__ void k_pack_geno(const int8_t* __restrict__ g, int64_t ld_g,
 // NBUF LDS stages with global_load_lds_dwordx4 (the copies of stages kt+1 ..
 // kt+NBUF-1 in flight while stage kt is multiplied), one barrier per 32-deep K
 // step. Genotype fragments are widened to X1 and X2 in one pass per column
-// block (the two share the code-to-byte selectors and the LDS read); each wave
-// widens the columns of its own tile, so WR = 8 (2 x 4 waves of 128 x 64)
-// widens each column twice per block where WR = 4 (4 x 2 waves of 64 x 128)
-// widens it four times. Per K step:
+// block. Per K step and accumulator:
 //   NPL = 2 (UT = hi + lo):        hi x X2, hi x X1, lo x X1   (3 products)
 //   NPL = 3 (UT = hi + mid + lo):  hi x X2, hi x X1, mid x X1, lo x X1
 template <int NPL, int NBUF, int WR>
@@ -375,8 +411,7 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
   constexpr int NWC = 16 / WC;  // waves across the 256 columns
   static_assert(WR * WC == 32 && (8 / NWC) * WR * 16 == V3_BM, "wave grid");
   constexpr int STAGE = v3_stage<NPL>();
-  constexpr int ADMA = NPL * 2;    // 1 KB UT copies per wave per stage (NPL x 16 KB / 8 waves)
-  constexpr int DPS = ADMA + 2;    // copies per wave per stage (+ 2 KB of genotype selectors)
+  constexpr int DPS = v3_ops<NPL>::DPS;
   static_assert(NBUF >= 2 && NBUF * STAGE <= 160 * 1024, "LDS stages");
   __shared__ __attribute__((aligned(1024))) uint8_t lds[NBUF * STAGE];
   const int tid = threadIdx.x;
@@ -385,28 +420,19 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
   int rb, cb;
   xcd_tile_2d(blockIdx.x, nrb, ncb, rb, cb);
   const int64_t i0 = (int64_t)rb * V3_BM, s0 = (int64_t)cb * V3_BN;
-  const int wr = (wave / NWC) * 16 * WR, wc = (wave % NWC) * 16 * WC;
-  const int lr = lane & 15, lk = lane >> 4;
-  // copy sources: plane pl's block (rb, s) and genotype block (cb, s)
-  const uint8_t* ap = (const uint8_t*)planes + (int64_t)rb * nks * UTP_BLOCK + lane * 16;
-  const uint8_t* bp = gp + (int64_t)cb * nks * GP_BLOCK + lane * 16;
-  const int64_t pstride = per_plane * 2;  // bytes per plane
-  auto issue = [&](uint8_t* base, int64_t sk) {
-#pragma unroll
-    for (int u = 0; u < ADMA; ++u) {
-      const int q = wave * ADMA + u;  // plane q >> 4, kilobyte q & 15
-      lds_copy16(ap + (q >> 4) * pstride + sk * UTP_BLOCK + (q & 15) * 1024,
-                 base + (q >> 4) * V3_APLANE + (q & 15) * 1024);
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-      lds_copy16(bp + sk * GP_BLOCK + (2 * wave + u) * 1024,
-                 base + NPL * V3_APLANE + (2 * wave + u) * 1024);
-  };
+  v3_ops<NPL> op;
+  op.ap = (const uint8_t*)planes + (int64_t)rb * nks * UTP_BLOCK + lane * 16;
+  op.bp = gp + (int64_t)cb * nks * GP_BLOCK + lane * 16;
+  op.pstride = per_plane * 2;
+  op.wave = wave;
+  op.wr = (wave / NWC) * 16 * WR;
+  op.wc = (wave % NWC) * 16 * WC;
+  op.lr = lane & 15;
+  op.lk = lane >> 4;
   widen_tab tab[WC];
 #pragma unroll
   for (int tj = 0; tj < WC; ++tj) {
-    const int64_t sc = s0 + wc + 16 * tj + lr;
+    const int64_t sc = s0 + op.wc + 16 * tj + op.lr;
     tab[tj] = widen_tables(sc < n_cols ? mmv[sc] : 0u);
   }
   f4_t acc[WR][WC];
@@ -415,43 +441,25 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
 #pragma unroll
     for (int b = 0; b < WC; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
   auto compute = [&](const uint8_t* base) {
-    auto gsel = [&](int tj, uint32_t (&sel)[4]) {  // selectors of 8 samples of column j
-      const int j = wc + 16 * tj + lr;
-      const int c = lk ^ ((j >> 2) & 3);
-      u32x4_t v;  // one 16-byte LDS read
-      memcpy(&v, base + NPL * V3_APLANE + j * 64 + 16 * c, 16);
-      sel[0] = v[0];
-      sel[1] = v[1];
-      sel[2] = v[2];
-      sel[3] = v[3];
-    };
     // every LDS read of the step is issued up front (hi plane, the genotype
-    // selectors, then the other planes), so the later planes' fragments arrive
-    // while the hi plane's MFMAs run instead of stalling each plane's first MFMA
+    // codes, then the other planes), so the later planes' fragments arrive
+    // while the hi plane's MFMAs run
     bf16x8_t af[NPL][WR];
-    uint32_t sel[WC][4];
+    uint32_t gw[WC][2];
 #pragma unroll
-    for (int ti = 0; ti < WR; ++ti) {
-      const int r = wr + 16 * ti + lr;
-      const int c = lk ^ ((r >> 2) & 3);
-      memcpy(&af[0][ti], base + r * 64 + 16 * c, 16);
-    }
+    for (int ti = 0; ti < WR; ++ti) af[0][ti] = op.a_frag(base, 0, ti);
 #pragma unroll
-    for (int tj = 0; tj < WC; ++tj) gsel(tj, sel[tj]);
+    for (int tj = 0; tj < WC; ++tj) op.g_codes(base, tj, gw[tj]);
 #pragma unroll
     for (int pl = 1; pl < NPL; ++pl)
 #pragma unroll
-      for (int ti = 0; ti < WR; ++ti) {
-        const int r = wr + 16 * ti + lr;
-        const int c = lk ^ ((r >> 2) & 3);
-        memcpy(&af[pl][ti], base + pl * V3_APLANE + r * 64 + 16 * c, 16);
-      }
+      for (int ti = 0; ti < WR; ++ti) af[pl][ti] = op.a_frag(base, pl, ti);
     // hi plane first: X1 and X2 of a column block from one widening
     bf16x8_t x1[WC];
 #pragma unroll
     for (int tj = 0; tj < WC; ++tj) {
       bf16x8_t x2;
-      widen8s(sel[tj], tab[tj], x1[tj], x2);
+      widen8c(gw[tj], tab[tj], x1[tj], x2);
 #pragma unroll
       for (int ti = 0; ti < WR; ++ti)
         acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][ti], x2, acc[ti][tj], 0, 0, 0);
@@ -467,7 +475,7 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
   };
 #pragma unroll
   for (int st = 0; st < NBUF - 1; ++st)
-    if (st < nks) issue(lds + st * STAGE, st);
+    if (st < nks) op.issue(lds + st * STAGE, st);
   int cur = 0;  // LDS stage of K step kt
   for (int kt = 0; kt < nks; ++kt) {
     // In flight: this wave's copies of K steps kt .. min(kt + NBUF - 2, nks - 1), issued
@@ -479,7 +487,7 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
     __syncthreads();  // every wave's copies of step kt landed; step kt-1's stage is free
     if (kt + NBUF - 1 < nks) {
       const int nxt = cur == 0 ? NBUF - 1 : cur - 1;  // (kt + NBUF - 1) % NBUF = stage of kt-1
-      issue(lds + nxt * STAGE, kt + NBUF - 1);
+      op.issue(lds + nxt * STAGE, kt + NBUF - 1);
     }
     compute(lds + cur * STAGE);
     cur = cur + 1 == NBUF ? 0 : cur + 1;
@@ -490,8 +498,160 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_packed(
     for (int tj = 0; tj < WC; ++tj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
-        const int64_t sc = s0 + wc + tj * 16 + lr;
+        const int64_t i = i0 + op.wr + ti * 16 + 4 * op.lk + r;
+        const int64_t sc = s0 + op.wc + tj * 16 + op.lr;
+        if (i < M && sc < n_cols) Z[i * ldz + sc] = acc[ti][tj][r];
+      }
+}
+
+// The two-plane rotation with its operand copies two K steps ahead and its LDS
+// reads software-pipelined behind the MFMAs. Same block tile, wave grid (2 x 4
+// waves of 128 x 64), operands, products and summation order per accumulator as
+// k_rotate_bf16x3_packed<2, *, 8>, so Z is the same bit for bit. What changes:
+//  * four LDS stages of 40 KB (the 8 KB genotype block stores one code byte per
+//    genotype): in the middle of K step kt (after row block 6) the block waits for
+//    step kt + 1 and copies step kt + 3 into the stage step kt - 1 used, so a copy
+//    has two K steps to land (the copies' latency under load is about one step's
+//    MFMA time);
+//  * a wave walks its 8 row blocks in order and holds the UT fragments of only two
+//    of them: the reads of row block ti + 1 are issued before the 12 MFMAs of row
+//    block ti (instruction groups fixed with sched_group_barrier), and during row
+//    block 7 the next step's codes and first UT fragments are read and widened
+//    into the other of two X register sets (the K loop is unrolled by two), so a
+//    step starts on operands already in registers.
+// Per row block the products are issued hi x X2, hi x X1, lo x X1, each over the 4
+// column blocks (consecutive MFMAs write different accumulators).
+__global__ void __launch_bounds__(512) k_rotate_bf16x3_pipe4(
+    const uint16_t* __restrict__ planes, int64_t per_plane, const uint8_t* __restrict__ gp,
+    const uint32_t* __restrict__ mmv, int64_t M, int64_t n_cols, int64_t nks,
+    float* __restrict__ Z, int64_t ldz) {
+  constexpr int NPL = 2, WR = 8, WC = 4, NWC = 4, NBUF = 4;
+  constexpr int STAGE = v3_stage<NPL>();
+  constexpr int DPS = v3_ops<NPL>::DPS;
+  static_assert(NBUF * STAGE <= 160 * 1024, "LDS stages");
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[NBUF * STAGE];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nrb = (int)((M + V3_BM - 1) / V3_BM), ncb = (int)((n_cols + V3_BN - 1) / V3_BN);
+  int rb, cb;
+  xcd_tile_2d(blockIdx.x, nrb, ncb, rb, cb);
+  const int64_t i0 = (int64_t)rb * V3_BM, s0 = (int64_t)cb * V3_BN;
+  v3_ops<NPL> op;
+  op.ap = (const uint8_t*)planes + (int64_t)rb * nks * UTP_BLOCK + lane * 16;
+  op.bp = gp + (int64_t)cb * nks * GP_BLOCK + lane * 16;
+  op.pstride = per_plane * 2;
+  op.wave = wave;
+  op.wr = (wave / NWC) * 16 * WR;
+  op.wc = (wave % NWC) * 16 * WC;
+  op.lr = lane & 15;
+  op.lk = lane >> 4;
+  widen_tab tab[WC];
+#pragma unroll
+  for (int tj = 0; tj < WC; ++tj) {
+    const int64_t sc = s0 + op.wc + 16 * tj + op.lr;
+    tab[tj] = widen_tables(sc < n_cols ? mmv[sc] : 0u);
+  }
+  f4_t acc[WR][WC];
+#pragma unroll
+  for (int a = 0; a < WR; ++a)
+#pragma unroll
+    for (int b = 0; b < WC; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
+
+  // prologue: steps 0, 1, 2 in flight; step 0's codes and first row block read
+#pragma unroll
+  for (int st = 0; st < NBUF - 1; ++st)
+    if (st < nks) op.issue(lds + st * STAGE, st);
+  if (nks > 2)
+    wait_vmcnt<2 * DPS>();
+  else if (nks > 1)
+    wait_vmcnt<DPS>();
+  else
+    wait_vmcnt<0>();
+  __syncthreads();
+  bf16x8_t af[2][NPL];  // rolling pair of row blocks
+  bf16x8_t xa1[WC], xa2[WC], xb1[WC], xb2[WC];  // X of even / odd K steps
+  {
+    uint32_t gw[WC][2];
+#pragma unroll
+    for (int tj = 0; tj < WC; ++tj) op.g_codes(lds, tj, gw[tj]);
+#pragma unroll
+    for (int pl = 0; pl < NPL; ++pl) af[0][pl] = op.a_frag(lds, pl, 0);
+#pragma unroll
+    for (int tj = 0; tj < WC; ++tj) widen8c(gw[tj], tab[tj], xa1[tj], xa2[tj]);
+  }
+  // one K step on X (x1, x2); widens the next step's X into (y1, y2)
+  auto step = [&](int64_t kt, int cur, bf16x8_t (&x1)[WC], bf16x8_t (&x2)[WC],
+                  bf16x8_t (&y1)[WC], bf16x8_t (&y2)[WC]) {
+    const int nxt = cur == NBUF - 1 ? 0 : cur + 1;
+    const uint8_t* bc = lds + cur * STAGE;
+    const uint8_t* bn = lds + nxt * STAGE;
+    auto row_block = [&](int ti) {
+      const int s = ti & 1;
+#pragma unroll
+      for (int tj = 0; tj < WC; ++tj)
+        acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[s][0], x2[tj], acc[ti][tj], 0, 0, 0);
+#pragma unroll
+      for (int pl = 0; pl < NPL; ++pl)
+#pragma unroll
+        for (int tj = 0; tj < WC; ++tj)
+          acc[ti][tj] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[s][pl], x1[tj], acc[ti][tj], 0, 0, 0);
+    };
+#pragma unroll
+    for (int ti = 0; ti < WR - 1; ++ti) {
+#pragma unroll
+      for (int pl = 0; pl < NPL; ++pl) af[(ti & 1) ^ 1][pl] = op.a_frag(bc, pl, ti + 1);
+      row_block(ti);
+    }
+    static_assert(WR == 8 && WC == 4 && NPL == 2, "the instruction group sizes below");
+#define JM_ROW_BLOCK                                              \
+  __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); /* DS reads */ \
+  __builtin_amdgcn_sched_group_barrier(0x008, 12, 0) /* MFMAs */
+    JM_ROW_BLOCK; JM_ROW_BLOCK; JM_ROW_BLOCK; JM_ROW_BLOCK; JM_ROW_BLOCK; JM_ROW_BLOCK; JM_ROW_BLOCK;
+#undef JM_ROW_BLOCK
+    // step kt + 1 landed for every wave (step kt + 2 may be in flight); nobody reads
+    // the stage of step kt - 1 any more: step kt + 3 goes there
+    if (kt + 2 < nks)
+      wait_vmcnt<DPS>();
+    else
+      wait_vmcnt<0>();
+    __syncthreads();
+    if (kt + 3 < nks) op.issue(lds + (nxt + 2) % NBUF * STAGE, kt + 3);
+    // row block 7 with the next step's operands read and widened behind it (after
+    // the last step these reads are harmless and unused)
+    uint32_t gw[WC][2];
+#pragma unroll
+    for (int tj = 0; tj < WC; ++tj) op.g_codes(bn, tj, gw[tj]);
+#pragma unroll
+    for (int pl = 0; pl < NPL; ++pl) af[0][pl] = op.a_frag(bn, pl, 0);
+    row_block(WR - 1);
+#pragma unroll
+    for (int tj = 0; tj < WC; ++tj) widen8c(gw[tj], tab[tj], y1[tj], y2[tj]);
+    __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);  // DS reads: codes, next row block
+    __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);  // MFMAs interleaved with
+    __builtin_amdgcn_sched_group_barrier(0x002, 16, 0); // the widening
+    __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+    __builtin_amdgcn_sched_group_barrier(0x002, 16, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+    __builtin_amdgcn_sched_group_barrier(0x002, 32, 0);
+  };
+  int cur = 0;  // LDS stage of K step kt
+  int64_t kt = 0;
+  for (; kt + 1 < nks; kt += 2) {
+    step(kt, cur, xa1, xa2, xb1, xb2);
+    cur = cur == NBUF - 1 ? 0 : cur + 1;
+    step(kt + 1, cur, xb1, xb2, xa1, xa2);
+    cur = cur == NBUF - 1 ? 0 : cur + 1;
+  }
+  if (kt < nks) step(kt, cur, xa1, xa2, xb1, xb2);
+#pragma unroll
+  for (int ti = 0; ti < WR; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < WC; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + op.wr + ti * 16 + 4 * op.lk + r;
+        const int64_t sc = s0 + op.wc + tj * 16 + op.lr;
         if (i < M && sc < n_cols) Z[i * ldz + sc] = acc[ti][tj][r];
       }
 }
@@ -655,7 +815,7 @@ extern "C" int jm_pack_genotypes(const int8_t* g, int64_t ld_g, const int32_t* c
   JM_CHECK_ARG(g && mean && gp && mm && ((uintptr_t)gp % 16) == 0, "null or misaligned pointer");
   JM_CHECK_ARG(ld_g % 8 == 0 && ((uintptr_t)g % 8) == 0, "genotype rows must be 8-byte aligned");
   const int64_t nks = (n + 31) / 32;
-  const int64_t chunks = ((n_snps + 255) / 256) * nks * 512;
+  const int64_t chunks = ((n_snps + 255) / 256) * nks * 1024;
   const unsigned grid = (unsigned)(chunks < 65536 ? (chunks + 255) / 256 : 256 * 64);
   hipLaunchKernelGGL(k_pack_geno, dim3(grid), dim3(256), 0, (hipStream_t)stream, g, ld_g, col_idx,
                      mean, n, n_snps, nks, gp, mm);
@@ -676,10 +836,17 @@ extern "C" int jm_rotate_bf16x3(const uint16_t* planes, int nplanes, const uint8
   const int64_t nks = (n + 31) / 32;
   const int64_t per_plane = jm_packed_ut_bytes(n, nplanes) / (2 * nplanes);  // bf16 per plane
   const int nrb = ceil_div(n, V3_BM), ncb = ceil_div(n_snps, V3_BN);
-  if (nplanes == 2)  // 2 x 48 KB stages, 128 x 64 wave tiles (measured fastest at n = 10k)
+  static const bool unpiped = [] {  // JM_ROTATE_UNPIPED=1: the kernel without read pipelining
+    const char* e = getenv("JM_ROTATE_UNPIPED");
+    return e && e[0] == '1';
+  }();
+  if (nplanes == 2 && !unpiped)  // 4 x 40 KB stages, copies 2 steps ahead, reads pipelined
+    hipLaunchKernelGGL(k_rotate_bf16x3_pipe4, dim3(nrb * ncb), dim3(512), 0, (hipStream_t)stream,
+                       planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
+  else if (nplanes == 2)  // 2 x 40 KB stages, 128 x 64 wave tiles
     hipLaunchKernelGGL((k_rotate_bf16x3_packed<2, 2, 8>), dim3(nrb * ncb), dim3(512), 0,
                        (hipStream_t)stream, planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
-  else  // 2 x 64 KB stages, 64 x 128 wave tiles (fewer registers live with three planes)
+  else  // 2 x 56 KB stages, 64 x 128 wave tiles (fewer registers live with three planes)
     hipLaunchKernelGGL((k_rotate_bf16x3_packed<3, 2, 4>), dim3(nrb * ncb), dim3(512), 0,
                        (hipStream_t)stream, planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
   JM_CHECK_LAUNCH();

@@ -4,8 +4,10 @@ Drop-in for src/jamma/kinship/compute.py:
   * compute_centered_kinship   :93-226  (in-memory genotypes)
   * compute_kinship_streaming   :502-690 (two passes over the PLINK file)
   * compute_loco_kinship(_streaming) :363-499, :693-916 (LOCO by subtraction)
-The 0/1/2 calls are packed to int8 and X0 X0^T is accumulated exactly in
-int32 (jm_kinship_syrk_i8); mean imputation and centring are applied as exact
+The 0/1/2 calls (missing as 0) are packed to 2 bits per genotype and X0 X0^T is
+accumulated exactly in int32 on int8 MFMA (jm_kinship_syrk_2bit; JM_KINSHIP_INT8=1
+selects the int8-operand form jm_kinship_syrk_i8, same S); mean imputation and
+centring are applied as exact
 FP64 low-rank / sparse corrections (jm_kinship_impute_terms,
 jm_kinship_finalize). Per-SNP statistics come from jm_decode_bed counts.
 """
@@ -13,6 +15,7 @@ jm_kinship_finalize). Per-SNP statistics come from jm_decode_bed counts.
 from __future__ import annotations
 
 import logging
+import os
 import warnings
 from collections.abc import Iterator
 from pathlib import Path
@@ -36,11 +39,13 @@ logger = logging.getLogger("jamma_amd")
 class KinshipAccumulator:
     """Device accumulators S (int32), u, C (float64) and s0 over SNP tiles.
 
-    The int8 GEMM operand of incoming tiles is staged (sample-major, missing calls
-    0, rows padded to a multiple of 256) in one buffer of up to ``stage_bytes``;
-    jm_kinship_syrk_i8 runs once per full buffer and when ``S`` is read, so the
-    reference's 10,000-SNP batches cost one large launch instead of many small ones.
-    The imputation terms u and C are applied per tile."""
+    The GEMM operand of incoming tiles (sample-major, missing calls 0, rows padded
+    to a multiple of 256) is staged in one buffer of up to ``stage_bytes``: 2-bit
+    packed (jm_pack_gram_2bit; each tile starts on a 16-SNP boundary, the gap reads
+    as zero genotypes) or, with JM_KINSHIP_INT8=1, int8. The GEMM runs once per full
+    buffer and when ``S`` is read, so the reference's 10,000-SNP batches cost one
+    large launch instead of many small ones. The imputation terms u and C are
+    applied per tile."""
 
     def __init__(self, n: int, device=None, stage_bytes: int = 1 << 31,
                  max_snps: int | None = None):
@@ -54,9 +59,13 @@ class KinshipAccumulator:
         self.C = None
         self.s0 = 0.0
         self.p = 0
-        self.stage_cols = int(min(1 << 17, max(128, (stage_bytes // self.n_pad) // 128 * 128)))
+        self.packed = os.environ.get("JM_KINSHIP_INT8", "0") != "1"
+        # staged SNP columns: a multiple of 256 (packed: 64 bytes per 256 SNPs and row)
+        per_snp = 0.25 if self.packed else 1.0
+        cap = (1 << 19) if self.packed else (1 << 17)
+        self.stage_cols = int(min(cap, max(256, int(stage_bytes / (self.n_pad * per_snp)) // 256 * 256)))
         if max_snps is not None:
-            self.stage_cols = int(min(self.stage_cols, max(128, round_up(max_snps, 128))))
+            self.stage_cols = int(min(self.stage_cols, max(256, round_up(max_snps + 16, 256))))
         self._stage = None
         self._fill = 0
         # optional list: each SYRK launch appends its (start, end) torch.cuda.Event pair
@@ -72,25 +81,30 @@ class KinshipAccumulator:
         self.flush()
         self._S = value
 
-    def _syrk(self, gT0: torch.Tensor, ld: int, n_snps: int) -> None:
+    def _syrk(self, op: torch.Tensor, ld: int, n_snps: int) -> None:
         ev = None
         if self.syrk_timer is not None:
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             ev[0].record()
-        _lib.call("jm_kinship_syrk_i8", ptr(gT0), ld, self.n, n_snps, ptr(self._S), stream_ptr())
+        name = "jm_kinship_syrk_2bit" if self.packed else "jm_kinship_syrk_i8"
+        _lib.call(name, ptr(op), ld, self.n, n_snps, ptr(self._S), stream_ptr())
         if ev is not None:
             ev[1].record()
             self.syrk_timer.append(ev)
+
+    def _stage_ld(self) -> int:
+        return self.stage_cols // 4 if self.packed else self.stage_cols
 
     def flush(self) -> None:
         """Run the GEMM over the staged SNPs (S += X0 X0^T)."""
         if self._fill == 0:
             return
         k, cols = self._fill, self.stage_cols
-        kp = min(round_up(k, 128), cols)
-        if kp > k:  # columns left from an earlier, longer fill must read as zero
-            self._stage[:, k:kp].zero_()
-        self._syrk(self._stage, cols, k)
+        if not self.packed:
+            kp = min(round_up(k, 128), cols)
+            if kp > k:  # columns left from an earlier, longer fill must read as zero
+                self._stage[:, k:kp].zero_()
+        self._syrk(self._stage, self._stage_ld(), k)
         self._fill = 0
 
     def release(self) -> None:
@@ -111,7 +125,27 @@ class KinshipAccumulator:
         gT = torch.empty((n, ldt), dtype=torch.int8, device=self.device)
         if ldt > n_snps:
             gT[:, n_snps:].zero_()
-        if n_snps > self.stage_cols:  # larger than the staging buffer: its own launch
+        if self.packed:
+            _lib.call("jm_transpose_i8", ptr(g), n_snps, n, ld_g, ptr(gT), ldt, None, 0, st)
+            if n_snps + 16 > self.stage_cols:  # larger than the staging buffer: its own launch
+                self.flush()
+                ldp = round_up(n_snps, 256) // 4
+                op = torch.zeros((self.n_pad, ldp), dtype=torch.uint8, device=self.device)
+                _lib.call("jm_pack_gram_2bit", ptr(g), ld_g, n_snps, n, ptr(op), ldp, 0, st)
+                self._syrk(op, ldp, n_snps)
+                del op
+            else:
+                col0 = round_up(self._fill, 16)
+                if col0 + n_snps > self.stage_cols:
+                    self.flush()
+                    col0 = 0
+                if self._stage is None:  # padded rows stay zero
+                    self._stage = torch.zeros((self.n_pad, self._stage_ld()), dtype=torch.uint8,
+                                              device=self.device)
+                _lib.call("jm_pack_gram_2bit", ptr(g), ld_g, n_snps, n, ptr(self._stage),
+                          self._stage_ld(), col0, st)
+                self._fill = col0 + n_snps
+        elif n_snps > self.stage_cols:  # larger than the staging buffer: its own launch
             self.flush()
             gT0 = torch.zeros((self.n_pad, ldt), dtype=torch.int8, device=self.device)
             _lib.call("jm_transpose_i8", ptr(g), n_snps, n, ld_g, ptr(gT), ldt, ptr(gT0), ldt, st)

@@ -95,23 +95,29 @@ def test_kinship_int8_matches_oracle(n, p, miss, batch):
     assert np.array_equal(K, K.T)
 
 
+@pytest.mark.parametrize("form", ["2bit", "int8"])
 @pytest.mark.parametrize("n,tiles,stage_cols", [
     (600, [300, 300, 700, 1200, 37], 512),  # flush when full, stale columns, oversized tiles
-    (5700, [3100], None),                   # 276 tiles: a full round + K-split tail round
+    (5700, [3100, 1500], None),             # 276 tiles: a full round + K-split tail round
     (129, [70], None),                      # one tile, fewer K steps than a split needs
 ])
-def test_syrk_int32_exact(n, tiles, stage_cols):
+def test_syrk_int32_exact(n, tiles, stage_cols, form, monkeypatch):
     """S = X0 X0^T (missing calls as 0) is exact integer arithmetic: compare the lower
-    triangle with a float64 product of the same integers (exact below 2^53)."""
+    triangle with a float64 product of the same integers (exact below 2^53), for the
+    2-bit packed operand (tiles on 16-SNP boundaries, gaps of zero genotypes) and the
+    int8 one."""
     from jamma_amd.device import round_up, to_dev
     from jamma_amd.kinship.compute import KinshipAccumulator
 
+    monkeypatch.setenv("JM_KINSHIP_INT8", "1" if form == "int8" else "0")
     rng = np.random.default_rng(n)
     p = sum(tiles)
     g = rng.integers(0, 3, size=(p, n)).astype(np.int8)
     g[rng.random(g.shape) < 0.01] = -1
-    stage = 1 << 31 if stage_cols is None else stage_cols * round_up(n, 256)
+    per_snp = 0.25 if form == "2bit" else 1.0
+    stage = 1 << 31 if stage_cols is None else int(stage_cols * round_up(n, 256) * per_snp)
     acc = KinshipAccumulator(n, stage_bytes=stage)
+    assert acc.packed == (form == "2bit")
     if stage_cols is not None:
         assert acc.stage_cols == stage_cols
     ld = round_up(n, 64)
@@ -126,6 +132,43 @@ def test_syrk_int32_exact(n, tiles, stage_cols):
     ref = (x0.T @ x0).astype(np.int64)
     low = np.tril_indices(n)
     assert np.array_equal(S[low], ref[low])
+
+
+@pytest.mark.parametrize("n,n_snps,col0", [(70, 37, 48), (300, 600, 0), (257, 255, 16)])
+def test_pack_gram_2bit_layout(n, n_snps, col0):
+    """The packed operand holds SNP c's code (0/1/2, missing 0) of each sample in the
+    interleaved word position the GEMM reads, zeros up to the 256-SNP group end, and
+    leaves the columns before col0 and the padded rows untouched."""
+    from jamma_amd import _lib
+    from jamma_amd.device import ptr, round_up, stream_ptr, to_dev
+
+    rng = np.random.default_rng(n_snps)
+    ld_g = round_up(n, 64)
+    g = np.zeros((n_snps, ld_g), np.int8)
+    g[:, :n] = rng.integers(0, 3, size=(n_snps, n))
+    g[:, :n][rng.random((n_snps, n)) < 0.05] = -1
+    end = round_up(col0 + n_snps, 256)
+    ldp = round_up(end, 256) // 4 + 64
+    gP = torch.full((round_up(n, 256), ldp), 0xAB, dtype=torch.uint8, device="cuda")
+    _lib.call("jm_pack_gram_2bit", ptr(to_dev(g, torch.int8)), ld_g, n_snps, n, ptr(gP), ldp,
+              col0, stream_ptr())
+    P = gP.cpu().numpy()
+    words = P[:, : ldp // 4 * 4].view(np.uint32)  # (rows, ldp / 4) words
+    codes = np.full((P.shape[0], end), -9, np.int64)
+    for Q in range(col0 // 16, end // 16):
+        off = (Q >> 4) * 64 + 16 * (Q & 3) + 4 * ((Q >> 2) & 3)
+        w = words[:, off // 4].astype(np.int64)
+        for e in range(16):
+            codes[:, 16 * Q + e] = (w >> (2 * e)) & 3
+    want = np.zeros((n, end - col0), np.int64)
+    want[:, :n_snps] = np.where(g[:, :n] < 0, 0, g[:, :n]).T
+    assert np.array_equal(codes[:n, col0:end], want)
+    # bytes outside the written words (before col0, after the group end, padded rows)
+    untouched = np.ones_like(P, dtype=bool)
+    for Q in range(col0 // 16, end // 16):
+        off = (Q >> 4) * 64 + 16 * (Q & 3) + 4 * ((Q >> 2) & 3)
+        untouched[:n, off:off + 4] = False
+    assert np.all(P[untouched] == 0xAB)
 
 
 def test_loco_kinship_matches_oracle():
