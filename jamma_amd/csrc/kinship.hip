@@ -217,11 +217,12 @@ __global__ void __launch_bounds__(64 * NW) k_syrk_i8(const int8_t* __restrict__ 
 // (ds_read_b128) and runs 4 sub-steps of 32 MFMAs, widening a fragment's word
 // (7 VALU) before the MFMAs that use it.
 //
-// Measured at n = 10,000 over 99,952 SNPs (one launch): 3.43 ms against 4.45 ms for
-// k_syrk_i8, bit-identical S. Tried: widening once per block into an int8 LDS
-// stage from register-staged packed loads (4.25 ms: each lane's 16-byte loads
-// touch 64 rows per instruction, and the int8 stage adds 32 KB of LDS writes per
-// 64 SNPs), and a fixed MFMA/VALU interleave order (3.56 ms).
+// Measured at n = 10,000 over 99,952 SNPs (one launch): 3.38 ms against 4.45 ms for
+// k_syrk_i8 (3.43 ms without the software-pipelined widening), bit-identical S.
+// Tried: widening once per block into an int8 LDS stage from register-staged packed
+// loads (4.25 ms: each lane's 16-byte loads touch 64 rows per instruction, and the
+// int8 stage adds 32 KB of LDS writes per 64 SNPs), and a fixed MFMA/VALU
+// interleave order (3.54 ms).
 constexpr int SP_G = 256;  // SNPs per stage (64 bytes of a packed row)
 
 __device__ __forceinline__ i4_t widen_codes(uint32_t w) {
@@ -303,17 +304,29 @@ __global__ void __launch_bounds__(512) k_syrk_p2(const uint8_t* __restrict__ gP,
       const int r = wc + 16 * tj + lr;
       br[tj] = *reinterpret_cast<const u4_t*>(base + SY_PANEL + r * 64 + 16 * (lk ^ ((r >> 1) & 3)));
     }
+    // software-pipelined widening: the next row block's A fragment and (early in a
+    // sub-step) the next sub-step's B fragments are widened while MFMAs issue
+    i4_t bq[2][WC];
+#pragma unroll
+    for (int tj = 0; tj < WC; ++tj) bq[0][tj] = widen_codes(br[tj][0]);
+    i4_t aq = widen_codes(ar[0][0]);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      i4_t bq[WC];
-#pragma unroll
-      for (int tj = 0; tj < WC; ++tj) bq[tj] = widen_codes(br[tj][s]);
 #pragma unroll
       for (int ti = 0; ti < WR; ++ti) {
-        const i4_t aq = widen_codes(ar[ti][s]);
+        i4_t an = aq;
+        if (ti + 1 < WR)
+          an = widen_codes(ar[ti + 1][s]);
+        else if (s + 1 < 4)
+          an = widen_codes(ar[0][s + 1]);
+        if (ti == 1 && s + 1 < 4) {
+#pragma unroll
+          for (int tj = 0; tj < WC; ++tj) bq[(s + 1) & 1][tj] = widen_codes(br[tj][s + 1]);
+        }
 #pragma unroll
         for (int tj = 0; tj < WC; ++tj)
-          acc[ti][tj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aq, bq[tj], acc[ti][tj], 0, 0, 0);
+          acc[ti][tj] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aq, bq[s & 1][tj], acc[ti][tj], 0, 0, 0);
+        aq = an;
       }
     }
     cur = cur + 1 == SY_NBUF ? 0 : cur + 1;
