@@ -624,6 +624,103 @@ __global__ void __launch_bounds__(256) k_impute_C(const uint32_t* __restrict__ g
     if (i0 + e < n) C[k * n + i0 + e] += acc[e];
 }
 
+// The same C[k][i] += sum_j mu_j Y_ij with the missing-call SNPs of sample k taken
+// in pairs (j1, j2): the 16 possible values mu_j1 Y(c1) + mu_j2 Y(c2) of a pair (c =
+// 2-bit code, Y(3) = mu / 2) are tabulated once per wavefront in LDS, and each lane
+// then adds, per sample, the table entry that its two codes index: per pair and
+// sample two bit-field extracts, an OR, an address, an LDS read and one FP64 add,
+// where the per-SNP form spends a conversion, a compare, a 64-bit select and an FMA
+// per SNP. The table covers the missing code too, so the both-missing pairs need no
+// separate pass. A last unpaired SNP pairs with a zero-weight partner. The 16
+// entries of a table occupy 32 banks, lanes reading one entry share the access.
+constexpr int CP_U = 8;  // pairs per table batch (16 SNPs of the list)
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__global__ void __launch_bounds__(256) k_impute_C2(const uint32_t* __restrict__ gc, int64_t ldc,
+                                                   int64_t n, const double* __restrict__ mu,
+                                                   const int32_t* __restrict__ off,
+                                                   const int32_t* __restrict__ list, int64_t cap,
+                                                   double* __restrict__ C) {
+  __shared__ double tab[C_WAVES][CP_U * 16];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ni = (int)((n + C_ICHUNK - 1) / C_ICHUNK);
+  const int nk = (int)((n + C_WAVES - 1) / C_WAVES);
+  const int total = ni * nk;
+  const int x = blockIdx.x % 8, jb = blockIdx.x / 8;
+  const int base = total / 8, rem = total % 8;
+  const int t = x * base + (x < rem ? x : rem) + jb;
+  const int ic = t / nk, kg = t % nk;
+  const int64_t k = (int64_t)kg * C_WAVES + w;
+  if (k >= n) return;  // wave-uniform; the kernel has no block-level barrier
+  const c_i32* soff = (const c_i32*)off;
+  const c_i32* slist = (const c_i32*)list;
+  const int64_t i0 = (int64_t)ic * C_ICHUNK + lane * 8;
+  const bool live = i0 < n;
+  const uint32_t* gl = gc + i0 / 16;  // word of this lane's 8 samples
+  const int sh = (lane & 1) * 16;
+  int64_t q0 = soff[k], q1 = soff[k + 1];
+  if (q1 > cap) q1 = cap;
+  double* T = tab[w];
+  double acc[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) acc[e] = 0.0;
+  for (int64_t q = q0; q < q1; q += 2 * CP_U) {
+    const int np = (int)min((int64_t)CP_U, (q1 - q + 1) / 2);  // pairs in this batch
+    {  // tables: lane l fills entries 2l, 2l + 1 of pair l >> 3
+      const int u = lane >> 3;
+      if (u < np) {
+        const int64_t qa = q + 2 * u, qb = qa + 1;
+        const bool hb = qb < q1;
+        const double ma = mu[list[qa]];
+        const double mb = hb ? mu[list[qb]] : 0.0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int idx = (2 * lane + h) & 15;
+          const int c1 = idx & 3, c2 = idx >> 2;
+          const double y1 = c1 == 3 ? 0.5 * ma : (double)c1;
+          const double y2 = c2 == 3 ? 0.5 * mb : (double)c2;
+          T[u * 16 + idx] = fma(mb, y2, ma * y1);
+        }
+      }
+    }
+    // the codes of the batch's SNPs, loaded while the tables are written
+    uint32_t va[CP_U], vb[CP_U];
+#pragma unroll
+    for (int u = 0; u < CP_U; ++u) {
+      if (u < np) {
+        const int64_t qa = q + 2 * u;
+        const int64_t ja = slist[qa];
+        const int64_t jb2 = qa + 1 < q1 ? (int64_t)slist[qa + 1] : ja;
+        va[u] = live ? gl[ja * ldc] >> sh : 0u;
+        vb[u] = live ? gl[jb2 * ldc] >> sh : 0u;
+      } else {
+        va[u] = vb[u] = 0u;
+      }
+    }
+    wave_lds_sync();  // tables visible to every lane
+#pragma unroll
+    for (int u = 0; u < CP_U; ++u) {
+      if (u < np) {
+        const double* Tu = T + u * 16;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const uint32_t idx = ((va[u] >> (2 * e)) & 3u) | (((vb[u] >> (2 * e)) & 3u) << 2);
+          acc[e] += Tu[idx];
+        }
+      }
+    }
+    wave_lds_sync();  // tables read before the next batch overwrites them
+  }
+#pragma unroll
+  for (int e = 0; e < 8; ++e)
+    if (i0 + e < n) C[k * n + i0 + e] += acc[e];
+}
+
 // K for the tile pair (ti, tk), ti >= tk, of 32 x 32 blocks: both K[i][k] (rows of
 // block ti) and its mirror K[k][i] are written, and C[i][k] + C[k][i] is formed from
 // two tiles staged in LDS, so every global access is a coalesced row segment (the
@@ -855,8 +952,16 @@ extern "C" int jm_kinship_impute_terms(const int8_t* g, int64_t ld_g, const int8
     JM_CHECK_LAUNCH();
     const int64_t blocks = ((n + C_ICHUNK - 1) / C_ICHUNK) * ((n + C_WAVES - 1) / C_WAVES);
     JM_CHECK_ARG(blocks < (1LL << 31), "n too large");
-    hipLaunchKernelGGL(k_impute_C, dim3((unsigned)blocks), dim3(256), 0, st, gc, ldc, n, mu, off,
-                       list, cap, C);
+    static const bool per_snp = [] {  // JM_IMPUTE_C_PER_SNP=1: the per-SNP form (comparisons)
+      const char* e = getenv("JM_IMPUTE_C_PER_SNP");
+      return e && e[0] == '1';
+    }();
+    if (per_snp)
+      hipLaunchKernelGGL(k_impute_C, dim3((unsigned)blocks), dim3(256), 0, st, gc, ldc, n, mu,
+                         off, list, cap, C);
+    else
+      hipLaunchKernelGGL(k_impute_C2, dim3((unsigned)blocks), dim3(256), 0, st, gc, ldc, n, mu,
+                         off, list, cap, C);
     JM_CHECK_LAUNCH();
   }
   return JM_OK;
