@@ -329,7 +329,8 @@ def main() -> int:
     rot_ms = mixed["kernel_ms_per_tile"][3]  # the rotation kernel alone
     tile_snps = pf / len(tiles)
     achieved = 2.0 * n * n * tile_snps / (rot_ms * 1e-3)
-    traffic, traffic_status = hbm_traffic("k_rotate_bf16x3_packed", n, args.tile)
+    rot_kernel = "k_rotate_bf16x3_pipe4" if args.planes == 2 else "k_rotate_bf16x3_packed"
+    traffic, traffic_status = hbm_traffic(rot_kernel, n, args.tile)
     vendor = vendor_bf16_gemm_tflops(n, int(tile_snps), n)
     issue_tflops = (args.planes + 1) * achieved / 1e12
     log(f"rotation: {achieved / 1e12:.0f} TFLOP/s algorithmic, {issue_tflops:.0f} issued; "
@@ -355,7 +356,7 @@ def main() -> int:
                    "n": n, "p": p, "n_cvt": args.ncov + 1, "tile_snps": args.tile},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["bf16"] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK["bf16"], "traffic": traffic,
-                     "kernel": "k_rotate_bf16x3_packed",
+                     "kernel": rot_kernel,
                      "algorithmic": f"2 n^2 FLOP per SNP (U^T x); the split with {args.planes} "
                                     f"planes of U^T issues {args.planes + 1} bf16 MFMA products "
                                     "per algorithmic FLOP",
@@ -371,6 +372,8 @@ def main() -> int:
         else None,
         "parity": parity,
         "kinship": {"seconds": kin_s, "int8_gemm_ms": syrk_ms, "int8_gemm_launches": syrk_launches,
+                    "int8_gemm_operand": "2-bit packed genotypes widened to int8 in registers"
+                    if os.environ.get("JM_KINSHIP_INT8", "0") != "1" else "int8",
                     "int8_gemm_frac_of_peak": (float(n) * (n + 1) * pf / (syrk_ms * 1e-3))
                     / PEAK["int8"],
                     "int8_gemm_ops": "n(n+1)p (symmetric rank-p update: the ops the SYRK needs)",
@@ -391,8 +394,11 @@ def main() -> int:
     return 0
 
 
-KERNEL_SOURCES = {"k_rotate_bf16x3_packed": "rotate.hip", "k_gemm_f64_geno": "rotate.hip",
+KERNEL_SOURCES = {"k_rotate_bf16x3_pipe4": "rotate.hip", "k_rotate_bf16x3_packed": "rotate.hip",
+                  "k_gemm_f64_geno": "rotate.hip",
+                  "k_syrk_p2_main": "kinship.hip", "k_syrk_p2_tail": "kinship.hip",
                   "k_syrk_i8_main": "kinship.hip", "k_syrk_i8_tail": "kinship.hip",
+                  "k_impute_C2": "kinship.hip",
                   "k_mom_seg_f32": "moments.hip", "k_mom_fold": "moments.hip",
                   "k_lmm_grid": "lmm.hip", "k_lmm_assoc16": "lmm.hip"}
 
