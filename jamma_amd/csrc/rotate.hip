@@ -656,6 +656,96 @@ __global__ void __launch_bounds__(512) k_rotate_bf16x3_pipe4(
       }
 }
 
+// The three-plane rotation with 128 x 64 wave tiles (2 x 4 waves): a wave holds the
+// UT fragments of only two row blocks (the reads of row block ti + 1, 3 planes, issued
+// before the 16 MFMAs of row block ti), so the 3 planes fit beside the 128
+// accumulators and each column block is widened by 2 waves instead of 4. Two LDS stages
+// of 56 KB (three would exceed the 160 KB); per K step the codes and the first row
+// block are read after the barrier. Per accumulator and K step: hi x X2, hi x X1,
+// mid x X1, lo x X1, as in k_rotate_bf16x3_packed<3, *, *>: Z is the same bit for bit.
+__global__ void __launch_bounds__(512) k_rotate_bf16x3_pipe3(
+    const uint16_t* __restrict__ planes, int64_t per_plane, const uint8_t* __restrict__ gp,
+    const uint32_t* __restrict__ mmv, int64_t M, int64_t n_cols, int64_t nks,
+    float* __restrict__ Z, int64_t ldz) {
+  constexpr int NPL = 3, WR = 8, WC = 4, NWC = 4, NBUF = 2;
+  constexpr int STAGE = v3_stage<NPL>();
+  static_assert(NBUF * STAGE <= 160 * 1024, "LDS stages");
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[NBUF * STAGE];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nrb = (int)((M + V3_BM - 1) / V3_BM), ncb = (int)((n_cols + V3_BN - 1) / V3_BN);
+  int rb, cb;
+  xcd_tile_2d(blockIdx.x, nrb, ncb, rb, cb);
+  const int64_t i0 = (int64_t)rb * V3_BM, s0 = (int64_t)cb * V3_BN;
+  v3_ops<NPL> op;
+  op.ap = (const uint8_t*)planes + (int64_t)rb * nks * UTP_BLOCK + lane * 16;
+  op.bp = gp + (int64_t)cb * nks * GP_BLOCK + lane * 16;
+  op.pstride = per_plane * 2;
+  op.wave = wave;
+  op.wr = (wave / NWC) * 16 * WR;
+  op.wc = (wave % NWC) * 16 * WC;
+  op.lr = lane & 15;
+  op.lk = lane >> 4;
+  widen_tab tab[WC];
+#pragma unroll
+  for (int tj = 0; tj < WC; ++tj) {
+    const int64_t sc = s0 + op.wc + 16 * tj + op.lr;
+    tab[tj] = widen_tables(sc < n_cols ? mmv[sc] : 0u);
+  }
+  f4_t acc[WR][WC];
+#pragma unroll
+  for (int a = 0; a < WR; ++a)
+#pragma unroll
+    for (int b = 0; b < WC; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
+  op.issue(lds, 0);
+  int cur = 0;
+  for (int64_t kt = 0; kt < nks; ++kt) {
+    wait_vmcnt<0>();  // this wave's copies of step kt (the only ones in flight)
+    __syncthreads();  // every wave's copies landed; nobody reads the other stage any more
+    if (kt + 1 < nks) op.issue(lds + (cur ^ 1) * STAGE, kt + 1);
+    const uint8_t* bc = lds + cur * STAGE;
+    bf16x8_t x1[WC], x2[WC];
+    bf16x8_t af[2][NPL];
+    {
+      uint32_t gw[WC][2];
+#pragma unroll
+      for (int tj = 0; tj < WC; ++tj) op.g_codes(bc, tj, gw[tj]);
+#pragma unroll
+      for (int pl = 0; pl < NPL; ++pl) af[0][pl] = op.a_frag(bc, pl, 0);
+#pragma unroll
+      for (int tj = 0; tj < WC; ++tj) widen8c(gw[tj], tab[tj], x1[tj], x2[tj]);
+    }
+#pragma unroll
+    for (int ti = 0; ti < WR; ++ti) {
+      const int s = ti & 1;
+      if (ti + 1 < WR) {
+#pragma unroll
+        for (int pl = 0; pl < NPL; ++pl) af[s ^ 1][pl] = op.a_frag(bc, pl, ti + 1);
+      }
+#pragma unroll
+      for (int tj = 0; tj < WC; ++tj)
+        acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[s][0], x2[tj], acc[ti][tj], 0, 0, 0);
+#pragma unroll
+      for (int pl = 0; pl < NPL; ++pl)
+#pragma unroll
+        for (int tj = 0; tj < WC; ++tj)
+          acc[ti][tj] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[s][pl], x1[tj], acc[ti][tj], 0, 0, 0);
+    }
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int ti = 0; ti < WR; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < WC; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + op.wr + ti * 16 + 4 * op.lk + r;
+        const int64_t sc = s0 + op.wc + tj * 16 + op.lr;
+        if (i < M && sc < n_cols) Z[i * ldz + sc] = acc[ti][tj][r];
+      }
+}
+
 }  // namespace jm
 
 using namespace jm;
@@ -846,7 +936,10 @@ extern "C" int jm_rotate_bf16x3(const uint16_t* planes, int nplanes, const uint8
   else if (nplanes == 2)  // 2 x 40 KB stages, 128 x 64 wave tiles
     hipLaunchKernelGGL((k_rotate_bf16x3_packed<2, 2, 8>), dim3(nrb * ncb), dim3(512), 0,
                        (hipStream_t)stream, planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
-  else  // 2 x 56 KB stages, 64 x 128 wave tiles (fewer registers live with three planes)
+  else if (!unpiped)  // 2 x 56 KB stages, 128 x 64 wave tiles, rolling UT fragments
+    hipLaunchKernelGGL(k_rotate_bf16x3_pipe3, dim3(nrb * ncb), dim3(512), 0, (hipStream_t)stream,
+                       planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
+  else  // 2 x 56 KB stages, 64 x 128 wave tiles (all fragments of a step in registers)
     hipLaunchKernelGGL((k_rotate_bf16x3_packed<3, 2, 4>), dim3(nrb * ncb), dim3(512), 0,
                        (hipStream_t)stream, planes, per_plane, gp, mm, n, n_snps, nks, Z, ldz);
   JM_CHECK_LAUNCH();
