@@ -624,15 +624,41 @@ __global__ void __launch_bounds__(256) k_impute_C(const uint32_t* __restrict__ g
     if (i0 + e < n) C[k * n + i0 + e] += acc[e];
 }
 
+// gc4[j][w] (ldc4 words per SNP row): the 2-bit codes of samples 8w .. 8w + 7 of SNP j,
+// one per 4-bit slot (low slots first); samples past n are code 0.
+__global__ void __launch_bounds__(256) k_pack_codes4(const int8_t* __restrict__ g, int64_t ld_g,
+                                                     int64_t n, int64_t K,
+                                                     uint32_t* __restrict__ gc4, int64_t ldc4) {
+  const int64_t words = (n + 7) / 8;
+  const int64_t total = K * words;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e / words, wd = e % words;
+    const int64_t i0 = wd * 8;
+    uint64_t b = 0;
+    if (i0 + 8 <= n) {
+      b = *reinterpret_cast<const uint64_t*>(g + j * ld_g + i0);  // rows 8-byte aligned
+    } else {
+      for (int q = 0; i0 + q < n; ++q) b |= (uint64_t)(uint8_t)g[j * ld_g + i0 + q] << (8 * q);
+    }
+    uint32_t code = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) code |= (uint32_t)((b >> (8 * q)) & 3u) << (4 * q);
+    gc4[j * ldc4 + wd] = code;
+  }
+}
+
 // The same C[k][i] += sum_j mu_j Y_ij with the missing-call SNPs of sample k taken
 // in pairs (j1, j2): the 16 possible values mu_j1 Y(c1) + mu_j2 Y(c2) of a pair (c =
 // 2-bit code, Y(3) = mu / 2) are tabulated once per wavefront in LDS, and each lane
-// then adds, per sample, the table entry that its two codes index: per pair and
-// sample two bit-field extracts, an OR, an address, an LDS read and one FP64 add,
-// where the per-SNP form spends a conversion, a compare, a 64-bit select and an FMA
-// per SNP. The table covers the missing code too, so the both-missing pairs need no
-// separate pass. A last unpaired SNP pairs with a zero-weight partner. The 16
-// entries of a table occupy 32 banks, lanes reading one entry share the access.
+// then adds, per sample, the table entry that its two codes index. The codes come one
+// per 4-bit slot (k_pack_codes4: a lane's 8 samples of a SNP are one word), so one OR
+// of the pair's two words (w1 | w2 << 2) holds the 8 table indices, and a sample costs
+// a bit-field extract, an address, an LDS read and one FP64 add per pair of SNPs,
+// where the per-SNP form (k_impute_C) spends a conversion, a compare, a 64-bit select
+// and an FMA per SNP. The table covers the missing code too, so the both-missing pairs
+// need no separate pass. A last unpaired SNP pairs with a zero-weight partner. The 16
+// entries of a table occupy 32 banks; lanes reading one entry share the access.
 constexpr int CP_U = 8;  // pairs per table batch (16 SNPs of the list)
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -640,7 +666,7 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-__global__ void __launch_bounds__(256) k_impute_C2(const uint32_t* __restrict__ gc, int64_t ldc,
+__global__ void __launch_bounds__(256) k_impute_C2(const uint32_t* __restrict__ gc4, int64_t ldc4,
                                                    int64_t n, const double* __restrict__ mu,
                                                    const int32_t* __restrict__ off,
                                                    const int32_t* __restrict__ list, int64_t cap,
@@ -661,8 +687,7 @@ __global__ void __launch_bounds__(256) k_impute_C2(const uint32_t* __restrict__ 
   const c_i32* slist = (const c_i32*)list;
   const int64_t i0 = (int64_t)ic * C_ICHUNK + lane * 8;
   const bool live = i0 < n;
-  const uint32_t* gl = gc + i0 / 16;  // word of this lane's 8 samples
-  const int sh = (lane & 1) * 16;
+  const uint32_t* gl = gc4 + i0 / 8;  // the word of this lane's 8 samples
   int64_t q0 = soff[k], q1 = soff[k + 1];
   if (q1 > cap) q1 = cap;
   double* T = tab[w];
@@ -688,18 +713,19 @@ __global__ void __launch_bounds__(256) k_impute_C2(const uint32_t* __restrict__ 
         }
       }
     }
-    // the codes of the batch's SNPs, loaded while the tables are written
-    uint32_t va[CP_U], vb[CP_U];
+    // the pair index words of the batch (slot e of word u: the table index of sample e)
+    uint32_t tw[CP_U];
 #pragma unroll
     for (int u = 0; u < CP_U; ++u) {
       if (u < np) {
         const int64_t qa = q + 2 * u;
         const int64_t ja = slist[qa];
         const int64_t jb2 = qa + 1 < q1 ? (int64_t)slist[qa + 1] : ja;
-        va[u] = live ? gl[ja * ldc] >> sh : 0u;
-        vb[u] = live ? gl[jb2 * ldc] >> sh : 0u;
+        const uint32_t va = live ? gl[ja * ldc4] : 0u;
+        const uint32_t vb = live ? gl[jb2 * ldc4] : 0u;
+        tw[u] = va | (vb << 2);
       } else {
-        va[u] = vb[u] = 0u;
+        tw[u] = 0u;
       }
     }
     wave_lds_sync();  // tables visible to every lane
@@ -708,10 +734,7 @@ __global__ void __launch_bounds__(256) k_impute_C2(const uint32_t* __restrict__ 
       if (u < np) {
         const double* Tu = T + u * 16;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const uint32_t idx = ((va[u] >> (2 * e)) & 3u) | (((vb[u] >> (2 * e)) & 3u) << 2);
-          acc[e] += Tu[idx];
-        }
+        for (int e = 0; e < 8; ++e) acc[e] += Tu[__builtin_amdgcn_ubfe(tw[u], 4 * e, 4)];
       }
     }
     wave_lds_sync();  // tables read before the next batch overwrites them
@@ -905,10 +928,11 @@ extern "C" int jm_kinship_syrk_2bit(const uint8_t* gP, int64_t ldp, int64_t n, i
 // workspace (int32 units): offsets (n + 1) | lists (n_missing) | 2-bit codes (K rows of
 // ceil(n / 16) words, rounded up to an even count so that rows stay 8-byte aligned)
 static int64_t codes_ld(int64_t n) { return (((n + 15) / 16) + 1) / 2 * 2; }
+static int64_t codes_ld4(int64_t n) { return (((n + 7) / 8) + 1) / 2 * 2; }
 
 extern "C" int64_t jm_kinship_miss_ws_len(int64_t n, int64_t n_snps, int64_t n_missing) {
   if (n < 1 || n_snps < 0 || n_missing < 0) return 0;
-  return n + 1 + (n_missing + 1) / 2 * 2 + n_snps * codes_ld(n);
+  return n + 1 + (n_missing + 1) / 2 * 2 + n_snps * codes_ld4(n);
 }
 
 extern "C" int jm_kinship_impute_terms(const int8_t* g, int64_t ld_g, const int8_t* gT,
@@ -948,20 +972,24 @@ extern "C" int jm_kinship_impute_terms(const int8_t* g, int64_t ld_g, const int8
     hipLaunchKernelGGL(k_miss_fill, dim3(rb), dim3(256), 0, st, gT, ld_gT, n, n_snps, off, list,
                        cap);
     JM_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_pack_codes, dim3(4096), dim3(256), 0, st, g, ld_g, n, n_snps, gc, ldc);
-    JM_CHECK_LAUNCH();
     const int64_t blocks = ((n + C_ICHUNK - 1) / C_ICHUNK) * ((n + C_WAVES - 1) / C_WAVES);
     JM_CHECK_ARG(blocks < (1LL << 31), "n too large");
     static const bool per_snp = [] {  // JM_IMPUTE_C_PER_SNP=1: the per-SNP form (comparisons)
       const char* e = getenv("JM_IMPUTE_C_PER_SNP");
       return e && e[0] == '1';
     }();
-    if (per_snp)
+    if (per_snp) {  // 2-bit codes, 16 samples per word
+      hipLaunchKernelGGL(k_pack_codes, dim3(4096), dim3(256), 0, st, g, ld_g, n, n_snps, gc, ldc);
+      JM_CHECK_LAUNCH();
       hipLaunchKernelGGL(k_impute_C, dim3((unsigned)blocks), dim3(256), 0, st, gc, ldc, n, mu,
                          off, list, cap, C);
-    else
-      hipLaunchKernelGGL(k_impute_C2, dim3((unsigned)blocks), dim3(256), 0, st, gc, ldc, n, mu,
+    } else {  // one code per 4-bit slot, 8 samples per word
+      const int64_t ldc4 = codes_ld4(n);
+      hipLaunchKernelGGL(k_pack_codes4, dim3(4096), dim3(256), 0, st, g, ld_g, n, n_snps, gc, ldc4);
+      JM_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_impute_C2, dim3((unsigned)blocks), dim3(256), 0, st, gc, ldc4, n, mu,
                          off, list, cap, C);
+    }
     JM_CHECK_LAUNCH();
   }
   return JM_OK;
