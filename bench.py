@@ -27,7 +27,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "SNPs/s at n=10k, mixed vs FP64 GPU path at |Δ-log10p|≤1e-4; req p50/p99"
-PEAK = {"bf16": 2.5e15, "int8": 5.0e15, "fp64": 78.6e12, "hbm": 8.0e12}
+PEAK = {"bf16": 2.5e15, "int8": 5.0e15, "fp4": 10.1e15, "fp64": 78.6e12, "hbm": 8.0e12}
 
 
 def log(msg: str) -> None:
@@ -169,9 +169,9 @@ def main() -> int:
     miss_f = counts[idx, 3]
     log(f"{pf} SNPs pass the filters")
 
-    # ---- kinship on int8 MFMA (kinship-filter = var > 0 over all samples) ----
-    # tiles of 16384 SNPs are staged by the accumulator; the int8 GEMM runs as one
-    # launch over all staged SNPs, timed with HIP events on its stream
+    # ---- kinship: exact integer GEMM on the FP4 MFMA (kinship-filter = var > 0 over all
+    # samples); tiles of 16384 SNPs are staged 2-bit packed by the accumulator, the GEMM runs
+    # as one launch over all staged SNPs, timed with HIP events on its stream
     acc = KinshipAccumulator(n)
     acc.syrk_timer = []
     torch.cuda.synchronize()
@@ -188,7 +188,7 @@ def main() -> int:
     torch.cuda.synchronize()
     kin_s = time.perf_counter() - t0
     del acc
-    log(f"kinship {kin_s:.2f}s (int8 GEMM {syrk_ms:.2f} ms in {syrk_launches} launch(es) over "
+    log(f"kinship {kin_s:.2f}s (GEMM {syrk_ms:.2f} ms in {syrk_launches} launch(es) over "
         f"{pf} SNPs, input tiles of {ktile})")
 
     t0 = time.perf_counter()
@@ -371,15 +371,7 @@ def main() -> int:
         "mixed_over_fp64": (mixed["snps_per_s"] / results["f64"]["snps_per_s"]) if "f64" in results
         else None,
         "parity": parity,
-        "kinship": {"seconds": kin_s, "int8_gemm_ms": syrk_ms, "int8_gemm_launches": syrk_launches,
-                    "int8_gemm_operand": "2-bit packed genotypes widened to int8 in registers"
-                    if os.environ.get("JM_KINSHIP_INT8", "0") != "1" else "int8",
-                    "int8_gemm_frac_of_peak": (float(n) * (n + 1) * pf / (syrk_ms * 1e-3))
-                    / PEAK["int8"],
-                    "int8_gemm_ops": "n(n+1)p (symmetric rank-p update: the ops the SYRK needs)",
-                    "int8_gemm_frac_of_peak_2n2p": (2.0 * n * n * pf / (syrk_ms * 1e-3))
-                    / PEAK["int8"],
-                    "int8_gemm_2n2p": "2n^2 p, the full-GEMM convention of the planning floors"},
+        "kinship": kinship_record(n, pf, kin_s, syrk_ms, syrk_launches),
         "eigendecomposition_s": eig_s,
         "spectrum_terms": cohort.n_terms,
     }
@@ -394,8 +386,28 @@ def main() -> int:
     return 0
 
 
+def kinship_record(n: int, pf: int, kin_s: float, syrk_ms: float, launches: int) -> dict:
+    """The kinship stage and its GEMM S = X0 X0^T: n(n+1)p ops (the symmetric rank-p update)
+    against the peak of the MFMA it runs on, and against the int8 peak."""
+    int8_operand = os.environ.get("JM_KINSHIP_INT8", "0") == "1"
+    i8mfma = int8_operand or os.environ.get("JM_KINSHIP_SYRK_I8MFMA", "0") == "1"
+    ops_s = float(n) * (n + 1) * pf / (syrk_ms * 1e-3)
+    mfma = "int8" if i8mfma else "fp4"
+    return {"seconds": kin_s, "gemm_ms": syrk_ms, "gemm_launches": launches,
+            "gemm_operand": "int8" if int8_operand else "2-bit packed genotypes (0/1/2, missing 0)",
+            "gemm_mfma": "v_mfma_i32_16x16x64_i8 (int32 sums)" if i8mfma else
+            "v_mfma_scale_f32_16x16x128_f8f6f4, e2m1 operands (codes widened in registers), "
+            "exact integer sums in fp32 (< 2^24 per launch)",
+            "gemm_ops": "n(n+1)p (symmetric rank-p update: the ops the SYRK needs)",
+            "gemm_pops": ops_s / 1e15,
+            "gemm_frac_of_mfma_peak": ops_s / PEAK[mfma], "mfma_peak": f"{mfma} {PEAK[mfma] / 1e15:g} POP/s",
+            "gemm_frac_of_int8_peak": ops_s / PEAK["int8"],
+            "gemm_frac_of_int8_peak_2n2p": 2.0 * n * n * pf / (syrk_ms * 1e-3) / PEAK["int8"]}
+
+
 KERNEL_SOURCES = {"k_rotate_bf16x3_pipe4": "rotate.hip", "k_rotate_bf16x3_packed": "rotate.hip",
                   "k_gemm_f64_geno": "rotate.hip",
+                  "k_syrk_f4_main": "kinship.hip", "k_syrk_f4_tail": "kinship.hip",
                   "k_syrk_p2_main": "kinship.hip", "k_syrk_p2_tail": "kinship.hip",
                   "k_syrk_i8_main": "kinship.hip", "k_syrk_i8_tail": "kinship.hip",
                   "k_impute_C2": "kinship.hip",

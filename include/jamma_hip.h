@@ -108,8 +108,12 @@ int jm_kinship_syrk_i8(const int8_t* gT0, int64_t ld_gT, int64_t n, int64_t n_sn
 
 /*
  * The same S += X0 X0^T (replacing the same reference code) from a 2-bit packed
- * operand, a quarter of the int8 operand's bytes; widened to int8 in registers,
- * exact int32 accumulation on int8 MFMA, S bit-identical to jm_kinship_syrk_i8.
+ * operand, a quarter of the int8 operand's bytes, S bit-identical to
+ * jm_kinship_syrk_i8. The codes are widened in registers to e2m1 (FP4) operands of
+ * v_mfma_scale_f32_16x16x128_f8f6f4 (twice the int8 MFMA rate): integer products
+ * summed in FP32, exact because a launch covers at most 2^22 SNPs (sums < 2^24;
+ * longer calls are split into such launches). JM_KINSHIP_SYRK_I8MFMA=1 widens to
+ * int8 for v_mfma_i32_16x16x64_i8 instead (same S).
  *   jm_pack_gram_2bit: SNP-major int8 genotypes g (n_snps rows of ld_g, -1 =
  *     missing, as jm_decode_bed writes them) -> SNP columns col0 .. col0 + n_snps
  *     of gP (col0 a multiple of 16): sample-major rows of ldp bytes (ldp a multiple

@@ -8,8 +8,8 @@
 //   Xc Xc^T = X0 X0^T - u 1^T - 1 u^T + s0 11^T + C + C^T
 // where u_i = sum_j (observed ? g_ij mu_j : mu_j^2), s0 = sum_j mu_j^2 and
 // C = Y D M^T with Y_ij = g_ij (observed) or mu_j / 2 (missing). X0 X0^T is an
-// exact int32 GEMM on v_mfma_i32_16x16x64_i8; the corrections are FP64 and
-// only C is sparse work (one FP64 FMA per (row, missing call) pair).
+// exact integer GEMM (int32 S) on the FP4 MFMA from 2-bit packed codes (or on
+// v_mfma_i32_16x16x64_i8); the corrections are FP64 and only C is sparse work.
 
 #include "jm_common.h"
 
@@ -345,6 +345,156 @@ __global__ void __launch_bounds__(512) k_syrk_p2(const uint8_t* __restrict__ gP,
             atomicAdd(&S[i * n + j], acc[ti][tj][r]);
           else
             S[i * n + j] += acc[ti][tj][r];
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The same product on the FP4 MFMA
+// ---------------------------------------------------------------------------
+// v_mfma_scale_f32_16x16x128_f8f6f4 with both operands in e2m1 multiplies 16 x 128
+// by 128 x 16 in the time v_mfma_i32_16x16x64_i8 takes for 16 x 64: twice the
+// int8 rate. A 2-bit code c in the low two bits of a nibble is the e2m1 value c / 2
+// (0, 0.5 subnormal, 1.0); a block scale of 2 (E8M0 128) on each operand makes the
+// products c1 c2. Every product and partial sum is an integer below 2^24 while a
+// launch covers at most 2^22 SNPs (|sum| <= 4 K), so the FP32 accumulation is exact
+// in any order and S gets the same integers as from the int8 MFMA.
+//
+// Operand, LDS stages, tiles, XCD patches and K-split tail as k_syrk_p2. Lane group
+// lk's 16 bytes of a row (words of 16-SNP groups lk, lk + 4, lk + 8, lk + 12) feed
+// two 128-deep MFMAs per stage: words 0, 1 the first, words 2, 3 the second. A word
+// widens to two dwords of 8 nibbles (w & 0x33333333 and (w >> 2) & 0x33333333: 3
+// VALU per 16 codes); lane l of an MFMA operand holds K positions 32 (l >> 4) ..
+// + 31 of its row in nibble order, so A and B see the same permutation of the
+// stage's 256 SNPs.
+constexpr int64_t SF4_KMAX = 1LL << 22;  // SNPs per launch for an exact FP32 sum
+
+typedef int v8i_t __attribute__((ext_vector_type(8)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v8i_t fp4_frag(uint32_t w0, uint32_t w1) {
+  const uint32_t m = 0x33333333u;
+  return (v8i_t){(int)(w0 & m), (int)((w0 >> 2) & m), (int)(w1 & m), (int)((w1 >> 2) & m),
+                 0, 0, 0, 0};
+}
+
+template <bool ATOMIC>
+__global__ void __launch_bounds__(512) k_syrk_f4(const uint8_t* __restrict__ gP, int64_t ldp,
+                                                 int64_t n, int64_t K, int32_t* __restrict__ S,
+                                                 int n_full, int split) {
+  constexpr int WR = 8, WC = 4, NWC = 4;
+  constexpr int CPO = 2;        // 1 KB copies per operand per wave per stage
+  constexpr int CPW = 2 * CPO;  // copies per wave per stage
+  constexpr int SC = 128;       // E8M0 block scale 2
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[SY_NBUF * SY_STAGE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nb = (int)((n + SY_B - 1) / SY_B);
+  int t, part = 0;
+  if (!ATOMIC) {
+    const int x = blockIdx.x % 8, j = blockIdx.x / 8;
+    const int base = n_full / 8, rem = n_full % 8;
+    t = x * base + (x < rem ? x : rem) + j;
+  } else {
+    t = n_full + blockIdx.x / split;
+    part = blockIdx.x % split;
+  }
+  int rb, cb;
+  group_tile(t, nb, rb, cb);
+  const int nk_all = (int)((K + SP_G - 1) / SP_G);
+  const int kb0 = ATOMIC ? (int)((int64_t)nk_all * part / split) : 0;
+  const int nk = (ATOMIC ? (int)((int64_t)nk_all * (part + 1) / split) : nk_all) - kb0;
+  const int64_t i0 = (int64_t)rb * SY_B, j0 = (int64_t)cb * SY_B;
+  const int wr = (wave / NWC) * 16 * WR, wc = (wave % NWC) * 16 * WC;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int d_r = lane >> 2, d_s = lane & 3;
+  const uint32_t voff = (uint32_t)(d_r * ldp + 16 * (d_s ^ ((d_r >> 1) & 3)));
+  auto issue = [&](uint8_t* stage, int64_t kb) {
+#pragma unroll
+    for (int op = 0; op < 2; ++op) {
+      const int64_t r0 = op ? j0 : i0;
+#pragma unroll
+      for (int u = 0; u < CPO; ++u) {
+        const int q = wave * CPO + u;  // rows 16q .. 16q+15 of the panel
+        lds_copy16_s(gP + (r0 + 16 * q) * ldp + kb * (SP_G / 4), voff,
+                     stage + op * SY_PANEL + q * 1024);
+      }
+    }
+  };
+  f4_t acc[WR][WC];
+#pragma unroll
+  for (int a = 0; a < WR; ++a)
+#pragma unroll
+    for (int b = 0; b < WC; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int st = 0; st < SY_NBUF - 1; ++st)
+    if (st < nk) issue(lds + st * SY_STAGE, kb0 + st);
+  int cur = 0;  // LDS stage of K step kt
+  for (int kt = 0; kt < nk; ++kt) {
+    if (kt + SY_NBUF - 2 < nk)
+      wait_vmcnt<CPW * (SY_NBUF - 2)>();
+    else
+      wait_vmcnt<0>();
+    __syncthreads();  // every wave's copies of step kt landed; step kt-1's stage is free
+    if (kt + SY_NBUF - 1 < nk) {
+      const int nxt = cur == 0 ? SY_NBUF - 1 : cur - 1;
+      issue(lds + nxt * SY_STAGE, kb0 + kt + SY_NBUF - 1);
+    }
+    const uint8_t* base = lds + cur * SY_STAGE;
+    u4_t ar[WR], br[WC];
+#pragma unroll
+    for (int ti = 0; ti < WR; ++ti) {
+      const int r = wr + 16 * ti + lr;
+      ar[ti] = *reinterpret_cast<const u4_t*>(base + r * 64 + 16 * (lk ^ ((r >> 1) & 3)));
+    }
+#pragma unroll
+    for (int tj = 0; tj < WC; ++tj) {
+      const int r = wc + 16 * tj + lr;
+      br[tj] = *reinterpret_cast<const u4_t*>(base + SY_PANEL + r * 64 + 16 * (lk ^ ((r >> 1) & 3)));
+    }
+    // widening pipelined as in k_syrk_p2: the next row block's A fragment and the
+    // second MFMA's B fragments are formed while MFMAs issue
+    v8i_t bq[2][WC];
+#pragma unroll
+    for (int tj = 0; tj < WC; ++tj) bq[0][tj] = fp4_frag(br[tj][0], br[tj][1]);
+    v8i_t aq = fp4_frag(ar[0][0], ar[0][1]);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+#pragma unroll
+      for (int ti = 0; ti < WR; ++ti) {
+        v8i_t an = aq;
+        if (ti + 1 < WR)
+          an = fp4_frag(ar[ti + 1][2 * s], ar[ti + 1][2 * s + 1]);
+        else if (s == 0)
+          an = fp4_frag(ar[0][2], ar[0][3]);
+        if (ti == 1 && s == 0) {
+#pragma unroll
+          for (int tj = 0; tj < WC; ++tj) bq[1][tj] = fp4_frag(br[tj][2], br[tj][3]);
+        }
+#pragma unroll
+        for (int tj = 0; tj < WC; ++tj)
+          acc[ti][tj] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+              aq, bq[s][tj], acc[ti][tj], 4, 4, 0, SC, 0, SC);
+        aq = an;
+      }
+    }
+    cur = cur + 1 == SY_NBUF ? 0 : cur + 1;
+  }
+#pragma unroll
+  for (int ti = 0; ti < WR; ++ti) {
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int tj = 0; tj < WC; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + wr + ti * 16 + 4 * lk + r;
+        const int64_t j = j0 + wc + tj * 16 + lr;
+        if (i < n && j < n) {
+          const int32_t v = (int32_t)acc[ti][tj][r];  // an exact integer
+          if (ATOMIC)
+            atomicAdd(&S[i * n + j], v);
+          else
+            S[i * n + j] += v;
         }
       }
   }
@@ -905,22 +1055,41 @@ extern "C" int jm_kinship_syrk_2bit(const uint8_t* gP, int64_t ldp, int64_t n, i
   const int64_t nb = (n + SY_B - 1) / SY_B;
   JM_CHECK_ARG(nb * (nb + 1) / 2 < (1LL << 28), "n too large");
   const int total = (int)(nb * (nb + 1) / 2);
-  const int nk = (int)((n_snps + SP_G - 1) / SP_G);  // 256-SNP stages
-  const int tail = total % 256;
-  int split = tail ? 256 / tail : 1;
-  if (split > nk / 4) split = nk / 4;
-  if (split < 1) split = 1;
-  const int n_full = split > 1 ? total - tail : total;
+  // FP4 MFMA (default) or the int8 MFMA on the same operand (JM_KINSHIP_SYRK_I8MFMA=1, for
+  // comparisons); the FP4 form runs at most SF4_KMAX SNPs per launch (exact FP32 sums)
+  static const bool i8mfma = [] {
+    const char* e = getenv("JM_KINSHIP_SYRK_I8MFMA");
+    return e && e[0] == '1';
+  }();
+  const int64_t kmax = i8mfma ? n_snps : SF4_KMAX;
   hipStream_t st = (hipStream_t)stream;
-  if (n_full > 0) {
-    hipLaunchKernelGGL(k_syrk_p2<false>, dim3(n_full), dim3(512), 0, st, gP, ldp, n, n_snps, S,
-                       n_full, 1);
-    JM_CHECK_LAUNCH();
-  }
-  if (total > n_full) {
-    hipLaunchKernelGGL(k_syrk_p2<true>, dim3((total - n_full) * split), dim3(512), 0, st, gP, ldp,
-                       n, n_snps, S, n_full, split);
-    JM_CHECK_LAUNCH();
+  for (int64_t k0 = 0; k0 < n_snps; k0 += kmax) {
+    const int64_t kn = n_snps - k0 < kmax ? n_snps - k0 : kmax;
+    const uint8_t* g0 = gP + k0 / 4;  // k0 is a multiple of 256 SNPs: 64-byte aligned
+    const int nk = (int)((kn + SP_G - 1) / SP_G);  // 256-SNP stages
+    const int tail = total % 256;
+    int split = tail ? 256 / tail : 1;
+    if (split > nk / 4) split = nk / 4;
+    if (split < 1) split = 1;
+    const int n_full = split > 1 ? total - tail : total;
+    if (n_full > 0) {
+      if (i8mfma)
+        hipLaunchKernelGGL(k_syrk_p2<false>, dim3(n_full), dim3(512), 0, st, g0, ldp, n, kn, S,
+                           n_full, 1);
+      else
+        hipLaunchKernelGGL(k_syrk_f4<false>, dim3(n_full), dim3(512), 0, st, g0, ldp, n, kn, S,
+                           n_full, 1);
+      JM_CHECK_LAUNCH();
+    }
+    if (total > n_full) {
+      if (i8mfma)
+        hipLaunchKernelGGL(k_syrk_p2<true>, dim3((total - n_full) * split), dim3(512), 0, st, g0,
+                           ldp, n, kn, S, n_full, split);
+      else
+        hipLaunchKernelGGL(k_syrk_f4<true>, dim3((total - n_full) * split), dim3(512), 0, st, g0,
+                           ldp, n, kn, S, n_full, split);
+      JM_CHECK_LAUNCH();
+    }
   }
   return JM_OK;
 }

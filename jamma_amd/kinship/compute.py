@@ -1,12 +1,13 @@
-"""Centred kinship (GEMMA -gk 1) on the device: int8 MFMA + FP64 corrections.
+"""Centred kinship (GEMMA -gk 1) on the device: exact integer GEMM on MFMA + FP64 corrections.
 
 Drop-in for src/jamma/kinship/compute.py:
   * compute_centered_kinship   :93-226  (in-memory genotypes)
   * compute_kinship_streaming   :502-690 (two passes over the PLINK file)
   * compute_loco_kinship(_streaming) :363-499, :693-916 (LOCO by subtraction)
 The 0/1/2 calls (missing as 0) are packed to 2 bits per genotype and X0 X0^T is
-accumulated exactly in int32 on int8 MFMA (jm_kinship_syrk_2bit; JM_KINSHIP_INT8=1
-selects the int8-operand form jm_kinship_syrk_i8, same S); mean imputation and
+accumulated exactly into int32 S (jm_kinship_syrk_2bit: FP4 MFMA with integer sums in
+FP32, or the int8 MFMA with JM_KINSHIP_SYRK_I8MFMA=1; JM_KINSHIP_INT8=1 selects the
+int8-operand form jm_kinship_syrk_i8; all give the same S); mean imputation and
 centring are applied as exact
 FP64 low-rank / sparse corrections (jm_kinship_impute_terms,
 jm_kinship_finalize). Per-SNP statistics come from jm_decode_bed counts.

@@ -11,6 +11,9 @@ significance / effect-direction agreement, the north-star bar.
 
 from __future__ import annotations
 
+import os
+from pathlib import Path
+
 import numpy as np
 import pytest
 import torch
@@ -132,6 +135,43 @@ def test_syrk_int32_exact(n, tiles, stage_cols, form, monkeypatch):
     ref = (x0.T @ x0).astype(np.int64)
     low = np.tril_indices(n)
     assert np.array_equal(S[low], ref[low])
+
+
+@pytest.mark.parametrize("form", ["fp4", "i8mfma"])
+def test_syrk_2bit_exact_past_2_24(form):
+    """The FP4-MFMA product sums in FP32, exact while a launch covers at most 2^22 SNPs;
+    over 2^22 + 768 SNPs of codes 2 (and 1 for half of the samples) every S entry is
+    K c_i c_j, past 2^24 for the code-2 pairs, so the K split across launches has to be
+    right. The int8-MFMA kernel on the same operand gives the same integers."""
+    import subprocess
+    import sys
+    import textwrap
+
+    # the kernel choice is read once per process: run the check in a child process
+    code = textwrap.dedent("""
+        import numpy as np, torch
+        from jamma_amd import _lib
+        from jamma_amd.device import ptr, stream_ptr
+        n, K = 300, (1 << 22) + 768
+        ldp = (K + 255) // 256 * 64
+        gP = torch.empty((512, ldp), dtype=torch.uint8, device="cuda")
+        gP[:150] = 0xAA   # every 2-bit code 2
+        gP[150:] = 0x55   # every 2-bit code 1
+        S = torch.zeros((n, n), dtype=torch.int32, device="cuda")
+        _lib.call("jm_kinship_syrk_2bit", ptr(gP), ldp, n, K, ptr(S), stream_ptr())
+        torch.cuda.synchronize()
+        c = np.where(np.arange(n) < 150, 2, 1).astype(np.int64)
+        ref = K * np.outer(c, c)
+        low = np.tril_indices(n)
+        got = S.cpu().numpy().astype(np.int64)
+        assert ref.max() > 1 << 24
+        assert np.array_equal(got[low], ref[low]), np.abs(got[low] - ref[low]).max()
+        print("ok")
+    """)
+    env = dict(os.environ, JM_KINSHIP_SYRK_I8MFMA="1" if form == "i8mfma" else "0")
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=300, cwd=str(Path(__file__).resolve().parents[1]))
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr
 
 
 @pytest.mark.parametrize("n,n_snps,col0", [(70, 37, 48), (300, 600, 0), (257, 255, 16)])
