@@ -184,9 +184,12 @@ def main() -> int:
     torch.cuda.synchronize()
     syrk_ms = sum(a.elapsed_time(b) for a, b in acc.syrk_timer)
     syrk_launches = len(acc.syrk_timer)
+    t1 = time.perf_counter()
+    i8mfma = syrk_int8_mfma(acc) if syrk_launches == 1 else None
+    t_check = time.perf_counter() - t1  # not part of the kinship stage
     K = acc.finalize(pf)
     torch.cuda.synchronize()
-    kin_s = time.perf_counter() - t0
+    kin_s = time.perf_counter() - t0 - t_check
     del acc
     log(f"kinship {kin_s:.2f}s (GEMM {syrk_ms:.2f} ms in {syrk_launches} launch(es) over "
         f"{pf} SNPs, input tiles of {ktile})")
@@ -371,7 +374,7 @@ def main() -> int:
         "mixed_over_fp64": (mixed["snps_per_s"] / results["f64"]["snps_per_s"]) if "f64" in results
         else None,
         "parity": parity,
-        "kinship": kinship_record(n, pf, kin_s, syrk_ms, syrk_launches),
+        "kinship": kinship_record(n, pf, kin_s, syrk_ms, syrk_launches, i8mfma),
         "eigendecomposition_s": eig_s,
         "spectrum_terms": cohort.n_terms,
     }
@@ -386,7 +389,36 @@ def main() -> int:
     return 0
 
 
-def kinship_record(n: int, pf: int, kin_s: float, syrk_ms: float, launches: int) -> dict:
+def syrk_int8_mfma(acc) -> dict | None:
+    """Outside the kinship's timing: the same GEMM over the staged 2-bit operand once more
+    with the int8 MFMA (v_mfma_i32_16x16x64_i8) instead of the FP4 one, into a scratch S;
+    its time and whether its S equals the FP4 kernel's bit for bit."""
+    import torch
+
+    from jamma_amd import _lib
+    from jamma_amd.device import ptr, stream_ptr
+
+    if not acc.packed or os.environ.get("JM_KINSHIP_SYRK_I8MFMA", "0") == "1":
+        return None
+    n, cols = acc.n, acc.last_flush_cols
+    S2 = torch.zeros_like(acc._S)
+    st = stream_ptr()
+    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    os.environ["JM_KINSHIP_SYRK_I8MFMA"] = "1"
+    try:
+        ev[0].record()
+        _lib.call("jm_kinship_syrk_2bit", ptr(acc._stage), acc._stage_ld(), n, cols, ptr(S2), st)
+        ev[1].record()
+    finally:
+        del os.environ["JM_KINSHIP_SYRK_I8MFMA"]
+    torch.cuda.synchronize()
+    low = torch.tril(torch.ones((n, n), dtype=torch.bool, device=S2.device))
+    same = bool(torch.equal(S2[low], acc._S[low]))
+    return {"ms": ev[0].elapsed_time(ev[1]), "S_identical_to_fp4": same}
+
+
+def kinship_record(n: int, pf: int, kin_s: float, syrk_ms: float, launches: int,
+                   i8form: dict | None = None) -> dict:
     """The kinship stage and its GEMM S = X0 X0^T: n(n+1)p ops (the symmetric rank-p update)
     against the peak of the MFMA it runs on, and against the int8 peak."""
     int8_operand = os.environ.get("JM_KINSHIP_INT8", "0") == "1"
@@ -402,7 +434,10 @@ def kinship_record(n: int, pf: int, kin_s: float, syrk_ms: float, launches: int)
             "gemm_pops": ops_s / 1e15,
             "gemm_frac_of_mfma_peak": ops_s / PEAK[mfma], "mfma_peak": f"{mfma} {PEAK[mfma] / 1e15:g} POP/s",
             "gemm_frac_of_int8_peak": ops_s / PEAK["int8"],
-            "gemm_frac_of_int8_peak_2n2p": 2.0 * n * n * pf / (syrk_ms * 1e-3) / PEAK["int8"]}
+            "gemm_frac_of_int8_peak_2n2p": 2.0 * n * n * pf / (syrk_ms * 1e-3) / PEAK["int8"],
+            "int8_mfma_form": None if i8form is None else {
+                **i8form, "frac_of_int8_peak": float(n) * (n + 1) * pf / (i8form["ms"] * 1e-3)
+                / PEAK["int8"]}}
 
 
 KERNEL_SOURCES = {"k_rotate_bf16x3_pipe4": "rotate.hip", "k_rotate_bf16x3_packed": "rotate.hip",

@@ -1055,12 +1055,11 @@ extern "C" int jm_kinship_syrk_2bit(const uint8_t* gP, int64_t ldp, int64_t n, i
   const int64_t nb = (n + SY_B - 1) / SY_B;
   JM_CHECK_ARG(nb * (nb + 1) / 2 < (1LL << 28), "n too large");
   const int total = (int)(nb * (nb + 1) / 2);
-  // FP4 MFMA (default) or the int8 MFMA on the same operand (JM_KINSHIP_SYRK_I8MFMA=1, for
-  // comparisons); the FP4 form runs at most SF4_KMAX SNPs per launch (exact FP32 sums)
-  static const bool i8mfma = [] {
-    const char* e = getenv("JM_KINSHIP_SYRK_I8MFMA");
-    return e && e[0] == '1';
-  }();
+  // FP4 MFMA (default) or the int8 MFMA on the same operand (JM_KINSHIP_SYRK_I8MFMA=1, read
+  // per call, for comparisons); the FP4 form runs at most SF4_KMAX SNPs per launch (exact
+  // FP32 sums)
+  const char* env_i8 = getenv("JM_KINSHIP_SYRK_I8MFMA");
+  const bool i8mfma = env_i8 && env_i8[0] == '1';
   const int64_t kmax = i8mfma ? n_snps : SF4_KMAX;
   hipStream_t st = (hipStream_t)stream;
   for (int64_t k0 = 0; k0 < n_snps; k0 += kmax) {

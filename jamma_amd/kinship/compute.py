@@ -69,6 +69,7 @@ class KinshipAccumulator:
             self.stage_cols = int(min(self.stage_cols, max(256, round_up(max_snps + 16, 256))))
         self._stage = None
         self._fill = 0
+        self.last_flush_cols = 0  # SNP columns of the stage's last GEMM
         # optional list: each SYRK launch appends its (start, end) torch.cuda.Event pair
         self.syrk_timer = None
 
@@ -106,6 +107,7 @@ class KinshipAccumulator:
             if kp > k:  # columns left from an earlier, longer fill must read as zero
                 self._stage[:, k:kp].zero_()
         self._syrk(self._stage, self._stage_ld(), k)
+        self.last_flush_cols = k
         self._fill = 0
 
     def release(self) -> None:

@@ -98,7 +98,7 @@ def test_kinship_int8_matches_oracle(n, p, miss, batch):
     assert np.array_equal(K, K.T)
 
 
-@pytest.mark.parametrize("form", ["2bit", "int8"])
+@pytest.mark.parametrize("form", ["2bit", "2bit-i8mfma", "int8"])
 @pytest.mark.parametrize("n,tiles,stage_cols", [
     (600, [300, 300, 700, 1200, 37], 512),  # flush when full, stale columns, oversized tiles
     (5700, [3100, 1500], None),             # 276 tiles: a full round + K-split tail round
@@ -107,20 +107,21 @@ def test_kinship_int8_matches_oracle(n, p, miss, batch):
 def test_syrk_int32_exact(n, tiles, stage_cols, form, monkeypatch):
     """S = X0 X0^T (missing calls as 0) is exact integer arithmetic: compare the lower
     triangle with a float64 product of the same integers (exact below 2^53), for the
-    2-bit packed operand (tiles on 16-SNP boundaries, gaps of zero genotypes) and the
-    int8 one."""
+    2-bit packed operand (tiles on 16-SNP boundaries, gaps of zero genotypes) on the FP4
+    and on the int8 MFMA, and the int8 operand."""
     from jamma_amd.device import round_up, to_dev
     from jamma_amd.kinship.compute import KinshipAccumulator
 
     monkeypatch.setenv("JM_KINSHIP_INT8", "1" if form == "int8" else "0")
+    monkeypatch.setenv("JM_KINSHIP_SYRK_I8MFMA", "1" if form == "2bit-i8mfma" else "0")
     rng = np.random.default_rng(n)
     p = sum(tiles)
     g = rng.integers(0, 3, size=(p, n)).astype(np.int8)
     g[rng.random(g.shape) < 0.01] = -1
-    per_snp = 0.25 if form == "2bit" else 1.0
+    per_snp = 1.0 if form == "int8" else 0.25
     stage = 1 << 31 if stage_cols is None else int(stage_cols * round_up(n, 256) * per_snp)
     acc = KinshipAccumulator(n, stage_bytes=stage)
-    assert acc.packed == (form == "2bit")
+    assert acc.packed == (form != "int8")
     if stage_cols is not None:
         assert acc.stage_cols == stage_cols
     ld = round_up(n, 64)
@@ -147,7 +148,7 @@ def test_syrk_2bit_exact_past_2_24(form):
     import sys
     import textwrap
 
-    # the kernel choice is read once per process: run the check in a child process
+    # a child process with the kernel choice in its environment
     code = textwrap.dedent("""
         import numpy as np, torch
         from jamma_amd import _lib
