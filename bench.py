@@ -332,7 +332,11 @@ def main() -> int:
     rot_ms = mixed["kernel_ms_per_tile"][3]  # the rotation kernel alone
     tile_snps = pf / len(tiles)
     achieved = 2.0 * n * n * tile_snps / (rot_ms * 1e-3)
-    rot_kernel = "k_rotate_bf16x3_pipe4" if args.planes == 2 else "k_rotate_bf16x3_packed"
+    # the rotation kernel jm_rotate_bf16x3 launches for this plane count (JM_ROTATE_UNPIPED=1
+    # selects the two-stage form for both)
+    unpiped = os.environ.get("JM_ROTATE_UNPIPED", "0") == "1"
+    rot_kernel = ("k_rotate_bf16x3_packed" if unpiped else
+                  "k_rotate_bf16x3_pipe4" if args.planes == 2 else "k_rotate_bf16x3_pipe3")
     traffic, traffic_status = hbm_traffic(rot_kernel, n, args.tile)
     vendor = vendor_bf16_gemm_tflops(n, int(tile_snps), n)
     issue_tflops = (args.planes + 1) * achieved / 1e12
@@ -440,7 +444,8 @@ def kinship_record(n: int, pf: int, kin_s: float, syrk_ms: float, launches: int,
                 / PEAK["int8"]}}
 
 
-KERNEL_SOURCES = {"k_rotate_bf16x3_pipe4": "rotate.hip", "k_rotate_bf16x3_packed": "rotate.hip",
+KERNEL_SOURCES = {"k_rotate_bf16x3_pipe4": "rotate.hip", "k_rotate_bf16x3_pipe3": "rotate.hip",
+                  "k_rotate_bf16x3_packed": "rotate.hip",
                   "k_gemm_f64_geno": "rotate.hip",
                   "k_syrk_f4_main": "kinship.hip", "k_syrk_f4_tail": "kinship.hip",
                   "k_syrk_p2_main": "kinship.hip", "k_syrk_p2_tail": "kinship.hip",
