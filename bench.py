@@ -1,11 +1,11 @@
 #!/usr/bin/env python
 """Benchmark of the LMM association hot path on one MI355X.
 
-Workload (BASELINE config "Precision form"): n = 10,000 samples x 100,000 SNPs,
+Workload ("precision form"): n = 10,000 samples x 100,000 SNPs,
 3 Balding-Nichols subpopulations (Fst = 0.05), 1% missing calls, h2 = 0.5 with
 200 causal SNPs, 3 Gaussian covariates + intercept, all tests (-lmm 4).
 Synthetic data is generated on the device. Untimed setup: genotype generation,
-int8 kinship (timed separately), eigendecomposition, null model.
+integer kinship GEMM (timed separately), eigendecomposition, null model.
 
 A step = association of every filtered SNP with the inputs resident in HBM
 (int8 genotypes, means, U^T and its bf16 split): rotation -> spectrum moments
@@ -644,7 +644,7 @@ def loco(args) -> int:
     line = {
         "metric": "LOCO SNP x phenotype tests/s (kinships + eigendecompositions + association)",
         "value": tests / timed, "unit": "tests/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "int8 kinship, f64 eigen, bf16x3 rotation + f64 lambda/tests",
+        "dtype": "integer kinship GEMM (fp4 MFMA, exact), f64 eigen, bf16x3 rotation + f64 lambda/tests",
         "data": "synthetic (Balding-Nichols genotypes generated on device)",
         "config": {"workload": f"loco: n={n} x {p} SNPs ({a.shape[0]} pass filters) on {n_chr} "
                                f"chromosomes (length ~ c^-0.5), {n_ph} phenotypes (genetic corr "
@@ -665,7 +665,7 @@ def pipeline(args) -> int:
     subpopulations, Fst 0.05, 1% missing, h2 = 0.5 / 200 causal, 3 covariates +
     intercept) written as a PLINK fileset, then the reference-signature gwas() run end to
     end (-lmm 4): .bed read from disk and copied to the device, decode, pass-1 statistics,
-    int8 kinship, host kinship matrix (the API's contract), eigendecomposition, rotation,
+    integer kinship GEMM, host kinship matrix (the API's contract), eigendecomposition, rotation,
     lambda search and tests, .assoc.txt written. Mixed precision, then FP64."""
     import tempfile
 
@@ -743,7 +743,7 @@ def small(args) -> int:
     SNP effects beta ~ N(0, 0.05) + N(0, 1); intercept only; centred kinship, its
     eigendecomposition, then the per-SNP REML Wald test (-lmm 1).
 
-    Device: int8 kinship + 64-bit eigendecomposition + FP64 rotation and lambda search
+    Device: integer kinship GEMM + 64-bit eigendecomposition + FP64 rotation and lambda search
     over every SNP. CPU: the oracle on the whole job (NumPy centred kinship, LAPACK
     eigh, the same association), on the host cores, with its p-values as the check."""
     import numpy as np

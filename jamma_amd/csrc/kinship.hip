@@ -1,4 +1,4 @@
-// Centred kinship (GEMMA -gk 1) on int8 MFMA with exact FP64 corrections.
+// Centred kinship (GEMMA -gk 1): exact integer GEMM on MFMA with exact FP64 corrections.
 //
 // Reference: K = Xc Xc^T / p with Xc = impute_and_center(X) per batch
 // (src/jamma/kinship/compute.py:46-57, 93-226, 502-690;

@@ -1,4 +1,4 @@
-"""Centred kinship (GEMMA -gk 1) on int8 MFMA (mirrors jamma.kinship exports)."""
+"""Centred kinship (GEMMA -gk 1): exact integer GEMM on MFMA (mirrors jamma.kinship exports)."""
 
 from jamma_amd.io.plink import get_chromosome_partitions
 from jamma_amd.kinship.compute import (

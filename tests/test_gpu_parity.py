@@ -6,7 +6,7 @@ must match the FP64 oracle to rounding (kinship < 1e-12 abs; association
 statistics rtol 1e-6, lambda rtol 5e-5 and -log10 p atol 1e-5, the width of
 the golden-section bracket, see _assert_matches_oracle); the
 mixed (BF16x3) rotation path is held to |delta -log10 p| <= 1e-4 and 100%
-significance / effect-direction agreement, the north-star bar.
+significance / effect-direction agreement, the acceptance bar of the mixed path.
 """
 
 from __future__ import annotations
@@ -404,7 +404,7 @@ def test_assoc_lane_groups_match_wave_kernel(monkeypatch, n, p, ncov, seed):
 
 def test_mixed_precision_meets_bar():
     """BF16x3 rotation vs the FP64 oracle: |delta -log10 p| <= 1e-4, 100% significance and
-    direction agreement (the north-star acceptance bar), at n = 1000."""
+    direction agreement (the mixed path's acceptance bar), at n = 1000."""
     from jamma_amd.lmm import run_lmm_association_jax
 
     G = balding_nichols(1000, 1400, miss=0.01, seed=7)

@@ -5,7 +5,7 @@
   U^T, and the eigendecomposition is the library's own 64-bit path at a size that is
   not a multiple of its 128-column panels. The eigendecomposition is checked by its
   residual and orthogonality (rocBLAS FP64 GEMMs, independent of the library), the
-  mixed path against the FP64 device path on 2,048 SNPs at the north-star bar, and
+  mixed path against the FP64 device path on 2,048 SNPs at the acceptance bar, and
   the FP64 path against the oracle on 128 SNPs downstream of the same U.
 * LOCO multi-phenotype form: n = 2,000, 1,000 SNPs on 5 chromosomes (lengths
   proportional to 1/sqrt(index)), 8 phenotypes with genetic correlation 0.3, an
