@@ -18,6 +18,7 @@
 #include "jm_common.h"
 
 #include <mutex>
+#include <type_traits>
 
 namespace jm {
 
@@ -77,13 +78,29 @@ __device__ __forceinline__ void mom_piece(const ZT* __restrict__ Z, int64_t ldz,
     for (int b = 0; b < kMomGroup; ++b)
 #pragma unroll
       for (int m = 0; m < M; ++m) acc[b][m] = 0.0;
+    // KIND (wave-uniform, fixed per group): 1 = all kMomGroup rows are V rows, 2 = V rows
+    // then the Z^2 row last, 0 = anything else (short last group, MULTI's per-column nv).
+    // 1 and 2 leave out the row selects (the same products as kind 0 where it has them)
+    const int kind = MULTI || ng != kMomGroup ? 0
+                     : g0 + kMomGroup <= nv   ? 1
+                     : g0 + kMomGroup - 1 == nv ? 2
+                                                : 0;
+    auto run = [&](auto kind_c) {
+    constexpr int KIND = decltype(kind_c)::value;
     // one eigen-index: prod_b = Z V_b (or Z^2), acc[b][m] += prod_b delta^m
     auto step = [&](double z, double dl, const double (&vb)[kMomGroup]) {
       double prod[kMomGroup];
 #pragma unroll
       for (int b = 0; b < kMomGroup; ++b) {
         const int row = g0 + b;
-        prod[b] = (b < ng && row <= nv) ? z * (row < nv ? vb[b] : z) : 0.0;
+        // the product is rounded on its own (__dmul_rn: never fused into the m = 0 sum),
+        // so every kind adds the same bits
+        if (KIND == 1)
+          prod[b] = __dmul_rn(z, vb[b]);
+        else if (KIND == 2)
+          prod[b] = __dmul_rn(z, b < kMomGroup - 1 ? vb[b] : z);
+        else
+          prod[b] = (b < ng && row <= nv) ? __dmul_rn(z, row < nv ? vb[b] : z) : 0.0;
       }
       double pw = 1.0;
 #pragma unroll
@@ -95,6 +112,8 @@ __device__ __forceinline__ void mom_piece(const ZT* __restrict__ Z, int64_t ldz,
     };
     auto load_v = [&](int row, int i) -> double {
       if (MULTI) return row < nv ? V[(int64_t)row * ldv + i] : 0.0;
+      if (KIND == 1 || (KIND == 2 && row < g0 + kMomGroup - 1)) return Vs[(int64_t)row * ldv + i];
+      if (KIND == 2) return 0.0;  // the Z^2 row reads no V
       return row < nv ? Vs[(int64_t)row * ldv + i] : 0.0;
     };
     // Z of the next kMomUnroll eigen-indices is loaded while the current ones
@@ -120,7 +139,8 @@ __device__ __forceinline__ void mom_piece(const ZT* __restrict__ Z, int64_t ldz,
       for (int u = 0; u < kMomUnroll; ++u) {
         dl[u] = MULTI ? delta[ib + u] : dls[ib + u];
 #pragma unroll
-        for (int b = 0; b < kMomGroup; ++b) vv[u][b] = b < ng ? load_v(g0 + b, ib + u) : 0.0;
+        for (int b = 0; b < kMomGroup; ++b)
+          vv[u][b] = (KIND != 0 || b < ng) ? load_v(g0 + b, ib + u) : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < kMomUnroll; ++u) step(zu[u], dl[u], vv[u]);
@@ -132,9 +152,16 @@ __device__ __forceinline__ void mom_piece(const ZT* __restrict__ Z, int64_t ldz,
     for (; ib < iend; ++ib) {
       double vv[kMomGroup];
 #pragma unroll
-      for (int b = 0; b < kMomGroup; ++b) vv[b] = b < ng ? load_v(g0 + b, ib) : 0.0;
+      for (int b = 0; b < kMomGroup; ++b) vv[b] = (KIND != 0 || b < ng) ? load_v(g0 + b, ib) : 0.0;
       step((double)Z[(int64_t)ib * ldz + sc], MULTI ? delta[ib] : dls[ib], vv);
     }
+    };
+    if (kind == 1)
+      run(std::integral_constant<int, 1>{});
+    else if (kind == 2)
+      run(std::integral_constant<int, 2>{});
+    else
+      run(std::integral_constant<int, 0>{});
     // transpose through LDS, one row at a time, so that the M consecutive slots
     // of a (column, row) are written by consecutive lanes
 #pragma unroll
