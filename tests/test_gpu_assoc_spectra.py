@@ -69,7 +69,9 @@ def test_search_kernels_match_oracle(monkeypatch, rng_d, ncov, p, t_range, seed)
 def test_moments_many_partner_rows(nv, z_f32):
     """The moments kernel against its definition, with more row groups than a block
     has wavefronts (nv = 14: 15 rows in 5 groups of 3 on 4 waves), a SNP count that
-    leaves a partial 64-column block, FP32 and FP64 Z:
+    leaves a partial 64-column block, FP32 and FP64 Z. The row groups take each of the
+    kernel's paths: three V rows (nv = 14, 5), two V rows and the Z^2 row (nv = 14, 5)
+    and a short last group (nv = 1):
       mom[s][b][t] = sum_{i in c(t)} Z[i][s] V[b][i] delta_i^m(t)   (b < nv)
       mom[s][nv][t] = sum_{i in c(t)} Z[i][s]^2 delta_i^m(t)."""
     from jamma_amd.lmm.engine import Cohort
