@@ -86,75 +86,75 @@ __device__ __forceinline__ void mom_piece(const ZT* __restrict__ Z, int64_t ldz,
                      : g0 + kMomGroup - 1 == nv ? 2
                                                 : 0;
     auto run = [&](auto kind_c) {
-    constexpr int KIND = decltype(kind_c)::value;
-    // one eigen-index: prod_b = Z V_b (or Z^2), acc[b][m] += prod_b delta^m
-    auto step = [&](double z, double dl, const double (&vb)[kMomGroup]) {
-      double prod[kMomGroup];
+      constexpr int KIND = decltype(kind_c)::value;
+      // one eigen-index: prod_b = Z V_b (or Z^2), acc[b][m] += prod_b delta^m
+      auto step = [&](double z, double dl, const double (&vb)[kMomGroup]) {
+        double prod[kMomGroup];
 #pragma unroll
-      for (int b = 0; b < kMomGroup; ++b) {
-        const int row = g0 + b;
-        // the product is rounded on its own (__dmul_rn: never fused into the m = 0 sum),
-        // so every kind adds the same bits
-        if (KIND == 1)
-          prod[b] = __dmul_rn(z, vb[b]);
-        else if (KIND == 2)
-          prod[b] = __dmul_rn(z, b < kMomGroup - 1 ? vb[b] : z);
-        else
-          prod[b] = (b < ng && row <= nv) ? __dmul_rn(z, row < nv ? vb[b] : z) : 0.0;
+        for (int b = 0; b < kMomGroup; ++b) {
+          const int row = g0 + b;
+          // the product is rounded on its own (__dmul_rn: never fused into the m = 0 sum),
+          // so every kind adds the same bits
+          if (KIND == 1)
+            prod[b] = __dmul_rn(z, vb[b]);
+          else if (KIND == 2)
+            prod[b] = __dmul_rn(z, b < kMomGroup - 1 ? vb[b] : z);
+          else
+            prod[b] = (b < ng && row <= nv) ? __dmul_rn(z, row < nv ? vb[b] : z) : 0.0;
+        }
+        double pw = 1.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+#pragma unroll
+          for (int b = 0; b < kMomGroup; ++b) acc[b][m] += prod[b] * pw;
+          pw *= dl;
+        }
+      };
+      auto load_v = [&](int row, int i) -> double {
+        if (MULTI) return row < nv ? V[(int64_t)row * ldv + i] : 0.0;
+        if (KIND == 1 || (KIND == 2 && row < g0 + kMomGroup - 1)) return Vs[(int64_t)row * ldv + i];
+        if (KIND == 2) return 0.0;  // the Z^2 row reads no V
+        return row < nv ? Vs[(int64_t)row * ldv + i] : 0.0;
+      };
+      // Z of the next kMomUnroll eigen-indices is loaded while the current ones
+      // are accumulated
+      int ib = i0;
+      ZT zc[kMomUnroll];
+      if (ib + kMomUnroll <= iend) {
+#pragma unroll
+        for (int u = 0; u < kMomUnroll; ++u) zc[u] = Z[(int64_t)(ib + u) * ldz + sc];
       }
-      double pw = 1.0;
+      for (; ib + kMomUnroll <= iend; ib += kMomUnroll) {
+        const bool more = ib + 2 * kMomUnroll <= iend;
+        ZT zn[kMomUnroll];
+        if (more) {
 #pragma unroll
-      for (int m = 0; m < M; ++m) {
+          for (int u = 0; u < kMomUnroll; ++u)
+            zn[u] = Z[(int64_t)(ib + kMomUnroll + u) * ldz + sc];
+        }
+        double zu[kMomUnroll], dl[kMomUnroll], vv[kMomUnroll][kMomGroup];
 #pragma unroll
-        for (int b = 0; b < kMomGroup; ++b) acc[b][m] += prod[b] * pw;
-        pw *= dl;
+        for (int u = 0; u < kMomUnroll; ++u) zu[u] = (double)zc[u];
+#pragma unroll
+        for (int u = 0; u < kMomUnroll; ++u) {
+          dl[u] = MULTI ? delta[ib + u] : dls[ib + u];
+#pragma unroll
+          for (int b = 0; b < kMomGroup; ++b)
+            vv[u][b] = (KIND != 0 || b < ng) ? load_v(g0 + b, ib + u) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kMomUnroll; ++u) step(zu[u], dl[u], vv[u]);
+        if (more) {
+#pragma unroll
+          for (int u = 0; u < kMomUnroll; ++u) zc[u] = zn[u];
+        }
       }
-    };
-    auto load_v = [&](int row, int i) -> double {
-      if (MULTI) return row < nv ? V[(int64_t)row * ldv + i] : 0.0;
-      if (KIND == 1 || (KIND == 2 && row < g0 + kMomGroup - 1)) return Vs[(int64_t)row * ldv + i];
-      if (KIND == 2) return 0.0;  // the Z^2 row reads no V
-      return row < nv ? Vs[(int64_t)row * ldv + i] : 0.0;
-    };
-    // Z of the next kMomUnroll eigen-indices is loaded while the current ones
-    // are accumulated
-    int ib = i0;
-    ZT zc[kMomUnroll];
-    if (ib + kMomUnroll <= iend) {
+      for (; ib < iend; ++ib) {
+        double vv[kMomGroup];
 #pragma unroll
-      for (int u = 0; u < kMomUnroll; ++u) zc[u] = Z[(int64_t)(ib + u) * ldz + sc];
-    }
-    for (; ib + kMomUnroll <= iend; ib += kMomUnroll) {
-      const bool more = ib + 2 * kMomUnroll <= iend;
-      ZT zn[kMomUnroll];
-      if (more) {
-#pragma unroll
-        for (int u = 0; u < kMomUnroll; ++u)
-          zn[u] = Z[(int64_t)(ib + kMomUnroll + u) * ldz + sc];
+        for (int b = 0; b < kMomGroup; ++b) vv[b] = (KIND != 0 || b < ng) ? load_v(g0 + b, ib) : 0.0;
+        step((double)Z[(int64_t)ib * ldz + sc], MULTI ? delta[ib] : dls[ib], vv);
       }
-      double zu[kMomUnroll], dl[kMomUnroll], vv[kMomUnroll][kMomGroup];
-#pragma unroll
-      for (int u = 0; u < kMomUnroll; ++u) zu[u] = (double)zc[u];
-#pragma unroll
-      for (int u = 0; u < kMomUnroll; ++u) {
-        dl[u] = MULTI ? delta[ib + u] : dls[ib + u];
-#pragma unroll
-        for (int b = 0; b < kMomGroup; ++b)
-          vv[u][b] = (KIND != 0 || b < ng) ? load_v(g0 + b, ib + u) : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < kMomUnroll; ++u) step(zu[u], dl[u], vv[u]);
-      if (more) {
-#pragma unroll
-        for (int u = 0; u < kMomUnroll; ++u) zc[u] = zn[u];
-      }
-    }
-    for (; ib < iend; ++ib) {
-      double vv[kMomGroup];
-#pragma unroll
-      for (int b = 0; b < kMomGroup; ++b) vv[b] = (KIND != 0 || b < ng) ? load_v(g0 + b, ib) : 0.0;
-      step((double)Z[(int64_t)ib * ldz + sc], MULTI ? delta[ib] : dls[ib], vv);
-    }
     };
     if (kind == 1)
       run(std::integral_constant<int, 1>{});
