@@ -612,13 +612,17 @@ __global__ void __launch_bounds__(256) k_lmm_grid(jm_lmm_problem pr_one, int mod
                                                   const double* xmom, int64_t ldm, int64_t n_snps,
                                                   double* out, const jm_lmm_problem* probs,
                                                   const int32_t* modes, const int32_t* snp_prob,
-                                                  const double* const* snp_xmom, int ni_max) {
+                                                  const double* const* snp_xmom, int ni_max,
+                                                  int ps) {
+  // ps: LDS stride of the Pab columns = the lanes that hold one (G + 1 when the grid fits
+  // one pass, else 64); the lanes past it compute nothing that is kept
   extern __shared__ double smem[];
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
-  double* P = smem + (size_t)wib * (ni_max * 64 + kMaxLevels);  // column per lane
-  double* iab = P + ni_max * 64;                                  // Iab level logs
+  const bool col = lane < ps;
+  double* P = smem + (size_t)wib * (ni_max * ps + kMaxLevels);  // column per lane
+  double* iab = P + ni_max * ps;                                  // Iab level logs
   for (int64_t s = (int64_t)blockIdx.x * wpb + wib; s < n_snps; s += (int64_t)gridDim.x * wpb) {
     const int pidx = MULTI ? snp_prob[s] : 0;
     const int mode = MULTI ? modes[pidx] : mode_one;
@@ -693,18 +697,19 @@ __global__ void __launch_bounds__(256) k_lmm_grid(jm_lmm_problem pr_one, int mod
           for (int r = 0; r < 8; ++r) acc[r] = fma(kn[u], xm[roff[r] + T4 + u], acc[r]);
 #pragma unroll
         for (int r = 0; r < 8; ++r)
-          if (r0 + r < nrows) P[xpair_index(r0 + r, nc) * 64 + lane] = acc[r];
+          if (col && r0 + r < nrows) P[xpair_index(r0 + r, nc) * ps + lane] = acc[r];
       }
-      for (int j = 0; j < ni; ++j)
-        if (pr.pair_row[j] < 0)
-          P[j * 64 + lane] = is_iab ? pr.iab_shared[j] : pr.grid_shared[(int64_t)g * ni + j];
+      if (col)
+        for (int j = 0; j < ni; ++j)
+          if (pr.pair_row[j] < 0)
+            P[j * ps + lane] = is_iab ? pr.iab_shared[j] : pr.grid_shared[(int64_t)g * ni + j];
       wave_sync();
       PabSummary o;
-      pab_recursion(P + lane, 64, nc, o, nullptr, nullptr);
+      if (col) pab_recursion(P + lane, ps, nc, o, nullptr, nullptr);
       // the level pivots stay on the column's diagonal: (p+1, p+1) is final after level p
       if (k0 == 0 && lane == 0) {
         for (int p = 0; p <= nc; ++p) {
-          const double v = P[ab_index(p + 1, p + 1, nc) * 64];
+          const double v = P[ab_index(p + 1, p + 1, nc) * ps];
           iab[p] = (v > 0.0) ? log(v) : 0.0;
         }
       }
@@ -712,7 +717,7 @@ __global__ void __launch_bounds__(256) k_lmm_grid(jm_lmm_problem pr_one, int mod
       if (act && !is_iab) {
         double hiw = 0.0;  // likelihood_jax.py:322-327 order
         for (int p = 0; p <= nc; ++p) {
-          const double v = P[ab_index(p + 1, p + 1, nc) * 64 + lane];
+          const double v = P[ab_index(p + 1, p + 1, nc) * ps + lane];
           hiw = hiw + ((v > 0.0) ? log(v) : 0.0);
           hiw = hiw - iab[p];
         }
@@ -1823,7 +1828,10 @@ static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* x
   hipStream_t st = (hipStream_t)stream;
   // grid stage (skips Score-only SNPs): one wave per SNP, a Pab column per lane in LDS
   {
-    const size_t gw = (size_t)(ni_max * 64 + kMaxLevels) * sizeof(double);
+    // Pab columns: G + 1 lanes hold one when a single-problem grid fits one pass (the
+    // reference's 50 points: 51), which lets 4 waves per SIMD fit the LDS instead of 3
+    const int ps = (one && one->n_grid + 1 <= 64) ? one->n_grid + 1 : 64;
+    const size_t gw = (size_t)(ni_max * ps + kMaxLevels) * sizeof(double);
     int gwpb = (int)((64 * 1024) / gw);
     if (gwpb > 4) gwpb = 4;
     if (gwpb < 1) {
@@ -1836,13 +1844,13 @@ static int launch_assoc(const jm_lmm_problem* one, int mode_one, const double* x
       if (mode_one != 3)
         hipLaunchKernelGGL(k_lmm_grid<false>, dim3((unsigned)gblocks), dim3(64 * gwpb), gw * gwpb,
                            st, *one, mode_one, xmom, ldm, n_snps, out, nullptr, nullptr, nullptr,
-                           nullptr, ni_max);
+                           nullptr, ni_max, ps);
     } else {
       jm_lmm_problem dummy;
       memset(&dummy, 0, sizeof(dummy));
       hipLaunchKernelGGL(k_lmm_grid<true>, dim3((unsigned)gblocks), dim3(64 * gwpb), gw * gwpb, st,
                          dummy, 0, nullptr, ldm, n_snps, out, probs, modes, snp_prob, snp_xmom,
-                         ni_max);
+                         ni_max, ps);
     }
     JM_CHECK_LAUNCH();
   }
