@@ -185,9 +185,12 @@ __device__ __forceinline__ void mom_piece(const ZT* __restrict__ Z, int64_t ldz,
 // One block per (64 SNP columns, cluster piece): grid (ceil(n_snps / 64), n_seg).
 // Pieces hold at most a fixed number of eigen-indices, so the blocks carry
 // balanced work however the spectrum clusters; the piece's term count
-// (1..kMomMaxTerms, fixed by the plan) selects the instantiation.
+// (1..kMomMaxTerms, fixed by the plan) selects the instantiation. Four waves per
+// SIMD (128 VGPRs): the instantiations with many terms spill some accumulators, but
+// the occupancy pays for it (n = 10k: 0.43 -> 0.41 ms per 8192-SNP tile; n = 50k:
+// 2.47 -> 2.38 ms per 10,000-SNP tile; against three waves per SIMD at 167 VGPRs).
 template <typename ZT, bool MULTI>
-__global__ void __launch_bounds__(64 * kMomWaves) __attribute__((amdgpu_waves_per_eu(3))) k_mom_seg(const ZT* __restrict__ Z, int64_t ldz,
+__global__ void __launch_bounds__(64 * kMomWaves) __attribute__((amdgpu_waves_per_eu(4))) k_mom_seg(const ZT* __restrict__ Z, int64_t ldz,
                                                 int64_t n_snps, const double* __restrict__ V_one,
                                                 int64_t ldv_one, int nv_one,
                                                 const jm_mom_col* __restrict__ cols, int nv_max,
